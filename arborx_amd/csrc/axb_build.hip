@@ -1,0 +1,644 @@
+// axb_build.hip -- BVH construction kernels for gfx950.
+//
+// Replaces the reference's build (spatial/ArborX_LinearBVH.hpp:167-256):
+//   scene box        TreeConstruction.hpp:26-39      -> k_scene_partial/final
+//   Morton64 codes   SpaceFillingCurves.hpp:63-84    -> k_morton_hist (also
+//                    builds the radix-sort digit histograms in the same pass)
+//   sortObjects      SortUtils.hpp:26-43             -> axb_sort.hip
+//   hierarchy        TreeConstruction.hpp:71-322     -> k_climb
+// and adds the pre-order linearisation the traversal kernels read
+// (k_lin_*): the same topology stored depth-first so that a node's left child
+// is the next record and its rope is `slot + skip`.
+//
+// Pre-order position of a node (no top-down pass needed): with c[j] = number
+// of internal nodes whose leaf range starts at leaf j and C its inclusive
+// prefix sum, an internal node X with first leaf f and left-spine height h
+// (internal nodes on the path from X down to leaf f, X included) is preceded
+// by C[f] - h internal nodes and f leaves; leaf i by C[i] internal nodes and
+// i leaves.  c[] is written by the climb: the top of every left spine is the
+// root or a right child, and its height is the spine length.
+#include "axb_internal.hpp"
+
+namespace axb
+{
+
+namespace
+{
+
+// ------------------------------------------------------------ scene box
+template <int G>
+__global__ __launch_bounds__(256) void k_scene_partial(const float *values,
+                                                       int64_t n,
+                                                       float *partials)
+{
+  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+  float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride)
+  {
+    const float *v = values + i * (G == GEOM_POINTS ? 3 : 6);
+    const float *w = v + (G == GEOM_POINTS ? 0 : 3);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      lo[d] = kmin(lo[d], v[d]);
+      hi[d] = kmax(hi[d], w[d]);
+    }
+  }
+  // min/max are exact and order-free: any reduction tree gives the
+  // reference's value.
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      lo[d] = kmin(lo[d], __shfl_xor(lo[d], o, 64));
+      hi[d] = kmax(hi[d], __shfl_xor(hi[d], o, 64));
+    }
+  __shared__ float s[4][6];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int d = 0; d < 3; ++d)
+      s[wave][d] = lo[d], s[wave][3 + d] = hi[d];
+  __syncthreads();
+  if (threadIdx.x < 6)
+  {
+    const int d = threadIdx.x;
+    float r = s[0][d];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      r = d < 3 ? kmin(r, s[w][d]) : kmax(r, s[w][d]);
+    partials[blockIdx.x * 6 + d] = r;
+  }
+}
+
+__global__ void k_scene_final(const float *partials, int nblocks,
+                              float *scene)
+{
+  if (threadIdx.x < 6)
+  {
+    const int d = threadIdx.x;
+    float r = d < 3 ? FLT_MAX : -FLT_MAX;
+    for (int b = 0; b < nblocks; ++b)
+      r = d < 3 ? kmin(r, partials[b * 6 + d]) : kmax(r, partials[b * 6 + d]);
+    scene[d] = r;
+  }
+}
+
+// ------------------------------------------------- Morton64 + histograms
+template <int G>
+__global__ __launch_bounds__(256) void k_morton_hist(
+    const float *__restrict__ values, int64_t n,
+    const float *__restrict__ scene, uint64_t *__restrict__ keys,
+    uint32_t *__restrict__ hist)
+{
+  __shared__ uint32_t s_h[8 * 256];
+  for (int j = threadIdx.x; j < 8 * 256; j += blockDim.x)
+    s_h[j] = 0;
+  const float a0 = scene[0], a1 = scene[1], a2 = scene[2];
+  const float b0 = scene[3], b1 = scene[4], b2 = scene[5];
+  __syncthreads();
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride)
+  {
+    float cx, cy, cz;
+    if (G == GEOM_POINTS)
+    {
+      cx = values[3 * i], cy = values[3 * i + 1], cz = values[3 * i + 2];
+    }
+    else
+    {
+      // Centroid.hpp:49-60
+      const float *v = values + 6 * i;
+      cx = (v[0] + v[3]) / 2, cy = (v[1] + v[4]) / 2, cz = (v[2] + v[5]) / 2;
+    }
+    uint64_t k = morton64(translate_and_scale(cx, a0, b0),
+                          translate_and_scale(cy, a1, b1),
+                          translate_and_scale(cz, a2, b2));
+    keys[i] = k;
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+      atomicAdd(&s_h[p * 256 + (uint32_t)((k >> (8 * p)) & 255)], 1u);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < 8 * 256; j += blockDim.x)
+    if (s_h[j])
+      atomicAdd(&hist[j], s_h[j]);
+}
+
+// ------------------------------------------------------------ climb
+// TreeConstruction.hpp:127-170.
+AXB_D int64_t apetrei_delta(const uint64_t *__restrict__ keys, int nint,
+                            int i)
+{
+  if (i < 0 || i >= nint)
+    return INT64_MAX;
+  uint64_t x = keys[i] ^ keys[i + 1];
+  return x ? (int64_t)(x - 1ull)
+           : (int64_t)((uint64_t)INT64_MIN + (uint64_t)(i ^ (i + 1)) - 1ull);
+}
+// TreeConstruction.hpp:172-195.
+AXB_D int32_t apetrei_rope(const uint64_t *__restrict__ keys, int nint, int n,
+                           int range_right, int64_t delta_right)
+{
+  if (range_right != nint)
+    return delta_right < apetrei_delta(keys, nint, range_right + 1)
+               ? range_right + 1
+               : range_right + 1 + n;
+  return -1;
+}
+
+AXB_D int cas_agent(int32_t *p, int32_t expected, int32_t desired)
+{
+  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_ACQ_REL,
+                                       __ATOMIC_ACQUIRE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return expected;
+}
+
+// TreeConstruction.hpp:197-311: one thread per leaf climbs toward the root;
+// the first thread to reach a parent stops (CAS on ranges), the second
+// finishes it.  Writes INode records, sorted leaf records, the spine data for
+// the pre-order positions and (optionally) the reference-form ropes.
+template <int G, bool ROPES>
+__global__ __launch_bounds__(256) void k_climb(
+    const float *__restrict__ values, int n, const uint64_t *__restrict__ keys,
+    const uint32_t *__restrict__ perm, int32_t *ranges, INode *inodes,
+    float4 *leafrec, int32_t *__restrict__ leaf_rope, int32_t *__restrict__ c,
+    int32_t *__restrict__ top)
+{
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i0 >= n)
+    return;
+  const int nint = n - 1;
+  const uint32_t orig = perm[i0];
+  Box6 bv;
+  if (G == GEOM_POINTS)
+  {
+    const float *v = values + 3 * (int64_t)orig;
+    float x = v[0], y = v[1], z = v[2];
+    // Box{} expanded by the point (Expand.hpp:42-57)
+    bv.lo[0] = kmin(FLT_MAX, x), bv.lo[1] = kmin(FLT_MAX, y),
+    bv.lo[2] = kmin(FLT_MAX, z);
+    bv.hi[0] = kmax(-FLT_MAX, x), bv.hi[1] = kmax(-FLT_MAX, y),
+    bv.hi[2] = kmax(-FLT_MAX, z);
+    leafrec[i0] = make_float4(x, y, z, bits_f(~orig));
+  }
+  else
+  {
+    const float *v = values + 6 * (int64_t)orig;
+    for (int d = 0; d < 3; ++d)
+      bv.lo[d] = kmin(FLT_MAX, v[d]), bv.hi[d] = kmax(-FLT_MAX, v[3 + d]);
+    leafrec[2 * i0] = make_float4(bv.lo[0], bv.lo[1], bv.lo[2], bits_f(~orig));
+    leafrec[2 * i0 + 1] = make_float4(bv.hi[0], bv.hi[1], bv.hi[2], 0.f);
+  }
+  int range_left = i0, range_right = i0;
+  int64_t dl = apetrei_delta(keys, nint, range_left - 1);
+  int64_t dr = apetrei_delta(keys, nint, range_right);
+  if (ROPES)
+    leaf_rope[i0] = apetrei_rope(keys, nint, n, range_right, dr);
+  int cur = i0;   // reference encoding of the current node
+  int cur_h = 0;  // left-spine height of the current node (0 for a leaf)
+  do
+  {
+    const bool is_left = dr < dl;
+    int ap, left_child, h_left, right_node, h_right;
+    Box6 sb;
+    if (is_left)
+    {
+      ap = range_right;
+      int old = cas_agent(&ranges[ap], -1, range_left);
+      if (old == -1)
+        break;
+      range_right = old;
+      left_child = cur;
+      h_left = cur_h;
+      const int rc = ap + 1;
+      const bool rc_leaf = (rc == range_right);
+      dr = apetrei_delta(keys, nint, range_right);
+      if (rc_leaf)
+      {
+        float4 a = leafrec[G == GEOM_POINTS ? rc : 2 * rc];
+        float4 b = G == GEOM_POINTS ? a : leafrec[2 * rc + 1];
+        sb = {{a.x, a.y, a.z}, {b.x, b.y, b.z}};
+        h_right = 0;
+        right_node = rc;
+      }
+      else
+      {
+        const INode &r = inodes[rc];
+        sb = {{r.lo[0], r.lo[1], r.lo[2]}, {r.hi[0], r.hi[1], r.hi[2]}};
+        h_right = r.h;
+        right_node = rc + n;
+      }
+    }
+    else
+    {
+      ap = range_left - 1;
+      int old = cas_agent(&ranges[ap], -1, range_right);
+      if (old == -1)
+        break;
+      range_left = old;
+      const int lc = ap;
+      const bool lc_leaf = (lc == range_left);
+      dl = apetrei_delta(keys, nint, range_left - 1);
+      if (lc_leaf)
+      {
+        float4 a = leafrec[G == GEOM_POINTS ? lc : 2 * lc];
+        float4 b = G == GEOM_POINTS ? a : leafrec[2 * lc + 1];
+        sb = {{a.x, a.y, a.z}, {b.x, b.y, b.z}};
+        h_left = 0;
+        left_child = lc;
+      }
+      else
+      {
+        const INode &l = inodes[lc];
+        sb = {{l.lo[0], l.lo[1], l.lo[2]}, {l.hi[0], l.hi[1], l.hi[2]}};
+        h_left = l.h;
+        left_child = lc + n;
+      }
+      right_node = cur;
+      h_right = cur_h;
+    }
+    for (int d = 0; d < 3; ++d)
+    {
+      bv.lo[d] = kmin(bv.lo[d], sb.lo[d]);
+      bv.hi[d] = kmax(bv.hi[d], sb.hi[d]);
+    }
+    const int kp = dr < dl ? range_right : range_left;
+    const int hp = left_child < n ? 1 : h_left + 1;
+    INode rec;
+    rec.lo[0] = bv.lo[0], rec.lo[1] = bv.lo[1], rec.lo[2] = bv.lo[2];
+    rec.hi[0] = bv.hi[0], rec.hi[1] = bv.hi[1], rec.hi[2] = bv.hi[2];
+    rec.left_child = left_child;
+    rec.rope = ROPES ? apetrei_rope(keys, nint, n, range_right, dr) : -1;
+    rec.first = range_left;
+    rec.last = range_right;
+    rec.h = hp;
+    rec.pad = 0;
+    inodes[kp] = rec;
+    if (right_node >= n)
+    {
+      // the right child tops a left spine starting at leaf ap + 1
+      c[ap + 1] = h_right;
+      top[ap + 1] = right_node - n;
+    }
+    cur = kp + n;
+    cur_h = hp;
+    if (kp == 0)
+    {
+      c[0] = hp;
+      top[0] = 0;
+    }
+  } while (cur != n);
+}
+
+// ------------------------------------------------------------ linearise
+template <int G>
+__global__ __launch_bounds__(256) void k_lin_fp32(const INode *__restrict__ inodes,
+                                                  int nint,
+                                                  const int32_t *__restrict__ C,
+                                                  uint4 *__restrict__ slots, uint32_t total,
+                                                  uint32_t *err)
+{
+  constexpr uint32_t SL = leaf_slots<G>(), SI = 2;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nint)
+    return;
+  const INode r = inodes[k];
+  const uint32_t pos = SI * (uint32_t)(C[r.first] - r.h) + SL * (uint32_t)r.first;
+  const uint32_t len = (uint32_t)(r.last - r.first + 1);
+  const uint32_t skip = SI * (len - 1) + SL * len;
+  if (pos + SI > total || pos + skip > total)
+  {
+    raise_error(err, KERR_LAYOUT);
+    return;
+  }
+  slots[pos] = make_uint4(f_bits(r.lo[0]), f_bits(r.lo[1]), f_bits(r.lo[2]), skip);
+  slots[pos + 1] = make_uint4(f_bits(r.hi[0]), f_bits(r.hi[1]), f_bits(r.hi[2]), 0u);
+}
+
+// Q16: one thread per left spine (rooted at the root or at a right child,
+// whose frame is the scene box); walking down the spine, each left child is
+// encoded relative to the decoded box of its parent.
+template <int G>
+__global__ __launch_bounds__(256) void k_lin_q16(const INode *__restrict__ inodes,
+                                                 int n,
+                                                 const int32_t *__restrict__ c,
+                                                 const int32_t *__restrict__ top,
+                                                 const int32_t *__restrict__ C,
+                                                 const float *__restrict__ scene,
+                                                 uint4 *__restrict__ slots,
+                                                 uint32_t total, uint32_t *err)
+{
+  constexpr uint32_t SL = leaf_slots<G>(), SI = 1;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || c[j] == 0)
+    return;
+  Box6 f;
+  for (int d = 0; d < 3; ++d)
+    f.lo[d] = scene[d], f.hi[d] = scene[3 + d];
+  int k = top[j];
+  while (true)
+  {
+    const INode r = inodes[k];
+    uint32_t q[6];
+    Box6 dec;
+    for (int d = 0; d < 3; ++d)
+    {
+      if (r.lo[d] < f.lo[d] || r.hi[d] > f.hi[d])
+        raise_error(err, KERR_FRAME);
+      q16_encode_axis(f.lo[d], f.hi[d], r.lo[d], r.hi[d], q[d], q[3 + d]);
+      const float s = q16_scale(f.lo[d], f.hi[d]);
+      dec.lo[d] = q16_lo(f.lo[d], s, q[d]);
+      dec.hi[d] = q16_hi(f.hi[d], s, q[3 + d]);
+    }
+    const uint32_t pos = SI * (uint32_t)(C[r.first] - r.h) + SL * (uint32_t)r.first;
+    const uint32_t len = (uint32_t)(r.last - r.first + 1);
+    const uint32_t skip = SI * (len - 1) + SL * len;
+    if (pos + SI > total || pos + skip > total)
+    {
+      raise_error(err, KERR_LAYOUT);
+      return;
+    }
+    slots[pos] = make_uint4(q[0] | (q[1] << 16), q[2] | (q[3] << 16),
+                            q[4] | (q[5] << 16), skip);
+    if (r.left_child < n)
+      break;
+    k = r.left_child - n;
+    f = dec;
+  }
+}
+
+template <int G, int SI>
+__global__ __launch_bounds__(256) void k_lin_leaves(const float4 *__restrict__ leafrec,
+                                                    int n,
+                                                    const int32_t *__restrict__ C,
+                                                    uint4 *__restrict__ slots, uint32_t total,
+                                                    uint32_t *err)
+{
+  constexpr uint32_t SL = leaf_slots<G>();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const uint32_t pos = SL * (uint32_t)i + (uint32_t)SI * (uint32_t)C[i];
+  if (pos + SL > total)
+  {
+    raise_error(err, KERR_LAYOUT);
+    return;
+  }
+  const uint4 *src = (const uint4 *)leafrec;
+  slots[pos] = src[SL * i];
+  if (SL == 2)
+    slots[pos + 1] = src[SL * i + 1];
+}
+
+template <int G>
+__global__ void k_single_leaf(const float *values, uint4 *slots, float *scene)
+{
+  if (threadIdx.x != 0)
+    return;
+  const float *v = values;
+  const float *w = values + (G == GEOM_POINTS ? 0 : 3);
+  float lo[3], hi[3];
+  for (int d = 0; d < 3; ++d)
+    lo[d] = kmin(FLT_MAX, v[d]), hi[d] = kmax(-FLT_MAX, w[d]);
+  slots[0] = make_uint4(f_bits(lo[0]), f_bits(lo[1]), f_bits(lo[2]), ~0u);
+  if (G == GEOM_BOXES)
+    slots[1] = make_uint4(f_bits(hi[0]), f_bits(hi[1]), f_bits(hi[2]), 0u);
+  for (int d = 0; d < 3; ++d)
+    scene[d] = lo[d], scene[3 + d] = hi[d];
+}
+
+__global__ void k_export_topology(const INode *inodes, int nint,
+                                  int32_t *left_child, int32_t *rope,
+                                  float *boxes)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nint)
+    return;
+  const INode r = inodes[k];
+  if (left_child)
+    left_child[k] = r.left_child;
+  if (rope)
+    rope[k] = r.rope;
+  if (boxes)
+    for (int d = 0; d < 3; ++d)
+      boxes[6 * k + d] = r.lo[d], boxes[6 * k + 3 + d] = r.hi[d];
+}
+
+inline unsigned grid_for(int64_t n, int block = 256)
+{
+  return (unsigned)div_up(n, block);
+}
+
+} // namespace
+
+// ------------------------------------------------------------ host driver
+
+int64_t total_slots_for(int64_t n, int geometry, int node_format)
+{
+  if (n == 0)
+    return 0;
+  const int sl = geometry == GEOM_POINTS ? 1 : 2;
+  const int si = node_format == NODES_FP32 ? 2 : 1;
+  return sl * n + si * (n - 1);
+}
+
+// Builds the tree; all device buffers in `out` are allocated here with
+// dalloc on `stream`.  keep_topology keeps keys/perm/leaf_rope/inodes.
+hipError_t build_tree(const float *values, int64_t n, int geometry,
+                      int node_format, bool keep_topology, uint32_t *err,
+                      hipStream_t stream, BuildOut &out)
+{
+  out = BuildOut{};
+  out.total_slots = total_slots_for(n, geometry, node_format);
+  out.scene = (float *)dalloc(6 * sizeof(float), stream);
+  out.slots = (uint4 *)dalloc((size_t)(out.total_slots + 1) * sizeof(uint4), stream);
+  if (!out.scene || !out.slots)
+    return hipErrorOutOfMemory;
+  // padding slot after the last record (kernels may read slot p + 1)
+  hipError_t e0 = fill_u32((uint32_t *)(out.slots + out.total_slots), 0u, 4, stream);
+  if (e0 != hipSuccess)
+    return e0;
+  if (n == 0)
+  {
+    const float empty[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+    return hipMemcpyAsync(out.scene, empty, sizeof(empty), hipMemcpyHostToDevice, stream);
+  }
+  if (n == 1)
+  {
+    out.C = (int32_t *)dalloc(sizeof(int32_t), stream);
+    e0 = fill_u32((uint32_t *)out.C, 0u, 1, stream);
+    if (e0 != hipSuccess)
+      return e0;
+    if (geometry == GEOM_POINTS)
+      AXB_LAUNCH(k_single_leaf<GEOM_POINTS>, dim3(1), dim3(64), 0, stream, values,
+                         out.slots, out.scene);
+    else
+      AXB_LAUNCH(k_single_leaf<GEOM_BOXES>, dim3(1), dim3(64), 0, stream, values,
+                         out.slots, out.scene);
+    return hipGetLastError();
+  }
+  const int ni = (int)n, nint = ni - 1;
+  const int leafrec_f4 = geometry == GEOM_POINTS ? 1 : 2;
+
+  // ---- workspace
+  const int partial_blocks = (int)std::min<int64_t>(1024, div_up(n, 256));
+  size_t sort_ws = sort_workspace_bytes(n, 8, 8);
+  size_t scan_ws = scan_workspace_bytes(n);
+  uint64_t *keys0 = (uint64_t *)dalloc(n * sizeof(uint64_t), stream);
+  uint64_t *keys1 = (uint64_t *)dalloc(n * sizeof(uint64_t), stream);
+  uint32_t *vals0 = (uint32_t *)dalloc(n * sizeof(uint32_t), stream);
+  uint32_t *vals1 = (uint32_t *)dalloc(n * sizeof(uint32_t), stream);
+  uint32_t *hist = (uint32_t *)dalloc(8 * 256 * sizeof(uint32_t), stream);
+  float *partials = (float *)dalloc(partial_blocks * 6 * sizeof(float), stream);
+  void *ws = dalloc(std::max(sort_ws, scan_ws), stream);
+  int32_t *ranges = (int32_t *)dalloc(nint * sizeof(int32_t), stream);
+  INode *inodes = (INode *)dalloc(nint * sizeof(INode), stream);
+  float4 *leafrec = (float4 *)dalloc((size_t)n * leafrec_f4 * sizeof(float4), stream);
+  int32_t *c = (int32_t *)dalloc(n * sizeof(int32_t), stream);
+  int32_t *top = (int32_t *)dalloc(n * sizeof(int32_t), stream);
+  out.C = (int32_t *)dalloc(n * sizeof(int32_t), stream);
+  int32_t *leaf_rope = keep_topology ? (int32_t *)dalloc(n * sizeof(int32_t), stream) : nullptr;
+  if (!keys0 || !keys1 || !vals0 || !vals1 || !hist || !partials || !ws || !ranges ||
+      !inodes || !leafrec || !c || !top || !out.C || (keep_topology && !leaf_rope))
+    return hipErrorOutOfMemory;
+
+  // ---- scene box
+  if (geometry == GEOM_POINTS)
+    AXB_LAUNCH(k_scene_partial<GEOM_POINTS>, dim3(partial_blocks), dim3(256), 0,
+                       stream, values, n, partials);
+  else
+    AXB_LAUNCH(k_scene_partial<GEOM_BOXES>, dim3(partial_blocks), dim3(256), 0,
+                       stream, values, n, partials);
+  AXB_LAUNCH(k_scene_final, dim3(1), dim3(64), 0, stream, partials, partial_blocks,
+                     out.scene);
+
+  // ---- Morton codes + digit histograms
+  e0 = fill_u32(hist, 0u, 8 * 256, stream);
+  if (e0 != hipSuccess)
+    return e0;
+  {
+    const unsigned g = (unsigned)std::min<int64_t>(2048, div_up(n, 256));
+    if (geometry == GEOM_POINTS)
+      AXB_LAUNCH(k_morton_hist<GEOM_POINTS>, dim3(g), dim3(256), 0, stream, values,
+                         n, out.scene, keys0, hist);
+    else
+      AXB_LAUNCH(k_morton_hist<GEOM_BOXES>, dim3(g), dim3(256), 0, stream, values,
+                         n, out.scene, keys0, hist);
+  }
+
+  // ---- sort (8 passes of 8 bits; result back in keys0 / vals0)
+  hipError_t e = radix_sort_pairs_u64(keys0, keys1, vals0, vals1, n, 8, hist, ws, err, stream);
+  if (e != hipSuccess)
+    return e;
+
+  // ---- hierarchy
+  e = fill_u32((uint32_t *)ranges, 0xffffffffu, nint, stream);
+  if (e == hipSuccess)
+    e = fill_u32((uint32_t *)c, 0u, n, stream);
+  if (e != hipSuccess)
+    return e;
+  {
+    const unsigned g = grid_for(n);
+    if (geometry == GEOM_POINTS)
+    {
+      if (keep_topology)
+        AXB_LAUNCH((k_climb<GEOM_POINTS, true>), dim3(g), dim3(256), 0, stream,
+                           values, ni, keys0, vals0, ranges, inodes, leafrec, leaf_rope, c, top);
+      else
+        AXB_LAUNCH((k_climb<GEOM_POINTS, false>), dim3(g), dim3(256), 0, stream,
+                           values, ni, keys0, vals0, ranges, inodes, leafrec, leaf_rope, c, top);
+    }
+    else
+    {
+      if (keep_topology)
+        AXB_LAUNCH((k_climb<GEOM_BOXES, true>), dim3(g), dim3(256), 0, stream, values,
+                           ni, keys0, vals0, ranges, inodes, leafrec, leaf_rope, c, top);
+      else
+        AXB_LAUNCH((k_climb<GEOM_BOXES, false>), dim3(g), dim3(256), 0, stream,
+                           values, ni, keys0, vals0, ranges, inodes, leafrec, leaf_rope, c, top);
+    }
+  }
+  e = inclusive_scan_i32(c, out.C, n, ws, err, stream);
+  if (e != hipSuccess)
+    return e;
+
+  // ---- pre-order linearisation
+  {
+    const unsigned gi = grid_for(nint), gl = grid_for(n);
+    if (node_format == NODES_FP32)
+    {
+      if (geometry == GEOM_POINTS)
+      {
+        AXB_LAUNCH(k_lin_fp32<GEOM_POINTS>, dim3(gi), dim3(256), 0, stream, inodes,
+                           nint, out.C, out.slots, (uint32_t)out.total_slots, err);
+        AXB_LAUNCH((k_lin_leaves<GEOM_POINTS, 2>), dim3(gl), dim3(256), 0, stream,
+                           leafrec, ni, out.C, out.slots, (uint32_t)out.total_slots, err);
+      }
+      else
+      {
+        AXB_LAUNCH(k_lin_fp32<GEOM_BOXES>, dim3(gi), dim3(256), 0, stream, inodes,
+                           nint, out.C, out.slots, (uint32_t)out.total_slots, err);
+        AXB_LAUNCH((k_lin_leaves<GEOM_BOXES, 2>), dim3(gl), dim3(256), 0, stream,
+                           leafrec, ni, out.C, out.slots, (uint32_t)out.total_slots, err);
+      }
+    }
+    else
+    {
+      if (geometry == GEOM_POINTS)
+      {
+        AXB_LAUNCH(k_lin_q16<GEOM_POINTS>, dim3(gl), dim3(256), 0, stream, inodes, ni,
+                           c, top, out.C, out.scene, out.slots, (uint32_t)out.total_slots, err);
+        AXB_LAUNCH((k_lin_leaves<GEOM_POINTS, 1>), dim3(gl), dim3(256), 0, stream,
+                           leafrec, ni, out.C, out.slots, (uint32_t)out.total_slots, err);
+      }
+      else
+      {
+        AXB_LAUNCH(k_lin_q16<GEOM_BOXES>, dim3(gl), dim3(256), 0, stream, inodes, ni,
+                           c, top, out.C, out.scene, out.slots, (uint32_t)out.total_slots, err);
+        AXB_LAUNCH((k_lin_leaves<GEOM_BOXES, 1>), dim3(gl), dim3(256), 0, stream,
+                           leafrec, ni, out.C, out.slots, (uint32_t)out.total_slots, err);
+      }
+    }
+  }
+  e = hipGetLastError();
+
+  dfree(keys1, stream);
+  dfree(vals1, stream);
+  dfree(hist, stream);
+  dfree(partials, stream);
+  dfree(ws, stream);
+  dfree(ranges, stream);
+  dfree(leafrec, stream);
+  dfree(c, stream);
+  dfree(top, stream);
+  if (keep_topology)
+  {
+    out.keys = keys0;
+    out.perm = vals0;
+    out.leaf_rope = leaf_rope;
+    out.inodes = inodes;
+  }
+  else
+  {
+    dfree(keys0, stream);
+    dfree(vals0, stream);
+    dfree(inodes, stream);
+  }
+  return e;
+}
+
+hipError_t export_topology(const INode *inodes, int64_t nint, int32_t *left_child,
+                           int32_t *rope, float *boxes, hipStream_t stream)
+{
+  if (nint <= 0)
+    return hipSuccess;
+  AXB_LAUNCH(k_export_topology, dim3(grid_for(nint)), dim3(256), 0, stream, inodes,
+                     (int)nint, left_child, rope, boxes);
+  return hipGetLastError();
+}
+
+} // namespace axb

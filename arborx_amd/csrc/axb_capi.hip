@@ -1,0 +1,690 @@
+// axb_capi.hip -- the extern "C" boundary (include/arborx_mi355x.h).
+//
+// Orchestrates the kernels of axb_build.hip / axb_sort.hip / axb_query.hip /
+// axb_energy.hip on the caller's stream, mirroring the reference's host-side
+// control flow (spatial/ArborX_LinearBVH.hpp:167-313,
+// detail/ArborX_CrsGraphWrapperImpl.hpp:145-466).  Host synchronisation
+// happens where the reference has one: after the build (bounds(),
+// TreeConstruction.hpp:108-113) and after the counting pass of a spatial
+// query (lastElement, CrsGraphWrapperImpl.hpp:248).
+#include "../../include/arborx_mi355x.h"
+#include "axb_internal.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <vector>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+namespace axb
+{
+namespace
+{
+thread_local std::string g_error;
+std::mutex g_alloc_mutex;
+axb_alloc_fn g_alloc = nullptr;
+axb_free_fn g_free = nullptr;
+void *g_alloc_ctx = nullptr;
+} // namespace
+
+void set_error(const std::string &msg) { g_error = msg; }
+
+// ------------------------------------------------------------ profiling
+namespace
+{
+struct ProfRec
+{
+  std::string name;
+  hipEvent_t a, b;
+};
+std::atomic<bool> g_prof_timing{false};
+std::atomic<bool> g_prof_visits{false};
+std::mutex g_prof_mutex;
+std::vector<ProfRec> g_prof_recs;
+unsigned long long *g_visits = nullptr; // device, 4 counters
+} // namespace
+
+ProfScope::ProfScope(const char *name, hipStream_t s) : name_(name), s_(s), on_(g_prof_timing.load())
+{
+  if (on_)
+  {
+    on_ = hipEventCreate(&a_) == hipSuccess && hipEventCreate(&b_) == hipSuccess &&
+          hipEventRecord(a_, s_) == hipSuccess;
+  }
+}
+ProfScope::~ProfScope()
+{
+  if (on_ && hipEventRecord(b_, s_) == hipSuccess)
+  {
+    std::lock_guard<std::mutex> lock(g_prof_mutex);
+    g_prof_recs.push_back({name_, a_, b_});
+  }
+}
+unsigned long long *visit_counters() { return g_prof_visits.load() ? g_visits : nullptr; }
+
+void *dalloc(size_t bytes, hipStream_t stream)
+{
+  void *p = nullptr;
+  if (bytes == 0)
+    bytes = 16;
+  if (hipMallocAsync(&p, bytes, stream) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
+}
+void dfree(void *p, hipStream_t stream)
+{
+  if (p)
+    (void)hipFreeAsync(p, stream);
+}
+
+// Result arrays the caller owns.
+static void *out_alloc(size_t bytes, hipStream_t stream)
+{
+  std::lock_guard<std::mutex> lock(g_alloc_mutex);
+  if (g_alloc)
+    return g_alloc(bytes == 0 ? 16 : bytes, g_alloc_ctx);
+  return dalloc(bytes, stream);
+}
+static void out_free(void *p, hipStream_t stream)
+{
+  if (!p)
+    return;
+  std::lock_guard<std::mutex> lock(g_alloc_mutex);
+  if (g_free)
+    g_free(p, g_alloc_ctx);
+  else if (!g_alloc)
+    dfree(p, stream);
+}
+
+} // namespace axb
+
+using namespace axb;
+
+struct axb_bvh
+{
+  int64_t n = 0;
+  int geometry = 0;
+  int node_format = 0;
+  bool keep_topology = false;
+  BuildOut b{};
+  float bounds[6];
+  hipStream_t stream = nullptr;
+  uint32_t *d_err = nullptr;
+};
+
+#define AXB_FAIL(code, msg)                                                                       \
+  do                                                                                              \
+  {                                                                                               \
+    ::axb::set_error(msg);                                                                        \
+    return code;                                                                                  \
+  } while (0)
+
+static axb_status hip_fail(hipError_t e, const char *where)
+{
+  set_error(std::string(where) + ": " + hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? AXB_ERR_OUT_OF_MEMORY : AXB_ERR_HIP;
+}
+
+// Reads and clears a kernel error word (the stream must be synchronised).
+static axb_status check_kernel_error(uint32_t *d_err, hipStream_t s, const char *where)
+{
+  uint32_t h = 0;
+  hipError_t e = hipMemcpyAsync(&h, d_err, sizeof(h), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(s);
+  if (e != hipSuccess)
+    return hip_fail(e, where);
+  if (h)
+  {
+    (void)fill_u32(d_err, 0u, 1, s);
+    std::string m = std::string(where) + ": kernel reported";
+    if (h & KERR_SPIN)
+      m += " bounded-wait timeout";
+    if (h & KERR_STACK)
+      m += " traversal stack overflow";
+    if (h & KERR_FRAME)
+      m += " Q16 box outside its frame";
+    if (h & KERR_LAYOUT)
+      m += " inconsistent node layout";
+    set_error(m);
+    return AXB_ERR_INTERNAL;
+  }
+  return AXB_OK;
+}
+
+static TreeView view_of(const axb_bvh *t)
+{
+  TreeView tv;
+  tv.slots = t->b.slots;
+  tv.total = (uint32_t)t->b.total_slots;
+  tv.n = (int32_t)t->n;
+  tv.C = t->b.C;
+  tv.err = t->d_err;
+  tv.visits = visit_counters();
+  for (int d = 0; d < 3; ++d)
+  {
+    tv.scene.lo[d] = t->bounds[d];
+    tv.scene.hi[d] = t->bounds[3 + d];
+    tv.ss[d] = q16_scale(t->bounds[d], t->bounds[3 + d]);
+  }
+  return tv;
+}
+
+extern "C" {
+
+const char *axb_last_error(void) { return g_error.c_str(); }
+const char *axb_version(void) { return "arborx-mi355x 0.1 (gfx950)"; }
+
+axb_status axb_set_output_allocator(axb_alloc_fn alloc, axb_free_fn free_fn, void *ctx)
+{
+  if ((alloc == nullptr) != (free_fn == nullptr))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_set_output_allocator: give both functions or none");
+  std::lock_guard<std::mutex> lock(g_alloc_mutex);
+  g_alloc = alloc;
+  g_free = free_fn;
+  g_alloc_ctx = ctx;
+  return AXB_OK;
+}
+
+axb_status axb_free(void *device_ptr, axb_stream_t stream)
+{
+  if (device_ptr)
+  {
+    hipError_t e = hipFreeAsync(device_ptr, (hipStream_t)stream);
+    if (e != hipSuccess)
+      return hip_fail(e, "axb_free");
+  }
+  return AXB_OK;
+}
+
+float axb_sqrt_threshold(float r) { return sqrt_threshold(r); }
+
+uint64_t axb_morton64_host(const float p[3], const float scene[6])
+{
+  return morton64(translate_and_scale(p[0], scene[0], scene[3]),
+                  translate_and_scale(p[1], scene[1], scene[4]),
+                  translate_and_scale(p[2], scene[2], scene[5]));
+}
+
+axb_status axb_bvh_create(const float *values, int64_t n, int32_t geometry, int32_t node_format,
+                          int32_t flags, axb_stream_t stream_, axb_bvh **out)
+{
+  if (!out)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_create: out is NULL");
+  *out = nullptr;
+  if (n < 0 || n >= (int64_t(1) << 30))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_create: n must be in [0, 2^30)");
+  if (geometry != AXB_GEOMETRY_POINTS && geometry != AXB_GEOMETRY_BOXES)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_create: unknown geometry");
+  if (node_format != AXB_NODES_FP32 && node_format != AXB_NODES_Q16)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_create: unknown node format");
+  if (n > 0 && !values)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_create: values is NULL");
+  hipStream_t s = (hipStream_t)stream_;
+  axb_bvh *t = new axb_bvh;
+  t->n = n;
+  t->geometry = geometry;
+  t->node_format = node_format;
+  t->keep_topology = (flags & AXB_BUILD_KEEP_TOPOLOGY) != 0;
+  t->stream = s;
+  t->d_err = (uint32_t *)dalloc(sizeof(uint32_t), s);
+  if (!t->d_err)
+  {
+    delete t;
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bvh_create: out of device memory");
+  }
+  if (fill_u32(t->d_err, 0u, 1, s) != hipSuccess)
+  {
+    dfree(t->d_err, s);
+    delete t;
+    AXB_FAIL(AXB_ERR_HIP, "axb_bvh_create: kernel launch failed");
+  }
+  hipError_t e = build_tree(values, n, geometry, node_format, t->keep_topology, t->d_err, s, t->b);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(t->bounds, t->b.scene, sizeof(t->bounds), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(s);
+  if (e != hipSuccess)
+  {
+    axb_status st = hip_fail(e, "axb_bvh_create");
+    axb_bvh_destroy(t);
+    return st;
+  }
+  axb_status st = check_kernel_error(t->d_err, s, "axb_bvh_create");
+  if (st != AXB_OK)
+  {
+    axb_bvh_destroy(t);
+    return st;
+  }
+  if (node_format == AXB_NODES_Q16 && n > 1)
+  {
+    for (int d = 0; d < 3; ++d)
+      if (!std::isfinite(t->bounds[3 + d] - t->bounds[d]))
+      {
+        axb_bvh_destroy(t);
+        AXB_FAIL(AXB_ERR_INVALID_ARGUMENT,
+                 "axb_bvh_create: Q16 nodes need a scene box of finite extent");
+      }
+  }
+  *out = t;
+  return AXB_OK;
+}
+
+void axb_bvh_destroy(axb_bvh *t)
+{
+  if (!t)
+    return;
+  hipStream_t s = t->stream;
+  dfree(t->b.slots, s);
+  dfree(t->b.scene, s);
+  dfree(t->b.C, s);
+  dfree(t->b.keys, s);
+  dfree(t->b.perm, s);
+  dfree(t->b.leaf_rope, s);
+  dfree(t->b.inodes, s);
+  dfree(t->d_err, s);
+  delete t;
+}
+
+int64_t axb_bvh_size(const axb_bvh *t) { return t ? t->n : 0; }
+
+axb_status axb_bvh_bounds(const axb_bvh *t, float out_host[6])
+{
+  if (!t || !out_host)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_bounds: NULL argument");
+  std::memcpy(out_host, t->bounds, sizeof(t->bounds));
+  return AXB_OK;
+}
+
+int64_t axb_bvh_num_slots(const axb_bvh *t) { return t ? t->b.total_slots : 0; }
+const void *axb_bvh_slots(const axb_bvh *t) { return t ? t->b.slots : nullptr; }
+
+axb_status axb_bvh_export_topology(const axb_bvh *t, uint64_t *keys, uint32_t *perm,
+                                   int32_t *leaf_rope, int32_t *left_child, int32_t *rope,
+                                   float *boxes, axb_stream_t stream_)
+{
+  if (!t)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_export_topology: NULL tree");
+  if (t->n < 2)
+    return AXB_OK;
+  if (!t->keep_topology)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT,
+             "axb_bvh_export_topology: tree was built without AXB_BUILD_KEEP_TOPOLOGY");
+  hipStream_t s = (hipStream_t)stream_;
+  const size_t n = (size_t)t->n;
+  if (keys)
+    AXB_HIP_TRY(hipMemcpyAsync(keys, t->b.keys, n * 8, hipMemcpyDeviceToDevice, s));
+  if (perm)
+    AXB_HIP_TRY(hipMemcpyAsync(perm, t->b.perm, n * 4, hipMemcpyDeviceToDevice, s));
+  if (leaf_rope)
+    AXB_HIP_TRY(hipMemcpyAsync(leaf_rope, t->b.leaf_rope, n * 4, hipMemcpyDeviceToDevice, s));
+  AXB_HIP_TRY(export_topology(t->b.inodes, t->n - 1, left_child, rope, boxes, s));
+  return AXB_OK;
+}
+
+// Morton-32 processing order of predicates (CrsGraphWrapperImpl.hpp:407-419).
+static axb_status predicate_order(const axb_bvh *t, int pred_kind, const float *preds,
+                                  int64_t nq, hipStream_t s, uint32_t **order_out,
+                                  void **to_free)
+{
+  *order_out = nullptr;
+  if (nq < 2)
+    return AXB_OK;
+  const size_t ws_bytes = sort_workspace_bytes(nq, 4, 4);
+  // one block: keys0 keys1 vals0 vals1 hist workspace
+  const size_t a = (size_t)nq * 4;
+  const size_t off_k1 = a, off_v0 = 2 * a, off_v1 = 3 * a, off_h = 4 * a,
+               off_ws = 4 * a + 4 * 256 * 4;
+  char *blk = (char *)dalloc(off_ws + ws_bytes, s);
+  if (!blk)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "query: out of device memory (predicate sort)");
+  uint32_t *k0 = (uint32_t *)blk, *k1 = (uint32_t *)(blk + off_k1);
+  uint32_t *v0 = (uint32_t *)(blk + off_v0), *v1 = (uint32_t *)(blk + off_v1);
+  uint32_t *hist = (uint32_t *)(blk + off_h);
+  hipError_t e = fill_u32(hist, 0u, 4 * 256, s);
+  if (e == hipSuccess)
+    e = launch_pred_morton32(pred_kind, preds, nq, t->b.scene, k0, hist, s);
+  if (e == hipSuccess)
+    e = radix_sort_pairs_u32(k0, k1, v0, v1, nq, 4, hist, blk + off_ws, t->d_err, s);
+  if (e != hipSuccess)
+  {
+    dfree(blk, s);
+    return hip_fail(e, "query: predicate sort");
+  }
+  *order_out = v0;
+  *to_free = blk;
+  return AXB_OK;
+}
+
+static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *preds, int64_t nq,
+                                float radius, axb_traversal_policy policy, int64_t *offsets,
+                                int32_t **indices, int64_t *nnz_host, axb_stream_t stream_)
+{
+  if (!t || !offsets || !indices || !nnz_host)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: NULL argument");
+  if (nq < 0 || nq >= (int64_t(1) << 31))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: nq must be in [0, 2^31)");
+  if (nq > 0 && !preds)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: predicates is NULL");
+  hipStream_t s = (hipStream_t)stream_;
+  *indices = nullptr;
+  *nnz_host = 0;
+  if (nq == 0 || t->n == 0)
+  {
+    AXB_HIP_TRY(fill_u64((uint64_t *)offsets, 0ull, nq + 1, s));
+    *indices = (int32_t *)out_alloc(0, s);
+    AXB_HIP_TRY(hipStreamSynchronize(s));
+    return AXB_OK;
+  }
+  const TreeView tv = view_of(t);
+  uint32_t *order = nullptr;
+  void *order_blk = nullptr;
+  if (policy.sort_predicates)
+  {
+    axb_status st = predicate_order(t, pred_kind, preds, nq, s, &order, &order_blk);
+    if (st != AXB_OK)
+      return st;
+  }
+  const int32_t B = policy.buffer_size < 0 ? -policy.buffer_size : policy.buffer_size;
+  void *scan_ws = dalloc(scan_workspace_bytes(nq + 1), s);
+  int32_t *buf = nullptr;
+  uint32_t *flag = nullptr;
+  axb_status st = AXB_OK;
+  hipError_t e = scan_ws ? hipSuccess : hipErrorOutOfMemory;
+  bool second_pass = (B == 0);
+  int64_t nnz = 0;
+  if (e == hipSuccess && B > 0)
+  {
+    buf = (int32_t *)dalloc((size_t)nq * B * sizeof(int32_t), s);
+    flag = (uint32_t *)dalloc(sizeof(uint32_t), s);
+    e = (buf && flag) ? hipSuccess : hipErrorOutOfMemory;
+    if (e == hipSuccess)
+      e = launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, nq, radius, order,
+                         offsets, buf, B, MODE_BUFFER, s);
+    if (e == hipSuccess)
+      e = fill_u32(flag, 0u, 1, s);
+    if (e == hipSuccess)
+      e = launch_max_count(offsets, nq, B, flag, s);
+    uint32_t hflag = 0;
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&hflag, flag, sizeof(hflag), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+      e = hipStreamSynchronize(s);
+    if (e == hipSuccess && hflag)
+    {
+      if (policy.buffer_size < 0)
+      {
+        set_error("query: a query has more results than the hard buffer size "
+                  "(TraversalPolicy buffer_size < 0)");
+        st = AXB_ERR_BUFFER_OVERFLOW;
+      }
+      second_pass = true;
+    }
+  }
+  else if (e == hipSuccess)
+  {
+    e = launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, nq, radius, order,
+                       offsets, nullptr, 0, MODE_COUNT, s);
+  }
+  // counts -> offsets (exclusive scan over nq + 1 entries, last count 0)
+  if (e == hipSuccess && st == AXB_OK)
+    e = fill_u64((uint64_t *)(offsets + nq), 0ull, 1, s);
+  if (e == hipSuccess && st == AXB_OK)
+    e = exclusive_scan_i64(offsets, offsets, nq + 1, scan_ws, t->d_err, s);
+  if (e == hipSuccess && st == AXB_OK)
+    e = hipMemcpyAsync(&nnz, offsets + nq, sizeof(nnz), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && st == AXB_OK)
+    e = hipStreamSynchronize(s);
+  if (e == hipSuccess && st == AXB_OK)
+  {
+    *indices = (int32_t *)out_alloc((size_t)nnz * sizeof(int32_t), s);
+    if (!*indices)
+      e = hipErrorOutOfMemory;
+  }
+  if (e == hipSuccess && st == AXB_OK && nnz > 0)
+  {
+    if (second_pass)
+      e = launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, nq, radius, order,
+                         offsets, *indices, 0, MODE_FILL, s);
+    else
+      e = launch_compact(buf, nq, B, offsets, *indices, s);
+  }
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(s);
+  dfree(scan_ws, s);
+  dfree(buf, s);
+  dfree(flag, s);
+  dfree(order_blk, s);
+  if (e != hipSuccess)
+  {
+    out_free(*indices, s);
+    *indices = nullptr;
+    return hip_fail(e, "query");
+  }
+  if (st != AXB_OK)
+    return st;
+  st = check_kernel_error(t->d_err, s, "query");
+  if (st != AXB_OK)
+  {
+    out_free(*indices, s);
+    *indices = nullptr;
+    return st;
+  }
+  *nnz_host = nnz;
+  return AXB_OK;
+}
+
+axb_status axb_bvh_query_intersects(const axb_bvh *t, int32_t predicate_kind,
+                                    const float *predicates, int64_t nq,
+                                    axb_traversal_policy policy, int64_t *offsets,
+                                    int32_t **indices, int64_t *nnz_host, axb_stream_t stream)
+{
+  if (predicate_kind != AXB_PREDICATE_SPHERES && predicate_kind != AXB_PREDICATE_BOXES)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_intersects: unknown predicate kind");
+  return query_spatial(t, predicate_kind, predicates, nq, 0.f, policy, offsets, indices,
+                       nnz_host, stream);
+}
+
+axb_status axb_bvh_query_intersects_radius(const axb_bvh *t, const float *points, int64_t nq,
+                                           float radius, axb_traversal_policy policy,
+                                           int64_t *offsets, int32_t **indices,
+                                           int64_t *nnz_host, axb_stream_t stream)
+{
+  // PredicateHelpers.hpp:102: KOKKOS_ASSERT(r > 0)
+  if (!(radius > 0.f))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_intersects_radius: radius must be > 0");
+  return query_spatial(t, PRED_RADIUS, points, nq, radius, policy, offsets, indices, nnz_host,
+                       stream);
+}
+
+axb_status axb_bvh_query_nearest(const axb_bvh *t, const float *points, int64_t nq, int32_t k,
+                                 axb_traversal_policy policy, int64_t *offsets,
+                                 int32_t **indices, float **distances, int64_t *nnz_host,
+                                 axb_stream_t stream_)
+{
+  if (!t || !offsets || !indices || !nnz_host)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_nearest: NULL argument");
+  if (nq < 0 || nq >= (int64_t(1) << 31))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_nearest: nq must be in [0, 2^31)");
+  if (nq > 0 && !points)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_nearest: points is NULL");
+  hipStream_t s = (hipStream_t)stream_;
+  const int64_t per = (k < 1 || t->n == 0) ? 0 : std::min<int64_t>(k, t->n);
+  const int64_t nnz = nq * per;
+  if (nnz >= (int64_t(1) << 40))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_nearest: result too large");
+  *indices = nullptr;
+  if (distances)
+    *distances = nullptr;
+  AXB_HIP_TRY(launch_knn_offsets(offsets, nq, per, s));
+  *indices = (int32_t *)out_alloc((size_t)nnz * sizeof(int32_t), s);
+  float *dist = distances ? (float *)out_alloc((size_t)nnz * sizeof(float), s) : nullptr;
+  float *tmp_dist = nullptr;
+  if (!*indices || (distances && !dist))
+  {
+    out_free(*indices, s);
+    out_free(dist, s);
+    *indices = nullptr;
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bvh_query_nearest: out of device memory");
+  }
+  hipError_t e = hipSuccess;
+  uint32_t *order = nullptr;
+  void *order_blk = nullptr;
+  if (nnz > 0)
+  {
+    if (k > 64 && !dist)
+    {
+      tmp_dist = (float *)dalloc((size_t)nnz * sizeof(float), s);
+      if (!tmp_dist)
+        e = hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess && policy.sort_predicates)
+    {
+      axb_status st = predicate_order(t, PRED_RADIUS, points, nq, s, &order, &order_blk);
+      if (st != AXB_OK)
+      {
+        out_free(*indices, s);
+        out_free(dist, s);
+        dfree(tmp_dist, s);
+        *indices = nullptr;
+        return st;
+      }
+    }
+    if (e == hipSuccess)
+      e = launch_knn(view_of(t), t->node_format, t->geometry, points, nq, k, order, *indices,
+                     dist ? dist : tmp_dist, s);
+  }
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(s);
+  dfree(order_blk, s);
+  dfree(tmp_dist, s);
+  axb_status st = e == hipSuccess ? check_kernel_error(t->d_err, s, "axb_bvh_query_nearest")
+                                  : hip_fail(e, "axb_bvh_query_nearest");
+  if (st != AXB_OK)
+  {
+    out_free(*indices, s);
+    out_free(dist, s);
+    *indices = nullptr;
+    return st;
+  }
+  if (distances)
+    *distances = dist;
+  *nnz_host = nnz;
+  return AXB_OK;
+}
+
+axb_status axb_profile_enable(int32_t flags)
+{
+  axb_profile_clear();
+  if ((flags & 2) && !g_visits)
+  {
+    hipError_t e = hipMalloc((void **)&g_visits, 4 * sizeof(unsigned long long));
+    if (e != hipSuccess)
+      return hip_fail(e, "axb_profile_enable");
+  }
+  if (g_visits)
+  {
+    hipError_t e = hipMemset(g_visits, 0, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess)
+      e = hipDeviceSynchronize();
+    if (e != hipSuccess)
+      return hip_fail(e, "axb_profile_enable");
+  }
+  g_prof_timing.store((flags & 1) != 0);
+  g_prof_visits.store((flags & 2) != 0);
+  return AXB_OK;
+}
+
+axb_status axb_profile_clear(void)
+{
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  for (auto &r : g_prof_recs)
+  {
+    (void)hipEventSynchronize(r.b);
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  g_prof_recs.clear();
+  return AXB_OK;
+}
+
+int32_t axb_profile_num_records(void)
+{
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  return (int32_t)g_prof_recs.size();
+}
+
+axb_status axb_profile_record(int32_t i, char *name, int32_t name_len, double *ms)
+{
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  if (i < 0 || i >= (int32_t)g_prof_recs.size() || !ms)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_profile_record: bad index");
+  const ProfRec &r = g_prof_recs[i];
+  hipError_t e = hipEventSynchronize(r.b);
+  float t = 0.f;
+  if (e == hipSuccess)
+    e = hipEventElapsedTime(&t, r.a, r.b);
+  if (e != hipSuccess)
+    return hip_fail(e, "axb_profile_record");
+  *ms = t;
+  if (name && name_len > 0)
+  {
+    std::strncpy(name, r.name.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  return AXB_OK;
+}
+
+axb_status axb_profile_visits(int64_t out_host[4])
+{
+  if (!out_host)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_profile_visits: NULL");
+  for (int i = 0; i < 4; ++i)
+    out_host[i] = 0;
+  if (!g_visits)
+    return AXB_OK;
+  unsigned long long h[4];
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess)
+    e = hipMemcpy(h, g_visits, sizeof(h), hipMemcpyDeviceToHost);
+  if (e != hipSuccess)
+    return hip_fail(e, "axb_profile_visits");
+  for (int i = 0; i < 4; ++i)
+    out_host[i] = (int64_t)h[i];
+  return AXB_OK;
+}
+
+axb_status axb_pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
+                           const int32_t *indices, float cutoff, double *energy, float *grad,
+                           axb_stream_t stream)
+{
+  if (n < 0 || (n > 0 && (!xyz || !offsets)))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_pair_energy: bad arguments");
+  if (!(cutoff > 0.f))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_pair_energy: cutoff must be > 0");
+  hipError_t e = pair_energy(xyz, n, offsets, indices, cutoff, energy, grad, (hipStream_t)stream);
+  if (e != hipSuccess)
+    return hip_fail(e, "axb_pair_energy");
+  return AXB_OK;
+}
+
+axb_status axb_pair_energy_fd(const float *xyz, int64_t n, const int64_t *offsets,
+                              const int32_t *indices, float cutoff, const int32_t *atoms,
+                              int64_t m, double h, double *fd, axb_stream_t stream)
+{
+  if (n < 0 || m < 0 || (m > 0 && (!xyz || !offsets || !indices || !atoms || !fd)))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_pair_energy_fd: bad arguments");
+  if (!(h > 0.0) || !(cutoff > 0.f))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_pair_energy_fd: h and cutoff must be > 0");
+  hipError_t e =
+      pair_energy_fd(xyz, offsets, indices, cutoff, atoms, m, h, fd, (hipStream_t)stream);
+  if (e != hipSuccess)
+    return hip_fail(e, "axb_pair_energy_fd");
+  return AXB_OK;
+}
+
+} // extern "C"

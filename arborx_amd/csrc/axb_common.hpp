@@ -1,0 +1,251 @@
+// axb_common.hpp -- geometry, Morton and node-format helpers shared by the
+// gfx950 kernels and the host code of the library.
+//
+// Every float expression here is compiled with -ffp-contract=off and
+// correctly rounded division/sqrt, so it rounds exactly like the
+// reference's CPU path (no FMA contraction, IEEE division and sqrt).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+
+#define AXB_HD __host__ __device__ __forceinline__
+#define AXB_D __device__ __forceinline__
+
+namespace axb
+{
+
+// ---------------------------------------------------------------- layout
+// Pre-order node array of 16-byte slots (uint4).  Word 3 of a record's
+// first slot tells the kind: >= 0 internal node (value = number of slots of
+// its whole subtree, i.e. the rope is `slot + skip`), < 0 leaf (value = ~index).
+//   FP32 internal : 2 slots {lo.x lo.y lo.z skip} {hi.x hi.y hi.z 0}
+//   Q16 internal  : 1 slot  {qlo.x|qlo.y<<16, qlo.z|qhi.x<<16, qhi.y|qhi.z<<16, skip}
+//   point leaf    : 1 slot  {x y z ~index}
+//   box leaf      : 2 slots {lo.x lo.y lo.z ~index} {hi.x hi.y hi.z 0}
+// The left child of an internal node is the next record; its right child is
+// the rope of its left child (HappyTreeFriends.hpp:73-78).
+constexpr int GEOM_POINTS = 0;
+constexpr int GEOM_BOXES = 1;
+constexpr int NODES_FP32 = 0;
+constexpr int NODES_Q16 = 1;
+
+template <int G>
+constexpr int leaf_slots()
+{
+  return G == GEOM_POINTS ? 1 : 2;
+}
+template <int F>
+constexpr int internal_slots()
+{
+  return F == NODES_FP32 ? 2 : 1;
+}
+
+// Kernel error flags (ORed into a device word, checked by the host).
+constexpr uint32_t KERR_SPIN = 1u;   // bounded wait exceeded
+constexpr uint32_t KERR_STACK = 2u;  // traversal stack overflow
+constexpr uint32_t KERR_FRAME = 4u;  // Q16 box outside its frame
+constexpr uint32_t KERR_LAYOUT = 8u; // node array inconsistent (guarded access)
+
+struct Box6
+{
+  float lo[3];
+  float hi[3];
+};
+
+// ---------------------------------------------------------------- Morton
+// ArborX_MortonCode.hpp:64-72 (expandBitsBy<2>, 32-bit).
+AXB_HD uint32_t expand2_u32(uint32_t x)
+{
+  x &= 0x000003ffu;
+  x = (x ^ (x << 16)) & 0xff0000ffu;
+  x = (x ^ (x << 8)) & 0x0300f00fu;
+  x = (x ^ (x << 4)) & 0x030c30c3u;
+  x = (x ^ (x << 2)) & 0x09249249u;
+  return x;
+}
+// ArborX_MortonCode.hpp:188-197 (expandBitsBy<2>, 64-bit).
+AXB_HD uint64_t expand2_u64(uint64_t x)
+{
+  x &= 0x1fffffull;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+// Kokkos::clamp semantics.
+AXB_HD float kclamp(float v, float lo, float hi)
+{
+  return (v < lo) ? lo : ((hi < v) ? hi : v);
+}
+AXB_HD float kmin(float a, float b) { return (b < a) ? b : a; }
+AXB_HD float kmax(float a, float b) { return (a < b) ? b : a; }
+
+// ArborX_TranslateAndScale.hpp:23-37 (one axis).
+AXB_HD float translate_and_scale(float c, float a, float b)
+{
+  return a != b ? (c - a) / (b - a) : 0.f;
+}
+// ArborX_MortonCode.hpp:318-347, DIM=3: N = 2^21 in float.
+AXB_HD uint64_t morton64(float px, float py, float pz)
+{
+  const float N = 2097152.f, Nm1 = 2097151.f;
+  uint64_t r = 0;
+  r += expand2_u64((uint64_t)kclamp(px * N, 0.f, Nm1)) << 2;
+  r += expand2_u64((uint64_t)kclamp(py * N, 0.f, Nm1)) << 1;
+  r += expand2_u64((uint64_t)kclamp(pz * N, 0.f, Nm1));
+  return r;
+}
+// ArborX_MortonCode.hpp:291-316, DIM=3: N = 2^10 in float.
+AXB_HD uint32_t morton32(float px, float py, float pz)
+{
+  const float N = 1024.f, Nm1 = 1023.f;
+  uint32_t r = 0;
+  r += expand2_u32((uint32_t)kclamp(px * N, 0.f, Nm1)) << 2;
+  r += expand2_u32((uint32_t)kclamp(py * N, 0.f, Nm1)) << 1;
+  r += expand2_u32((uint32_t)kclamp(pz * N, 0.f, Nm1));
+  return r;
+}
+
+// ---------------------------------------------------------------- distance
+// Squared distances summed in the reference's order (ArborX_Distance.hpp:
+// 52-70: t = b - a; s += t*t starting from 0).  The sqrt is applied by the
+// caller only where a distance value is needed.
+AXB_HD float dist2_pp(float ax, float ay, float az, float bx, float by,
+                      float bz)
+{
+  float tx = bx - ax, ty = by - ay, tz = bz - az;
+  float s = tx * tx;
+  s = s + ty * ty;
+  s = s + tz * tz;
+  return s;
+}
+// ClosestPoint.hpp:48-66 + Distance.hpp:72-80.
+AXB_HD float closest(float p, float lo, float hi)
+{
+  return p < lo ? lo : (p > hi ? hi : p);
+}
+AXB_HD float dist2_pbox(float px, float py, float pz, const Box6 &b)
+{
+  return dist2_pp(px, py, pz, closest(px, b.lo[0], b.hi[0]),
+                  closest(py, b.lo[1], b.hi[1]),
+                  closest(pz, b.lo[2], b.hi[2]));
+}
+// ArborX_Intersects.hpp:51-65.
+AXB_HD bool intersects_bb(const Box6 &a, const Box6 &b)
+{
+  return !(a.lo[0] > b.hi[0] || a.hi[0] < b.lo[0] || a.lo[1] > b.hi[1] ||
+           a.hi[1] < b.lo[1] || a.lo[2] > b.hi[2] || a.hi[2] < b.lo[2]);
+}
+
+AXB_HD float bits_f(uint32_t u)
+{
+  union
+  {
+    uint32_t u;
+    float f;
+  } v;
+  v.u = u;
+  return v.f;
+}
+AXB_HD uint32_t f_bits(float f)
+{
+  union
+  {
+    uint32_t u;
+    float f;
+  } v;
+  v.f = f;
+  return v.u;
+}
+
+// Correctly rounded float sqrt: a double sqrt rounded to float is correctly
+// rounded (53 >= 2*24 + 2 bits).
+AXB_HD float sqrt_rn(float x) { return (float)sqrt((double)x); }
+
+// Largest float s with RN(sqrt(s)) <= r.  sqrt is monotone and never lands
+// exactly on the midpoint m between r and the next float (m has 25
+// significant bits, sqrt of a float that rounds there would need m^2 to be a
+// float), so RN(sqrt(s)) <= r  <=>  s < m^2, and m^2 is exact in double.
+AXB_HD float sqrt_threshold(float r)
+{
+  if (!(r >= 0.f))
+    return r < 0.f ? -1.f : r; // negative radius: nothing; NaN: nothing
+  if (r == INFINITY)
+    return INFINITY;
+  float rn = bits_f(f_bits(r) + 1u); // next float above r (may be +inf)
+  double m = 0.5 * ((double)r + (double)rn);
+  double m2 = m * m;
+  float t = (float)m2;
+  if ((double)t >= m2)
+    t = bits_f(f_bits(t) - 1u); // step below m^2 (t > 0 here)
+  return t;
+}
+
+// ---------------------------------------------------------------- Q16
+// Node box quantised on the 65535-step grid of a frame box F:
+//   lo(q) = F.lo + q * s,  hi(q) = F.hi - (65535 - q) * s,
+//   s = (F.hi - F.lo) * (1/65535)  (float, no FMA).
+// qlo is the largest q with lo(q) <= box.lo and qhi the smallest with
+// hi(q) >= box.hi, so the decoded box always contains the exact one.
+constexpr float Q16_INV = 1.0f / 65535.0f;
+
+AXB_HD float q16_scale(float flo, float fhi) { return (fhi - flo) * Q16_INV; }
+AXB_HD float q16_lo(float flo, float s, uint32_t q)
+{
+  return flo + (float)q * s;
+}
+AXB_HD float q16_hi(float fhi, float s, uint32_t q)
+{
+  return fhi - (float)(65535u - q) * s;
+}
+AXB_HD void q16_encode_axis(float flo, float fhi, float vlo, float vhi,
+                            uint32_t &qlo, uint32_t &qhi)
+{
+  float s = q16_scale(flo, fhi);
+  if (!(s > 0.f))
+  {
+    qlo = 0, qhi = 65535;
+    return;
+  }
+  int a = 0, b = 65535;
+  while (a < b)
+  {
+    int m = (a + b + 1) >> 1;
+    if (q16_lo(flo, s, (uint32_t)m) <= vlo)
+      a = m;
+    else
+      b = m - 1;
+  }
+  qlo = (uint32_t)a;
+  a = 0, b = 65535;
+  while (a < b)
+  {
+    int m = (a + b) >> 1;
+    if (q16_hi(fhi, s, (uint32_t)m) >= vhi)
+      b = m;
+    else
+      a = m + 1;
+  }
+  qhi = (uint32_t)a;
+}
+
+// Decodes a Q16 record (words x, y, z) against frame (flo, fhi, scale).
+AXB_HD void q16_decode(uint32_t w0, uint32_t w1, uint32_t w2, const Box6 &f,
+                       const float s[3], Box6 &out)
+{
+  out.lo[0] = q16_lo(f.lo[0], s[0], w0 & 0xffffu);
+  out.lo[1] = q16_lo(f.lo[1], s[1], w0 >> 16);
+  out.lo[2] = q16_lo(f.lo[2], s[2], w1 & 0xffffu);
+  out.hi[0] = q16_hi(f.hi[0], s[0], w1 >> 16);
+  out.hi[1] = q16_hi(f.hi[1], s[1], w2 & 0xffffu);
+  out.hi[2] = q16_hi(f.hi[2], s[2], w2 >> 16);
+}
+
+} // namespace axb

@@ -1,0 +1,170 @@
+// axb_energy.hip -- pair energy over a fixed neighbour CRS and its gradient.
+//
+// No reference counterpart; the pattern is the reference example's
+// Lennard-Jones loop over a radius CRS
+// (examples/molecular_dynamics/example_molecular_dynamics.cpp:128-194).
+// phi(r) = 4 (r^-12 - r^-6) - phi(cutoff);  E = 1/2 sum_i sum_{j in row i,
+// j != i} phi(r_ij);  dE/dx_i = sum_{j in row i, j != i} phi'(r)/r (x_i - x_j)
+// (rows of a radius query are symmetric).  Pair terms are float, every sum
+// is double; the block partials are reduced in a fixed order, so E is
+// bit-reproducible run to run.  One lane per row: forward and backward read
+// the row once (the fused kernel costs one gather of x_j per pair).
+#include "axb_internal.hpp"
+
+namespace axb
+{
+namespace
+{
+
+AXB_D double wave_sum_f64(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <bool WANT_G>
+__global__ __launch_bounds__(256) void k_energy(const float *__restrict__ xyz, int64_t n,
+                                                const int64_t *__restrict__ offsets,
+                                                const int32_t *__restrict__ indices, float phic,
+                                                double *__restrict__ partials,
+                                                float *__restrict__ grad)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (i < n)
+  {
+    const float xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    const int64_t b = offsets[i], end = offsets[i + 1];
+    for (int64_t p = b; p < end; ++p)
+    {
+      const int64_t j = indices[p];
+      if (j == i)
+        continue;
+      const float dx = xi - xyz[3 * j], dy = yi - xyz[3 * j + 1], dz = zi - xyz[3 * j + 2];
+      float r2 = dx * dx;
+      r2 = r2 + dy * dy;
+      r2 = r2 + dz * dz;
+      const float ir2 = 1.0f / r2;
+      const float ir6 = ir2 * ir2 * ir2;
+      e += (double)(4.0f * ir6 * (ir6 - 1.0f) - phic);
+      if (WANT_G)
+      {
+        const float f = 24.0f * ir2 * ir6 * (1.0f - 2.0f * ir6);
+        gx += (double)(f * dx);
+        gy += (double)(f * dy);
+        gz += (double)(f * dz);
+      }
+    }
+    e *= 0.5;
+    if (WANT_G)
+    {
+      grad[3 * i] = (float)gx;
+      grad[3 * i + 1] = (float)gy;
+      grad[3 * i + 2] = (float)gz;
+    }
+  }
+  __shared__ double s[4];
+  e = wave_sum_f64(e);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    s[wave] = e;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    partials[blockIdx.x] = ((s[0] + s[1]) + s[2]) + s[3];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_f64(const double *__restrict__ partials,
+                                                    int64_t m, double *__restrict__ out)
+{
+  double v = 0.0;
+  for (int64_t j = threadIdx.x; j < m; j += 256)
+    v += partials[j];
+  __shared__ double s[4];
+  v = wave_sum_f64(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    s[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    *out = ((s[0] + s[1]) + s[2]) + s[3];
+}
+
+// FP64 central differences of the row energy of each listed atom (the only
+// terms of E that depend on x_i when rows are symmetric).
+__global__ __launch_bounds__(256) void k_energy_fd(const float *__restrict__ xyz,
+                                                   const int64_t *__restrict__ offsets,
+                                                   const int32_t *__restrict__ indices,
+                                                   double phic, const int32_t *__restrict__ atoms,
+                                                   int64_t m, double h, double *__restrict__ fd)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * m)
+    return;
+  const int64_t a = t / 3;
+  const int d = (int)(t % 3);
+  const int64_t i = atoms[a];
+  double e2[2];
+  for (int sgn = 0; sgn < 2; ++sgn)
+  {
+    double x[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    x[d] += sgn == 0 ? h : -h;
+    double e = 0.0;
+    for (int64_t p = offsets[i]; p < offsets[i + 1]; ++p)
+    {
+      const int64_t j = indices[p];
+      if (j == i)
+        continue;
+      const double dx = x[0] - (double)xyz[3 * j], dy = x[1] - (double)xyz[3 * j + 1],
+                   dz = x[2] - (double)xyz[3 * j + 2];
+      const double ir2 = 1.0 / (dx * dx + dy * dy + dz * dz);
+      const double ir6 = ir2 * ir2 * ir2;
+      e += 4.0 * ir6 * (ir6 - 1.0) - phic;
+    }
+    e2[sgn] = e;
+  }
+  fd[t] = (e2[0] - e2[1]) / (2.0 * h);
+}
+
+} // namespace
+
+hipError_t pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
+                       const int32_t *indices, float cutoff, double *energy, float *grad,
+                       hipStream_t s)
+{
+  const double irc2 = 1.0 / ((double)cutoff * (double)cutoff);
+  const double irc6 = irc2 * irc2 * irc2;
+  const float phic = (float)(4.0 * irc6 * (irc6 - 1.0));
+  const int64_t nb = div_up(n > 0 ? n : 1, 256);
+  double *partials = (double *)dalloc(nb * sizeof(double), s);
+  if (!partials)
+    return hipErrorOutOfMemory;
+  if (grad)
+    AXB_LAUNCH(k_energy<true>, dim3((unsigned)nb), dim3(256), 0, s, xyz, n, offsets,
+                       indices, phic, partials, grad);
+  else
+    AXB_LAUNCH(k_energy<false>, dim3((unsigned)nb), dim3(256), 0, s, xyz, n, offsets,
+                       indices, phic, partials, grad);
+  if (energy)
+    AXB_LAUNCH(k_reduce_f64, dim3(1), dim3(256), 0, s, partials, nb, energy);
+  dfree(partials, s);
+  return hipGetLastError();
+}
+
+hipError_t pair_energy_fd(const float *xyz, const int64_t *offsets, const int32_t *indices,
+                          float cutoff, const int32_t *atoms, int64_t m, double h, double *fd,
+                          hipStream_t s)
+{
+  if (m <= 0)
+    return hipSuccess;
+  const double irc2 = 1.0 / ((double)cutoff * (double)cutoff);
+  const double irc6 = irc2 * irc2 * irc2;
+  const double phic = 4.0 * irc6 * (irc6 - 1.0);
+  AXB_LAUNCH(k_energy_fd, dim3((unsigned)div_up(3 * m, 256)), dim3(256), 0, s, xyz,
+                     offsets, indices, phic, atoms, m, h, fd);
+  return hipGetLastError();
+}
+
+} // namespace axb

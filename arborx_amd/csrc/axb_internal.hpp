@@ -1,0 +1,193 @@
+// axb_internal.hpp -- declarations shared by the library's translation units.
+#pragma once
+
+#include "axb_common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace axb
+{
+
+// ------------------------------------------------------------ errors
+void set_error(const std::string &msg);
+#define AXB_HIP_TRY(expr)                                                      \
+  do                                                                           \
+  {                                                                            \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+    {                                                                          \
+      ::axb::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));    \
+      return AXB_ERR_HIP;                                                      \
+    }                                                                          \
+  } while (0)
+
+// Stream-ordered device allocation (hipMallocAsync on `stream`).
+void *dalloc(size_t bytes, hipStream_t stream);
+void dfree(void *p, hipStream_t stream);
+
+inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ------------------------------------------------------------ profiling
+// While per-kernel timing is enabled (axb_profile_enable), every launch is
+// bracketed by HIP events recorded on its own stream.
+struct ProfScope
+{
+  ProfScope(const char *name, hipStream_t s);
+  ~ProfScope();
+  const char *name_;
+  hipStream_t s_;
+  bool on_;
+  hipEvent_t a_, b_;
+};
+#define AXB_LAUNCH(kernel, grid, block, shmem, stream, ...)                    \
+  do                                                                           \
+  {                                                                            \
+    ::axb::ProfScope axb_ps_(#kernel, stream);                                 \
+    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);       \
+  } while (0)
+// Device counters {queries, internal-node records read, leaf records read,
+// results} while visit counting is enabled, else null.
+unsigned long long *visit_counters();
+
+// ------------------------------------------------------------ sort / scan
+constexpr int SORT_BLOCK = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+// Workspace bytes for radix_sort_pairs over n keys with `passes` 8-bit digits.
+size_t sort_workspace_bytes(int64_t n, int passes, int key_bytes);
+
+// Stable LSD radix sort of keys (u32 or u64) carrying the permutation of
+// original indices.  keys[0] holds the input; after the call keys[0] /
+// vals[0] hold the sorted keys and permutation (passes must be even).
+// `hist` (passes x 256 uint32 bucket counts) must be filled beforehand by
+// the caller's key-generation kernel, or be null to compute it here.
+hipError_t radix_sort_pairs_u64(uint64_t *keys0, uint64_t *keys1,
+                                uint32_t *vals0, uint32_t *vals1, int64_t n,
+                                int passes, uint32_t *hist, void *workspace,
+                                uint32_t *err, hipStream_t stream);
+hipError_t radix_sort_pairs_u32(uint32_t *keys0, uint32_t *keys1,
+                                uint32_t *vals0, uint32_t *vals1, int64_t n,
+                                int passes, uint32_t *hist, void *workspace,
+                                uint32_t *err, hipStream_t stream);
+
+// Device-wide scans with decoupled look-back (in place allowed).
+size_t scan_workspace_bytes(int64_t n);
+hipError_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n,
+                              void *workspace, uint32_t *err,
+                              hipStream_t stream);
+hipError_t inclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n,
+                              void *workspace, uint32_t *err,
+                              hipStream_t stream);
+
+// Fill kernels (used instead of hipMemsetAsync so that the code stays
+// graph-capture friendly).
+hipError_t fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t stream);
+hipError_t fill_u64(uint64_t *p, uint64_t v, int64_t n, hipStream_t stream);
+
+
+// ------------------------------------------------------------ tree data
+// Internal node as produced by the climb (48 B).
+struct INode
+{
+  float lo[3];
+  int32_t left_child; // reference encoding: leaf j -> j, internal k -> k + n
+  float hi[3];
+  int32_t rope; // reference encoding, -1 sentinel
+  int32_t first, last, h, pad;
+};
+
+struct BuildOut
+{
+  uint4 *slots;        // total_slots + 1 (one padding slot)
+  int64_t total_slots;
+  float *scene;        // 6 floats (device)
+  int32_t *C;          // n: inclusive prefix of left-spine counts
+  // reference-form topology (kept on request)
+  uint64_t *keys;
+  uint32_t *perm;
+  int32_t *leaf_rope;
+  INode *inodes;
+};
+
+struct TreeView
+{
+  const uint4 *slots;
+  uint32_t total;
+  int32_t n;
+  const int32_t *C;
+  Box6 scene;
+  float ss[3];   // Q16 scale of the scene frame
+  uint32_t *err; // kernel error word
+  unsigned long long *visits; // optional visit counters (see visit_counters)
+};
+
+constexpr int PRED_SPHERES = 0;
+constexpr int PRED_BOXES = 1;
+constexpr int PRED_RADIUS = 2; // points + one radius
+
+constexpr int MODE_COUNT = 0;
+constexpr int MODE_FILL = 1;
+constexpr int MODE_BUFFER = 2;
+
+int64_t total_slots_for(int64_t n, int geometry, int node_format);
+hipError_t build_tree(const float *values, int64_t n, int geometry, int node_format,
+                      bool keep_topology, uint32_t *err, hipStream_t stream, BuildOut &out);
+hipError_t export_topology(const INode *inodes, int64_t nint, int32_t *left_child, int32_t *rope,
+                           float *boxes, hipStream_t stream);
+hipError_t launch_spatial(const TreeView &tv, int node_format, int geometry, int pred_kind,
+                          const float *preds, int64_t nq, float radius, const uint32_t *order,
+                          int64_t *offsets, int32_t *out, int32_t buffer, int mode,
+                          hipStream_t s);
+hipError_t launch_knn(const TreeView &tv, int node_format, int geometry, const float *pts,
+                      int64_t nq, int k, const uint32_t *order, int32_t *oi, float *od,
+                      hipStream_t s);
+hipError_t launch_knn_offsets(int64_t *offsets, int64_t nq, int64_t per, hipStream_t s);
+hipError_t launch_pred_morton32(int pred_kind, const float *preds, int64_t nq,
+                                const float *scene, uint32_t *keys, uint32_t *hist,
+                                hipStream_t s);
+hipError_t launch_compact(const int32_t *buf, int64_t nq, int32_t buffer,
+                          const int64_t *offsets, int32_t *out, hipStream_t s);
+hipError_t launch_max_count(const int64_t *counts, int64_t nq, int32_t buffer, uint32_t *flag,
+                            hipStream_t s);
+hipError_t pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
+                       const int32_t *indices, float cutoff, double *energy, float *grad,
+                       hipStream_t s);
+hipError_t pair_energy_fd(const float *xyz, const int64_t *offsets, const int32_t *indices,
+                          float cutoff, const int32_t *atoms, int64_t m, double h, double *fd,
+                          hipStream_t s);
+
+// Bounded spin count for decoupled look-back waits.
+constexpr uint32_t SPIN_LIMIT = 1u << 22;
+
+// Atomic helpers at device (agent) scope.
+AXB_D uint32_t ld_agent(const uint32_t *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+AXB_D void st_agent(uint32_t *p, uint32_t v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+AXB_D uint64_t ld_agent64(const uint64_t *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+AXB_D void st_agent64(uint64_t *p, uint64_t v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+AXB_D void raise_error(uint32_t *err, uint32_t flag)
+{
+  if (err)
+    __hip_atomic_fetch_or(err, flag, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
+} // namespace axb

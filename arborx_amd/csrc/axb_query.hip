@@ -1,0 +1,768 @@
+// axb_query.hip -- spatial and nearest traversals on gfx950.
+//
+// Spatial: the reference's stackless rope walk (detail/ArborX_TreeTraversal.hpp:
+// 97-119) over the pre-order slot array: an internal node that passes its
+// test continues at the next record (its left child), one that fails jumps
+// to `slot + skip` (its rope); a leaf is tested and the walk moves to the
+// next record.  One lane per predicate, predicates in Morton-32 order.
+// Sphere tests use `d2 <= sqrt_threshold(r)`, which decides exactly the
+// reference's `sqrt(d2) <= r` (Intersects.hpp:82-114).
+//
+// Nearest: result = the k smallest (distance, index) pairs, distance =
+// RN(sqrt(d2)) as in Distance.hpp:52-80.  Two phases per lane, no stack:
+//   1. greedy descent to the leaf nearest in tree order, then the k leaves
+//      around it in Morton order (contiguous in pre-order) seed a sorted
+//      list, giving an upper bound R on the k-th distance;
+//   2. a rope walk that skips every node farther than the current k-th
+//      distance (ties kept: pruning is `d2 > T(R)`), inserting closer leaves.
+// The list lives in registers (k <= 64, unrolled insertion, static
+// indexing only); larger k use a binary max-heap in the output row
+// (the reference's NearestBufferProvider scheme, NearestBufferProvider.hpp:
+// 23-73).
+#include "axb_internal.hpp"
+
+namespace axb
+{
+
+namespace
+{
+
+template <int F>
+AXB_D void node_box(const TreeView &tv, uint4 s0, uint4 s1, bool desc,
+                    const Box6 &par, const float *ps, Box6 &b)
+{
+  if (F == NODES_FP32)
+  {
+    b.lo[0] = bits_f(s0.x), b.lo[1] = bits_f(s0.y), b.lo[2] = bits_f(s0.z);
+    b.hi[0] = bits_f(s1.x), b.hi[1] = bits_f(s1.y), b.hi[2] = bits_f(s1.z);
+  }
+  else
+  {
+    if (desc)
+      q16_decode(s0.x, s0.y, s0.z, par, ps, b);
+    else
+      q16_decode(s0.x, s0.y, s0.z, tv.scene, tv.ss, b);
+  }
+}
+
+template <int F>
+AXB_D void set_frame(const Box6 &b, Box6 &par, float *ps)
+{
+  if (F == NODES_Q16)
+  {
+    par = b;
+    ps[0] = q16_scale(b.lo[0], b.hi[0]);
+    ps[1] = q16_scale(b.lo[1], b.hi[1]);
+    ps[2] = q16_scale(b.lo[2], b.hi[2]);
+  }
+}
+
+// ------------------------------------------------------------ spatial
+template <int F, int G, int PK, int MODE>
+__global__ __launch_bounds__(256) void k_spatial(
+    TreeView tv, const float *__restrict__ preds, int64_t nq, float radius,
+    const uint32_t *__restrict__ order, int64_t *__restrict__ offsets,
+    int32_t *__restrict__ out, int32_t buffer)
+{
+  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq)
+    return;
+  const int64_t q = order ? (int64_t)order[t] : t;
+  float cx, cy, cz, thr;
+  Box6 qb;
+  if (PK == PRED_BOXES)
+  {
+    const float *p = preds + 6 * q;
+    qb = {{p[0], p[1], p[2]}, {p[3], p[4], p[5]}};
+    cx = cy = cz = thr = 0.f;
+  }
+  else
+  {
+    const float *p = preds + (PK == PRED_SPHERES ? 4 : 3) * q;
+    cx = p[0], cy = p[1], cz = p[2];
+    thr = sqrt_threshold(PK == PRED_SPHERES ? p[3] : radius);
+  }
+  int64_t base = 0, limit = 0;
+  if (MODE == MODE_FILL)
+  {
+    base = offsets[q];
+    limit = offsets[q + 1] - base;
+  }
+  else if (MODE == MODE_BUFFER)
+    base = q * (int64_t)buffer;
+  int64_t cnt = 0;
+
+  Box6 par;
+  float ps[3];
+  bool desc = false;
+  uint32_t p = 0;
+  uint32_t v_int = 0, v_leaf = 0;
+  const uint32_t end = tv.total;
+  while (p < end)
+  {
+    const uint4 s0 = tv.slots[p];
+    const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
+    const int32_t w = (int32_t)s0.w;
+    if (w < 0)
+    {
+      ++v_leaf;
+      bool hit;
+      const float x = bits_f(s0.x), y = bits_f(s0.y), z = bits_f(s0.z);
+      if (G == GEOM_POINTS)
+      {
+        if (PK == PRED_BOXES)
+          hit = !(x > qb.hi[0] || x < qb.lo[0] || y > qb.hi[1] || y < qb.lo[1] ||
+                  z > qb.hi[2] || z < qb.lo[2]);
+        else
+          hit = dist2_pp(cx, cy, cz, x, y, z) <= thr;
+      }
+      else
+      {
+        const Box6 lb = {{x, y, z}, {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
+        hit = PK == PRED_BOXES ? intersects_bb(qb, lb) : dist2_pbox(cx, cy, cz, lb) <= thr;
+      }
+      if (hit)
+      {
+        if (MODE == MODE_FILL && cnt < limit)
+          out[base + cnt] = ~w;
+        else if (MODE == MODE_BUFFER && cnt < buffer)
+          out[base + cnt] = ~w;
+        ++cnt;
+      }
+      p += SL;
+      desc = false;
+    }
+    else
+    {
+      ++v_int;
+      Box6 b;
+      node_box<F>(tv, s0, s1, desc, par, ps, b);
+      const bool pass = PK == PRED_BOXES ? intersects_bb(qb, b)
+                                         : dist2_pbox(cx, cy, cz, b) <= thr;
+      if (pass)
+      {
+        set_frame<F>(b, par, ps);
+        desc = true;
+        p += SI;
+      }
+      else
+      {
+        if (w == 0)
+        {
+          raise_error(tv.err, KERR_LAYOUT);
+          break;
+        }
+        desc = false;
+        p += (uint32_t)w;
+      }
+    }
+  }
+  if (MODE != MODE_FILL)
+    offsets[q] = cnt;
+  else if (cnt != limit)
+    raise_error(tv.err, KERR_LAYOUT);
+  if (tv.visits && MODE != MODE_FILL)
+  {
+    atomicAdd(tv.visits + 0, 1ull);
+    atomicAdd(tv.visits + 1, (unsigned long long)v_int);
+    atomicAdd(tv.visits + 2, (unsigned long long)v_leaf);
+    atomicAdd(tv.visits + 3, (unsigned long long)cnt);
+  }
+}
+
+// ------------------------------------------------------------ nearest
+AXB_D bool pair_less(float da, int32_t ia, float db, int32_t ib)
+{
+  return da < db || (da == db && ia < ib);
+}
+
+// Sorted list of K (distance, index) pairs in registers; only the first k
+// (k <= K) entries are meaningful.
+template <int K>
+struct RegList
+{
+  float d[K];
+  int32_t i[K];
+  int k, count;
+  float kth_d;
+  int32_t kth_i;
+  AXB_D void init(int k_)
+  {
+    k = k_;
+    count = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      d[j] = INFINITY, i[j] = INT_MAX;
+    kth_d = INFINITY;
+    kth_i = INT_MAX;
+  }
+  AXB_D bool full() const { return count >= k; }
+  AXB_D bool accepts(float dn, int32_t in) const
+  {
+    return count < k || pair_less(dn, in, kth_d, kth_i);
+  }
+  AXB_D void insert(float dn, int32_t in)
+  {
+#pragma unroll
+    for (int j = K - 1; j > 0; --j)
+    {
+      const bool lt_prev = pair_less(dn, in, d[j - 1], i[j - 1]);
+      const bool lt_cur = pair_less(dn, in, d[j], i[j]);
+      const float nd = lt_prev ? d[j - 1] : (lt_cur ? dn : d[j]);
+      const int32_t ni = lt_prev ? i[j - 1] : (lt_cur ? in : i[j]);
+      d[j] = nd;
+      i[j] = ni;
+    }
+    if (pair_less(dn, in, d[0], i[0]))
+      d[0] = dn, i[0] = in;
+    ++count;
+    float kd = d[0];
+    int32_t ki = i[0];
+#pragma unroll
+    for (int j = 1; j < K; ++j)
+      if (j == k - 1)
+        kd = d[j], ki = i[j];
+    kth_d = kd;
+    kth_i = ki;
+  }
+  AXB_D void write(int32_t *oi, float *od, int per) const
+  {
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (j < per)
+      {
+        oi[j] = i[j];
+        if (od)
+          od[j] = d[j];
+      }
+  }
+};
+
+// Binary max-heap on (distance, index) stored in the output row (global).
+struct RowHeap
+{
+  int32_t *hi;
+  float *hd;
+  int k, count;
+  float kth_d;
+  int32_t kth_i;
+  AXB_D void init(int k_, int32_t *oi, float *od)
+  {
+    hi = oi, hd = od, k = k_, count = 0;
+    kth_d = INFINITY, kth_i = INT_MAX;
+  }
+  AXB_D bool full() const { return count >= k; }
+  AXB_D bool accepts(float dn, int32_t in) const
+  {
+    return count < k || pair_less(dn, in, kth_d, kth_i);
+  }
+  AXB_D void insert(float dn, int32_t in)
+  {
+    if (count < k)
+    {
+      // push: sift up
+      int j = count++;
+      while (j > 0)
+      {
+        int par = (j - 1) >> 1;
+        if (!pair_less(hd[par], hi[par], dn, in))
+          break;
+        hd[j] = hd[par], hi[j] = hi[par];
+        j = par;
+      }
+      hd[j] = dn, hi[j] = in;
+    }
+    else
+    {
+      // pop-push: replace the top, sift down (PriorityQueue.hpp:87-93)
+      int j = 0;
+      while (true)
+      {
+        int l = 2 * j + 1;
+        if (l >= k)
+          break;
+        int r = l + 1;
+        int big = (r < k && pair_less(hd[l], hi[l], hd[r], hi[r])) ? r : l;
+        if (!pair_less(dn, in, hd[big], hi[big]))
+          break;
+        hd[j] = hd[big], hi[j] = hi[big];
+        j = big;
+      }
+      hd[j] = dn, hi[j] = in;
+    }
+    if (count >= k)
+      kth_d = hd[0], kth_i = hi[0];
+  }
+  // heap sort ascending in place (Heap.hpp sortHeap)
+  AXB_D void finish(int per)
+  {
+    for (int end = per - 1; end > 0; --end)
+    {
+      float td = hd[0];
+      int32_t ti = hi[0];
+      float xd = hd[end];
+      int32_t xi = hi[end];
+      hd[end] = td, hi[end] = ti;
+      int j = 0;
+      while (true)
+      {
+        int l = 2 * j + 1;
+        if (l >= end)
+          break;
+        int r = l + 1;
+        int big = (r < end && pair_less(hd[l], hi[l], hd[r], hi[r])) ? r : l;
+        if (!pair_less(xd, xi, hd[big], hi[big]))
+          break;
+        hd[j] = hd[big], hi[j] = hi[big];
+        j = big;
+      }
+      hd[j] = xd, hi[j] = xi;
+    }
+  }
+};
+
+template <int G>
+AXB_D float leaf_d2(float qx, float qy, float qz, uint4 s0, uint4 s1)
+{
+  const float x = bits_f(s0.x), y = bits_f(s0.y), z = bits_f(s0.z);
+  if (G == GEOM_POINTS)
+    return dist2_pp(qx, qy, qz, x, y, z);
+  const Box6 lb = {{x, y, z}, {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
+  return dist2_pbox(qx, qy, qz, lb);
+}
+
+template <int F, int G, class List>
+AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
+                        List &L, uint32_t &v_int, uint32_t &v_leaf)
+{
+  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  const uint32_t end = tv.total;
+  const int n = tv.n;
+  // ---- phase 1: greedy descent (closer child first, ties left), tracking
+  // the Morton rank of the first leaf of the current node.
+  int first = 0;
+  uint32_t p = 0;
+  Box6 cur;
+  float cs[3];
+  {
+    const uint4 s0 = tv.slots[p], s1 = tv.slots[p + 1];
+    if ((int32_t)s0.w >= 0)
+    {
+      ++v_int;
+      node_box<F>(tv, s0, s1, false, cur, cs, cur);
+      set_frame<F>(cur, cur, cs);
+    }
+  }
+  while (true)
+  {
+    const uint4 s0 = tv.slots[p];
+    if ((int32_t)s0.w < 0)
+      break;
+    const uint32_t lp = p + SI;
+    const uint4 l0 = tv.slots[lp], l1 = tv.slots[lp + 1];
+    const int32_t lw = (int32_t)l0.w;
+    const uint32_t lskip = lw < 0 ? SL : (uint32_t)lw;
+    const int llen = lw < 0 ? 1 : (int)((lskip + SI) / (SI + SL));
+    const uint32_t rp = lp + lskip;
+    if (rp >= end)
+    {
+      raise_error(tv.err, KERR_LAYOUT);
+      return;
+    }
+    const uint4 r0 = tv.slots[rp], r1 = tv.slots[rp + 1];
+    const int32_t rw = (int32_t)r0.w;
+    Box6 lb, rb;
+    float dl, dr;
+    v_leaf += (lw < 0) + (rw < 0);
+    v_int += (lw >= 0) + (rw >= 0);
+    if (lw < 0)
+      dl = leaf_d2<G>(qx, qy, qz, l0, l1);
+    else
+    {
+      node_box<F>(tv, l0, l1, true, cur, cs, lb);
+      dl = dist2_pbox(qx, qy, qz, lb);
+    }
+    if (rw < 0)
+      dr = leaf_d2<G>(qx, qy, qz, r0, r1);
+    else
+    {
+      node_box<F>(tv, r0, r1, false, cur, cs, rb);
+      dr = dist2_pbox(qx, qy, qz, rb);
+    }
+    if (dl <= dr)
+    {
+      p = lp;
+      if (lw >= 0)
+        set_frame<F>(lb, cur, cs);
+    }
+    else
+    {
+      p = rp;
+      first += llen;
+      if (rw >= 0)
+        set_frame<F>(rb, cur, cs);
+    }
+  }
+  // ---- seed: the m leaves of ranks [j0, j0 + m) around `first`
+  int j0 = first - (m - 1) / 2;
+  j0 = j0 < 0 ? 0 : (j0 > n - m ? n - m : j0);
+  {
+    uint32_t s = SL * (uint32_t)j0 + SI * (uint32_t)tv.C[j0];
+    int got = 0;
+    while (got < m)
+    {
+      if (s >= end)
+      {
+        raise_error(tv.err, KERR_LAYOUT);
+        return;
+      }
+      const uint4 s0 = tv.slots[s];
+      if ((int32_t)s0.w < 0)
+      {
+        ++v_leaf;
+        const uint4 s1 = SL == 2 ? tv.slots[s + 1] : s0;
+        const float dist = sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1));
+        const int32_t idx = ~(int32_t)s0.w;
+        if (L.accepts(dist, idx))
+          L.insert(dist, idx);
+        ++got;
+        s += SL;
+      }
+      else
+      {
+        ++v_int;
+        s += SI;
+      }
+    }
+  }
+  if (m >= n)
+    return; // every leaf is already in the list
+  // ---- phase 2: rope walk with the current k-th distance as the bound
+  float thr = L.full() ? sqrt_threshold(L.kth_d) : INFINITY;
+  Box6 par;
+  float ps[3];
+  bool desc = false;
+  int rank = 0;
+  p = 0;
+  while (p < end)
+  {
+    const uint4 s0 = tv.slots[p];
+    const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
+    const int32_t w = (int32_t)s0.w;
+    if (w < 0)
+    {
+      ++v_leaf;
+      if (rank < j0 || rank >= j0 + m)
+      {
+        const float d2 = leaf_d2<G>(qx, qy, qz, s0, s1);
+        if (d2 <= thr)
+        {
+          const float dist = sqrt_rn(d2);
+          const int32_t idx = ~w;
+          if (L.accepts(dist, idx))
+          {
+            L.insert(dist, idx);
+            if (L.full())
+              thr = sqrt_threshold(L.kth_d);
+          }
+        }
+      }
+      ++rank;
+      p += SL;
+      desc = false;
+    }
+    else
+    {
+      ++v_int;
+      Box6 b;
+      node_box<F>(tv, s0, s1, desc, par, ps, b);
+      if (dist2_pbox(qx, qy, qz, b) <= thr)
+      {
+        set_frame<F>(b, par, ps);
+        desc = true;
+        p += SI;
+      }
+      else
+      {
+        if (w == 0)
+        {
+          raise_error(tv.err, KERR_LAYOUT);
+          return;
+        }
+        desc = false;
+        p += (uint32_t)w;
+        rank += (int)(((uint32_t)w + SI) / (SI + SL));
+      }
+    }
+  }
+}
+
+template <int F, int G, int K>
+__global__ __launch_bounds__(256) void k_knn(TreeView tv,
+                                             const float *__restrict__ pts,
+                                             int64_t nq, int k,
+                                             const uint32_t *__restrict__ order,
+                                             int32_t *__restrict__ out_idx,
+                                             float *__restrict__ out_dist)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq)
+    return;
+  const int64_t q = order ? (int64_t)order[t] : t;
+  const int m = k < tv.n ? k : tv.n;
+  const float qx = pts[3 * q], qy = pts[3 * q + 1], qz = pts[3 * q + 2];
+  RegList<K> L;
+  L.init(k);
+  uint32_t v_int = 0, v_leaf = 0;
+  knn_traverse<F, G>(tv, qx, qy, qz, m, L, v_int, v_leaf);
+  if (tv.visits)
+  {
+    atomicAdd(tv.visits + 0, 1ull);
+    atomicAdd(tv.visits + 1, (unsigned long long)v_int);
+    atomicAdd(tv.visits + 2, (unsigned long long)v_leaf);
+    atomicAdd(tv.visits + 3, (unsigned long long)m);
+  }
+  L.write(out_idx + q * m, out_dist ? out_dist + q * m : nullptr, m);
+}
+
+template <int F, int G>
+__global__ __launch_bounds__(256) void k_knn_heap(TreeView tv,
+                                                  const float *__restrict__ pts,
+                                                  int64_t nq, int k,
+                                                  const uint32_t *__restrict__ order,
+                                                  int32_t *__restrict__ out_idx,
+                                                  float *__restrict__ out_dist)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq)
+    return;
+  const int64_t q = order ? (int64_t)order[t] : t;
+  const int m = k < tv.n ? k : tv.n;
+  const float qx = pts[3 * q], qy = pts[3 * q + 1], qz = pts[3 * q + 2];
+  RowHeap L;
+  L.init(m, out_idx + q * m, out_dist + q * m);
+  uint32_t v_int = 0, v_leaf = 0;
+  knn_traverse<F, G>(tv, qx, qy, qz, m, L, v_int, v_leaf);
+  if (tv.visits)
+  {
+    atomicAdd(tv.visits + 0, 1ull);
+    atomicAdd(tv.visits + 1, (unsigned long long)v_int);
+    atomicAdd(tv.visits + 2, (unsigned long long)v_leaf);
+    atomicAdd(tv.visits + 3, (unsigned long long)m);
+  }
+  L.finish(m);
+}
+
+__global__ void k_knn_offsets(int64_t *offsets, int64_t nq, int64_t per)
+{
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= nq)
+    offsets[i] = i * per;
+}
+
+// Morton-32 code of predicate centroids in the tree's scene box
+// (SpaceFillingCurves.hpp:31-43, CrsGraphWrapperImpl.hpp:407-419) plus the
+// digit histograms of the 4 radix passes.
+template <int PK>
+__global__ __launch_bounds__(256) void k_pred_morton32(
+    const float *__restrict__ preds, int64_t nq, const float *__restrict__ scene,
+    uint32_t *__restrict__ keys, uint32_t *__restrict__ hist)
+{
+  __shared__ uint32_t s_h[4 * 256];
+  for (int j = threadIdx.x; j < 4 * 256; j += blockDim.x)
+    s_h[j] = 0;
+  const float a0 = scene[0], a1 = scene[1], a2 = scene[2];
+  const float b0 = scene[3], b1 = scene[4], b2 = scene[5];
+  __syncthreads();
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += stride)
+  {
+    float cx, cy, cz;
+    if (PK == PRED_BOXES)
+    {
+      const float *v = preds + 6 * i;
+      cx = (v[0] + v[3]) / 2, cy = (v[1] + v[4]) / 2, cz = (v[2] + v[5]) / 2;
+    }
+    else
+    {
+      const float *v = preds + (PK == PRED_SPHERES ? 4 : 3) * i;
+      cx = v[0], cy = v[1], cz = v[2];
+    }
+    uint32_t k = morton32(translate_and_scale(cx, a0, b0), translate_and_scale(cy, a1, b1),
+                          translate_and_scale(cz, a2, b2));
+    keys[i] = k;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < 4 * 256; j += blockDim.x)
+    if (s_h[j])
+      atomicAdd(&hist[j], s_h[j]);
+}
+
+// Buffered first pass -> CRS: row q holds min(count, buffer) results at
+// buf + q * buffer (CrsGraphWrapperImpl.hpp:296-318, copy_valid_values).
+__global__ void k_compact(const int32_t *__restrict__ buf, int64_t nq, int32_t buffer,
+                          const int64_t *__restrict__ offsets, int32_t *__restrict__ out)
+{
+  const int64_t q = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (q >= nq)
+    return;
+  const int lane = threadIdx.x & 63;
+  const int64_t o = offsets[q];
+  const int64_t cnt = offsets[q + 1] - o;
+  for (int64_t j = lane; j < cnt; j += 64)
+    out[o + j] = buf[q * buffer + j];
+}
+
+__global__ void k_max_count(const int64_t *__restrict__ counts, int64_t nq, int32_t buffer,
+                            uint32_t *flag)
+{
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nq && counts[i] > buffer)
+    *flag = 1u;
+}
+
+} // namespace
+
+// ------------------------------------------------------------ host launchers
+inline unsigned grid_of(int64_t n) { return (unsigned)div_up(n, 256); }
+
+template <int F, int G, int PK>
+static hipError_t launch_spatial_fg(const TreeView &tv, const float *preds, int64_t nq,
+                                    float radius, const uint32_t *order, int64_t *offsets,
+                                    int32_t *out, int32_t buffer, int mode, hipStream_t s)
+{
+  if (mode == MODE_COUNT)
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_COUNT>), dim3(grid_of(nq)), dim3(256), 0, s,
+                       tv, preds, nq, radius, order, offsets, out, buffer);
+  else if (mode == MODE_FILL)
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_FILL>), dim3(grid_of(nq)), dim3(256), 0, s,
+                       tv, preds, nq, radius, order, offsets, out, buffer);
+  else
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_BUFFER>), dim3(grid_of(nq)), dim3(256), 0, s,
+                       tv, preds, nq, radius, order, offsets, out, buffer);
+  return hipGetLastError();
+}
+
+template <int PK>
+static hipError_t launch_spatial_p(const TreeView &tv, int F, int G, const float *preds,
+                                   int64_t nq, float radius, const uint32_t *order,
+                                   int64_t *offsets, int32_t *out, int32_t buffer, int mode,
+                                   hipStream_t s)
+{
+  if (F == NODES_FP32 && G == GEOM_POINTS)
+    return launch_spatial_fg<NODES_FP32, GEOM_POINTS, PK>(tv, preds, nq, radius, order, offsets,
+                                                          out, buffer, mode, s);
+  if (F == NODES_FP32 && G == GEOM_BOXES)
+    return launch_spatial_fg<NODES_FP32, GEOM_BOXES, PK>(tv, preds, nq, radius, order, offsets,
+                                                         out, buffer, mode, s);
+  if (F == NODES_Q16 && G == GEOM_POINTS)
+    return launch_spatial_fg<NODES_Q16, GEOM_POINTS, PK>(tv, preds, nq, radius, order, offsets,
+                                                         out, buffer, mode, s);
+  return launch_spatial_fg<NODES_Q16, GEOM_BOXES, PK>(tv, preds, nq, radius, order, offsets,
+                                                      out, buffer, mode, s);
+}
+
+hipError_t launch_spatial(const TreeView &tv, int node_format, int geometry, int pred_kind,
+                          const float *preds, int64_t nq, float radius, const uint32_t *order,
+                          int64_t *offsets, int32_t *out, int32_t buffer, int mode,
+                          hipStream_t s)
+{
+  if (nq <= 0)
+    return hipSuccess;
+  if (pred_kind == PRED_SPHERES)
+    return launch_spatial_p<PRED_SPHERES>(tv, node_format, geometry, preds, nq, radius, order,
+                                          offsets, out, buffer, mode, s);
+  if (pred_kind == PRED_BOXES)
+    return launch_spatial_p<PRED_BOXES>(tv, node_format, geometry, preds, nq, radius, order,
+                                        offsets, out, buffer, mode, s);
+  return launch_spatial_p<PRED_RADIUS>(tv, node_format, geometry, preds, nq, radius, order,
+                                       offsets, out, buffer, mode, s);
+}
+
+template <int F, int G>
+static hipError_t launch_knn_fg(const TreeView &tv, const float *pts, int64_t nq, int k,
+                                const uint32_t *order, int32_t *oi, float *od, hipStream_t s)
+{
+  const dim3 g(grid_of(nq)), b(256);
+  if (k <= 1)
+    AXB_LAUNCH((k_knn<F, G, 1>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
+  else if (k <= 2)
+    AXB_LAUNCH((k_knn<F, G, 2>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
+  else if (k <= 4)
+    AXB_LAUNCH((k_knn<F, G, 4>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
+  else if (k <= 8)
+    AXB_LAUNCH((k_knn<F, G, 8>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
+  else if (k <= 16)
+    AXB_LAUNCH((k_knn<F, G, 16>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
+  else if (k <= 32)
+    AXB_LAUNCH((k_knn<F, G, 32>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
+  else if (k <= 64)
+    AXB_LAUNCH((k_knn<F, G, 64>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
+  else
+    AXB_LAUNCH((k_knn_heap<F, G>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
+  return hipGetLastError();
+}
+
+// od must be non-null when k > 64 (the heap keeps its distances there).
+hipError_t launch_knn(const TreeView &tv, int node_format, int geometry, const float *pts,
+                      int64_t nq, int k, const uint32_t *order, int32_t *oi, float *od,
+                      hipStream_t s)
+{
+  if (nq <= 0 || k < 1 || tv.n == 0)
+    return hipSuccess;
+  if (node_format == NODES_FP32 && geometry == GEOM_POINTS)
+    return launch_knn_fg<NODES_FP32, GEOM_POINTS>(tv, pts, nq, k, order, oi, od, s);
+  if (node_format == NODES_FP32 && geometry == GEOM_BOXES)
+    return launch_knn_fg<NODES_FP32, GEOM_BOXES>(tv, pts, nq, k, order, oi, od, s);
+  if (node_format == NODES_Q16 && geometry == GEOM_POINTS)
+    return launch_knn_fg<NODES_Q16, GEOM_POINTS>(tv, pts, nq, k, order, oi, od, s);
+  return launch_knn_fg<NODES_Q16, GEOM_BOXES>(tv, pts, nq, k, order, oi, od, s);
+}
+
+hipError_t launch_knn_offsets(int64_t *offsets, int64_t nq, int64_t per, hipStream_t s)
+{
+  AXB_LAUNCH(k_knn_offsets, dim3(grid_of(nq + 1)), dim3(256), 0, s, offsets, nq, per);
+  return hipGetLastError();
+}
+
+hipError_t launch_pred_morton32(int pred_kind, const float *preds, int64_t nq,
+                                const float *scene, uint32_t *keys, uint32_t *hist,
+                                hipStream_t s)
+{
+  const unsigned g = (unsigned)std::min<int64_t>(2048, div_up(nq, 256));
+  if (pred_kind == PRED_SPHERES)
+    AXB_LAUNCH(k_pred_morton32<PRED_SPHERES>, dim3(g), dim3(256), 0, s, preds, nq,
+                       scene, keys, hist);
+  else if (pred_kind == PRED_BOXES)
+    AXB_LAUNCH(k_pred_morton32<PRED_BOXES>, dim3(g), dim3(256), 0, s, preds, nq, scene,
+                       keys, hist);
+  else
+    AXB_LAUNCH(k_pred_morton32<PRED_RADIUS>, dim3(g), dim3(256), 0, s, preds, nq,
+                       scene, keys, hist);
+  return hipGetLastError();
+}
+
+hipError_t launch_compact(const int32_t *buf, int64_t nq, int32_t buffer,
+                          const int64_t *offsets, int32_t *out, hipStream_t s)
+{
+  if (nq <= 0)
+    return hipSuccess;
+  AXB_LAUNCH(k_compact, dim3((unsigned)div_up(nq, 4)), dim3(256), 0, s, buf, nq, buffer,
+                     offsets, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_count(const int64_t *counts, int64_t nq, int32_t buffer, uint32_t *flag,
+                            hipStream_t s)
+{
+  if (nq <= 0)
+    return hipSuccess;
+  AXB_LAUNCH(k_max_count, dim3(grid_of(nq)), dim3(256), 0, s, counts, nq, buffer, flag);
+  return hipGetLastError();
+}
+
+} // namespace axb

@@ -1,0 +1,224 @@
+/* arborx_mi355x.h -- C-ABI of the MI355X (gfx950) BVH library.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every `values`,
+ * `predicates`, `points`, `offsets`, `indices` and `distances` pointer is a
+ * DEVICE pointer; arrays of 3-D points are N x 3 float (AoS, x y z), arrays of
+ * boxes are N x 6 float (min x y z, max x y z), spheres are N x 4 float
+ * (centre x y z, radius).  All work is enqueued on `stream` (a hipStream_t;
+ * NULL = the null stream).  Functions return AXB_OK (0) or a negative
+ * axb_status; axb_last_error() gives the message of the calling thread's last
+ * failure.
+ *
+ * Each entry point names the ArborX interface it replaces (paths relative to
+ * the ArborX source tree).  The reference's Kokkos Views are replaced by
+ * device pointers; views the reference reallocates itself
+ * (CrsGraphWrapperImpl.hpp:376-377) are allocated here through the output
+ * allocator (axb_set_output_allocator, default hipMallocAsync on `stream`)
+ * and released with axb_free().
+ */
+#ifndef ARBORX_MI355X_H
+#define ARBORX_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same type as hipStream_t, declared here so that callers need no HIP
+ * headers. */
+typedef struct ihipStream_t *axb_stream_t;
+
+/* Opaque tree handle; replaces ArborX::BoundingVolumeHierarchy
+ * (src/spatial/ArborX_LinearBVH.hpp:50-142). */
+typedef struct axb_bvh axb_bvh;
+
+typedef enum axb_status
+{
+  AXB_OK = 0,
+  AXB_ERR_INVALID_ARGUMENT = -1,
+  AXB_ERR_HIP = -2,
+  AXB_ERR_OUT_OF_MEMORY = -3,
+  /* A query produced more results than a hard (negative) buffer_size allows.
+   * Replaces ArborX::SearchException thrown by ARBORX_ASSERT at
+   * src/spatial/detail/ArborX_CrsGraphWrapperImpl.hpp:268. */
+  AXB_ERR_BUFFER_OVERFLOW = -4,
+  /* A kernel reported an internal limit (bounded wait exceeded, kNN stack
+   * overflow). */
+  AXB_ERR_INTERNAL = -5
+} axb_status;
+
+enum
+{
+  AXB_GEOMETRY_POINTS = 0, /* values are N x 3 points */
+  AXB_GEOMETRY_BOXES = 1   /* values are N x 6 boxes */
+};
+
+enum
+{
+  AXB_NODES_FP32 = 0, /* internal node boxes as 6 x float (32 B / node) */
+  AXB_NODES_Q16 = 1   /* internal node boxes as 6 x uint16, outward-rounded
+                         relative to a frame box (16 B / node) */
+};
+
+enum
+{
+  AXB_BUILD_KEEP_TOPOLOGY = 1 /* keep the reference-form arrays (sorted Morton
+                                 codes, permutation, left_child, rope, boxes)
+                                 for axb_bvh_export_topology */
+};
+
+enum
+{
+  AXB_PREDICATE_SPHERES = 0, /* Intersects<Sphere>: nq x 4 {x,y,z,r} */
+  AXB_PREDICATE_BOXES = 1    /* Intersects<Box>:    nq x 6 {lo, hi} */
+};
+
+/* ArborX::Experimental::TraversalPolicy
+ * (src/spatial/detail/ArborX_TraversalPolicy.hpp:19-47).
+ * buffer_size 0: two passes (count, fill); > 0: one pass into a buffer of
+ * buffer_size results per query, second pass only if some query overflows;
+ * < 0: like > 0 but overflow is an error (AXB_ERR_BUFFER_OVERFLOW).
+ * sort_predicates != 0: process queries in Morton-32 order of their centroid
+ * (results are still returned in the caller's query order). */
+typedef struct axb_traversal_policy
+{
+  int32_t buffer_size;
+  int32_t sort_predicates;
+} axb_traversal_policy;
+
+typedef void *(*axb_alloc_fn)(size_t bytes, void *ctx);
+typedef void (*axb_free_fn)(void *ptr, void *ctx);
+
+const char *axb_last_error(void);
+const char *axb_version(void);
+
+/* Allocator for the result arrays the library sizes itself (CRS indices,
+ * distances).  alloc/free NULL restores the default (hipMallocAsync /
+ * hipFreeAsync on the query stream). */
+axb_status axb_set_output_allocator(axb_alloc_fn alloc, axb_free_fn free_fn,
+                                    void *ctx);
+/* Releases an array returned by a query when the default allocator is in use. */
+axb_status axb_free(void *device_ptr, axb_stream_t stream);
+
+/* Build: BoundingVolumeHierarchy(space, values[, indexable_getter, Morton64])
+ * (src/spatial/ArborX_LinearBVH.hpp:68-73,167-256).  Values are points or
+ * boxes; leaf i's value is its index in `values` (the reference's benchmark
+ * form BVH<MemorySpace, int, ...> over Iota / attach_indices,
+ * benchmarks/bvh_driver/benchmark_registration.hpp:151-161).
+ * flags: AXB_BUILD_KEEP_TOPOLOGY or 0.  n must be < 2^30.  The caller keeps
+ * ownership of `values`; the tree copies what it needs. */
+axb_status axb_bvh_create(const float *values, int64_t n, int32_t geometry,
+                          int32_t node_format, int32_t flags,
+                          axb_stream_t stream, axb_bvh **out);
+void axb_bvh_destroy(axb_bvh *bvh);
+/* size() / empty() (ArborX_LinearBVH.hpp:75-79). */
+int64_t axb_bvh_size(const axb_bvh *bvh);
+/* bounds() (ArborX_LinearBVH.hpp:81-82): host copy {lo xyz, hi xyz}; an
+ * empty tree returns the empty box {+FLT_MAX, -FLT_MAX}. */
+axb_status axb_bvh_bounds(const axb_bvh *bvh, float out_host[6]);
+
+/* Spatial query to CRS: bvh.query(space, predicates, indices, offsets,
+ * policy) with Intersects<Sphere> or Intersects<Box> predicates
+ * (ArborX_LinearBVH.hpp:90-110, ArborX_CrsGraphWrapper.hpp:22-34,
+ * detail/ArborX_CrsGraphWrapperImpl.hpp:376-446).
+ * offsets: caller-allocated, nq + 1 entries (int64; the reference uses the
+ * offset View's type).  *indices: allocated by the library, *nnz_host
+ * entries, each the index of a value whose geometry satisfies the
+ * predicate; row i lists them in traversal order. */
+axb_status axb_bvh_query_intersects(const axb_bvh *bvh, int32_t predicate_kind,
+                                    const float *predicates, int64_t nq,
+                                    axb_traversal_policy policy,
+                                    int64_t *offsets, int32_t **indices,
+                                    int64_t *nnz_host, axb_stream_t stream);
+
+/* make_intersects(points, r) (detail/ArborX_PredicateHelpers.hpp:99-105,
+ * 163-186): spheres of one radius r (stored as float) around nq points. */
+axb_status axb_bvh_query_intersects_radius(const axb_bvh *bvh,
+                                           const float *points, int64_t nq,
+                                           float radius,
+                                           axb_traversal_policy policy,
+                                           int64_t *offsets, int32_t **indices,
+                                           int64_t *nnz_host,
+                                           axb_stream_t stream);
+
+/* Nearest query to CRS: bvh.query(space, make_nearest(points, k), indices,
+ * offsets) (detail/ArborX_PredicateHelpers.hpp:114-119,188-206,
+ * CrsGraphWrapperImpl.hpp:353-374, TreeTraversal.hpp:122-336).  Row i holds
+ * min(k, size()) indices ordered by (distance, index); *distances (optional,
+ * pass NULL to skip) receives the matching float distances.  offsets:
+ * caller-allocated nq + 1 entries. */
+axb_status axb_bvh_query_nearest(const axb_bvh *bvh, const float *points,
+                                 int64_t nq, int32_t k,
+                                 axb_traversal_policy policy, int64_t *offsets,
+                                 int32_t **indices, float **distances,
+                                 int64_t *nnz_host, axb_stream_t stream);
+
+/* Pair energy over a fixed neighbour CRS (no reference counterpart; pattern of
+ * examples/molecular_dynamics/example_molecular_dynamics.cpp:128-194).
+ * Shifted Lennard-Jones phi(r) = 4(r^-12 - r^-6) - phi(cutoff);
+ * E = 1/2 sum_i sum_{j in row i, j != i} phi(|x_i - x_j|), pair terms in
+ * float, sums in double.  energy (device double, may be NULL) receives E;
+ * grad (device N x 3 float, may be NULL) receives dE/dx. */
+axb_status axb_pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
+                           const int32_t *indices, float cutoff,
+                           double *energy, float *grad, axb_stream_t stream);
+
+/* Central finite differences of the same energy in double precision for the
+ * atoms listed in `atoms` (device int32, m entries): fd[3*a + d] =
+ * (E(x + h e_ad) - E(x - h e_ad)) / 2h with the neighbour set held fixed
+ * (device double, m x 3). */
+axb_status axb_pair_energy_fd(const float *xyz, int64_t n,
+                              const int64_t *offsets, const int32_t *indices,
+                              float cutoff, const int32_t *atoms, int64_t m,
+                              double h, double *fd, axb_stream_t stream);
+
+/* ---- introspection (parity tests, benchmarks) ---- */
+
+/* Number of 16-byte slots of the pre-order node array and a device pointer
+ * to it (uint32 x 4 per slot; read-only, owned by the tree). */
+int64_t axb_bvh_num_slots(const axb_bvh *bvh);
+const void *axb_bvh_slots(const axb_bvh *bvh);
+
+/* Copies the reference-form topology (tree built with
+ * AXB_BUILD_KEEP_TOPOLOGY): sorted Morton codes (n), permutation (n), leaf
+ * ropes (n), left_child (n-1), internal ropes (n-1), internal boxes
+ * ((n-1) x 6).  Node indices follow detail/ArborX_HappyTreeFriends.hpp:23-86
+ * (leaves 0..n-1, internal node k is n+k, rope -1 = sentinel).  Device
+ * destination pointers; any may be NULL. */
+axb_status axb_bvh_export_topology(const axb_bvh *bvh, uint64_t *keys,
+                                   uint32_t *perm, int32_t *leaf_rope,
+                                   int32_t *left_child, int32_t *rope,
+                                   float *boxes, axb_stream_t stream);
+
+/* ---- profiling (replaces the Kokkos Tools labels the reference puts on
+ * every region and kernel, spatial/ArborX_LinearBVH.hpp:201,217-255) ----
+ * flags bit 0: bracket every kernel launch with HIP events on its stream;
+ * bit 1: count the node records each traversal reads.  0 disables.
+ * Enabling clears previous records. */
+axb_status axb_profile_enable(int32_t flags);
+axb_status axb_profile_clear(void);
+int32_t axb_profile_num_records(void);
+/* Kernel name and duration (ms) of record i (waits for its events). */
+axb_status axb_profile_record(int32_t i, char *name, int32_t name_len,
+                              double *ms);
+/* {queries, internal-node records read, leaf records read, results} summed
+ * over the traversals run since axb_profile_enable(2 | ...). */
+axb_status axb_profile_visits(int64_t out_host[4]);
+
+/* Host helper used by the query kernels, exported for unit tests: the largest
+ * float s with RN(sqrt(s)) <= r, so that `d2 <= t` decides exactly the
+ * reference's `sqrt(d2) <= r` (detail/../geometry/algorithms/
+ * ArborX_Intersects.hpp:105-114 with ArborX_Distance.hpp:52-70). */
+float axb_sqrt_threshold(float r);
+
+/* Morton-64 code of one point in a scene box (host, for unit tests;
+ * detail/ArborX_MortonCode.hpp:318-347 with TranslateAndScale.hpp:23-37). */
+uint64_t axb_morton64_host(const float p[3], const float scene[6]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARBORX_MI355X_H */

@@ -1,0 +1,342 @@
+"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the checker:
+against the reference's own known-answer tests (tests/golden), against brute
+force with the same float formulas, against std::stable_sort, and against
+closed forms / invariants.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from arborx_amd import synthetic
+from tests.helpers import (assert_crs_equal_as_sets, diagonal_boxes, manufactured_grid,
+                           manufactured_queries, rows, spheres_from_points)
+
+
+# ------------------------------------------------------------ golden vectors
+def test_expand_bits_golden(kat):
+    e = kat["expand_bits"]
+    assert oracle.expand2_u32(e["u32_in"]) == e["u32_expand2"]
+    assert oracle.expand2_u64(e["u64_in"]) == e["u64_expand2"]
+
+
+def test_morton_3d_golden(kat):
+    for p, code in kat["morton_3d"]["morton32"]:
+        assert oracle.morton32(p) == code
+    for p, code in kat["morton_3d"]["morton64"]:
+        assert oracle.morton64(p) == code
+
+
+def test_assign_morton_codes_golden(kat):
+    a = kat["assign_morton_codes"]
+    boxes = np.array(a["boxes"], np.float32)
+    scene = oracle.scene_box(boxes, geometry=1)
+    np.testing.assert_array_equal(scene, np.array(a["scene"], np.float32))
+    codes = oracle.morton64_values(boxes, scene, geometry=1)
+    assert [int(c) for c in codes] == a["codes"]
+
+
+def test_indirect_sort_golden(kat):
+    a = kat["indirect_sort"]
+    for stable in (False, True):
+        k, perm = oracle.sort_u64(a["keys"], stable_sort=stable)
+        assert list(k) == a["sorted"]
+        assert list(perm) == a["perm"]
+
+
+def _dfs_string(exp, n):
+    out = []
+
+    def trav(node):
+        while True:
+            if node < n:
+                out.append(f"L{node}")
+                r = exp["leaf_rope"][node]
+                if r == -1:
+                    return
+                node = r
+            else:
+                out.append(f"I{node - n}")
+                node = exp["left_child"][node - n]
+
+    trav(n)
+    return "".join(out)
+
+
+def test_karras_example_golden(kat):
+    a = kat["karras_8"]
+    n = len(a["codes"])
+    t = oracle.Tree(np.zeros((n, 3), np.float32), sorted_codes=a["codes"])
+    assert _dfs_string(t.export(), n) == a["dfs"]
+
+
+# ------------------------------------------------------------ sort
+@pytest.mark.parametrize("n", [1, 2, 1000, 200_000])
+def test_radix_sort_matches_stable_sort(n):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 1 << 63, n, dtype=np.uint64)
+    keys[: n // 3] = keys[n // 2: n // 2 + n // 3]  # many duplicates
+    k1, p1 = oracle.sort_u64(keys)
+    k2, p2 = oracle.sort_u64(keys, stable_sort=True)
+    np.testing.assert_array_equal(k1, k2)
+    np.testing.assert_array_equal(p1, p2)
+    # closed form: numpy's stable argsort
+    p3 = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(p1, p3.astype(np.uint32))
+
+
+# ------------------------------------------------------------ Morton closed form
+def _interleave(i, j, k, bits=21):
+    r = 0
+    for b in range(bits):
+        r |= ((i >> b) & 1) << (3 * b + 2) | ((j >> b) & 1) << (3 * b + 1) | ((k >> b) & 1) << (3 * b)
+    return r
+
+
+def test_morton64_closed_form():
+    pts = synthetic.uniform_points(2000, 11)
+    scene = oracle.scene_box(pts)
+    codes = oracle.morton64_values(pts, scene)
+    for p, c in zip(pts[:300], codes[:300]):
+        q = []
+        for d in range(3):
+            a, b = scene[d], scene[3 + d]
+            t = np.float32((p[d] - a) / np.float32(b - a)) if a != b else np.float32(0)
+            x = np.float32(t * np.float32(2097152.0))
+            x = min(max(x, np.float32(0)), np.float32(2097151))
+            q.append(int(x))
+        assert int(c) == _interleave(*q)
+
+
+# ------------------------------------------------------------ hierarchy
+def _check_tree_invariants(t, values, geometry):
+    e = t.export()
+    n = t.n
+    lo = values[:, :3]
+    hi = values[:, 3:] if geometry == 1 else values[:, :3]
+    perm = e["perm"].astype(np.int64)
+    # keys sorted, permutation
+    assert np.all(np.diff(e["keys"].astype(np.float64)) >= 0) or np.all(e["keys"][1:] >= e["keys"][:-1])
+    np.testing.assert_array_equal(np.sort(perm), np.arange(n))
+    # leaf order in a pre-order walk is 0..n-1 and every internal box is the
+    # union of its children's boxes
+    boxes = e["boxes"]
+    lc = e["left_child"]
+    leaf_rope = e["leaf_rope"]
+    rope = e["rope"]
+
+    def right_of(k):
+        c = lc[k]
+        return leaf_rope[c] if c < n else rope[c - n]
+
+    def box_of(node):
+        if node < n:
+            o = perm[node]
+            return np.concatenate([lo[o], hi[o]])
+        return boxes[node - n]
+
+    for k in range(n - 1):
+        l, r = lc[k], right_of(k)
+        bl, br = box_of(l), box_of(r)
+        u = np.concatenate([np.minimum(bl[:3], br[:3]), np.maximum(bl[3:], br[3:])])
+        np.testing.assert_array_equal(boxes[k], u)
+    np.testing.assert_array_equal(t.bounds(), oracle.scene_box(values, geometry))
+    seen = []
+    node = n
+    while node != -1:
+        if node < n:
+            seen.append(node)
+            node = leaf_rope[node]
+        else:
+            node = lc[node - n]
+    np.testing.assert_array_equal(seen, np.arange(n))
+
+
+@pytest.mark.parametrize("kind", ["uniform", "clustered", "duplicates", "boxes"])
+def test_hierarchy_invariants(kind):
+    n = 3000
+    if kind == "uniform":
+        v, g = synthetic.uniform_points(n, 1), 0
+    elif kind == "clustered":
+        v, g = synthetic.gaussian_mixture_points(n, 3, n_centres=5, sigma=0.01), 0
+    elif kind == "duplicates":
+        v = synthetic.uniform_points(n, 2)
+        v[::3] = v[0]
+        v, g = v, 0
+    else:
+        v, g = synthetic.lognormal_boxes(n, 4, mu=math.log(0.02)), 1
+    t = oracle.Tree(v, geometry=g)
+    _check_tree_invariants(t, v, g)
+
+
+@pytest.mark.parametrize("node_format", [0, 1])
+def test_linearized_layout(node_format):
+    """Pre-order slots: leaves appear in Morton order with ~original index;
+    every rope (slot + skip) lands on the next record after the subtree;
+    Q16 boxes contain the exact FP32 boxes."""
+    n = 2000
+    v = synthetic.uniform_points(n, 5)
+    t = oracle.Tree(v)
+    w = t.linearize(node_format)
+    e = t.export()
+    s_int = 2 if node_format == 0 else 1
+    assert w.shape[0] == n + s_int * (n - 1)
+    leaves = []
+    p = 0
+    while p < w.shape[0]:
+        x = np.int32(w[p, 3].astype(np.int64) - (1 << 32) if w[p, 3] >= (1 << 31) else w[p, 3])
+        if x < 0:
+            leaves.append(~int(x))
+            np.testing.assert_array_equal(w[p, :3].view(np.float32), v[~int(x)])
+            p += 1
+        else:
+            p += s_int
+    np.testing.assert_array_equal(leaves, e["perm"])
+
+
+def test_q16_encoder_is_tight_and_conservative():
+    rng = np.random.default_rng(7)
+    for _ in range(2000):
+        f = np.sort(rng.uniform(-5, 5, (2, 3)).astype(np.float32), axis=0)
+        frame = np.concatenate([f[0], f[1]])
+        b = np.sort(rng.uniform(0, 1, (2, 3)), axis=0)
+        box = np.concatenate([f[0] + (f[1] - f[0]) * b[0], f[0] + (f[1] - f[0]) * b[1]]).astype(np.float32)
+        box = np.concatenate([np.maximum(box[:3], frame[:3]), np.minimum(box[3:], frame[3:])])
+        q, dec = oracle.q16_encode_box(frame, box)
+        assert np.all(dec[:3] <= box[:3]) and np.all(dec[3:] >= box[3:])
+        for d in range(3):
+            s = np.float32((frame[3 + d] - frame[d]) * np.float32(1.0 / 65535.0))
+            if s > 0 and q[d] < 65535:  # one step up would cut the box
+                assert np.float32(frame[d] + np.float32(int(q[d]) + 1) * s) > box[d]
+            if s > 0 and q[3 + d] > 0:
+                assert np.float32(frame[3 + d] - np.float32(65535 - (int(q[3 + d]) - 1)) * s) < box[3 + d]
+
+
+# ------------------------------------------------------------ queries vs brute force
+@pytest.mark.parametrize("geometry", [0, 1])
+def test_spatial_spheres_vs_bruteforce(geometry):
+    n, nq = 6000, 1500
+    v = synthetic.uniform_points(n, 21) if geometry == 0 else synthetic.lognormal_boxes(n, 22, mu=math.log(0.01))
+    q = spheres_from_points(synthetic.uniform_points(nq, 23), 0.05)
+    t = oracle.Tree(v, geometry)
+    o1, i1 = t.query_spatial(0, q)
+    o2, i2 = oracle.bruteforce_spatial(v, geometry, 0, q)
+    assert_crs_equal_as_sets(o1, i1, o2, i2)
+    assert o1[-1] > nq  # non-trivial
+
+
+@pytest.mark.parametrize("geometry", [0, 1])
+def test_spatial_boxes_vs_bruteforce(geometry):
+    n, nq = 6000, 1500
+    v = synthetic.uniform_points(n, 31) if geometry == 0 else synthetic.lognormal_boxes(n, 32, mu=math.log(0.02))
+    q = synthetic.lognormal_boxes(nq, 33, mu=math.log(0.05))
+    t = oracle.Tree(v, geometry)
+    o1, i1 = t.query_spatial(1, q)
+    o2, i2 = oracle.bruteforce_spatial(v, geometry, 1, q)
+    assert_crs_equal_as_sets(o1, i1, o2, i2)
+    assert o1[-1] > 0
+
+
+@pytest.mark.parametrize("k", [1, 5, 16, 40])
+@pytest.mark.parametrize("data", ["uniform", "grid"])
+def test_knn_vs_bruteforce(k, data):
+    if data == "uniform":
+        v = synthetic.uniform_points(4000, 41)
+        q = synthetic.uniform_points(800, 42)
+    else:  # integer grid: many exactly tied distances
+        g = np.arange(16, dtype=np.float32)
+        v = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+        q = np.floor(synthetic.uniform_points(500, 43) * 16).astype(np.float32) + np.float32(0.5)
+        q[::2] = np.floor(q[::2])
+    t = oracle.Tree(v)
+    off, idx, dist = t.query_knn(q, k)
+    idx2, dist2 = oracle.bruteforce_knn(v, 0, q, k)
+    np.testing.assert_array_equal(idx, idx2)
+    np.testing.assert_array_equal(dist, dist2)
+    assert off[-1] == q.shape[0] * min(k, v.shape[0])
+
+
+# ------------------------------------------------------------ reference test cases
+def test_degenerate_cases(kat):
+    for c in kat["degenerate"]["cases"]:
+        tree = np.array(c["tree"], np.float32).reshape(-1, 6)
+        t = oracle.Tree(tree, geometry=1)
+        if c["kind"] == "nearest":
+            preds = np.array(c["preds"], np.float64).reshape(-1, 4)
+            if preds.shape[0] == 0:
+                continue
+            ks = preds[:, 3].astype(int)
+            off = [0]
+            idx = []
+            for p, k in zip(preds[:, :3].astype(np.float32), ks):
+                o, i, _ = t.query_knn(p[None], int(k))
+                idx += list(i)
+                off.append(len(idx))
+            assert_crs_equal_as_sets(off, idx, c["offsets"], c["indices"])
+        else:
+            kind = 0 if c["kind"] == "spheres" else 1
+            preds = np.array(c["preds"], np.float32).reshape(-1, 4 if kind == 0 else 6)
+            o, i = t.query_spatial(kind, preds)
+            assert_crs_equal_as_sets(o, i, c["offsets"], c["indices"]), c["line"]
+
+
+def test_manufactured_solution():
+    boxes, *_ = manufactured_grid()
+    t = oracle.Tree(boxes, geometry=1)
+    for name, q, off, ind in manufactured_queries():
+        o, i = t.query_spatial(1, q)
+        assert_crs_equal_as_sets(o, i, off, ind)
+
+
+def test_stack_capacity(kat):
+    b = diagonal_boxes(kat["stack_capacity"]["n"])
+    t = oracle.Tree(b, geometry=1)
+    FM = np.finfo(np.float32).max
+    q = np.array([[FM, FM, FM, -FM, -FM, -FM], [0, 0, 0, 4096, 4096, 4096]], np.float32)
+    o, _ = t.query_spatial(1, q)
+    assert o[-1] == kat["stack_capacity"]["spatial_total"]
+    o, _, _ = t.query_knn(np.zeros((1, 3), np.float32), 4096)
+    assert o[-1] == kat["stack_capacity"]["nearest_total"]
+
+
+def test_nearest_collinear(kat):
+    a = kat["nearest_collinear"]
+    t = oracle.Tree(np.array(a["points"], np.float32))
+    o, i, d = t.query_knn(np.array([a["query"]], np.float32), a["k"])
+    assert list(i) == a["indices"]
+    np.testing.assert_array_equal(d, np.array(a["distances"], np.float32))
+
+
+# ------------------------------------------------------------ energy
+def test_pair_energy_gradient_fd_f64():
+    """FP64 oracle gradient vs numpy FP64 central differences (h = 1e-6) on a
+    small jittered FCC block with a fixed neighbour set."""
+    x = synthetic.fcc_lattice((3, 3, 3), seed=6)
+    rc = 2.5
+    t = oracle.Tree(x)
+    off, idx = t.query_spatial(0, spheres_from_points(x, rc))
+    e, g = oracle.pair_energy_f64(x, off, idx, rc)
+    assert np.isfinite(e) and e < 0
+    xd = x.astype(np.float64)
+    irc6 = rc ** -6
+    phic = 4 * irc6 * (irc6 - 1)
+    R = rows(off, idx)
+
+    def e_row(i, xi):
+        s = 0.0
+        for j in R[i]:
+            if j == i:
+                continue
+            r2 = float(np.sum((xi - xd[j]) ** 2))
+            s += 4 * r2 ** -6 - 4 * r2 ** -3 - phic
+        return s
+
+    h = 1e-6
+    for i in range(0, x.shape[0], 7):
+        for d in range(3):
+            a = xd[i].copy()
+            b = xd[i].copy()
+            a[d] += h
+            b[d] -= h
+            fd = (e_row(i, a) - e_row(i, b)) / (2 * h)
+            assert abs(fd - g[i, d]) <= 1e-6 * max(1.0, abs(g[i, d])), (i, d, fd, g[i, d])
