@@ -149,12 +149,67 @@ AXB_D int32_t apetrei_rope(const uint64_t *__restrict__ keys, int nint, int n,
   return -1;
 }
 
-AXB_D int cas_agent(int32_t *p, int32_t expected, int32_t desired)
+// Hand-off between the two threads that meet at a parent.  The data one
+// thread leaves for the other (its node's box and spine height, or its leaf)
+// is written and read with relaxed agent-scope atomics, which are coherent
+// across the XCDs' L2s on their own; the writer waits for those stores to
+// complete before its CAS.  This gives the release/acquire pairing of
+// TreeConstruction.hpp:241-265 (CAS + load_fence) for exactly the words that
+// cross threads, without the whole-cache writeback/invalidate that an
+// acq_rel CAS at agent scope costs on gfx950.
+AXB_D void st_coh64(void *p, uint64_t v)
 {
-  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_ACQ_REL,
-                                       __ATOMIC_ACQUIRE,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store((uint64_t *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+AXB_D uint64_t ld_coh64(const void *p)
+{
+  return __hip_atomic_load((const uint64_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+AXB_D uint64_t pack2(uint32_t a, uint32_t b) { return (uint64_t)a | ((uint64_t)b << 32); }
+AXB_D uint32_t lo32(uint64_t w) { return (uint32_t)w; }
+AXB_D uint32_t hi32(uint64_t w) { return (uint32_t)(w >> 32); }
+
+AXB_D int cas_handoff(int32_t *p, int32_t expected, int32_t desired)
+{
+  // every coherent store issued so far has completed before the flag moves
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   return expected;
+}
+
+// Reads the box (and spine height) of a completed sibling.
+template <int G>
+AXB_D void read_sibling(bool is_leaf, int j, const float4 *leafrec, const INode *inodes,
+                        Box6 &b, int &h)
+{
+  if (is_leaf)
+  {
+    const uint64_t *w = (const uint64_t *)(leafrec + (G == GEOM_POINTS ? j : 2 * j));
+    const uint64_t w0 = ld_coh64(w), w1 = ld_coh64(w + 1);
+    b.lo[0] = bits_f(lo32(w0)), b.lo[1] = bits_f(hi32(w0)), b.lo[2] = bits_f(lo32(w1));
+    if (G == GEOM_POINTS)
+    {
+      b.hi[0] = b.lo[0], b.hi[1] = b.lo[1], b.hi[2] = b.lo[2];
+    }
+    else
+    {
+      const uint64_t w2 = ld_coh64(w + 2), w3 = ld_coh64(w + 3);
+      b.hi[0] = bits_f(lo32(w2)), b.hi[1] = bits_f(hi32(w2)), b.hi[2] = bits_f(lo32(w3));
+    }
+    h = 0;
+  }
+  else
+  {
+    const uint64_t *w = (const uint64_t *)(inodes + j);
+    const uint64_t w0 = ld_coh64(w), w1 = ld_coh64(w + 1), w2 = ld_coh64(w + 2),
+                   w3 = ld_coh64(w + 3), w5 = ld_coh64(w + 5);
+    b.lo[0] = bits_f(lo32(w0)), b.lo[1] = bits_f(hi32(w0)), b.lo[2] = bits_f(lo32(w1));
+    b.hi[0] = bits_f(lo32(w2)), b.hi[1] = bits_f(hi32(w2)), b.hi[2] = bits_f(lo32(w3));
+    h = (int)lo32(w5);
+  }
 }
 
 // TreeConstruction.hpp:197-311: one thread per leaf climbs toward the root;
@@ -183,15 +238,20 @@ __global__ __launch_bounds__(256) void k_climb(
     bv.lo[2] = kmin(FLT_MAX, z);
     bv.hi[0] = kmax(-FLT_MAX, x), bv.hi[1] = kmax(-FLT_MAX, y),
     bv.hi[2] = kmax(-FLT_MAX, z);
-    leafrec[i0] = make_float4(x, y, z, bits_f(~orig));
+    uint64_t *w = (uint64_t *)(leafrec + i0);
+    st_coh64(w, pack2(f_bits(x), f_bits(y)));
+    st_coh64(w + 1, pack2(f_bits(z), ~orig));
   }
   else
   {
     const float *v = values + 6 * (int64_t)orig;
     for (int d = 0; d < 3; ++d)
       bv.lo[d] = kmin(FLT_MAX, v[d]), bv.hi[d] = kmax(-FLT_MAX, v[3 + d]);
-    leafrec[2 * i0] = make_float4(bv.lo[0], bv.lo[1], bv.lo[2], bits_f(~orig));
-    leafrec[2 * i0 + 1] = make_float4(bv.hi[0], bv.hi[1], bv.hi[2], 0.f);
+    uint64_t *w = (uint64_t *)(leafrec + 2 * i0);
+    st_coh64(w, pack2(f_bits(bv.lo[0]), f_bits(bv.lo[1])));
+    st_coh64(w + 1, pack2(f_bits(bv.lo[2]), ~orig));
+    st_coh64(w + 2, pack2(f_bits(bv.hi[0]), f_bits(bv.hi[1])));
+    st_coh64(w + 3, pack2(f_bits(bv.hi[2]), 0u));
   }
   int range_left = i0, range_right = i0;
   int64_t dl = apetrei_delta(keys, nint, range_left - 1);
@@ -208,7 +268,7 @@ __global__ __launch_bounds__(256) void k_climb(
     if (is_left)
     {
       ap = range_right;
-      int old = cas_agent(&ranges[ap], -1, range_left);
+      int old = cas_handoff(&ranges[ap], -1, range_left);
       if (old == -1)
         break;
       range_right = old;
@@ -217,47 +277,21 @@ __global__ __launch_bounds__(256) void k_climb(
       const int rc = ap + 1;
       const bool rc_leaf = (rc == range_right);
       dr = apetrei_delta(keys, nint, range_right);
-      if (rc_leaf)
-      {
-        float4 a = leafrec[G == GEOM_POINTS ? rc : 2 * rc];
-        float4 b = G == GEOM_POINTS ? a : leafrec[2 * rc + 1];
-        sb = {{a.x, a.y, a.z}, {b.x, b.y, b.z}};
-        h_right = 0;
-        right_node = rc;
-      }
-      else
-      {
-        const INode &r = inodes[rc];
-        sb = {{r.lo[0], r.lo[1], r.lo[2]}, {r.hi[0], r.hi[1], r.hi[2]}};
-        h_right = r.h;
-        right_node = rc + n;
-      }
+      read_sibling<G>(rc_leaf, rc, leafrec, inodes, sb, h_right);
+      right_node = rc_leaf ? rc : rc + n;
     }
     else
     {
       ap = range_left - 1;
-      int old = cas_agent(&ranges[ap], -1, range_right);
+      int old = cas_handoff(&ranges[ap], -1, range_right);
       if (old == -1)
         break;
       range_left = old;
       const int lc = ap;
       const bool lc_leaf = (lc == range_left);
       dl = apetrei_delta(keys, nint, range_left - 1);
-      if (lc_leaf)
-      {
-        float4 a = leafrec[G == GEOM_POINTS ? lc : 2 * lc];
-        float4 b = G == GEOM_POINTS ? a : leafrec[2 * lc + 1];
-        sb = {{a.x, a.y, a.z}, {b.x, b.y, b.z}};
-        h_left = 0;
-        left_child = lc;
-      }
-      else
-      {
-        const INode &l = inodes[lc];
-        sb = {{l.lo[0], l.lo[1], l.lo[2]}, {l.hi[0], l.hi[1], l.hi[2]}};
-        h_left = l.h;
-        left_child = lc + n;
-      }
+      read_sibling<G>(lc_leaf, lc, leafrec, inodes, sb, h_left);
+      left_child = lc_leaf ? lc : lc + n;
       right_node = cur;
       h_right = cur_h;
     }
@@ -268,16 +302,14 @@ __global__ __launch_bounds__(256) void k_climb(
     }
     const int kp = dr < dl ? range_right : range_left;
     const int hp = left_child < n ? 1 : h_left + 1;
-    INode rec;
-    rec.lo[0] = bv.lo[0], rec.lo[1] = bv.lo[1], rec.lo[2] = bv.lo[2];
-    rec.hi[0] = bv.hi[0], rec.hi[1] = bv.hi[1], rec.hi[2] = bv.hi[2];
-    rec.left_child = left_child;
-    rec.rope = ROPES ? apetrei_rope(keys, nint, n, range_right, dr) : -1;
-    rec.first = range_left;
-    rec.last = range_right;
-    rec.h = hp;
-    rec.pad = 0;
-    inodes[kp] = rec;
+    const int32_t rope = ROPES ? apetrei_rope(keys, nint, n, range_right, dr) : -1;
+    uint64_t *w = (uint64_t *)(inodes + kp);
+    st_coh64(w, pack2(f_bits(bv.lo[0]), f_bits(bv.lo[1])));
+    st_coh64(w + 1, pack2(f_bits(bv.lo[2]), (uint32_t)left_child));
+    st_coh64(w + 2, pack2(f_bits(bv.hi[0]), f_bits(bv.hi[1])));
+    st_coh64(w + 3, pack2(f_bits(bv.hi[2]), (uint32_t)rope));
+    st_coh64(w + 4, pack2((uint32_t)range_left, (uint32_t)range_right));
+    st_coh64(w + 5, pack2((uint32_t)hp, 0u));
     if (right_node >= n)
     {
       // the right child tops a left spine starting at leaf ap + 1

@@ -165,9 +165,9 @@ AXB_HD uint32_t f_bits(float f)
   return v.u;
 }
 
-// Correctly rounded float sqrt: a double sqrt rounded to float is correctly
-// rounded (53 >= 2*24 + 2 bits).
-AXB_HD float sqrt_rn(float x) { return (float)sqrt((double)x); }
+// Correctly rounded float sqrt (IEEE sqrt under
+// -fhip-fp32-correctly-rounded-divide-sqrt on the device, libm on the host).
+AXB_HD float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 
 // Largest float s with RN(sqrt(s)) <= r.  sqrt is monotone and never lands
 // exactly on the midpoint m between r and the next float (m has 25

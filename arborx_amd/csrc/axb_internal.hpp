@@ -94,7 +94,7 @@ hipError_t fill_u64(uint64_t *p, uint64_t v, int64_t n, hipStream_t stream);
 
 // ------------------------------------------------------------ tree data
 // Internal node as produced by the climb (48 B).
-struct INode
+struct alignas(16) INode
 {
   float lo[3];
   int32_t left_child; // reference encoding: leaf j -> j, internal k -> k + n

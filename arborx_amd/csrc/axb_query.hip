@@ -332,121 +332,27 @@ AXB_D float leaf_d2(float qx, float qy, float qz, uint4 s0, uint4 s1)
   return dist2_pbox(qx, qy, qz, lb);
 }
 
+// Rope walk over [p, pend) for nearest search: a node farther than the
+// current k-th distance is skipped (`d2 > thr`, ties kept), a leaf closer
+// than it is inserted.  `rank` is the Morton rank of the next leaf; leaves
+// of rank [j0, j0 + m) were seeded already.  The subtree at skip_at (already
+// walked) is jumped over.
 template <int F, int G, class List>
-AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
-                        List &L, uint32_t &v_int, uint32_t &v_leaf)
+AXB_D void knn_walk(const TreeView &tv, float qx, float qy, float qz, List &L, float &thr,
+                    uint32_t p, uint32_t pend, int rank, bool desc, Box6 par, float ps[3],
+                    uint32_t skip_at, uint32_t skip_slots, int skip_leaves, int j0, int m,
+                    uint32_t &v_int, uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
-  const uint32_t end = tv.total;
-  const int n = tv.n;
-  // ---- phase 1: greedy descent (closer child first, ties left), tracking
-  // the Morton rank of the first leaf of the current node.
-  int first = 0;
-  uint32_t p = 0;
-  Box6 cur;
-  float cs[3];
+  while (p < pend)
   {
-    const uint4 s0 = tv.slots[p], s1 = tv.slots[p + 1];
-    if ((int32_t)s0.w >= 0)
+    if (p == skip_at)
     {
-      ++v_int;
-      node_box<F>(tv, s0, s1, false, cur, cs, cur);
-      set_frame<F>(cur, cur, cs);
+      p += skip_slots;
+      rank += skip_leaves;
+      desc = false;
+      continue;
     }
-  }
-  while (true)
-  {
-    const uint4 s0 = tv.slots[p];
-    if ((int32_t)s0.w < 0)
-      break;
-    const uint32_t lp = p + SI;
-    const uint4 l0 = tv.slots[lp], l1 = tv.slots[lp + 1];
-    const int32_t lw = (int32_t)l0.w;
-    const uint32_t lskip = lw < 0 ? SL : (uint32_t)lw;
-    const int llen = lw < 0 ? 1 : (int)((lskip + SI) / (SI + SL));
-    const uint32_t rp = lp + lskip;
-    if (rp >= end)
-    {
-      raise_error(tv.err, KERR_LAYOUT);
-      return;
-    }
-    const uint4 r0 = tv.slots[rp], r1 = tv.slots[rp + 1];
-    const int32_t rw = (int32_t)r0.w;
-    Box6 lb, rb;
-    float dl, dr;
-    v_leaf += (lw < 0) + (rw < 0);
-    v_int += (lw >= 0) + (rw >= 0);
-    if (lw < 0)
-      dl = leaf_d2<G>(qx, qy, qz, l0, l1);
-    else
-    {
-      node_box<F>(tv, l0, l1, true, cur, cs, lb);
-      dl = dist2_pbox(qx, qy, qz, lb);
-    }
-    if (rw < 0)
-      dr = leaf_d2<G>(qx, qy, qz, r0, r1);
-    else
-    {
-      node_box<F>(tv, r0, r1, false, cur, cs, rb);
-      dr = dist2_pbox(qx, qy, qz, rb);
-    }
-    if (dl <= dr)
-    {
-      p = lp;
-      if (lw >= 0)
-        set_frame<F>(lb, cur, cs);
-    }
-    else
-    {
-      p = rp;
-      first += llen;
-      if (rw >= 0)
-        set_frame<F>(rb, cur, cs);
-    }
-  }
-  // ---- seed: the m leaves of ranks [j0, j0 + m) around `first`
-  int j0 = first - (m - 1) / 2;
-  j0 = j0 < 0 ? 0 : (j0 > n - m ? n - m : j0);
-  {
-    uint32_t s = SL * (uint32_t)j0 + SI * (uint32_t)tv.C[j0];
-    int got = 0;
-    while (got < m)
-    {
-      if (s >= end)
-      {
-        raise_error(tv.err, KERR_LAYOUT);
-        return;
-      }
-      const uint4 s0 = tv.slots[s];
-      if ((int32_t)s0.w < 0)
-      {
-        ++v_leaf;
-        const uint4 s1 = SL == 2 ? tv.slots[s + 1] : s0;
-        const float dist = sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1));
-        const int32_t idx = ~(int32_t)s0.w;
-        if (L.accepts(dist, idx))
-          L.insert(dist, idx);
-        ++got;
-        s += SL;
-      }
-      else
-      {
-        ++v_int;
-        s += SI;
-      }
-    }
-  }
-  if (m >= n)
-    return; // every leaf is already in the list
-  // ---- phase 2: rope walk with the current k-th distance as the bound
-  float thr = L.full() ? sqrt_threshold(L.kth_d) : INFINITY;
-  Box6 par;
-  float ps[3];
-  bool desc = false;
-  int rank = 0;
-  p = 0;
-  while (p < end)
-  {
     const uint4 s0 = tv.slots[p];
     const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
     const int32_t w = (int32_t)s0.w;
@@ -495,6 +401,146 @@ AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
         rank += (int)(((uint32_t)w + SI) / (SI + SL));
       }
     }
+  }
+}
+
+template <int F, int G, class List>
+AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
+                        List &L, uint32_t &v_int, uint32_t &v_leaf)
+{
+  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  const uint32_t end = tv.total;
+  const int n = tv.n;
+  // ---- phase 1: greedy descent (closer child first, ties left), tracking
+  // the Morton rank of the first leaf of the current node and the deepest
+  // node X on the path that still holds >= lmin leaves.
+  const int lmin = 4 * m > 32 ? 4 * m : 32;
+  int first = 0;
+  uint32_t p = 0;
+  Box6 cur;
+  float cs[3];
+  uint32_t pX = 0, sX = end;
+  int fX = 0, lenX = n;
+  Box6 DX;
+  {
+    const uint4 s0 = tv.slots[p], s1 = tv.slots[p + 1];
+    if ((int32_t)s0.w >= 0)
+    {
+      ++v_int;
+      node_box<F>(tv, s0, s1, false, cur, cs, cur);
+      set_frame<F>(cur, cur, cs);
+      DX = cur;
+    }
+  }
+  while (true)
+  {
+    const uint4 s0 = tv.slots[p];
+    if ((int32_t)s0.w < 0)
+      break;
+    const uint32_t lp = p + SI;
+    const uint4 l0 = tv.slots[lp], l1 = tv.slots[lp + 1];
+    const int32_t lw = (int32_t)l0.w;
+    const uint32_t lskip = lw < 0 ? SL : (uint32_t)lw;
+    const int llen = lw < 0 ? 1 : (int)((lskip + SI) / (SI + SL));
+    const uint32_t rp = lp + lskip;
+    if (rp >= end)
+    {
+      raise_error(tv.err, KERR_LAYOUT);
+      return;
+    }
+    const uint4 r0 = tv.slots[rp], r1 = tv.slots[rp + 1];
+    const int32_t rw = (int32_t)r0.w;
+    Box6 lb, rb;
+    float dl, dr;
+    v_leaf += (lw < 0) + (rw < 0);
+    v_int += (lw >= 0) + (rw >= 0);
+    if (lw < 0)
+      dl = leaf_d2<G>(qx, qy, qz, l0, l1);
+    else
+    {
+      node_box<F>(tv, l0, l1, true, cur, cs, lb);
+      dl = dist2_pbox(qx, qy, qz, lb);
+    }
+    if (rw < 0)
+      dr = leaf_d2<G>(qx, qy, qz, r0, r1);
+    else
+    {
+      node_box<F>(tv, r0, r1, false, cur, cs, rb);
+      dr = dist2_pbox(qx, qy, qz, rb);
+    }
+    if (dl <= dr)
+    {
+      p = lp;
+      if (lw >= 0)
+      {
+        set_frame<F>(lb, cur, cs);
+        if (llen >= lmin)
+          pX = lp, sX = lskip, fX = first, lenX = llen, DX = lb;
+      }
+    }
+    else
+    {
+      p = rp;
+      first += llen;
+      if (rw >= 0)
+      {
+        set_frame<F>(rb, cur, cs);
+        const int rlen = (int)(((uint32_t)rw + SI) / (SI + SL));
+        if (rlen >= lmin)
+          pX = rp, sX = (uint32_t)rw, fX = first, lenX = rlen, DX = rb;
+      }
+    }
+  }
+  // ---- seed: the m leaves of ranks [j0, j0 + m) around `first`
+  int j0 = first - (m - 1) / 2;
+  j0 = j0 < 0 ? 0 : (j0 > n - m ? n - m : j0);
+  {
+    uint32_t s = SL * (uint32_t)j0 + SI * (uint32_t)tv.C[j0];
+    int got = 0;
+    while (got < m)
+    {
+      if (s >= end)
+      {
+        raise_error(tv.err, KERR_LAYOUT);
+        return;
+      }
+      const uint4 s0 = tv.slots[s];
+      if ((int32_t)s0.w < 0)
+      {
+        ++v_leaf;
+        const uint4 s1 = SL == 2 ? tv.slots[s + 1] : s0;
+        const float dist = sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1));
+        const int32_t idx = ~(int32_t)s0.w;
+        if (L.accepts(dist, idx))
+          L.insert(dist, idx);
+        ++got;
+        s += SL;
+      }
+      else
+      {
+        ++v_int;
+        s += SI;
+      }
+    }
+  }
+  if (m >= n)
+    return; // every leaf is already in the list
+  float thr = L.full() ? sqrt_threshold(L.kth_d) : INFINITY;
+  float ps[3];
+  if (lenX < n)
+  {
+    // ---- phase 2a: the local subtree X first (tightens the k-th distance)
+    set_frame<F>(DX, DX, ps);
+    knn_walk<F, G>(tv, qx, qy, qz, L, thr, pX + SI, pX + sX, fX, true, DX, ps, UINT32_MAX, 0,
+                   0, j0, m, v_int, v_leaf);
+    // ---- phase 2b: the rest of the tree
+    knn_walk<F, G>(tv, qx, qy, qz, L, thr, 0, end, 0, false, DX, ps, pX, sX, lenX, j0, m, v_int,
+                   v_leaf);
+  }
+  else
+  {
+    knn_walk<F, G>(tv, qx, qy, qz, L, thr, 0, end, 0, false, DX, ps, UINT32_MAX, 0, 0, j0, m,
+                   v_int, v_leaf);
   }
 }
 
