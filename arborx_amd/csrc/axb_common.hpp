@@ -190,8 +190,9 @@ AXB_HD float sqrt_threshold(float r)
 
 // ---------------------------------------------------------------- Q16
 // Node box quantised on the 65535-step grid of a frame box F:
-//   lo(q) = F.lo + q * s,  hi(q) = F.hi - (65535 - q) * s,
-//   s = (F.hi - F.lo) * (1/65535)  (float, no FMA).
+//   lo(q) = fma(q, s, F.lo),  hi(q) = fma(-(65535 - q), s, F.hi),
+//   s = (F.hi - F.lo) * (1/65535)
+// (one correctly rounded fma each, identical on the host and the device).
 // qlo is the largest q with lo(q) <= box.lo and qhi the smallest with
 // hi(q) >= box.hi, so the decoded box always contains the exact one.
 constexpr float Q16_INV = 1.0f / 65535.0f;
@@ -199,11 +200,11 @@ constexpr float Q16_INV = 1.0f / 65535.0f;
 AXB_HD float q16_scale(float flo, float fhi) { return (fhi - flo) * Q16_INV; }
 AXB_HD float q16_lo(float flo, float s, uint32_t q)
 {
-  return flo + (float)q * s;
+  return __builtin_fmaf((float)q, s, flo);
 }
 AXB_HD float q16_hi(float fhi, float s, uint32_t q)
 {
-  return fhi - (float)(65535u - q) * s;
+  return __builtin_fmaf(-(float)(65535u - q), s, fhi);
 }
 AXB_HD void q16_encode_axis(float flo, float fhi, float vlo, float vhi,
                             uint32_t &qlo, uint32_t &qhi)

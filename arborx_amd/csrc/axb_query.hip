@@ -21,11 +21,17 @@
 // 23-73).
 #include "axb_internal.hpp"
 
+#include <algorithm>
+
 namespace axb
 {
 
 namespace
 {
+
+// int32 x 4 with 4-byte alignment (gfx9 global stores take dword-aligned
+// multi-dword accesses).
+typedef int32_t i32x4_u __attribute__((ext_vector_type(4), aligned(4)));
 
 template <int F>
 AXB_D void node_box(const TreeView &tv, uint4 s0, uint4 s1, bool desc,
@@ -92,6 +98,8 @@ __global__ __launch_bounds__(256) void k_spatial(
   else if (MODE == MODE_BUFFER)
     base = q * (int64_t)buffer;
   int64_t cnt = 0;
+  int32_t h0 = 0, h1 = 0, h2 = 0; // fill pass: hits not yet written
+  int np = 0;
 
   Box6 par;
   float ps[3];
@@ -124,8 +132,23 @@ __global__ __launch_bounds__(256) void k_spatial(
       }
       if (hit)
       {
-        if (MODE == MODE_FILL && cnt < limit)
-          out[base + cnt] = ~w;
+        if (MODE == MODE_FILL)
+        {
+          // results leave in 16-byte groups (one store per 4 hits)
+          if (np == 3)
+          {
+            if (cnt < limit)
+              *(i32x4_u *)(out + base + cnt - 3) = (i32x4_u){h0, h1, h2, ~w};
+            np = 0;
+          }
+          else
+          {
+            h0 = np == 0 ? ~w : h0;
+            h1 = np == 1 ? ~w : h1;
+            h2 = np == 2 ? ~w : h2;
+            ++np;
+          }
+        }
         else if (MODE == MODE_BUFFER && cnt < buffer)
           out[base + cnt] = ~w;
         ++cnt;
@@ -158,6 +181,14 @@ __global__ __launch_bounds__(256) void k_spatial(
       }
     }
   }
+  if (MODE == MODE_FILL && np > 0 && cnt <= limit)
+  {
+    out[base + cnt - np] = h0;
+    if (np > 1)
+      out[base + cnt - np + 1] = h1;
+    if (np > 2)
+      out[base + cnt - np + 2] = h2;
+  }
   if (MODE != MODE_FILL)
     offsets[q] = cnt;
   else if (cnt != limit)
@@ -177,54 +208,57 @@ AXB_D bool pair_less(float da, int32_t ia, float db, int32_t ib)
   return da < db || (da == db && ia < ib);
 }
 
-// Sorted list of K (distance, index) pairs in registers; only the first k
-// (k <= K) entries are meaningful.
+// (distance, index) packed into one 64-bit key: distances are >= 0, whose
+// float bit patterns order like the values, so the key order is exactly the
+// lexicographic (distance, index) order.
+AXB_D uint64_t pack_key(float d, int32_t i)
+{
+  return ((uint64_t)f_bits(d) << 32) | (uint32_t)i;
+}
+
+// Sorted list of K keys in registers; only the first k (k <= K) entries are
+// meaningful.  Insertion is a branch-free network a'[j] = min(a[j],
+// max(a[j-1], x)) with static indexing only.
 template <int K>
 struct RegList
 {
-  float d[K];
-  int32_t i[K];
+  uint64_t a[K];
   int k, count;
   float kth_d;
-  int32_t kth_i;
+  uint64_t kth;
   AXB_D void init(int k_)
   {
     k = k_;
     count = 0;
 #pragma unroll
     for (int j = 0; j < K; ++j)
-      d[j] = INFINITY, i[j] = INT_MAX;
+      a[j] = ~0ull;
     kth_d = INFINITY;
-    kth_i = INT_MAX;
+    kth = ~0ull;
   }
   AXB_D bool full() const { return count >= k; }
-  AXB_D bool accepts(float dn, int32_t in) const
-  {
-    return count < k || pair_less(dn, in, kth_d, kth_i);
-  }
+  AXB_D bool accepts(float dn, int32_t in) const { return pack_key(dn, in) < kth; }
   AXB_D void insert(float dn, int32_t in)
   {
+    const uint64_t x = pack_key(dn, in);
 #pragma unroll
     for (int j = K - 1; j > 0; --j)
     {
-      const bool lt_prev = pair_less(dn, in, d[j - 1], i[j - 1]);
-      const bool lt_cur = pair_less(dn, in, d[j], i[j]);
-      const float nd = lt_prev ? d[j - 1] : (lt_cur ? dn : d[j]);
-      const int32_t ni = lt_prev ? i[j - 1] : (lt_cur ? in : i[j]);
-      d[j] = nd;
-      i[j] = ni;
+      const uint64_t hi = a[j - 1] < x ? x : a[j - 1];
+      a[j] = hi < a[j] ? hi : a[j];
     }
-    if (pair_less(dn, in, d[0], i[0]))
-      d[0] = dn, i[0] = in;
+    a[0] = x < a[0] ? x : a[0];
     ++count;
-    float kd = d[0];
-    int32_t ki = i[0];
+    uint64_t kk = a[0];
 #pragma unroll
     for (int j = 1; j < K; ++j)
       if (j == k - 1)
-        kd = d[j], ki = i[j];
-    kth_d = kd;
-    kth_i = ki;
+        kk = a[j];
+    if (count >= k)
+    {
+      kth = kk;
+      kth_d = bits_f((uint32_t)(kk >> 32));
+    }
   }
   AXB_D void write(int32_t *oi, float *od, int per) const
   {
@@ -232,9 +266,9 @@ struct RegList
     for (int j = 0; j < K; ++j)
       if (j < per)
       {
-        oi[j] = i[j];
+        oi[j] = (int32_t)(uint32_t)a[j];
         if (od)
-          od[j] = d[j];
+          od[j] = bits_f((uint32_t)(a[j] >> 32));
       }
   }
 };
@@ -337,49 +371,74 @@ AXB_D float leaf_d2(float qx, float qy, float qz, uint4 s0, uint4 s1)
 // than it is inserted.  `rank` is the Morton rank of the next leaf; leaves
 // of rank [j0, j0 + m) were seeded already.  The subtree at skip_at (already
 // walked) is jumped over.
-template <int F, int G, class List>
-AXB_D void knn_walk(const TreeView &tv, float qx, float qy, float qz, List &L, float &thr,
-                    uint32_t p, uint32_t pend, int rank, bool desc, Box6 par, float ps[3],
-                    uint32_t skip_at, uint32_t skip_slots, int skip_leaves, int j0, int m,
-                    uint32_t &v_int, uint32_t &v_leaf)
+// Candidates (leaves within the current k-th distance) wait in a per-lane
+// LDS buffer; the list is updated in flushes that the whole wavefront takes
+// together once some lane's buffer is full, so the insertion network runs a
+// few times per wave instead of at almost every step of a divergent walk.
+constexpr int KNN_BUF = 8;
+
+struct CandBuf
 {
-  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
-  while (p < pend)
+  float *d2;   // KNN_BUF x blockDim, strided by lane
+  int32_t *id;
+  int stride, lane, cnt;
+};
+
+template <class List>
+AXB_D void knn_flush(List &L, float &thr, CandBuf &B)
+{
+#pragma unroll 1
+  for (int j = 0; j < KNN_BUF; ++j)
   {
-    if (p == skip_at)
+    if (j < B.cnt)
     {
-      p += skip_slots;
-      rank += skip_leaves;
-      desc = false;
-      continue;
-    }
-    const uint4 s0 = tv.slots[p];
-    const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
-    const int32_t w = (int32_t)s0.w;
-    if (w < 0)
-    {
-      ++v_leaf;
-      if (rank < j0 || rank >= j0 + m)
+      const float d2 = B.d2[j * B.stride + B.lane];
+      if (d2 <= thr)
       {
-        const float d2 = leaf_d2<G>(qx, qy, qz, s0, s1);
-        if (d2 <= thr)
+        const float dist = sqrt_rn(d2);
+        const int32_t idx = B.id[j * B.stride + B.lane];
+        if (L.accepts(dist, idx))
         {
-          const float dist = sqrt_rn(d2);
-          const int32_t idx = ~w;
-          if (L.accepts(dist, idx))
-          {
-            L.insert(dist, idx);
-            if (L.full())
-              thr = sqrt_threshold(L.kth_d);
-          }
+          L.insert(dist, idx);
+          if (L.full())
+            thr = sqrt_threshold(L.kth_d);
         }
       }
-      ++rank;
-      p += SL;
-      desc = false;
     }
-    else
+  }
+  B.cnt = 0;
+}
+
+template <int F, int G, class List>
+AXB_D void knn_walk(const TreeView &tv, float qx, float qy, float qz, List &L, float &thr,
+                    CandBuf &B, uint32_t p, uint32_t pend, int rank, bool desc, Box6 par,
+                    float ps[3], uint32_t skip_at, uint32_t skip_slots, int skip_leaves, int j0,
+                    int m, uint32_t &v_int, uint32_t &v_leaf)
+{
+  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  uint4 s0 = make_uint4(0, 0, 0, 0), s1 = s0;
+  // Outer loop: the whole wavefront; inner loops ("while-while"): a run of
+  // internal nodes up to the next leaf, then a run of leaves up to the next
+  // internal node (or a full candidate buffer).  Lanes that finish an inner
+  // run early wait instead of executing the other record kind's code.
+  while (__ballot(p < pend))
+  {
+    if (__ballot(B.cnt == KNN_BUF))
+      knn_flush(L, thr, B);
+    while (p < pend)
     {
+      if (p == skip_at)
+      {
+        p += skip_slots;
+        rank += skip_leaves;
+        desc = false;
+        continue;
+      }
+      s0 = tv.slots[p];
+      s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
+      const int32_t w = (int32_t)s0.w;
+      if (w < 0)
+        break;
       ++v_int;
       Box6 b;
       node_box<F>(tv, s0, s1, desc, par, ps, b);
@@ -394,19 +453,42 @@ AXB_D void knn_walk(const TreeView &tv, float qx, float qy, float qz, List &L, f
         if (w == 0)
         {
           raise_error(tv.err, KERR_LAYOUT);
-          return;
+          p = pend;
+          break;
         }
         desc = false;
         p += (uint32_t)w;
         rank += (int)(((uint32_t)w + SI) / (SI + SL));
       }
     }
+    while (p < pend && (int32_t)s0.w < 0 && B.cnt < KNN_BUF)
+    {
+      ++v_leaf;
+      if (rank < j0 || rank >= j0 + m)
+      {
+        const float d2 = leaf_d2<G>(qx, qy, qz, s0, s1);
+        if (d2 <= thr)
+        {
+          B.d2[B.cnt * B.stride + B.lane] = d2;
+          B.id[B.cnt * B.stride + B.lane] = ~(int32_t)s0.w;
+          ++B.cnt;
+        }
+      }
+      ++rank;
+      p += SL;
+      desc = false;
+      if (p >= pend || p == skip_at)
+        break;
+      s0 = tv.slots[p];
+      s1 = G == GEOM_BOXES ? tv.slots[p + 1] : s0;
+    }
   }
+  knn_flush(L, thr, B);
 }
 
 template <int F, int G, class List>
 AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
-                        List &L, uint32_t &v_int, uint32_t &v_leaf)
+                        List &L, CandBuf &B, uint32_t &v_int, uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
   const uint32_t end = tv.total;
@@ -527,49 +609,269 @@ AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
     return; // every leaf is already in the list
   float thr = L.full() ? sqrt_threshold(L.kth_d) : INFINITY;
   float ps[3];
-  if (lenX < n)
+  set_frame<F>(DX, DX, ps);
+  // ---- phase 2: the local subtree X first (tightens the k-th distance),
+  // then the whole tree with X jumped over.  One walk per segment.
+  const bool local = lenX < n;
+  for (int seg = local ? 0 : 1; seg < 2; ++seg)
   {
-    // ---- phase 2a: the local subtree X first (tightens the k-th distance)
-    set_frame<F>(DX, DX, ps);
-    knn_walk<F, G>(tv, qx, qy, qz, L, thr, pX + SI, pX + sX, fX, true, DX, ps, UINT32_MAX, 0,
-                   0, j0, m, v_int, v_leaf);
-    // ---- phase 2b: the rest of the tree
-    knn_walk<F, G>(tv, qx, qy, qz, L, thr, 0, end, 0, false, DX, ps, pX, sX, lenX, j0, m, v_int,
-                   v_leaf);
-  }
-  else
-  {
-    knn_walk<F, G>(tv, qx, qy, qz, L, thr, 0, end, 0, false, DX, ps, UINT32_MAX, 0, 0, j0, m,
-                   v_int, v_leaf);
+    const bool first_seg = seg == 0;
+    knn_walk<F, G>(tv, qx, qy, qz, L, thr, B, first_seg ? pX + SI : 0u,
+                   first_seg ? pX + sX : end, first_seg ? fX : 0, first_seg, DX, ps,
+                   (local && !first_seg) ? pX : UINT32_MAX, sX, lenX, j0, m, v_int, v_leaf);
   }
 }
 
+// ------------------------------------------------ nearest: collect + select
+// For k <= 64 the search runs as three kernels:
+//  collect  greedy descent, then the k-th distance R0 of the W = 4k leaves
+//           around the query in Morton order (an upper bound of the true
+//           k-th distance: those are k real points within R0); then a plain
+//           fixed-radius rope walk that appends every leaf with
+//           RN(sqrt(d2)) <= R0 to the query's candidate row (cap entries);
+//  select   exact (distance, index) top-k of each row in a register list;
+//  fallback the few queries whose row overflowed run the exact adaptive walk
+//           (knn_traverse) instead.
+// The walk of `collect` carries no list, so it runs at the register budget
+// of the spatial traversal.
 template <int F, int G, int K>
-__global__ __launch_bounds__(256) void k_knn(TreeView tv,
-                                             const float *__restrict__ pts,
-                                             int64_t nq, int k,
-                                             const uint32_t *__restrict__ order,
-                                             int32_t *__restrict__ out_idx,
-                                             float *__restrict__ out_dist)
+AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, int W,
+                     uint32_t &v_int, uint32_t &v_leaf)
 {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nq)
-    return;
-  const int64_t q = order ? (int64_t)order[t] : t;
-  const int m = k < tv.n ? k : tv.n;
-  const float qx = pts[3 * q], qy = pts[3 * q + 1], qz = pts[3 * q + 2];
+  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  const uint32_t end = tv.total;
+  const int n = tv.n;
+  int first = 0;
+  uint32_t p = 0;
+  Box6 cur;
+  float cs[3];
+  {
+    const uint4 s0 = tv.slots[0], s1 = tv.slots[1];
+    if ((int32_t)s0.w >= 0)
+    {
+      ++v_int;
+      node_box<F>(tv, s0, s1, false, cur, cs, cur);
+      set_frame<F>(cur, cur, cs);
+    }
+  }
+  while (true)
+  {
+    const uint4 s0 = tv.slots[p];
+    if ((int32_t)s0.w < 0)
+      break;
+    const uint32_t lp = p + SI;
+    const uint4 l0 = tv.slots[lp], l1 = tv.slots[lp + 1];
+    const int32_t lw = (int32_t)l0.w;
+    const uint32_t lskip = lw < 0 ? SL : (uint32_t)lw;
+    const uint32_t rp = lp + lskip;
+    if (rp >= end)
+    {
+      raise_error(tv.err, KERR_LAYOUT);
+      return INFINITY;
+    }
+    const uint4 r0 = tv.slots[rp], r1 = tv.slots[rp + 1];
+    const int32_t rw = (int32_t)r0.w;
+    Box6 lb, rb;
+    float dl, dr;
+    v_leaf += (lw < 0) + (rw < 0);
+    v_int += (lw >= 0) + (rw >= 0);
+    if (lw < 0)
+      dl = leaf_d2<G>(qx, qy, qz, l0, l1);
+    else
+    {
+      node_box<F>(tv, l0, l1, true, cur, cs, lb);
+      dl = dist2_pbox(qx, qy, qz, lb);
+    }
+    if (rw < 0)
+      dr = leaf_d2<G>(qx, qy, qz, r0, r1);
+    else
+    {
+      node_box<F>(tv, r0, r1, false, cur, cs, rb);
+      dr = dist2_pbox(qx, qy, qz, rb);
+    }
+    if (dl <= dr)
+    {
+      p = lp;
+      if (lw >= 0)
+        set_frame<F>(lb, cur, cs);
+    }
+    else
+    {
+      p = rp;
+      first += lw < 0 ? 1 : (int)((lskip + SI) / (SI + SL));
+      if (rw >= 0)
+        set_frame<F>(rb, cur, cs);
+    }
+  }
+  int j0 = first - (W - 1) / 2;
+  j0 = j0 < 0 ? 0 : (j0 > n - W ? n - W : j0);
   RegList<K> L;
   L.init(k);
+  uint32_t s = SL * (uint32_t)j0 + SI * (uint32_t)tv.C[j0];
+  int got = 0;
+  while (got < W)
+  {
+    if (s >= end)
+    {
+      raise_error(tv.err, KERR_LAYOUT);
+      return INFINITY;
+    }
+    const uint4 s0 = tv.slots[s];
+    if ((int32_t)s0.w < 0)
+    {
+      ++v_leaf;
+      const uint4 s1 = SL == 2 ? tv.slots[s + 1] : s0;
+      const float dist = sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1));
+      const int32_t idx = ~(int32_t)s0.w;
+      if (L.accepts(dist, idx))
+        L.insert(dist, idx);
+      ++got;
+      s += SL;
+    }
+    else
+    {
+      ++v_int;
+      s += SI;
+    }
+  }
+  return L.full() ? L.kth_d : INFINITY;
+}
+
+template <int F, int G, int K>
+__global__ __launch_bounds__(256) void k_knn_collect(
+    TreeView tv, const float *__restrict__ pts, int64_t t0, int64_t tn, int k, int W,
+    const uint32_t *__restrict__ order, uint2 *__restrict__ cand, int cap,
+    int32_t *__restrict__ counts, uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_n)
+{
+  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= tn)
+    return;
+  const int64_t t = t0 + g;
+  const int64_t q = order ? (int64_t)order[t] : t;
+  const float qx = pts[3 * q], qy = pts[3 * q + 1], qz = pts[3 * q + 2];
   uint32_t v_int = 0, v_leaf = 0;
-  knn_traverse<F, G>(tv, qx, qy, qz, m, L, v_int, v_leaf);
+  const float thr = sqrt_threshold(knn_seed<F, G, K>(tv, qx, qy, qz, k, W, v_int, v_leaf));
+  uint2 *row = cand + g * cap;
+  int c = 0;
+  Box6 par;
+  float ps[3];
+  bool desc = false;
+  uint32_t p = 0;
+  const uint32_t end = tv.total;
+  while (p < end)
+  {
+    const uint4 s0 = tv.slots[p];
+    const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
+    const int32_t w = (int32_t)s0.w;
+    if (w < 0)
+    {
+      ++v_leaf;
+      const float d2 = leaf_d2<G>(qx, qy, qz, s0, s1);
+      if (d2 <= thr)
+      {
+        if (c < cap)
+          row[c] = make_uint2(f_bits(d2), (uint32_t)~w);
+        ++c;
+      }
+      p += SL;
+      desc = false;
+    }
+    else
+    {
+      ++v_int;
+      Box6 b;
+      node_box<F>(tv, s0, s1, desc, par, ps, b);
+      if (dist2_pbox(qx, qy, qz, b) <= thr)
+      {
+        set_frame<F>(b, par, ps);
+        desc = true;
+        p += SI;
+      }
+      else
+      {
+        if (w == 0)
+        {
+          raise_error(tv.err, KERR_LAYOUT);
+          break;
+        }
+        desc = false;
+        p += (uint32_t)w;
+      }
+    }
+  }
+  counts[g] = c;
+  if (c > cap)
+    ovf[atomicAdd(ovf_n, 1u)] = (uint32_t)t;
   if (tv.visits)
   {
     atomicAdd(tv.visits + 0, 1ull);
     atomicAdd(tv.visits + 1, (unsigned long long)v_int);
     atomicAdd(tv.visits + 2, (unsigned long long)v_leaf);
-    atomicAdd(tv.visits + 3, (unsigned long long)m);
+    atomicAdd(tv.visits + 3, (unsigned long long)c);
   }
-  L.write(out_idx + q * m, out_dist ? out_dist + q * m : nullptr, m);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_knn_select(const uint2 *__restrict__ cand, int cap,
+                                                    const int32_t *__restrict__ counts,
+                                                    int64_t t0, int64_t tn, int k, int m,
+                                                    const uint32_t *__restrict__ order,
+                                                    int32_t *__restrict__ oi,
+                                                    float *__restrict__ od)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= tn)
+    return;
+  const int c = counts[g];
+  if (c > cap)
+    return; // handled by the fallback
+  const int64_t t = t0 + g;
+  const int64_t q = order ? (int64_t)order[t] : t;
+  RegList<K> L;
+  L.init(k);
+  const uint2 *row = cand + g * cap;
+  for (int j = 0; j < c; ++j)
+  {
+    const uint2 e = row[j];
+    const float dist = sqrt_rn(bits_f(e.x));
+    const int32_t idx = (int32_t)e.y;
+    if (L.accepts(dist, idx))
+      L.insert(dist, idx);
+  }
+  L.write(oi + q * m, od ? od + q * m : nullptr, m);
+}
+
+template <int F, int G, int K>
+__global__ __launch_bounds__(256) void k_knn_fallback(TreeView tv, const float *__restrict__ pts,
+                                                      int k, const uint32_t *__restrict__ order,
+                                                      const uint32_t *__restrict__ ovf,
+                                                      const uint32_t *__restrict__ ovf_n,
+                                                      int32_t *__restrict__ oi,
+                                                      float *__restrict__ od)
+{
+  __shared__ float s_d2[KNN_BUF * 256];
+  __shared__ int32_t s_id[KNN_BUF * 256];
+  const uint32_t cnt = *ovf_n;
+  const int m = k < tv.n ? k : tv.n;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt;
+       i += gridDim.x * blockDim.x)
+  {
+    const int64_t t = ovf[i];
+    const int64_t q = order ? (int64_t)order[t] : t;
+    const float qx = pts[3 * q], qy = pts[3 * q + 1], qz = pts[3 * q + 2];
+    CandBuf B{s_d2, s_id, 256, (int)threadIdx.x, 0};
+    RegList<K> L;
+    L.init(k);
+    uint32_t v_int = 0, v_leaf = 0;
+    knn_traverse<F, G>(tv, qx, qy, qz, m, L, B, v_int, v_leaf);
+    if (tv.visits)
+    {
+      atomicAdd(tv.visits + 1, (unsigned long long)v_int);
+      atomicAdd(tv.visits + 2, (unsigned long long)v_leaf);
+    }
+    L.write(oi + q * m, od ? od + q * m : nullptr, m);
+  }
 }
 
 template <int F, int G>
@@ -580,6 +882,9 @@ __global__ __launch_bounds__(256) void k_knn_heap(TreeView tv,
                                                   int32_t *__restrict__ out_idx,
                                                   float *__restrict__ out_dist)
 {
+  __shared__ float s_d2[KNN_BUF * 256];
+  __shared__ int32_t s_id[KNN_BUF * 256];
+  CandBuf B{s_d2, s_id, 256, (int)threadIdx.x, 0};
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nq)
     return;
@@ -589,7 +894,7 @@ __global__ __launch_bounds__(256) void k_knn_heap(TreeView tv,
   RowHeap L;
   L.init(m, out_idx + q * m, out_dist + q * m);
   uint32_t v_int = 0, v_leaf = 0;
-  knn_traverse<F, G>(tv, qx, qy, qz, m, L, v_int, v_leaf);
+  knn_traverse<F, G>(tv, qx, qy, qz, m, L, B, v_int, v_leaf);
   if (tv.visits)
   {
     atomicAdd(tv.visits + 0, 1ull);
@@ -729,27 +1034,61 @@ hipError_t launch_spatial(const TreeView &tv, int node_format, int geometry, int
                                        offsets, out, buffer, mode, s);
 }
 
+template <int F, int G, int K>
+static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq, int k,
+                               const uint32_t *order, int32_t *oi, float *od, hipStream_t s)
+{
+  const int n = tv.n;
+  const int m = k < n ? k : n;
+  const int W = std::min(n, std::max(4 * k, 16));
+  const int cap = 8 * k > 64 ? 8 * k : 64;
+  const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t(1) << 31) / (cap * 8)));
+  uint2 *cand = (uint2 *)dalloc((size_t)batch * cap * sizeof(uint2), s);
+  int32_t *counts = (int32_t *)dalloc((size_t)batch * sizeof(int32_t), s);
+  uint32_t *ovf = (uint32_t *)dalloc((size_t)batch * sizeof(uint32_t), s);
+  uint32_t *ovf_n = (uint32_t *)dalloc(sizeof(uint32_t), s);
+  hipError_t e = (cand && counts && ovf && ovf_n) ? hipSuccess : hipErrorOutOfMemory;
+  for (int64_t t0 = 0; e == hipSuccess && t0 < nq; t0 += batch)
+  {
+    const int64_t tn = std::min(batch, nq - t0);
+    e = fill_u32(ovf_n, 0u, 1, s);
+    if (e != hipSuccess)
+      break;
+    AXB_LAUNCH((k_knn_collect<F, G, K>), dim3(grid_of(tn)), dim3(256), 0, s, tv, pts, t0, tn, k,
+               W, order, cand, cap, counts, ovf, ovf_n);
+    AXB_LAUNCH((k_knn_select<K>), dim3(grid_of(tn)), dim3(256), 0, s, cand, cap, counts, t0, tn,
+               k, m, order, oi, od);
+    AXB_LAUNCH((k_knn_fallback<F, G, K>), dim3((unsigned)std::min<int64_t>(grid_of(tn), 512)),
+               dim3(256), 0, s, tv, pts, k, order, ovf, ovf_n, oi, od);
+    e = hipGetLastError();
+  }
+  dfree(cand, s);
+  dfree(counts, s);
+  dfree(ovf, s);
+  dfree(ovf_n, s);
+  return e;
+}
+
 template <int F, int G>
 static hipError_t launch_knn_fg(const TreeView &tv, const float *pts, int64_t nq, int k,
                                 const uint32_t *order, int32_t *oi, float *od, hipStream_t s)
 {
-  const dim3 g(grid_of(nq)), b(256);
   if (k <= 1)
-    AXB_LAUNCH((k_knn<F, G, 1>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
-  else if (k <= 2)
-    AXB_LAUNCH((k_knn<F, G, 2>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
-  else if (k <= 4)
-    AXB_LAUNCH((k_knn<F, G, 4>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
-  else if (k <= 8)
-    AXB_LAUNCH((k_knn<F, G, 8>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
-  else if (k <= 16)
-    AXB_LAUNCH((k_knn<F, G, 16>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
-  else if (k <= 32)
-    AXB_LAUNCH((k_knn<F, G, 32>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
-  else if (k <= 64)
-    AXB_LAUNCH((k_knn<F, G, 64>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
-  else
-    AXB_LAUNCH((k_knn_heap<F, G>), g, b, 0, s, tv, pts, nq, k, order, oi, od);
+    return knn_pipeline<F, G, 1>(tv, pts, nq, k, order, oi, od, s);
+  if (k <= 2)
+    return knn_pipeline<F, G, 2>(tv, pts, nq, k, order, oi, od, s);
+  if (k <= 4)
+    return knn_pipeline<F, G, 4>(tv, pts, nq, k, order, oi, od, s);
+  if (k <= 8)
+    return knn_pipeline<F, G, 8>(tv, pts, nq, k, order, oi, od, s);
+  if (k <= 16)
+    return knn_pipeline<F, G, 16>(tv, pts, nq, k, order, oi, od, s);
+  if (k <= 32)
+    return knn_pipeline<F, G, 32>(tv, pts, nq, k, order, oi, od, s);
+  if (k <= 64)
+    return knn_pipeline<F, G, 64>(tv, pts, nq, k, order, oi, od, s);
+  AXB_LAUNCH((k_knn_heap<F, G>), dim3(grid_of(nq)), dim3(256), 0, s, tv, pts, nq, k, order, oi,
+             od);
   return hipGetLastError();
 }
 

@@ -569,17 +569,18 @@ void orc_export(void *h, uint64_t *keys, uint32_t *perm, int32_t *leaf_rope,
 //
 // Q16 encoding (no reference counterpart): a node's box is quantised on a
 // 65535-step grid of a frame box F with s = (F.hi - F.lo) * (1/65535):
-//   lo(q) = F.lo + q*s,   hi(q) = F.hi - (65535-q)*s      (float, no FMA)
+//   lo(q) = fma(q, s, F.lo),   hi(q) = fma(-(65535-q), s, F.hi)
+// (correctly rounded fused multiply-add, as on the GPU)
 // qmin = largest q with lo(q) <= box.lo, qmax = smallest q with
 // hi(q) >= box.hi (outward rounding => decoded box contains the exact one).
 // Frame = decoded box of the parent for a left child (reached by descent),
 // the scene box for the root and every right child (reached by a rope).
 static const float Q16_INV = 1.0f / 65535.0f;
 
-static inline float q16_lo(float flo, float s, int q) { return flo + (float)q * s; }
+static inline float q16_lo(float flo, float s, int q) { return std::fma((float)q, s, flo); }
 static inline float q16_hi(float fhi, float s, int q)
 {
-  return fhi - (float)(65535 - q) * s;
+  return std::fma(-(float)(65535 - q), s, fhi);
 }
 static inline void q16_encode_axis(float flo, float fhi, float vlo, float vhi,
                                    uint16_t &qlo, uint16_t &qhi)

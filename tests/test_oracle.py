@@ -206,10 +206,12 @@ def test_q16_encoder_is_tight_and_conservative():
         assert np.all(dec[:3] <= box[:3]) and np.all(dec[3:] >= box[3:])
         for d in range(3):
             s = np.float32((frame[3 + d] - frame[d]) * np.float32(1.0 / 65535.0))
+            fma = lambda a, b, c: np.float32(float(a) * float(b) + float(c))  # noqa: E731
+            # (the float64 product of two floats is exact; one rounding to float)
             if s > 0 and q[d] < 65535:  # one step up would cut the box
-                assert np.float32(frame[d] + np.float32(int(q[d]) + 1) * s) > box[d]
+                assert fma(np.float32(int(q[d]) + 1), s, frame[d]) > box[d]
             if s > 0 and q[3 + d] > 0:
-                assert np.float32(frame[3 + d] - np.float32(65535 - (int(q[3 + d]) - 1)) * s) < box[3 + d]
+                assert fma(-np.float32(65535 - (int(q[3 + d]) - 1)), s, frame[3 + d]) < box[3 + d]
 
 
 # ------------------------------------------------------------ queries vs brute force
