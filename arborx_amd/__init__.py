@@ -427,9 +427,11 @@ class profile:
             _check(L.axb_profile_record(i, buf, 256, ctypes.byref(ms)))
             self.kernels.setdefault(buf.value.decode(), []).append(ms.value)
         if self.flags & 2:
-            v = (_I64 * 4)()
+            v = (_I64 * 8)()
             _check(L.axb_profile_visits(v))
-            self.visits = dict(queries=v[0], internal=v[1], leaf=v[2], results=v[3])
+            self.visits = dict(queries=v[0], internal=v[1], leaf=v[2], results=v[3],
+                               fallback_queries=v[4], fallback_internal=v[5],
+                               fallback_leaf=v[6])
         _check(L.axb_profile_enable(0))
         return False
 

@@ -582,13 +582,13 @@ axb_status axb_profile_enable(int32_t flags)
   axb_profile_clear();
   if ((flags & 2) && !g_visits)
   {
-    hipError_t e = hipMalloc((void **)&g_visits, 4 * sizeof(unsigned long long));
+    hipError_t e = hipMalloc((void **)&g_visits, 8 * sizeof(unsigned long long));
     if (e != hipSuccess)
       return hip_fail(e, "axb_profile_enable");
   }
   if (g_visits)
   {
-    hipError_t e = hipMemset(g_visits, 0, 4 * sizeof(unsigned long long));
+    hipError_t e = hipMemset(g_visits, 0, 8 * sizeof(unsigned long long));
     if (e == hipSuccess)
       e = hipDeviceSynchronize();
     if (e != hipSuccess)
@@ -639,21 +639,21 @@ axb_status axb_profile_record(int32_t i, char *name, int32_t name_len, double *m
   return AXB_OK;
 }
 
-axb_status axb_profile_visits(int64_t out_host[4])
+axb_status axb_profile_visits(int64_t out_host[8])
 {
   if (!out_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_profile_visits: NULL");
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 8; ++i)
     out_host[i] = 0;
   if (!g_visits)
     return AXB_OK;
-  unsigned long long h[4];
+  unsigned long long h[8];
   hipError_t e = hipDeviceSynchronize();
   if (e == hipSuccess)
     e = hipMemcpy(h, g_visits, sizeof(h), hipMemcpyDeviceToHost);
   if (e != hipSuccess)
     return hip_fail(e, "axb_profile_visits");
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 8; ++i)
     out_host[i] = (int64_t)h[i];
   return AXB_OK;
 }

@@ -487,7 +487,7 @@ AXB_D void knn_walk(const TreeView &tv, float qx, float qy, float qz, List &L, f
 }
 
 template <int F, int G, class List>
-AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
+AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m, int W,
                         List &L, CandBuf &B, uint32_t &v_int, uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
@@ -573,13 +573,13 @@ AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
       }
     }
   }
-  // ---- seed: the m leaves of ranks [j0, j0 + m) around `first`
-  int j0 = first - (m - 1) / 2;
-  j0 = j0 < 0 ? 0 : (j0 > n - m ? n - m : j0);
+  // ---- seed: the W >= m leaves of ranks [j0, j0 + W) around `first`
+  int j0 = first - (W - 1) / 2;
+  j0 = j0 < 0 ? 0 : (j0 > n - W ? n - W : j0);
   {
     uint32_t s = SL * (uint32_t)j0 + SI * (uint32_t)tv.C[j0];
     int got = 0;
-    while (got < m)
+    while (got < W)
     {
       if (s >= end)
       {
@@ -605,7 +605,7 @@ AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
       }
     }
   }
-  if (m >= n)
+  if (W >= n)
     return; // every leaf is already in the list
   float thr = L.full() ? sqrt_threshold(L.kth_d) : INFINITY;
   float ps[3];
@@ -618,7 +618,7 @@ AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
     const bool first_seg = seg == 0;
     knn_walk<F, G>(tv, qx, qy, qz, L, thr, B, first_seg ? pX + SI : 0u,
                    first_seg ? pX + sX : end, first_seg ? fX : 0, first_seg, DX, ps,
-                   (local && !first_seg) ? pX : UINT32_MAX, sX, lenX, j0, m, v_int, v_leaf);
+                   (local && !first_seg) ? pX : UINT32_MAX, sX, lenX, j0, W, v_int, v_leaf);
   }
 }
 
@@ -634,9 +634,18 @@ AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
 //           (knn_traverse) instead.
 // The walk of `collect` carries no list, so it runs at the register budget
 // of the spatial traversal.
+// Local subtree of a query: the deepest node on its greedy descent path that
+// still holds at least W leaves (slot, slots, decoded box).
+struct LocalSub
+{
+  uint32_t p, slots;
+  Box6 box;
+  bool valid;
+};
+
 template <int F, int G, int K>
 AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, int W,
-                     uint32_t &v_int, uint32_t &v_leaf)
+                     LocalSub &X, uint32_t &v_int, uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
   const uint32_t end = tv.total;
@@ -645,6 +654,9 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
   uint32_t p = 0;
   Box6 cur;
   float cs[3];
+  X.valid = false;
+  X.p = 0;
+  X.slots = 0;
   {
     const uint4 s0 = tv.slots[0], s1 = tv.slots[1];
     if ((int32_t)s0.w >= 0)
@@ -652,6 +664,7 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
       ++v_int;
       node_box<F>(tv, s0, s1, false, cur, cs, cur);
       set_frame<F>(cur, cur, cs);
+      X.box = cur;
     }
   }
   while (true)
@@ -693,14 +706,22 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
     {
       p = lp;
       if (lw >= 0)
+      {
         set_frame<F>(lb, cur, cs);
+        if ((int)((lskip + SI) / (SI + SL)) >= W)
+          X = LocalSub{lp, lskip, lb, true};
+      }
     }
     else
     {
       p = rp;
       first += lw < 0 ? 1 : (int)((lskip + SI) / (SI + SL));
       if (rw >= 0)
+      {
         set_frame<F>(rb, cur, cs);
+        if ((int)(((uint32_t)rw + SI) / (SI + SL)) >= W)
+          X = LocalSub{rp, (uint32_t)rw, rb, true};
+      }
     }
   }
   int j0 = first - (W - 1) / 2;
@@ -737,71 +758,131 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
   return L.full() ? L.kth_d : INFINITY;
 }
 
+// Fixed-radius rope walk appending every leaf with d2 <= thr to `row`:
+// the local subtree X first, then the rest of the tree with X jumped over,
+// so that the first entries of the row are the query's near leaves.  Stops
+// as soon as more than `cap` leaves qualify; returns the count (cap + 1 on
+// overflow).
+template <int F, int G>
+AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, float thr,
+                           const LocalSub &X, uint2 *row, int cap, uint32_t &v_int,
+                           uint32_t &v_leaf)
+{
+  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  int c = 0;
+  for (int seg = X.valid ? 0 : 1; seg < 2 && c <= cap; ++seg)
+  {
+    const bool local = seg == 0;
+    uint32_t p = local ? X.p + SI : 0u;
+    const uint32_t end = local ? X.p + X.slots : tv.total;
+    const uint32_t skip_at = (!local && X.valid) ? X.p : UINT32_MAX;
+    Box6 par = X.box;
+    float ps[3];
+    set_frame<F>(X.box, par, ps);
+    bool desc = local;
+    while (p < end)
+    {
+      if (p == skip_at)
+      {
+        p += X.slots;
+        desc = false;
+        continue;
+      }
+      const uint4 s0 = tv.slots[p];
+      const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
+      const int32_t w = (int32_t)s0.w;
+      if (w < 0)
+      {
+        ++v_leaf;
+        const float d2 = leaf_d2<G>(qx, qy, qz, s0, s1);
+        if (d2 <= thr)
+        {
+          if (c < cap)
+            row[c] = make_uint2(f_bits(d2), (uint32_t)~w);
+          if (++c > cap)
+            break;
+        }
+        p += SL;
+        desc = false;
+      }
+      else
+      {
+        ++v_int;
+        Box6 b;
+        node_box<F>(tv, s0, s1, desc, par, ps, b);
+        if (dist2_pbox(qx, qy, qz, b) <= thr)
+        {
+          set_frame<F>(b, par, ps);
+          desc = true;
+          p += SI;
+        }
+        else
+        {
+          if (w == 0)
+          {
+            raise_error(tv.err, KERR_LAYOUT);
+            return c;
+          }
+          desc = false;
+          p += (uint32_t)w;
+        }
+      }
+    }
+  }
+  return c;
+}
+
+// One lane per query: seed radius, then up to KNN_ROUNDS rounds of
+// collect + exact select.  When a round overflows its row, the k-th
+// distance of the candidates it did keep (k real points) is the next,
+// smaller radius.  Queries still overflowing go to the fallback list.
+constexpr int KNN_ROUNDS = 3;
+
 template <int F, int G, int K>
 __global__ __launch_bounds__(256) void k_knn_collect(
     TreeView tv, const float *__restrict__ pts, int64_t t0, int64_t tn, int k, int W,
     const uint32_t *__restrict__ order, uint2 *__restrict__ cand, int cap,
-    int32_t *__restrict__ counts, uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_n)
+    uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_n, int32_t *__restrict__ oi,
+    float *__restrict__ od)
 {
-  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= tn)
     return;
   const int64_t t = t0 + g;
   const int64_t q = order ? (int64_t)order[t] : t;
   const float qx = pts[3 * q], qy = pts[3 * q + 1], qz = pts[3 * q + 2];
+  const int m = k < tv.n ? k : tv.n;
   uint32_t v_int = 0, v_leaf = 0;
-  const float thr = sqrt_threshold(knn_seed<F, G, K>(tv, qx, qy, qz, k, W, v_int, v_leaf));
+  LocalSub X;
+  float R = knn_seed<F, G, K>(tv, qx, qy, qz, k, W, X, v_int, v_leaf);
   uint2 *row = cand + g * cap;
   int c = 0;
-  Box6 par;
-  float ps[3];
-  bool desc = false;
-  uint32_t p = 0;
-  const uint32_t end = tv.total;
-  while (p < end)
+  bool done = false;
+  for (int round = 0; round < KNN_ROUNDS && !done; ++round)
   {
-    const uint4 s0 = tv.slots[p];
-    const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
-    const int32_t w = (int32_t)s0.w;
-    if (w < 0)
+    c = knn_collect_walk<F, G>(tv, qx, qy, qz, sqrt_threshold(R), X, row, cap, v_int, v_leaf);
+    RegList<K> L;
+    L.init(k);
+    const int kept = c < cap ? c : cap;
+    for (int j = 0; j < kept; ++j)
     {
-      ++v_leaf;
-      const float d2 = leaf_d2<G>(qx, qy, qz, s0, s1);
-      if (d2 <= thr)
-      {
-        if (c < cap)
-          row[c] = make_uint2(f_bits(d2), (uint32_t)~w);
-        ++c;
-      }
-      p += SL;
-      desc = false;
+      const uint2 e = row[j];
+      const float dist = sqrt_rn(bits_f(e.x));
+      const int32_t idx = (int32_t)e.y;
+      if (L.accepts(dist, idx))
+        L.insert(dist, idx);
+    }
+    if (c <= cap)
+    {
+      L.write(oi + q * m, od ? od + q * m : nullptr, m);
+      done = true;
     }
     else
     {
-      ++v_int;
-      Box6 b;
-      node_box<F>(tv, s0, s1, desc, par, ps, b);
-      if (dist2_pbox(qx, qy, qz, b) <= thr)
-      {
-        set_frame<F>(b, par, ps);
-        desc = true;
-        p += SI;
-      }
-      else
-      {
-        if (w == 0)
-        {
-          raise_error(tv.err, KERR_LAYOUT);
-          break;
-        }
-        desc = false;
-        p += (uint32_t)w;
-      }
+      R = L.kth_d; // kept >= cap >= k entries, so the list is full
     }
   }
-  counts[g] = c;
-  if (c > cap)
+  if (!done)
     ovf[atomicAdd(ovf_n, 1u)] = (uint32_t)t;
   if (tv.visits)
   {
@@ -810,36 +891,6 @@ __global__ __launch_bounds__(256) void k_knn_collect(
     atomicAdd(tv.visits + 2, (unsigned long long)v_leaf);
     atomicAdd(tv.visits + 3, (unsigned long long)c);
   }
-}
-
-template <int K>
-__global__ __launch_bounds__(256) void k_knn_select(const uint2 *__restrict__ cand, int cap,
-                                                    const int32_t *__restrict__ counts,
-                                                    int64_t t0, int64_t tn, int k, int m,
-                                                    const uint32_t *__restrict__ order,
-                                                    int32_t *__restrict__ oi,
-                                                    float *__restrict__ od)
-{
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= tn)
-    return;
-  const int c = counts[g];
-  if (c > cap)
-    return; // handled by the fallback
-  const int64_t t = t0 + g;
-  const int64_t q = order ? (int64_t)order[t] : t;
-  RegList<K> L;
-  L.init(k);
-  const uint2 *row = cand + g * cap;
-  for (int j = 0; j < c; ++j)
-  {
-    const uint2 e = row[j];
-    const float dist = sqrt_rn(bits_f(e.x));
-    const int32_t idx = (int32_t)e.y;
-    if (L.accepts(dist, idx))
-      L.insert(dist, idx);
-  }
-  L.write(oi + q * m, od ? od + q * m : nullptr, m);
 }
 
 template <int F, int G, int K>
@@ -864,11 +915,14 @@ __global__ __launch_bounds__(256) void k_knn_fallback(TreeView tv, const float *
     RegList<K> L;
     L.init(k);
     uint32_t v_int = 0, v_leaf = 0;
-    knn_traverse<F, G>(tv, qx, qy, qz, m, L, B, v_int, v_leaf);
+    const int W = min(tv.n, max(4 * k, 16));
+    knn_traverse<F, G>(tv, qx, qy, qz, m, W, L, B, v_int, v_leaf);
     if (tv.visits)
     {
-      atomicAdd(tv.visits + 1, (unsigned long long)v_int);
-      atomicAdd(tv.visits + 2, (unsigned long long)v_leaf);
+      atomicAdd(tv.visits + 4, 1ull);
+      atomicAdd(tv.visits + 5, (unsigned long long)v_int);
+      atomicAdd(tv.visits + 6, (unsigned long long)v_leaf);
+      atomicAdd(tv.visits + 7, (unsigned long long)m);
     }
     L.write(oi + q * m, od ? od + q * m : nullptr, m);
   }
@@ -894,7 +948,7 @@ __global__ __launch_bounds__(256) void k_knn_heap(TreeView tv,
   RowHeap L;
   L.init(m, out_idx + q * m, out_dist + q * m);
   uint32_t v_int = 0, v_leaf = 0;
-  knn_traverse<F, G>(tv, qx, qy, qz, m, L, B, v_int, v_leaf);
+  knn_traverse<F, G>(tv, qx, qy, qz, m, min(tv.n, 2 * m), L, B, v_int, v_leaf);
   if (tv.visits)
   {
     atomicAdd(tv.visits + 0, 1ull);
@@ -1039,15 +1093,13 @@ static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq,
                                const uint32_t *order, int32_t *oi, float *od, hipStream_t s)
 {
   const int n = tv.n;
-  const int m = k < n ? k : n;
   const int W = std::min(n, std::max(4 * k, 16));
   const int cap = 8 * k > 64 ? 8 * k : 64;
   const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t(1) << 31) / (cap * 8)));
   uint2 *cand = (uint2 *)dalloc((size_t)batch * cap * sizeof(uint2), s);
-  int32_t *counts = (int32_t *)dalloc((size_t)batch * sizeof(int32_t), s);
   uint32_t *ovf = (uint32_t *)dalloc((size_t)batch * sizeof(uint32_t), s);
   uint32_t *ovf_n = (uint32_t *)dalloc(sizeof(uint32_t), s);
-  hipError_t e = (cand && counts && ovf && ovf_n) ? hipSuccess : hipErrorOutOfMemory;
+  hipError_t e = (cand && ovf && ovf_n) ? hipSuccess : hipErrorOutOfMemory;
   for (int64_t t0 = 0; e == hipSuccess && t0 < nq; t0 += batch)
   {
     const int64_t tn = std::min(batch, nq - t0);
@@ -1055,15 +1107,12 @@ static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq,
     if (e != hipSuccess)
       break;
     AXB_LAUNCH((k_knn_collect<F, G, K>), dim3(grid_of(tn)), dim3(256), 0, s, tv, pts, t0, tn, k,
-               W, order, cand, cap, counts, ovf, ovf_n);
-    AXB_LAUNCH((k_knn_select<K>), dim3(grid_of(tn)), dim3(256), 0, s, cand, cap, counts, t0, tn,
-               k, m, order, oi, od);
+               W, order, cand, cap, ovf, ovf_n, oi, od);
     AXB_LAUNCH((k_knn_fallback<F, G, K>), dim3((unsigned)std::min<int64_t>(grid_of(tn), 512)),
                dim3(256), 0, s, tv, pts, k, order, ovf, ovf_n, oi, od);
     e = hipGetLastError();
   }
   dfree(cand, s);
-  dfree(counts, s);
   dfree(ovf, s);
   dfree(ovf_n, s);
   return e;

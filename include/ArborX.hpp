@@ -1,0 +1,432 @@
+// ArborX.hpp -- header-only C++17 facade of the MI355X BVH library.
+//
+// Mirrors the host interface of ArborX's BVH hot path (src/ArborX.hpp,
+// src/spatial/ArborX_LinearBVH.hpp:50-142, ArborX_CrsGraphWrapper.hpp:22-34,
+// detail/ArborX_Predicates.hpp, detail/ArborX_PredicateHelpers.hpp,
+// detail/ArborX_TraversalPolicy.hpp, misc/ArborX_Exception.hpp) on top of the
+// C ABI in arborx_mi355x.h, without Kokkos.  Kokkos Views become
+// ArborX::DeviceVector (device memory owned by the facade); the execution
+// space becomes ArborX::HipStream.  Values are points or boxes in device
+// memory; a leaf's value is its index (the reference's benchmark form
+// BVH<MemorySpace, int> over Iota, benchmarks/bvh_driver/
+// benchmark_registration.hpp:151-161).
+//
+//   ArborX::HipStream space;
+//   ArborX::DeviceVector<ArborX::Point> points = ...;
+//   ArborX::BoundingVolumeHierarchy bvh(space, points);
+//   ArborX::DeviceVector<int> indices;  ArborX::DeviceVector<int64_t> offsets;
+//   bvh.query(space, ArborX::Experimental::make_intersects(points, r), indices, offsets);
+//   ArborX::query(bvh, space, ArborX::Experimental::make_nearest(points, k), indices, offsets);
+#pragma once
+
+#include "arborx_mi355x.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace ArborX
+{
+
+// misc/ArborX_Exception.hpp:19-37
+class SearchException : public std::logic_error
+{
+public:
+  using std::logic_error::logic_error;
+};
+
+namespace Details
+{
+inline void check(axb_status s)
+{
+  if (s == AXB_OK)
+    return;
+  const std::string msg = axb_last_error();
+  if (s == AXB_ERR_BUFFER_OVERFLOW)
+    throw SearchException(msg);
+  throw std::runtime_error("arborx_mi355x: " + msg);
+}
+inline void hip_check(hipError_t e, const char *what)
+{
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+} // namespace Details
+
+// Execution space: one HIP stream (the default stream unless given one).
+struct HipStream
+{
+  hipStream_t stream = nullptr;
+  void fence() const { Details::hip_check(hipStreamSynchronize(stream), "fence"); }
+};
+
+// geometry/ArborX_Point.hpp, ArborX_Box.hpp (Box.hpp:34-44 empty box),
+// ArborX_Sphere.hpp -- 3-D float, same bytes as the C ABI arrays.
+struct Point
+{
+  float _coords[3];
+  float &operator[](int d) { return _coords[d]; }
+  float const &operator[](int d) const { return _coords[d]; }
+};
+struct Box
+{
+  Point _min_corner{{FLT_MAX, FLT_MAX, FLT_MAX}};
+  Point _max_corner{{-FLT_MAX, -FLT_MAX, -FLT_MAX}};
+  Box() = default;
+  Box(Point const &lo, Point const &hi) : _min_corner(lo), _max_corner(hi) {}
+  Point &minCorner() { return _min_corner; }
+  Point const &minCorner() const { return _min_corner; }
+  Point &maxCorner() { return _max_corner; }
+  Point const &maxCorner() const { return _max_corner; }
+};
+struct Sphere
+{
+  Point _centroid;
+  float _radius;
+  Point const &centroid() const { return _centroid; }
+  float radius() const { return _radius; }
+};
+static_assert(sizeof(Point) == 12 && sizeof(Box) == 24 && sizeof(Sphere) == 16,
+              "facade geometry must match the C ABI layout");
+
+// Owning device array (stands in for Kokkos::View<T*, HIP memory space>).
+template <class T>
+class DeviceVector
+{
+public:
+  DeviceVector() = default;
+  explicit DeviceVector(size_t n) { resize(n); }
+  explicit DeviceVector(std::vector<T> const &host) { assign(host); }
+  DeviceVector(DeviceVector const &) = delete;
+  DeviceVector &operator=(DeviceVector const &) = delete;
+  DeviceVector(DeviceVector &&o) noexcept { swap(o); }
+  DeviceVector &operator=(DeviceVector &&o) noexcept
+  {
+    swap(o);
+    return *this;
+  }
+  ~DeviceVector() { release(); }
+
+  void resize(size_t n)
+  {
+    release();
+    if (n)
+      Details::hip_check(hipMalloc((void **)&_p, n * sizeof(T)), "hipMalloc");
+    _n = n;
+  }
+  void assign(std::vector<T> const &host)
+  {
+    resize(host.size());
+    if (_n)
+      Details::hip_check(hipMemcpy(_p, host.data(), _n * sizeof(T), hipMemcpyHostToDevice),
+                         "hipMemcpy");
+  }
+  std::vector<T> to_host() const
+  {
+    std::vector<T> h(_n);
+    if (_n)
+      Details::hip_check(hipMemcpy(h.data(), _p, _n * sizeof(T), hipMemcpyDeviceToHost),
+                         "hipMemcpy");
+    return h;
+  }
+  // Takes ownership of memory the C ABI allocated (hipMallocAsync).
+  void adopt(T *p, size_t n, hipStream_t s)
+  {
+    release();
+    _p = p;
+    _n = n;
+    _async = true;
+    _stream = s;
+  }
+  T *data() const { return _p; }
+  size_t size() const { return _n; }
+  size_t extent(int) const { return _n; }
+
+private:
+  void release()
+  {
+    if (_p)
+    {
+      if (_async)
+        (void)axb_free(_p, (axb_stream_t)_stream);
+      else
+        (void)hipFree(_p);
+    }
+    _p = nullptr;
+    _n = 0;
+    _async = false;
+  }
+  void swap(DeviceVector &o) noexcept
+  {
+    std::swap(_p, o._p);
+    std::swap(_n, o._n);
+    std::swap(_async, o._async);
+    std::swap(_stream, o._stream);
+  }
+  T *_p = nullptr;
+  size_t _n = 0;
+  bool _async = false;
+  hipStream_t _stream = nullptr;
+};
+
+namespace Experimental
+{
+// detail/ArborX_TraversalPolicy.hpp:19-47
+struct TraversalPolicy
+{
+  int _buffer_size = 0;
+  bool _sort_predicates = true;
+  TraversalPolicy &setBufferSize(int buffer_size)
+  {
+    _buffer_size = buffer_size;
+    return *this;
+  }
+  TraversalPolicy &setPredicateSorting(bool sort_predicates)
+  {
+    _sort_predicates = sort_predicates;
+    return *this;
+  }
+};
+
+// Predicate batches (the results of make_intersects / make_nearest,
+// detail/ArborX_PredicateHelpers.hpp:55-206).
+struct PointsWithRadius
+{
+  Point const *points;
+  size_t n;
+  float r;
+};
+struct PointsNearestK
+{
+  Point const *points;
+  size_t n;
+  int k;
+};
+struct IntersectsSpheres
+{
+  Sphere const *spheres;
+  size_t n;
+};
+struct IntersectsBoxes
+{
+  Box const *boxes;
+  size_t n;
+};
+
+// make_intersects(points, r): spheres of radius r (stored as float),
+// PredicateHelpers.hpp:99-105 (r > 0 asserted at :102).
+inline PointsWithRadius make_intersects(DeviceVector<Point> const &points, float r)
+{
+  if (!(r > 0))
+    throw std::invalid_argument("make_intersects: r must be > 0");
+  return {points.data(), points.size(), r};
+}
+inline PointsNearestK make_nearest(DeviceVector<Point> const &points, int k)
+{
+  return {points.data(), points.size(), k};
+}
+} // namespace Experimental
+
+// intersects(geometry) / nearest(geometry, k) over a batch
+// (detail/ArborX_Predicates.hpp:129-140).
+inline Experimental::IntersectsSpheres intersects(DeviceVector<Sphere> const &s)
+{
+  return {s.data(), s.size()};
+}
+inline Experimental::IntersectsBoxes intersects(DeviceVector<Box> const &b)
+{
+  return {b.data(), b.size()};
+}
+inline Experimental::PointsNearestK nearest(DeviceVector<Point> const &p, int k = 1)
+{
+  return {p.data(), p.size(), k};
+}
+
+enum class NodeFormat
+{
+  FP32 = AXB_NODES_FP32,
+  Q16 = AXB_NODES_Q16
+};
+
+// spatial/ArborX_LinearBVH.hpp:50-142.
+class BoundingVolumeHierarchy
+{
+public:
+  using size_type = int64_t;
+  using bounding_volume_type = Box;
+
+  BoundingVolumeHierarchy() = default; // empty tree
+
+  BoundingVolumeHierarchy(HipStream const &space, DeviceVector<Point> const &values,
+                          NodeFormat fmt = NodeFormat::FP32)
+  {
+    build(space, (float const *)values.data(), (int64_t)values.size(), AXB_GEOMETRY_POINTS,
+          fmt);
+  }
+  BoundingVolumeHierarchy(HipStream const &space, DeviceVector<Box> const &values,
+                          NodeFormat fmt = NodeFormat::FP32)
+  {
+    build(space, (float const *)values.data(), (int64_t)values.size(), AXB_GEOMETRY_BOXES,
+          fmt);
+  }
+  BoundingVolumeHierarchy(BoundingVolumeHierarchy const &) = delete;
+  BoundingVolumeHierarchy &operator=(BoundingVolumeHierarchy const &) = delete;
+  BoundingVolumeHierarchy(BoundingVolumeHierarchy &&o) noexcept { std::swap(_h, o._h); }
+  BoundingVolumeHierarchy &operator=(BoundingVolumeHierarchy &&o) noexcept
+  {
+    std::swap(_h, o._h);
+    return *this;
+  }
+  ~BoundingVolumeHierarchy()
+  {
+    if (_h)
+      axb_bvh_destroy(_h);
+  }
+
+  size_type size() const noexcept { return _h ? axb_bvh_size(_h) : 0; }
+  bool empty() const noexcept { return size() == 0; }
+  Box bounds() const
+  {
+    Box b;
+    if (_h)
+    {
+      float f[6];
+      Details::check(axb_bvh_bounds(_h, f));
+      b = Box({{f[0], f[1], f[2]}}, {{f[3], f[4], f[5]}});
+    }
+    return b;
+  }
+
+  // query(space, predicates, indices, offsets[, policy])
+  // (ArborX_LinearBVH.hpp:90-110): indices/offsets are (re)allocated.
+  void query(HipStream const &space, Experimental::PointsWithRadius const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    prepare(offsets, preds.n);
+    int32_t *ind = nullptr;
+    int64_t nnz = 0;
+    if (_h)
+      Details::check(axb_bvh_query_intersects_radius(
+          _h, (float const *)preds.points, (int64_t)preds.n, preds.r, c_policy(policy),
+          offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
+    finish(space, indices, ind, nnz, offsets, preds.n);
+  }
+  void query(HipStream const &space, Experimental::IntersectsSpheres const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    prepare(offsets, preds.n);
+    int32_t *ind = nullptr;
+    int64_t nnz = 0;
+    if (_h)
+      Details::check(axb_bvh_query_intersects(
+          _h, AXB_PREDICATE_SPHERES, (float const *)preds.spheres, (int64_t)preds.n,
+          c_policy(policy), offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
+    finish(space, indices, ind, nnz, offsets, preds.n);
+  }
+  void query(HipStream const &space, Experimental::IntersectsBoxes const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    prepare(offsets, preds.n);
+    int32_t *ind = nullptr;
+    int64_t nnz = 0;
+    if (_h)
+      Details::check(axb_bvh_query_intersects(
+          _h, AXB_PREDICATE_BOXES, (float const *)preds.boxes, (int64_t)preds.n,
+          c_policy(policy), offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
+    finish(space, indices, ind, nnz, offsets, preds.n);
+  }
+  void query(HipStream const &space, Experimental::PointsNearestK const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    DeviceVector<float> *no_dist = nullptr;
+    nearest_impl(space, preds, indices, offsets, no_dist, policy);
+  }
+  // Nearest with the float distances of every result (same row layout).
+  void query(HipStream const &space, Experimental::PointsNearestK const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             DeviceVector<float> &distances,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    nearest_impl(space, preds, indices, offsets, &distances, policy);
+  }
+
+  axb_bvh const *c_handle() const { return _h; }
+
+private:
+  static axb_traversal_policy c_policy(Experimental::TraversalPolicy const &p)
+  {
+    return axb_traversal_policy{p._buffer_size, p._sort_predicates ? 1 : 0};
+  }
+  void build(HipStream const &space, float const *v, int64_t n, int32_t geometry,
+             NodeFormat fmt)
+  {
+    Details::check(axb_bvh_create(v, n, geometry, (int32_t)fmt, 0,
+                                  (axb_stream_t)space.stream, &_h));
+  }
+  static void prepare(DeviceVector<int64_t> &offsets, size_t nq)
+  {
+    if (offsets.size() != nq + 1)
+      offsets.resize(nq + 1);
+  }
+  void finish(HipStream const &space, DeviceVector<int> &indices, int32_t *ind, int64_t nnz,
+              DeviceVector<int64_t> &offsets, size_t nq) const
+  {
+    if (!_h)
+    {
+      // empty tree: every row is empty
+      std::vector<int64_t> z(nq + 1, 0);
+      offsets.assign(z);
+      indices.resize(0);
+      return;
+    }
+    indices.adopt(ind, (size_t)nnz, space.stream);
+  }
+  void nearest_impl(HipStream const &space, Experimental::PointsNearestK const &preds,
+                    DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+                    DeviceVector<float> *distances,
+                    Experimental::TraversalPolicy const &policy) const
+  {
+    prepare(offsets, preds.n);
+    if (!_h)
+    {
+      finish(space, indices, nullptr, 0, offsets, preds.n);
+      if (distances)
+        distances->resize(0);
+      return;
+    }
+    int32_t *ind = nullptr;
+    float *dist = nullptr;
+    int64_t nnz = 0;
+    Details::check(axb_bvh_query_nearest(_h, (float const *)preds.points, (int64_t)preds.n,
+                                         preds.k, c_policy(policy), offsets.data(), &ind,
+                                         distances ? &dist : nullptr, &nnz,
+                                         (axb_stream_t)space.stream));
+    indices.adopt(ind, (size_t)nnz, space.stream);
+    if (distances)
+      distances->adopt(dist, (size_t)nnz, space.stream);
+  }
+
+  axb_bvh *_h = nullptr;
+};
+
+using BVH = BoundingVolumeHierarchy;
+
+// Free function ArborX::query(tree, space, predicates, indices, offsets
+// [, policy]) (src/spatial/ArborX_CrsGraphWrapper.hpp:22-34).
+template <class Predicates, class... Args>
+void query(BoundingVolumeHierarchy const &tree, HipStream const &space,
+           Predicates const &predicates, Args &&...args)
+{
+  tree.query(space, predicates, std::forward<Args>(args)...);
+}
+
+} // namespace ArborX

@@ -204,8 +204,9 @@ int32_t axb_profile_num_records(void);
 axb_status axb_profile_record(int32_t i, char *name, int32_t name_len,
                               double *ms);
 /* {queries, internal-node records read, leaf records read, results} summed
- * over the traversals run since axb_profile_enable(2 | ...). */
-axb_status axb_profile_visits(int64_t out_host[4]);
+ * over the traversals run since axb_profile_enable(2 | ...), then the same
+ * four counters for the nearest queries that took the exact fallback walk. */
+axb_status axb_profile_visits(int64_t out_host[8]);
 
 /* Host helper used by the query kernels, exported for unit tests: the largest
  * float s with RN(sqrt(s)) <= r, so that `d2 <= t` decides exactly the

@@ -1,0 +1,291 @@
+// test_facade.cpp -- the C++ facade (include/ArborX.hpp) exercised like the
+// reference's own Boost.Test suites, on the GPU:
+//   tstQueryTreeDegenerate.cpp (empty / one leaf / two leaves / duplicates /
+//   4096-leaf capacity), tstQueryTreeCallbacks.cpp:138-179 (collinear kNN),
+//   tstQueryTreeManufacturedSolution.cpp (11^3 grid), and
+//   tstQueryTreeTraversalPolicy.cpp:101-130 (buffer policy, SearchException),
+// plus a brute-force comparison on random points.  Rows are compared as
+// multisets (test/boost_ext/CompressedStorageComparison.hpp:21-61).
+#include <ArborX.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <set>
+#include <vector>
+
+static int g_fail = 0, g_pass = 0;
+#define CHECK(cond, what)                                                                   \
+  do                                                                                        \
+  {                                                                                         \
+    if (cond)                                                                               \
+      ++g_pass;                                                                             \
+    else                                                                                    \
+    {                                                                                       \
+      ++g_fail;                                                                             \
+      std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, what);                             \
+    }                                                                                       \
+  } while (0)
+
+using namespace ArborX;
+using Rows = std::vector<std::multiset<int>>;
+
+static Rows rows_of(DeviceVector<int> const &ind, DeviceVector<int64_t> const &off)
+{
+  auto i = ind.to_host();
+  auto o = off.to_host();
+  Rows r(o.empty() ? 0 : o.size() - 1);
+  for (size_t q = 0; q + 1 < o.size(); ++q)
+    r[q].insert(i.begin() + o[q], i.begin() + o[q + 1]);
+  return r;
+}
+static Rows rows(std::vector<std::vector<int>> const &v)
+{
+  Rows r;
+  for (auto &x : v)
+    r.emplace_back(x.begin(), x.end());
+  return r;
+}
+static Box pbox(float a) { return Box({{a, a, a}}, {{a, a, a}}); }
+
+static void degenerate(HipStream const &space, NodeFormat fmt)
+{
+  // empty tree (tstQueryTreeDegenerate.cpp:24-120)
+  {
+    BoundingVolumeHierarchy bvh;
+    CHECK(bvh.empty() && bvh.size() == 0, "default tree is empty");
+    DeviceVector<Box> none;
+    BoundingVolumeHierarchy bvh2(space, none, fmt);
+    CHECK(bvh2.empty(), "tree over no values is empty");
+    Box b = bvh2.bounds();
+    CHECK(b.minCorner()[0] == FLT_MAX && b.maxCorner()[0] == -FLT_MAX, "empty bounds");
+    DeviceVector<Box> q(std::vector<Box>{Box(), Box()});
+    DeviceVector<int> ind;
+    DeviceVector<int64_t> off;
+    bvh2.query(space, intersects(q), ind, off);
+    CHECK(rows_of(ind, off) == rows({{}, {}}), "empty tree, box queries");
+    DeviceVector<Point> p(std::vector<Point>{{{0, 0, 0}}, {{1, 1, 1}}});
+    bvh2.query(space, nearest(p, 2), ind, off);
+    CHECK(rows_of(ind, off) == rows({{}, {}}), "empty tree, nearest");
+  }
+  // single leaf (:123-225)
+  {
+    DeviceVector<Box> v(std::vector<Box>{Box({{0, 0, 0}}, {{1, 1, 1}})});
+    BoundingVolumeHierarchy bvh(space, v, fmt);
+    CHECK(bvh.size() == 1, "one leaf");
+    DeviceVector<Box> q(std::vector<Box>{pbox(5), pbox(.5f)});
+    DeviceVector<int> ind;
+    DeviceVector<int64_t> off;
+    bvh.query(space, intersects(q), ind, off);
+    CHECK(rows_of(ind, off) == rows({{}, {0}}), "one leaf, boxes");
+    DeviceVector<Sphere> s(
+        std::vector<Sphere>{{{{0, 0, 0}}, 1.f}, {{{1, 1, 1}}, 3.f}, {{{5, 5, 5}}, 2.f}});
+    bvh.query(space, intersects(s), ind, off);
+    CHECK(rows_of(ind, off) == rows({{0}, {0}, {}}), "one leaf, spheres");
+    DeviceVector<Point> p(std::vector<Point>{{{0, 0, 0}}, {{4, 5, 1}}});
+    bvh.query(space, nearest(p, 3), ind, off);
+    CHECK(rows_of(ind, off) == rows({{0}, {0}}), "one leaf, nearest");
+  }
+  // two leaves (:227-331)
+  {
+    DeviceVector<Box> v(std::vector<Box>{pbox(0), pbox(1)});
+    BoundingVolumeHierarchy bvh(space, v, fmt);
+    DeviceVector<Box> q(std::vector<Box>{Box(), Box({{0, 0, 0}}, {{1, 1, 1}}),
+                                         Box({{.5f, .5f, .5f}}, {{1.5f, 1.5f, 1.5f}}), pbox(0)});
+    DeviceVector<int> ind;
+    DeviceVector<int64_t> off;
+    bvh.query(space, intersects(q), ind, off);
+    CHECK(rows_of(ind, off) == rows({{}, {0, 1}, {1}, {0}}), "two leaves, boxes");
+    DeviceVector<Point> p(std::vector<Point>{{{0, 0, 0}}, {{1, 0, 0}}});
+    bvh.query(space, nearest(p, 4), ind, off);
+    CHECK(rows_of(ind, off) == rows({{0, 1}, {0, 1}}), "two leaves, nearest k > n");
+  }
+  // duplicated leaves (:333-368)
+  {
+    DeviceVector<Box> v(std::vector<Box>{pbox(0), pbox(1), pbox(1), pbox(1)});
+    BoundingVolumeHierarchy bvh(space, v, fmt);
+    DeviceVector<Sphere> s(std::vector<Sphere>{
+        {{{0, 0, 0}}, 1.f}, {{{1, 1, 1}}, 1.f}, {{{.5f, .5f, .5f}}, 1.f}});
+    DeviceVector<int> ind;
+    DeviceVector<int64_t> off;
+    bvh.query(space, intersects(s), ind, off);
+    CHECK(rows_of(ind, off) == rows({{0}, {1, 2, 3}, {0, 1, 2, 3}}), "duplicated leaves");
+  }
+  // stack capacity: 4096 diagonal boxes (:375-446)
+  {
+    std::vector<Box> b;
+    for (int i = 0; i < 4096; ++i)
+      b.push_back(Box({{(float)i, (float)i, (float)i}},
+                      {{(float)i + 1, (float)i + 1, (float)i + 1}}));
+    DeviceVector<Box> v(b);
+    BoundingVolumeHierarchy bvh(space, v, fmt);
+    DeviceVector<Box> q(std::vector<Box>{Box(), Box({{0, 0, 0}}, {{4096, 4096, 4096}})});
+    DeviceVector<int> ind;
+    DeviceVector<int64_t> off;
+    bvh.query(space, intersects(q), ind, off);
+    CHECK(off.to_host().back() == 4096, "spatial query over 4096 leaves");
+    DeviceVector<Point> p(std::vector<Point>{{{0, 0, 0}}});
+    bvh.query(space, nearest(p, 4096), ind, off);
+    CHECK(off.to_host().back() == 4096, "nearest k = n = 4096");
+  }
+}
+
+static void callbacks_nearest(HipStream const &space, NodeFormat fmt)
+{
+  // tstQueryTreeCallbacks.cpp:138-179: points (i,i,i), nearest(origin, 10)
+  std::vector<Point> pts;
+  for (int i = 0; i < 10; ++i)
+    pts.push_back({{(float)i, (float)i, (float)i}});
+  DeviceVector<Point> v(pts);
+  BoundingVolumeHierarchy bvh(space, v, fmt);
+  DeviceVector<Point> q(std::vector<Point>{{{0, 0, 0}}});
+  DeviceVector<int> ind;
+  DeviceVector<int64_t> off;
+  DeviceVector<float> dist;
+  bvh.query(space, nearest(q, 10), ind, off, dist);
+  auto i = ind.to_host();
+  auto d = dist.to_host();
+  bool ok = i.size() == 10;
+  for (int j = 0; ok && j < 10; ++j)
+  {
+    float s = 0, t = (float)j;
+    for (int a = 0; a < 3; ++a)
+      s += t * t;
+    ok = i[j] == j && d[j] == std::sqrt(s);
+  }
+  CHECK(ok, "collinear nearest: indices and distances");
+}
+
+static void manufactured(HipStream const &space, NodeFormat fmt)
+{
+  // tstQueryTreeManufacturedSolution.cpp:29-270 (cases i and ii)
+  const int nx = 11, ny = 11, nz = 11, n = nx * ny * nz;
+  const float h = 100.f / 10.f;
+  auto ind = [&](int i, int j, int k) { return i + j * nx + k * nx * ny; };
+  std::vector<Box> boxes(n), q(n);
+  std::vector<std::vector<int>> ref(n);
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k)
+      {
+        Point p{{i * h, j * h, k * h}};
+        boxes[ind(i, j, k)] = Box(p, p);
+        q[ind(i, j, k)] = Box({{(i - 1) * h, (j - 1) * h, (k - 1) * h}},
+                              {{(i + 1) * h, (j + 1) * h, (k + 1) * h}});
+        for (int a = -1; a <= 1; ++a)
+          for (int b = -1; b <= 1; ++b)
+            for (int c = -1; c <= 1; ++c)
+              if (i + a >= 0 && i + a < nx && j + b >= 0 && j + b < ny && k + c >= 0 &&
+                  k + c < nz)
+                ref[ind(i, j, k)].push_back(ind(i + a, j + b, k + c));
+      }
+  DeviceVector<Box> v(boxes);
+  BoundingVolumeHierarchy bvh(space, v, fmt);
+  DeviceVector<int> I;
+  DeviceVector<int64_t> O;
+  bvh.query(space, intersects(v), I, O);
+  std::vector<std::vector<int>> self(n);
+  for (int i = 0; i < n; ++i)
+    self[i] = {i};
+  CHECK(rows_of(I, O) == rows(self), "manufactured (i): objects themselves");
+  DeviceVector<Box> qd(q);
+  bvh.query(space, intersects(qd), I, O);
+  CHECK(rows_of(I, O) == rows(ref), "manufactured (ii): first neighbours");
+}
+
+static void policy(HipStream const &space)
+{
+  // tstQueryTreeTraversalPolicy.cpp:101-130
+  std::vector<Point> pts;
+  for (int i = 0; i < 64; ++i)
+    pts.push_back({{(float)(i % 4), (float)((i / 4) % 4), (float)(i / 16)}});
+  DeviceVector<Point> v(pts);
+  BoundingVolumeHierarchy bvh(space, v);
+  DeviceVector<int> I, I0;
+  DeviceVector<int64_t> O, O0;
+  bvh.query(space, Experimental::make_intersects(v, 1.5f), I0, O0);
+  for (int b : {64, 2, 1})
+  {
+    bvh.query(space, Experimental::make_intersects(v, 1.5f), I, O,
+              Experimental::TraversalPolicy().setBufferSize(b));
+    CHECK(rows_of(I, O) == rows_of(I0, O0), "buffer policy gives the same rows");
+  }
+  bool threw = false;
+  try
+  {
+    bvh.query(space, Experimental::make_intersects(v, 1.5f), I, O,
+              Experimental::TraversalPolicy().setBufferSize(-1));
+  }
+  catch (SearchException const &)
+  {
+    threw = true;
+  }
+  CHECK(threw, "hard buffer overflow throws SearchException");
+  bvh.query(space, Experimental::make_intersects(v, 1.5f), I, O,
+            Experimental::TraversalPolicy().setPredicateSorting(false));
+  CHECK(rows_of(I, O) == rows_of(I0, O0), "predicate sorting off gives the same rows");
+}
+
+static void brute_force(HipStream const &space, NodeFormat fmt)
+{
+  std::mt19937 gen(42);
+  std::uniform_real_distribution<float> U(0.f, 1.f);
+  const int n = 20000, nq = 2000, k = 12;
+  const float r = 0.06f;
+  std::vector<Point> pts(n), qp(nq);
+  for (auto &p : pts)
+    p = {{U(gen), U(gen), U(gen)}};
+  for (auto &p : qp)
+    p = {{U(gen), U(gen), U(gen)}};
+  DeviceVector<Point> v(pts), q(qp);
+  BoundingVolumeHierarchy bvh(space, v, fmt);
+  DeviceVector<int> I;
+  DeviceVector<int64_t> O;
+  bvh.query(space, Experimental::make_intersects(q, r), I, O);
+  Rows got = rows_of(I, O);
+  DeviceVector<float> D;
+  bvh.query(space, Experimental::make_nearest(q, k), I, O, D);
+  auto ki = I.to_host();
+  auto kd = D.to_host();
+  bool ok_r = true, ok_k = true;
+  for (int a = 0; a < nq; ++a)
+  {
+    std::multiset<int> want;
+    std::vector<std::pair<float, int>> all;
+    for (int b = 0; b < n; ++b)
+    {
+      float s = 0;
+      for (int d = 0; d < 3; ++d)
+      {
+        float t = pts[b][d] - qp[a][d];
+        s += t * t;
+      }
+      float dist = std::sqrt(s);
+      if (dist <= r)
+        want.insert(b);
+      all.push_back({dist, b});
+    }
+    std::partial_sort(all.begin(), all.begin() + k, all.end());
+    ok_r = ok_r && (got[a] == want);
+    for (int j = 0; j < k; ++j)
+      ok_k = ok_k && ki[a * k + j] == all[j].second && kd[a * k + j] == all[j].first;
+  }
+  CHECK(ok_r, "radius query equals brute force");
+  CHECK(ok_k, "kNN equals brute force (distance, index)");
+}
+
+int main()
+{
+  HipStream space;
+  for (NodeFormat fmt : {NodeFormat::FP32, NodeFormat::Q16})
+  {
+    degenerate(space, fmt);
+    callbacks_nearest(space, fmt);
+    manufactured(space, fmt);
+    brute_force(space, fmt);
+  }
+  policy(space);
+  std::printf("%d passed, %d failed\n", g_pass, g_fail);
+  return g_fail == 0 ? 0 : 1;
+}
