@@ -47,7 +47,9 @@ EXPORTED_SYMBOLS = (
     "axb_last_error", "axb_version", "axb_set_output_allocator", "axb_free",
     "axb_bvh_create", "axb_bvh_destroy", "axb_bvh_size", "axb_bvh_bounds",
     "axb_bvh_query_intersects", "axb_bvh_query_intersects_radius",
-    "axb_bvh_query_nearest", "axb_pair_energy", "axb_pair_energy_fd",
+    "axb_bvh_query_nearest", "axb_bvh_neighbor_list", "axb_find_neighbor_list",
+    "axb_expand_half_to_full", "axb_pair_energy", "axb_pair_energy_fd",
+    "axb_bvh_view", "axb_bvh_predicate_order", "axb_exclusive_scan_i64",
     "axb_bvh_num_slots", "axb_bvh_slots", "axb_bvh_export_topology",
     "axb_sqrt_threshold", "axb_morton64_host", "axb_profile_enable", "axb_profile_clear",
     "axb_profile_num_records", "axb_profile_record", "axb_profile_visits",
@@ -112,6 +114,12 @@ def lib():
             "axb_bvh_query_nearest": (_I32, [_P, _P, _I64, _I32, _Policy, _P,
                                              ctypes.POINTER(_P), ctypes.POINTER(_P),
                                              ctypes.POINTER(_I64), _P]),
+            "axb_bvh_neighbor_list": (_I32, [_P, ctypes.c_float, _I32, _Policy, _P,
+                                             ctypes.POINTER(_P), ctypes.POINTER(_I64), _P]),
+            "axb_find_neighbor_list": (_I32, [_P, _I64, ctypes.c_float, _I32, _P,
+                                              ctypes.POINTER(_P), ctypes.POINTER(_I64), _P]),
+            "axb_expand_half_to_full": (_I32, [_P, _P, _I64, _P, ctypes.POINTER(_P),
+                                               ctypes.POINTER(_I64), _P]),
             "axb_pair_energy": (_I32, [_P, _I64, _P, _P, ctypes.c_float, _P, _P, _P]),
             "axb_pair_energy_fd": (_I32, [_P, _I64, _P, _P, ctypes.c_float, _P, _I64,
                                           ctypes.c_double, _P, _P]),
@@ -379,7 +387,70 @@ class BoundingVolumeHierarchy:
         raise TypeError("unsupported predicates")
 
 
+    def neighbor_list(self, radius, full=True, policy=None, stream=None):
+        """Full or half neighbour list of a points tree with itself
+        (detail/ArborX_NeighborList.hpp:49-188 semantics, rows by value
+        index) -> (offsets, indices)."""
+        policy = policy or TraversalPolicy()
+        n = self.size()
+        offsets = torch.empty(n + 1, dtype=torch.int64, device="cuda")
+        ind = ctypes.c_void_p()
+        nnz = ctypes.c_int64()
+        _check(lib().axb_bvh_neighbor_list(self._h, float(radius), 1 if full else 0,
+                                           policy._c(), _ptr(offsets), ctypes.byref(ind),
+                                           ctypes.byref(nnz), _stream(stream)))
+        return offsets, _take_output(ind.value, nnz.value, torch.int32)
+
+
 BVH = BoundingVolumeHierarchy
+
+
+def _neighbor_list(points, radius, full, stream):
+    x = _dev_f32(points, 3)
+    n = x.shape[0]
+    offsets = torch.empty(n + 1, dtype=torch.int64, device="cuda")
+    ind = ctypes.c_void_p()
+    nnz = ctypes.c_int64()
+    _use_torch_allocator()
+    _check(lib().axb_find_neighbor_list(_ptr(x), n, float(radius), 1 if full else 0,
+                                        _ptr(offsets), ctypes.byref(ind), ctypes.byref(nnz),
+                                        _stream(stream)))
+    return offsets, _take_output(ind.value, nnz.value, torch.int32)
+
+
+def find_full_neighbor_list(points, radius, stream=None):
+    """Experimental::findFullNeighborList(space, points, radius, offsets,
+    indices) (detail/ArborX_NeighborList.hpp:105-188) -> (offsets, indices):
+    row i lists every j != i within radius of point i."""
+    return _neighbor_list(points, radius, True, stream)
+
+
+def find_half_neighbor_list(points, radius, stream=None):
+    """Experimental::findHalfNeighborList (detail/ArborX_NeighborList.hpp:49-103)
+    -> (offsets, indices): each pair once, in the row of the point whose
+    leaf comes later in the tree."""
+    return _neighbor_list(points, radius, False, stream)
+
+
+def expand_half_to_full(offsets, indices, stream=None):
+    """Details::expandHalfToFull (detail/ArborX_ExpandHalfToFull.hpp:23-68)
+    -> (offsets, indices) of the full list."""
+    o = torch.as_tensor(offsets).to(device="cuda", dtype=torch.int64).contiguous()
+    i = torch.as_tensor(indices).to(device="cuda", dtype=torch.int32).contiguous()
+    n = o.shape[0] - 1
+    out_o = torch.empty(n + 1, dtype=torch.int64, device="cuda")
+    ind = ctypes.c_void_p()
+    nnz = ctypes.c_int64()
+    _use_torch_allocator()
+    _check(lib().axb_expand_half_to_full(_ptr(o), _ptr(i) if i.numel() else None, n,
+                                         _ptr(out_o), ctypes.byref(ind), ctypes.byref(nnz),
+                                         _stream(stream)))
+    return out_o, _take_output(ind.value, nnz.value, torch.int32)
+
+
+findFullNeighborList = find_full_neighbor_list
+findHalfNeighborList = find_half_neighbor_list
+expandHalfToFull = expand_half_to_full
 
 
 def query(tree, predicates, policy=None, **kw):
@@ -427,11 +498,11 @@ class profile:
             _check(L.axb_profile_record(i, buf, 256, ctypes.byref(ms)))
             self.kernels.setdefault(buf.value.decode(), []).append(ms.value)
         if self.flags & 2:
-            v = (_I64 * 8)()
+            v = (_I64 * 12)()
             _check(L.axb_profile_visits(v))
             self.visits = dict(queries=v[0], internal=v[1], leaf=v[2], results=v[3],
                                fallback_queries=v[4], fallback_internal=v[5],
-                               fallback_leaf=v[6])
+                               fallback_leaf=v[6], rounds=v[8], multi_round_queries=v[9])
         _check(L.axb_profile_enable(0))
         return False
 

@@ -34,6 +34,7 @@ CFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
     "-Wno-unused-function",
+    "-I" + os.path.join(os.path.dirname(HERE), "include"),
 ]
 
 

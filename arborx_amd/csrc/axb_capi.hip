@@ -360,6 +360,120 @@ static axb_status predicate_order(const axb_bvh *t, int pred_kind, const float *
   return AXB_OK;
 }
 
+} // extern "C"
+
+// One CRS query of nq rows (CrsGraphWrapperImpl.hpp:145-374 semantics).
+// first(mode, out, B) traverses every row once: MODE_COUNT, or MODE_BUFFER
+// keeping up to B results per row in a scratch row.  Counts become offsets
+// by an exclusive scan; the rows that fit are compacted, and only the rows
+// that overflowed are traversed again by fill(list, n_list, out) with
+// MODE_FILL (list == nullptr: every row).  The overflow list walks the
+// processing order `ovf_order` (processing index -> row, nullptr: identity)
+// and holds rows, or processing indices when `ovf_emit_index`.
+// TraversalPolicy buffer_size: 0 -> a scratch row of up to 64 results when
+// the scratch fits in 4 GiB, else plain count + fill; > 0 -> that many;
+// < 0 -> that many, and a row that overflows is an error
+// (AXB_ERR_BUFFER_OVERFLOW, the reference's SearchException).
+template <class First, class Fill>
+static axb_status crs_query(const axb_bvh *t, int64_t nq, axb_traversal_policy policy,
+                            const uint32_t *ovf_order, bool ovf_emit_index, First first,
+                            Fill fill, int64_t *offsets, int32_t **indices, int64_t *nnz_host,
+                            hipStream_t s, const char *what)
+{
+  const bool hard = policy.buffer_size < 0;
+  int64_t B = policy.buffer_size < 0 ? -(int64_t)policy.buffer_size : policy.buffer_size;
+  if (B == 0)
+  {
+    const int64_t fit = ((int64_t(4) << 30) / (4 * nq)) & ~int64_t(3);
+    B = fit >= 64 ? 64 : (fit >= 16 ? fit : 0);
+  }
+  void *scan_ws = dalloc(scan_workspace_bytes(nq + 1), s);
+  int32_t *buf = nullptr;
+  uint32_t *ovf = nullptr, *ovf_n = nullptr;
+  axb_status st = AXB_OK;
+  hipError_t e = scan_ws ? hipSuccess : hipErrorOutOfMemory;
+  int64_t nnz = 0;
+  uint32_t n_ovf = 0;
+  if (e == hipSuccess && B > 0)
+  {
+    buf = (int32_t *)dalloc((size_t)nq * B * sizeof(int32_t), s);
+    ovf = (uint32_t *)dalloc((size_t)nq * sizeof(uint32_t), s);
+    ovf_n = (uint32_t *)dalloc(sizeof(uint32_t), s);
+    e = (buf && ovf && ovf_n) ? hipSuccess : hipErrorOutOfMemory;
+    if (e == hipSuccess)
+      e = first(MODE_BUFFER, buf, (int32_t)B);
+  }
+  else if (e == hipSuccess)
+    e = first(MODE_COUNT, nullptr, 0);
+  // counts -> offsets (exclusive scan over nq + 1 entries, last count 0)
+  if (e == hipSuccess)
+    e = fill_u64((uint64_t *)(offsets + nq), 0ull, 1, s);
+  if (e == hipSuccess)
+    e = exclusive_scan_i64(offsets, offsets, nq + 1, scan_ws, t->d_err, s);
+  if (e == hipSuccess && B > 0)
+  {
+    e = fill_u32(ovf_n, 0u, 1, s);
+    if (e == hipSuccess)
+      e = launch_overflow_list(offsets, ovf_order, nq, (int32_t)B, ovf_emit_index, ovf, ovf_n,
+                               s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&n_ovf, ovf_n, sizeof(n_ovf), hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&nnz, offsets + nq, sizeof(nnz), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(s);
+  if (e == hipSuccess && hard && n_ovf > 0)
+  {
+    set_error(std::string(what) + ": a row has more results than the hard buffer size "
+                                  "(TraversalPolicy buffer_size < 0)");
+    st = AXB_ERR_BUFFER_OVERFLOW;
+  }
+  if (e == hipSuccess && st == AXB_OK)
+  {
+    *indices = (int32_t *)out_alloc((size_t)nnz * sizeof(int32_t), s);
+    if (!*indices)
+      e = hipErrorOutOfMemory;
+  }
+  if (e == hipSuccess && st == AXB_OK && nnz > 0)
+  {
+    if (B == 0)
+      e = fill(nullptr, nq, *indices);
+    else
+    {
+      e = launch_compact(buf, nq, (int32_t)B, offsets, *indices, s);
+      // rows that did not fit: traverse those again, writing in place
+      if (e == hipSuccess && n_ovf > 0)
+        e = fill(ovf, n_ovf, *indices);
+    }
+  }
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(s);
+  dfree(scan_ws, s);
+  dfree(buf, s);
+  dfree(ovf, s);
+  dfree(ovf_n, s);
+  if (e != hipSuccess)
+  {
+    out_free(*indices, s);
+    *indices = nullptr;
+    return hip_fail(e, what);
+  }
+  if (st != AXB_OK)
+    return st;
+  st = check_kernel_error(t->d_err, s, what);
+  if (st != AXB_OK)
+  {
+    out_free(*indices, s);
+    *indices = nullptr;
+    return st;
+  }
+  *nnz_host = nnz;
+  return AXB_OK;
+}
+
+extern "C" {
+
 static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *preds, int64_t nq,
                                 float radius, axb_traversal_policy policy, int64_t *offsets,
                                 int32_t **indices, int64_t *nnz_host, axb_stream_t stream_)
@@ -389,93 +503,19 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
     if (st != AXB_OK)
       return st;
   }
-  const int32_t B = policy.buffer_size < 0 ? -policy.buffer_size : policy.buffer_size;
-  void *scan_ws = dalloc(scan_workspace_bytes(nq + 1), s);
-  int32_t *buf = nullptr;
-  uint32_t *flag = nullptr;
-  axb_status st = AXB_OK;
-  hipError_t e = scan_ws ? hipSuccess : hipErrorOutOfMemory;
-  bool second_pass = (B == 0);
-  int64_t nnz = 0;
-  if (e == hipSuccess && B > 0)
-  {
-    buf = (int32_t *)dalloc((size_t)nq * B * sizeof(int32_t), s);
-    flag = (uint32_t *)dalloc(sizeof(uint32_t), s);
-    e = (buf && flag) ? hipSuccess : hipErrorOutOfMemory;
-    if (e == hipSuccess)
-      e = launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, nq, radius, order,
-                         offsets, buf, B, MODE_BUFFER, s);
-    if (e == hipSuccess)
-      e = fill_u32(flag, 0u, 1, s);
-    if (e == hipSuccess)
-      e = launch_max_count(offsets, nq, B, flag, s);
-    uint32_t hflag = 0;
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(&hflag, flag, sizeof(hflag), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess)
-      e = hipStreamSynchronize(s);
-    if (e == hipSuccess && hflag)
-    {
-      if (policy.buffer_size < 0)
-      {
-        set_error("query: a query has more results than the hard buffer size "
-                  "(TraversalPolicy buffer_size < 0)");
-        st = AXB_ERR_BUFFER_OVERFLOW;
-      }
-      second_pass = true;
-    }
-  }
-  else if (e == hipSuccess)
-  {
-    e = launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, nq, radius, order,
-                       offsets, nullptr, 0, MODE_COUNT, s);
-  }
-  // counts -> offsets (exclusive scan over nq + 1 entries, last count 0)
-  if (e == hipSuccess && st == AXB_OK)
-    e = fill_u64((uint64_t *)(offsets + nq), 0ull, 1, s);
-  if (e == hipSuccess && st == AXB_OK)
-    e = exclusive_scan_i64(offsets, offsets, nq + 1, scan_ws, t->d_err, s);
-  if (e == hipSuccess && st == AXB_OK)
-    e = hipMemcpyAsync(&nnz, offsets + nq, sizeof(nnz), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && st == AXB_OK)
-    e = hipStreamSynchronize(s);
-  if (e == hipSuccess && st == AXB_OK)
-  {
-    *indices = (int32_t *)out_alloc((size_t)nnz * sizeof(int32_t), s);
-    if (!*indices)
-      e = hipErrorOutOfMemory;
-  }
-  if (e == hipSuccess && st == AXB_OK && nnz > 0)
-  {
-    if (second_pass)
-      e = launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, nq, radius, order,
-                         offsets, *indices, 0, MODE_FILL, s);
-    else
-      e = launch_compact(buf, nq, B, offsets, *indices, s);
-  }
-  if (e == hipSuccess)
-    e = hipStreamSynchronize(s);
-  dfree(scan_ws, s);
-  dfree(buf, s);
-  dfree(flag, s);
+  auto first = [&](int mode, int32_t *out, int32_t B) {
+    return launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, nq, radius, order,
+                          offsets, out, B, mode, s);
+  };
+  // the overflow list holds rows; the fill pass takes them as its order
+  auto fill = [&](const uint32_t *list, int64_t n_list, int32_t *out) {
+    return launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, n_list, radius,
+                          list ? list : order, offsets, out, 0, MODE_FILL, s);
+  };
+  axb_status st = crs_query(t, nq, policy, order, false, first, fill, offsets, indices, nnz_host,
+                            s, "query");
   dfree(order_blk, s);
-  if (e != hipSuccess)
-  {
-    out_free(*indices, s);
-    *indices = nullptr;
-    return hip_fail(e, "query");
-  }
-  if (st != AXB_OK)
-    return st;
-  st = check_kernel_error(t->d_err, s, "query");
-  if (st != AXB_OK)
-  {
-    out_free(*indices, s);
-    *indices = nullptr;
-    return st;
-  }
-  *nnz_host = nnz;
-  return AXB_OK;
+  return st;
 }
 
 axb_status axb_bvh_query_intersects(const axb_bvh *t, int32_t predicate_kind,
@@ -499,6 +539,201 @@ axb_status axb_bvh_query_intersects_radius(const axb_bvh *t, const float *points
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_intersects_radius: radius must be > 0");
   return query_spatial(t, PRED_RADIUS, points, nq, radius, policy, offsets, indices, nnz_host,
                        stream);
+}
+
+axb_status axb_bvh_view(const axb_bvh *t, axb_tree_view *out)
+{
+  if (!t || !out)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_view: NULL argument");
+  const TreeView tv = view_of(t);
+  out->slots = t->n > 0 ? (const void *)tv.slots : nullptr;
+  out->end_slot = t->n > 0 ? (int64_t)tv.total : 0;
+  out->n = t->n;
+  out->first_leaf_prefix = t->n > 0 ? tv.C : nullptr;
+  for (int d = 0; d < 3; ++d)
+  {
+    out->scene[d] = tv.scene.lo[d];
+    out->scene[3 + d] = tv.scene.hi[d];
+    out->q16_scale[d] = tv.ss[d];
+  }
+  out->geometry = t->geometry;
+  out->node_format = t->node_format;
+  return AXB_OK;
+}
+
+axb_status axb_bvh_predicate_order(const axb_bvh *t, int32_t predicate_kind,
+                                   const float *preds, int64_t nq, uint32_t *order,
+                                   axb_stream_t stream_)
+{
+  if (!t || (nq > 0 && (!preds || !order)))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: NULL argument");
+  if (predicate_kind != AXB_PREDICATE_SPHERES && predicate_kind != AXB_PREDICATE_BOXES &&
+      predicate_kind != PRED_RADIUS)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: unknown predicate kind");
+  if (nq < 0 || nq >= (int64_t(1) << 31))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: nq must be in [0, 2^31)");
+  hipStream_t s = (hipStream_t)stream_;
+  if (nq == 0)
+    return AXB_OK;
+  if (nq == 1 || t->n == 0)
+  {
+    AXB_HIP_TRY(launch_iota_u32(order, nq, s));
+    return AXB_OK;
+  }
+  uint32_t *ord = nullptr;
+  void *blk = nullptr;
+  axb_status st = predicate_order(t, predicate_kind, preds, nq, s, &ord, &blk);
+  if (st != AXB_OK)
+    return st;
+  hipError_t e = hipMemcpyAsync(order, ord, (size_t)nq * sizeof(uint32_t),
+                                hipMemcpyDeviceToDevice, s);
+  dfree(blk, s);
+  if (e != hipSuccess)
+    return hip_fail(e, "axb_bvh_predicate_order");
+  return check_kernel_error(t->d_err, s, "axb_bvh_predicate_order");
+}
+
+axb_status axb_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n,
+                                  axb_stream_t stream_)
+{
+  if (n < 0 || (n > 0 && (!in || !out)))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_exclusive_scan_i64: bad argument");
+  if (n == 0)
+    return AXB_OK;
+  hipStream_t s = (hipStream_t)stream_;
+  void *ws = dalloc(scan_workspace_bytes(n), s);
+  uint32_t *err = (uint32_t *)dalloc(sizeof(uint32_t), s);
+  hipError_t e = (ws && err) ? hipSuccess : hipErrorOutOfMemory;
+  if (e == hipSuccess)
+    e = fill_u32(err, 0u, 1, s);
+  if (e == hipSuccess)
+    e = exclusive_scan_i64(in, out, n, ws, err, s);
+  axb_status st = AXB_OK;
+  if (e == hipSuccess)
+    st = check_kernel_error(err, s, "axb_exclusive_scan_i64");
+  dfree(ws, s);
+  dfree(err, s);
+  if (e != hipSuccess)
+    return hip_fail(e, "axb_exclusive_scan_i64");
+  return st;
+}
+
+axb_status axb_bvh_neighbor_list(const axb_bvh *t, float radius, int32_t full,
+                                 axb_traversal_policy policy, int64_t *offsets,
+                                 int32_t **indices, int64_t *nnz_host, axb_stream_t stream_)
+{
+  if (!t || !offsets || !indices || !nnz_host)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_neighbor_list: NULL argument");
+  if (t->geometry != AXB_GEOMETRY_POINTS)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_neighbor_list: the tree must hold points");
+  if (!(radius >= 0.f))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_neighbor_list: radius must be >= 0");
+  hipStream_t s = (hipStream_t)stream_;
+  *indices = nullptr;
+  *nnz_host = 0;
+  const int64_t n = t->n;
+  if (n <= 1) // HalfTraversal.hpp:33-44: no pairs
+  {
+    AXB_HIP_TRY(fill_u64((uint64_t *)offsets, 0ull, n + 1, s));
+    *indices = (int32_t *)out_alloc(0, s);
+    AXB_HIP_TRY(hipStreamSynchronize(s));
+    return AXB_OK;
+  }
+  const TreeView tv = view_of(t);
+  uint32_t *rowof = (uint32_t *)dalloc((size_t)n * sizeof(uint32_t), s);
+  if (!rowof)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bvh_neighbor_list: out of device memory");
+  const bool f = full != 0;
+  // lanes walk the leaves in tree order (already Morton-sorted)
+  auto first = [&](int mode, int32_t *out, int32_t B) {
+    return launch_nlist(tv, t->node_format, f, radius, n, nullptr, offsets, out, B, rowof, mode,
+                        s);
+  };
+  // the overflow list holds leaf positions
+  auto fill = [&](const uint32_t *list, int64_t n_list, int32_t *out) {
+    return launch_nlist(tv, t->node_format, f, radius, n_list, list, offsets, out, 0, nullptr,
+                        MODE_FILL, s);
+  };
+  axb_status st = crs_query(t, n, policy, rowof, true, first, fill, offsets, indices, nnz_host,
+                            s, "axb_bvh_neighbor_list");
+  dfree(rowof, s);
+  return st;
+}
+
+axb_status axb_find_neighbor_list(const float *points, int64_t n, float radius, int32_t full,
+                                  int64_t *offsets, int32_t **indices, int64_t *nnz_host,
+                                  axb_stream_t stream)
+{
+  axb_bvh *t = nullptr;
+  axb_status st = axb_bvh_create(points, n, AXB_GEOMETRY_POINTS, AXB_NODES_Q16, 0, stream, &t);
+  if (st == AXB_ERR_INVALID_ARGUMENT && n > 1 && points)
+    // scene box without finite extent: 32-byte nodes have no such limit
+    st = axb_bvh_create(points, n, AXB_GEOMETRY_POINTS, AXB_NODES_FP32, 0, stream, &t);
+  if (st != AXB_OK)
+    return st;
+  st = axb_bvh_neighbor_list(t, radius, full, axb_traversal_policy{0, 1}, offsets, indices,
+                             nnz_host, stream);
+  axb_bvh_destroy(t);
+  return st;
+}
+
+axb_status axb_expand_half_to_full(const int64_t *offsets, const int32_t *indices, int64_t n,
+                                   int64_t *out_offsets, int32_t **out_indices,
+                                   int64_t *nnz_host, axb_stream_t stream_)
+{
+  if (!offsets || !out_offsets || !out_indices || !nnz_host)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_expand_half_to_full: NULL argument");
+  if (n < 0 || n >= (int64_t(1) << 31))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_expand_half_to_full: n must be in [0, 2^31)");
+  hipStream_t s = (hipStream_t)stream_;
+  *out_indices = nullptr;
+  *nnz_host = 0;
+  int64_t half = 0;
+  AXB_HIP_TRY(hipMemcpyAsync(&half, offsets + n, sizeof(half), hipMemcpyDeviceToHost, s));
+  AXB_HIP_TRY(hipStreamSynchronize(s));
+  if (half > 0 && !indices)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_expand_half_to_full: indices is NULL");
+  int64_t *cursor = (int64_t *)dalloc((size_t)(n + 1) * sizeof(int64_t), s);
+  void *scan_ws = dalloc(scan_workspace_bytes(n + 1), s);
+  uint32_t *err = (uint32_t *)dalloc(sizeof(uint32_t), s);
+  hipError_t e = (cursor && scan_ws && err) ? hipSuccess : hipErrorOutOfMemory;
+  if (e == hipSuccess)
+    e = fill_u32(err, 0u, 1, s);
+  if (e == hipSuccess)
+    e = fill_u64((uint64_t *)out_offsets, 0ull, n + 1, s);
+  if (e == hipSuccess)
+    e = launch_h2f_count(offsets, indices, n, out_offsets, err, s);
+  if (e == hipSuccess)
+    e = exclusive_scan_i64(out_offsets, out_offsets, n + 1, scan_ws, err, s);
+  int64_t m = 0;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&m, out_offsets + n, sizeof(m), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(s);
+  if (e == hipSuccess)
+  {
+    *out_indices = (int32_t *)out_alloc((size_t)m * sizeof(int32_t), s);
+    if (!*out_indices)
+      e = hipErrorOutOfMemory;
+  }
+  if (e == hipSuccess && m > 0)
+    e = launch_h2f_fill(offsets, indices, n, out_offsets, cursor, *out_indices, s);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(s);
+  axb_status st = AXB_OK;
+  if (e == hipSuccess)
+    st = check_kernel_error(err, s, "axb_expand_half_to_full");
+  dfree(cursor, s);
+  dfree(scan_ws, s);
+  dfree(err, s);
+  if (e != hipSuccess || st != AXB_OK)
+  {
+    out_free(*out_indices, s);
+    *out_indices = nullptr;
+    return e != hipSuccess ? hip_fail(e, "axb_expand_half_to_full") : st;
+  }
+  *nnz_host = m;
+  return AXB_OK;
 }
 
 axb_status axb_bvh_query_nearest(const axb_bvh *t, const float *points, int64_t nq, int32_t k,
@@ -582,13 +817,13 @@ axb_status axb_profile_enable(int32_t flags)
   axb_profile_clear();
   if ((flags & 2) && !g_visits)
   {
-    hipError_t e = hipMalloc((void **)&g_visits, 8 * sizeof(unsigned long long));
+    hipError_t e = hipMalloc((void **)&g_visits, 12 * sizeof(unsigned long long));
     if (e != hipSuccess)
       return hip_fail(e, "axb_profile_enable");
   }
   if (g_visits)
   {
-    hipError_t e = hipMemset(g_visits, 0, 8 * sizeof(unsigned long long));
+    hipError_t e = hipMemset(g_visits, 0, 12 * sizeof(unsigned long long));
     if (e == hipSuccess)
       e = hipDeviceSynchronize();
     if (e != hipSuccess)
@@ -639,21 +874,21 @@ axb_status axb_profile_record(int32_t i, char *name, int32_t name_len, double *m
   return AXB_OK;
 }
 
-axb_status axb_profile_visits(int64_t out_host[8])
+axb_status axb_profile_visits(int64_t out_host[12])
 {
   if (!out_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_profile_visits: NULL");
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < 12; ++i)
     out_host[i] = 0;
   if (!g_visits)
     return AXB_OK;
-  unsigned long long h[8];
+  unsigned long long h[12];
   hipError_t e = hipDeviceSynchronize();
   if (e == hipSuccess)
     e = hipMemcpy(h, g_visits, sizeof(h), hipMemcpyDeviceToHost);
   if (e != hipSuccess)
     return hip_fail(e, "axb_profile_visits");
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < 12; ++i)
     out_host[i] = (int64_t)h[i];
   return AXB_OK;
 }

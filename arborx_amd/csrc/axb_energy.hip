@@ -31,7 +31,10 @@ __global__ __launch_bounds__(256) void k_energy(const float *__restrict__ xyz, i
                                                 double *__restrict__ partials,
                                                 float *__restrict__ grad)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // rows of neighbouring points share neighbours: one contiguous range of
+  // blocks per XCD (its L2)
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t i = (int64_t)blk * blockDim.x + threadIdx.x;
   double e = 0.0;
   if (i < n)
   {
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(256) void k_energy(const float *__restrict__ xyz, i
     s[wave] = e;
   __syncthreads();
   if (threadIdx.x == 0)
-    partials[blockIdx.x] = ((s[0] + s[1]) + s[2]) + s[3];
+    partials[blk] = ((s[0] + s[1]) + s[2]) + s[3];
 }
 
 __global__ __launch_bounds__(256) void k_reduce_f64(const double *__restrict__ partials,

@@ -1,7 +1,7 @@
 // axb_internal.hpp -- declarations shared by the library's translation units.
 #pragma once
 
-#include "axb_common.hpp"
+#include "arborx_mi355x_device.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -89,6 +89,7 @@ hipError_t inclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n,
 // Fill kernels (used instead of hipMemsetAsync so that the code stays
 // graph-capture friendly).
 hipError_t fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t stream);
+hipError_t launch_iota_u32(uint32_t *p, int64_t n, hipStream_t stream); // p[i] = i
 hipError_t fill_u64(uint64_t *p, uint64_t v, int64_t n, hipStream_t stream);
 
 
@@ -128,10 +129,6 @@ struct TreeView
   unsigned long long *visits; // optional visit counters (see visit_counters)
 };
 
-constexpr int PRED_SPHERES = 0;
-constexpr int PRED_BOXES = 1;
-constexpr int PRED_RADIUS = 2; // points + one radius
-
 constexpr int MODE_COUNT = 0;
 constexpr int MODE_FILL = 1;
 constexpr int MODE_BUFFER = 2;
@@ -154,8 +151,22 @@ hipError_t launch_pred_morton32(int pred_kind, const float *preds, int64_t nq,
                                 hipStream_t s);
 hipError_t launch_compact(const int32_t *buf, int64_t nq, int32_t buffer,
                           const int64_t *offsets, int32_t *out, hipStream_t s);
-hipError_t launch_max_count(const int64_t *counts, int64_t nq, int32_t buffer, uint32_t *flag,
-                            hipStream_t s);
+hipError_t launch_overflow_list(const int64_t *offsets, const uint32_t *order, int64_t nq,
+                                int32_t buffer, bool emit_index, uint32_t *ovf,
+                                uint32_t *ovf_n, hipStream_t s);
+// Neighbour list of a point tree: lane t takes leaf position list[t] (t when
+// list is nullptr); rows are the leaves' values.  rowof (leaf position ->
+// row) is written when non-null.
+hipError_t launch_nlist(const TreeView &tv, int node_format, bool full, float radius, int64_t nl,
+                        const uint32_t *list, int64_t *offsets, int32_t *out, int32_t buffer,
+                        uint32_t *rowof, int mode, hipStream_t s);
+// Half -> full neighbour list: counts into cnt (zeroed, n + 1 entries), then
+// after the scan into out_offsets, the fill (cursor: n scratch entries).
+hipError_t launch_h2f_count(const int64_t *offsets, const int32_t *indices, int64_t n,
+                            int64_t *cnt, uint32_t *err, hipStream_t s);
+hipError_t launch_h2f_fill(const int64_t *offsets, const int32_t *indices, int64_t n,
+                           const int64_t *out_offsets, int64_t *cursor, int32_t *out,
+                           hipStream_t s);
 hipError_t pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
                        const int32_t *indices, float cutoff, double *energy, float *grad,
                        hipStream_t s);
@@ -183,11 +194,4 @@ AXB_D void st_agent64(uint64_t *p, uint64_t v)
 {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-AXB_D void raise_error(uint32_t *err, uint32_t flag)
-{
-  if (err)
-    __hip_atomic_fetch_or(err, flag, __ATOMIC_RELAXED,
-                          __HIP_MEMORY_SCOPE_AGENT);
-}
-
 } // namespace axb

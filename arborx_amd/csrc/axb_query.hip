@@ -33,173 +33,138 @@ namespace
 // multi-dword accesses).
 typedef int32_t i32x4_u __attribute__((ext_vector_type(4), aligned(4)));
 
-template <int F>
-AXB_D void node_box(const TreeView &tv, uint4 s0, uint4 s1, bool desc,
-                    const Box6 &par, const float *ps, Box6 &b)
-{
-  if (F == NODES_FP32)
-  {
-    b.lo[0] = bits_f(s0.x), b.lo[1] = bits_f(s0.y), b.lo[2] = bits_f(s0.z);
-    b.hi[0] = bits_f(s1.x), b.hi[1] = bits_f(s1.y), b.hi[2] = bits_f(s1.z);
-  }
-  else
-  {
-    if (desc)
-      q16_decode(s0.x, s0.y, s0.z, par, ps, b);
-    else
-      q16_decode(s0.x, s0.y, s0.z, tv.scene, tv.ss, b);
-  }
-}
-
-template <int F>
-AXB_D void set_frame(const Box6 &b, Box6 &par, float *ps)
-{
-  if (F == NODES_Q16)
-  {
-    par = b;
-    ps[0] = q16_scale(b.lo[0], b.hi[0]);
-    ps[1] = q16_scale(b.lo[1], b.hi[1]);
-    ps[2] = q16_scale(b.lo[2], b.hi[2]);
-  }
-}
-
 // ------------------------------------------------------------ spatial
-template <int F, int G, int PK, int MODE>
-__global__ __launch_bounds__(256) void k_spatial(
-    TreeView tv, const float *__restrict__ preds, int64_t nq, float radius,
-    const uint32_t *__restrict__ order, int64_t *__restrict__ offsets,
-    int32_t *__restrict__ out, int32_t buffer)
+// Result sink of one row.  MODE_COUNT only counts; MODE_BUFFER keeps the
+// first `buffer` hits in the row's scratch slot; MODE_FILL writes the row of
+// the CRS.  Hits leave in 16-byte groups (one store per 4 hits).
+template <int MODE>
+struct RowWriter
 {
-  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nq)
-    return;
-  const int64_t q = order ? (int64_t)order[t] : t;
-  float cx, cy, cz, thr;
-  Box6 qb;
-  if (PK == PRED_BOXES)
-  {
-    const float *p = preds + 6 * q;
-    qb = {{p[0], p[1], p[2]}, {p[3], p[4], p[5]}};
-    cx = cy = cz = thr = 0.f;
-  }
-  else
-  {
-    const float *p = preds + (PK == PRED_SPHERES ? 4 : 3) * q;
-    cx = p[0], cy = p[1], cz = p[2];
-    thr = sqrt_threshold(PK == PRED_SPHERES ? p[3] : radius);
-  }
-  int64_t base = 0, limit = 0;
-  if (MODE == MODE_FILL)
-  {
-    base = offsets[q];
-    limit = offsets[q + 1] - base;
-  }
-  else if (MODE == MODE_BUFFER)
-    base = q * (int64_t)buffer;
+  int32_t *out;
+  int64_t base, limit; // FILL: row start and length; BUFFER: limit = buffer
   int64_t cnt = 0;
-  int32_t h0 = 0, h1 = 0, h2 = 0; // fill pass: hits not yet written
-  int np = 0;
+  int32_t h0 = 0, h1 = 0, h2 = 0;
 
-  Box6 par;
-  float ps[3];
-  bool desc = false;
-  uint32_t p = 0;
-  uint32_t v_int = 0, v_leaf = 0;
-  const uint32_t end = tv.total;
-  while (p < end)
+  AXB_D bool hit(int32_t v)
   {
-    const uint4 s0 = tv.slots[p];
-    const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
-    const int32_t w = (int32_t)s0.w;
-    if (w < 0)
+    if (MODE != MODE_COUNT && cnt < limit)
     {
-      ++v_leaf;
-      bool hit;
-      const float x = bits_f(s0.x), y = bits_f(s0.y), z = bits_f(s0.z);
-      if (G == GEOM_POINTS)
-      {
-        if (PK == PRED_BOXES)
-          hit = !(x > qb.hi[0] || x < qb.lo[0] || y > qb.hi[1] || y < qb.lo[1] ||
-                  z > qb.hi[2] || z < qb.lo[2]);
-        else
-          hit = dist2_pp(cx, cy, cz, x, y, z) <= thr;
-      }
+      const int j = (int)(cnt & 3);
+      if (j == 3)
+        *(i32x4_u *)(out + base + cnt - 3) = (i32x4_u){h0, h1, h2, v};
       else
       {
-        const Box6 lb = {{x, y, z}, {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
-        hit = PK == PRED_BOXES ? intersects_bb(qb, lb) : dist2_pbox(cx, cy, cz, lb) <= thr;
-      }
-      if (hit)
-      {
-        if (MODE == MODE_FILL)
-        {
-          // results leave in 16-byte groups (one store per 4 hits)
-          if (np == 3)
-          {
-            if (cnt < limit)
-              *(i32x4_u *)(out + base + cnt - 3) = (i32x4_u){h0, h1, h2, ~w};
-            np = 0;
-          }
-          else
-          {
-            h0 = np == 0 ? ~w : h0;
-            h1 = np == 1 ? ~w : h1;
-            h2 = np == 2 ? ~w : h2;
-            ++np;
-          }
-        }
-        else if (MODE == MODE_BUFFER && cnt < buffer)
-          out[base + cnt] = ~w;
-        ++cnt;
-      }
-      p += SL;
-      desc = false;
-    }
-    else
-    {
-      ++v_int;
-      Box6 b;
-      node_box<F>(tv, s0, s1, desc, par, ps, b);
-      const bool pass = PK == PRED_BOXES ? intersects_bb(qb, b)
-                                         : dist2_pbox(cx, cy, cz, b) <= thr;
-      if (pass)
-      {
-        set_frame<F>(b, par, ps);
-        desc = true;
-        p += SI;
-      }
-      else
-      {
-        if (w == 0)
-        {
-          raise_error(tv.err, KERR_LAYOUT);
-          break;
-        }
-        desc = false;
-        p += (uint32_t)w;
+        h0 = j == 0 ? v : h0;
+        h1 = j == 1 ? v : h1;
+        h2 = j == 2 ? v : h2;
       }
     }
+    ++cnt;
+    return true;
   }
-  if (MODE == MODE_FILL && np > 0 && cnt <= limit)
+  // Writes the last partial group of the kept hits.
+  AXB_D void finish()
   {
-    out[base + cnt - np] = h0;
-    if (np > 1)
-      out[base + cnt - np + 1] = h1;
-    if (np > 2)
-      out[base + cnt - np + 2] = h2;
+    if (MODE == MODE_COUNT)
+      return;
+    const int64_t kept = cnt < limit ? cnt : limit;
+    const int rem = (int)(kept & 3);
+    if (rem > 0)
+    {
+      const int64_t at = base + kept - rem;
+      out[at] = h0;
+      if (rem > 1)
+        out[at + 1] = h1;
+      if (rem > 2)
+        out[at + 2] = h2;
+    }
   }
+};
+
+template <int MODE>
+AXB_D void row_epilogue(const TreeView &tv, RowWriter<MODE> &rw, int64_t *offsets, int64_t q,
+                        uint32_t v_int, uint32_t v_leaf)
+{
+  rw.finish();
   if (MODE != MODE_FILL)
-    offsets[q] = cnt;
-  else if (cnt != limit)
+    offsets[q] = rw.cnt;
+  else if (rw.cnt != rw.limit)
     raise_error(tv.err, KERR_LAYOUT);
   if (tv.visits && MODE != MODE_FILL)
   {
     atomicAdd(tv.visits + 0, 1ull);
     atomicAdd(tv.visits + 1, (unsigned long long)v_int);
     atomicAdd(tv.visits + 2, (unsigned long long)v_leaf);
-    atomicAdd(tv.visits + 3, (unsigned long long)cnt);
+    atomicAdd(tv.visits + 3, (unsigned long long)rw.cnt);
   }
+}
+
+// One lane per predicate (in `order` when given): rope traversal of the
+// whole tree; row q of the CRS (counts, scratch rows or the final rows).
+template <int F, int G, int PK, int MODE>
+__global__ __launch_bounds__(256) void k_spatial(
+    TreeView tv, const float *__restrict__ preds, int64_t nq, float radius,
+    const uint32_t *__restrict__ order, int64_t *__restrict__ offsets,
+    int32_t *__restrict__ out, int32_t buffer)
+{
+  const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= nq)
+    return;
+  const int64_t q = order ? (int64_t)order[t] : t;
+  SpatialPred pr;
+  if (PK == PRED_BOXES)
+  {
+    const float *p = preds + 6 * q;
+    pr = box_pred(Box6{{p[0], p[1], p[2]}, {p[3], p[4], p[5]}});
+  }
+  else
+  {
+    const float *p = preds + (PK == PRED_SPHERES ? 4 : 3) * q;
+    pr = sphere_pred(p[0], p[1], p[2], PK == PRED_SPHERES ? p[3] : radius);
+  }
+  RowWriter<MODE> rw;
+  rw.out = out;
+  rw.base = MODE == MODE_FILL ? offsets[q] : (MODE == MODE_BUFFER ? q * (int64_t)buffer : 0);
+  rw.limit = MODE == MODE_FILL ? offsets[q + 1] - rw.base : (int64_t)buffer;
+  uint32_t v_int = 0, v_leaf = 0;
+  rope_walk<F, G, PK>(tv, pr, 0u, tv.total, ~0u, rw, v_int, v_leaf);
+  row_epilogue(tv, rw, offsets, q, v_int, v_leaf);
+}
+
+// Neighbour lists over a point tree (NeighborList.hpp:49-188 semantics):
+// lane t takes the leaf at position L = list ? list[t] : t and the sphere
+// {its point, radius}; row ~w (its value) receives the values of the other
+// leaves in the sphere -- all of them (FULL) or only those whose leaf comes
+// before L (half list: HalfTraversal.hpp pairs (earlier, later) land in the
+// later leaf's row).  The half walk covers the slots before L's record,
+// which every ancestor of L precedes and passes (distance 0).  rowof[L] is
+// written in the first pass (leaf position -> row) for the overflow list.
+template <int F, bool FULL, int MODE>
+__global__ __launch_bounds__(256) void k_nlist(TreeView tv, float radius, int64_t nl,
+                                               const uint32_t *__restrict__ list,
+                                               int64_t *__restrict__ offsets,
+                                               int32_t *__restrict__ out, int32_t buffer,
+                                               uint32_t *__restrict__ rowof)
+{
+  constexpr uint32_t SL = leaf_slots<GEOM_POINTS>(), SI = internal_slots<F>();
+  const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= nl)
+    return;
+  const uint32_t L = list ? list[t] : (uint32_t)t;
+  const uint32_t self = SL * L + SI * (uint32_t)tv.C[L];
+  const uint4 s = tv.slots[self];
+  const int64_t q = ~(int32_t)s.w;
+  if (rowof)
+    rowof[L] = (uint32_t)q;
+  const SpatialPred pr = sphere_pred(bits_f(s.x), bits_f(s.y), bits_f(s.z), radius);
+  RowWriter<MODE> rw;
+  rw.out = out;
+  rw.base = MODE == MODE_FILL ? offsets[q] : (MODE == MODE_BUFFER ? q * (int64_t)buffer : 0);
+  rw.limit = MODE == MODE_FILL ? offsets[q + 1] - rw.base : (int64_t)buffer;
+  uint32_t v_int = 0, v_leaf = 0;
+  rope_walk<F, GEOM_POINTS, PRED_SPHERES>(tv, pr, 0u, FULL ? tv.total : self, self, rw, v_int,
+                                          v_leaf);
+  row_epilogue(tv, rw, offsets, q, v_int, v_leaf);
 }
 
 // ------------------------------------------------------------ nearest
@@ -270,6 +235,41 @@ struct RegList
         if (od)
           od[j] = bits_f((uint32_t)(a[j] >> 32));
       }
+  }
+};
+
+// The K smallest distances seen (float bits, ascending; no indices): the
+// seed only needs the k-th distance, in half the registers of RegList.
+template <int K>
+struct DistList
+{
+  uint32_t a[K];
+  AXB_D void init()
+  {
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      a[j] = ~0u;
+  }
+  AXB_D void insert(float d)
+  {
+    const uint32_t x = f_bits(d); // d >= 0: bit order = value order
+#pragma unroll
+    for (int j = K - 1; j > 0; --j)
+    {
+      const uint32_t hi = a[j - 1] < x ? x : a[j - 1];
+      a[j] = hi < a[j] ? hi : a[j];
+    }
+    a[0] = x < a[0] ? x : a[0];
+  }
+  // k-th smallest (1 <= k <= K); +inf bits while fewer than k were inserted
+  AXB_D float kth(int k) const
+  {
+    uint32_t v = a[0];
+#pragma unroll
+    for (int j = 1; j < K; ++j)
+      if (j == k - 1)
+        v = a[j];
+    return v == ~0u ? INFINITY : bits_f(v);
   }
 };
 
@@ -726,8 +726,8 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
   }
   int j0 = first - (W - 1) / 2;
   j0 = j0 < 0 ? 0 : (j0 > n - W ? n - W : j0);
-  RegList<K> L;
-  L.init(k);
+  DistList<K> L;
+  L.init();
   uint32_t s = SL * (uint32_t)j0 + SI * (uint32_t)tv.C[j0];
   int got = 0;
   while (got < W)
@@ -742,10 +742,7 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
     {
       ++v_leaf;
       const uint4 s1 = SL == 2 ? tv.slots[s + 1] : s0;
-      const float dist = sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1));
-      const int32_t idx = ~(int32_t)s0.w;
-      if (L.accepts(dist, idx))
-        L.insert(dist, idx);
+      L.insert(sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1)));
       ++got;
       s += SL;
     }
@@ -755,7 +752,7 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
       s += SI;
     }
   }
-  return L.full() ? L.kth_d : INFINITY;
+  return L.kth(k);
 }
 
 // Fixed-radius rope walk appending every leaf with d2 <= thr to `row`:
@@ -832,20 +829,24 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
   return c;
 }
 
-// One lane per query: seed radius, then up to KNN_ROUNDS rounds of
-// collect + exact select.  When a round overflows its row, the k-th
-// distance of the candidates it did keep (k real points) is the next,
-// smaller radius.  Queries still overflowing go to the fallback list.
-constexpr int KNN_ROUNDS = 3;
+// One lane per query: seed radius, then up to KNN_ROUNDS collect walks until
+// one finds between k and cap leaves within the radius (then every point
+// within the true k-th distance is in the row), followed by the exact
+// top-k selection of that row.  On overflow (the walk stops at cap + 1) the
+// radius is an upper bound and the next one shrinks by the volume ratio
+// (2k / (cap + 1))^(1/3); with fewer than k leaves the radius was below the
+// k-th distance and the next one bisects towards the upper bound.  Queries
+// still unresolved go to the exact fallback.
+constexpr int KNN_ROUNDS = 6;
 
 template <int F, int G, int K>
 __global__ __launch_bounds__(256) void k_knn_collect(
     TreeView tv, const float *__restrict__ pts, int64_t t0, int64_t tn, int k, int W,
-    const uint32_t *__restrict__ order, uint2 *__restrict__ cand, int cap,
+    const uint32_t *__restrict__ order, uint2 *__restrict__ cand, int cap, float shrink,
     uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_n, int32_t *__restrict__ oi,
     float *__restrict__ od)
 {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t g = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= tn)
     return;
   const int64_t t = t0 + g;
@@ -858,13 +859,39 @@ __global__ __launch_bounds__(256) void k_knn_collect(
   uint2 *row = cand + g * cap;
   int c = 0;
   bool done = false;
-  for (int round = 0; round < KNN_ROUNDS && !done; ++round)
+  int rounds = 0;
+  float R_hi = R;   // radius known to hold >= k points
+  float R_lo = 0.f; // radius known to hold < k points
+  for (int round = 0; round < KNN_ROUNDS; ++round)
   {
+    ++rounds;
     c = knn_collect_walk<F, G>(tv, qx, qy, qz, sqrt_threshold(R), X, row, cap, v_int, v_leaf);
+    if (c >= m && c <= cap)
+    {
+      done = true;
+      break;
+    }
+    if (c < m)
+    {
+      R_lo = R;
+      R = 0.5f * (R_lo + R_hi);
+      if (!(R > R_lo && R < R_hi))
+        R = R_hi;
+    }
+    else
+    {
+      // c == cap + 1: shrink towards 2m leaves, staying above R_lo
+      R_hi = R;
+      float next = R * shrink;
+      R = next > R_lo ? next : 0.5f * (R_lo + R_hi);
+    }
+  }
+  if (done)
+  {
+    // exact top-k of the row (every leaf within the k-th distance is in it)
     RegList<K> L;
     L.init(k);
-    const int kept = c < cap ? c : cap;
-    for (int j = 0; j < kept; ++j)
+    for (int j = 0; j < c; ++j)
     {
       const uint2 e = row[j];
       const float dist = sqrt_rn(bits_f(e.x));
@@ -872,15 +899,7 @@ __global__ __launch_bounds__(256) void k_knn_collect(
       if (L.accepts(dist, idx))
         L.insert(dist, idx);
     }
-    if (c <= cap)
-    {
-      L.write(oi + q * m, od ? od + q * m : nullptr, m);
-      done = true;
-    }
-    else
-    {
-      R = L.kth_d; // kept >= cap >= k entries, so the list is full
-    }
+    L.write(oi + q * m, od ? od + q * m : nullptr, m);
   }
   if (!done)
     ovf[atomicAdd(ovf_n, 1u)] = (uint32_t)t;
@@ -890,6 +909,9 @@ __global__ __launch_bounds__(256) void k_knn_collect(
     atomicAdd(tv.visits + 1, (unsigned long long)v_int);
     atomicAdd(tv.visits + 2, (unsigned long long)v_leaf);
     atomicAdd(tv.visits + 3, (unsigned long long)c);
+    atomicAdd(tv.visits + 8, (unsigned long long)rounds);
+    if (rounds > 1)
+      atomicAdd(tv.visits + 9, 1ull);
   }
 }
 
@@ -939,7 +961,7 @@ __global__ __launch_bounds__(256) void k_knn_heap(TreeView tv,
   __shared__ float s_d2[KNN_BUF * 256];
   __shared__ int32_t s_id[KNN_BUF * 256];
   CandBuf B{s_d2, s_id, 256, (int)threadIdx.x, 0};
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= nq)
     return;
   const int64_t q = order ? (int64_t)order[t] : t;
@@ -1007,27 +1029,118 @@ __global__ __launch_bounds__(256) void k_pred_morton32(
       atomicAdd(&hist[j], s_h[j]);
 }
 
-// Buffered first pass -> CRS: row q holds min(count, buffer) results at
-// buf + q * buffer (CrsGraphWrapperImpl.hpp:296-318, copy_valid_values).
+// Buffered first pass -> CRS: row q's results sit at buf + q * buffer when
+// they fit (CrsGraphWrapperImpl.hpp:296-318, copy_valid_values); rows that
+// did not fit are written by a second traversal of those queries.
+// Scratch rows that fit -> their CRS rows: 16 lanes per row, 16 bytes per
+// lane per step (a 64-entry row is one step).
 __global__ void k_compact(const int32_t *__restrict__ buf, int64_t nq, int32_t buffer,
                           const int64_t *__restrict__ offsets, int32_t *__restrict__ out)
 {
-  const int64_t q = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
   if (q >= nq)
     return;
-  const int lane = threadIdx.x & 63;
+  const int l = threadIdx.x & 15;
   const int64_t o = offsets[q];
   const int64_t cnt = offsets[q + 1] - o;
-  for (int64_t j = lane; j < cnt; j += 64)
-    out[o + j] = buf[q * buffer + j];
+  if (cnt > buffer)
+    return;
+  const int32_t *src = buf + q * (int64_t)buffer;
+  for (int64_t j = 4 * l; j < cnt; j += 64)
+  {
+    if (j + 4 <= cnt)
+      *(i32x4_u *)(out + o + j) = *(const i32x4_u *)(src + j);
+    else
+      for (int64_t u = j; u < cnt; ++u)
+        out[o + u] = src[u];
+  }
 }
 
-__global__ void k_max_count(const int64_t *__restrict__ counts, int64_t nq, int32_t buffer,
-                            uint32_t *flag)
+// Rows whose result count exceeds the buffer, listed in processing
+// (Morton) order: one atomic per wavefront, lanes in order behind it.  The
+// list holds the rows, or the processing indices when emit_index is set.
+__global__ void k_overflow_list(const int64_t *__restrict__ offsets,
+                                const uint32_t *__restrict__ order, int64_t nq, int32_t buffer,
+                                int emit_index, uint32_t *__restrict__ ovf,
+                                uint32_t *__restrict__ ovf_n)
 {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nq && counts[i] > buffer)
-    *flag = 1u;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint32_t q = 0;
+  bool f = false;
+  if (t < nq)
+  {
+    q = order ? order[t] : (uint32_t)t;
+    f = offsets[q + 1] - offsets[q] > buffer;
+  }
+  const uint64_t m = __ballot(f);
+  if (m == 0)
+    return;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader)
+    base = atomicAdd(ovf_n, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  if (f)
+    ovf[base + __popcll(m & lt)] = emit_index ? (uint32_t)t : q;
+}
+
+// ------------------------------------------------------------ half -> full
+// ExpandHalfToFull.hpp:23-68.  Counts: row i gains its own entries and one
+// for each row that lists it.  Entries outside [0, n) raise KERR_LAYOUT.
+__global__ void k_h2f_count(const int64_t *__restrict__ offsets,
+                            const int32_t *__restrict__ indices, int64_t n,
+                            int64_t *__restrict__ cnt, uint32_t *err)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const int64_t b = offsets[i], e = offsets[i + 1];
+  if (e > b)
+    atomicAdd((unsigned long long *)(cnt + i), (unsigned long long)(e - b));
+  for (int64_t p = b; p < e; ++p)
+  {
+    const int32_t k = indices[p];
+    if (k < 0 || k >= n)
+    {
+      raise_error(err, KERR_LAYOUT);
+      continue;
+    }
+    atomicAdd((unsigned long long *)(cnt + k), 1ull);
+  }
+}
+
+__global__ void k_h2f_cursor(const int64_t *__restrict__ offsets,
+                             const int64_t *__restrict__ out_offsets, int64_t n,
+                             int64_t *__restrict__ cursor)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    cursor[i] = out_offsets[i] + (offsets[i + 1] - offsets[i]);
+}
+
+// Row i of the full list: its own entries first (in order), then the
+// mirrored ones, appended through the row's cursor.
+__global__ void k_h2f_fill(const int64_t *__restrict__ offsets,
+                           const int32_t *__restrict__ indices, int64_t n,
+                           const int64_t *__restrict__ out_offsets, int64_t *__restrict__ cursor,
+                           int32_t *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const int64_t b = offsets[i], e = offsets[i + 1], o = out_offsets[i];
+  for (int64_t p = b; p < e; ++p)
+  {
+    const int32_t k = indices[p];
+    if (k < 0 || k >= n)
+      continue;
+    out[o + (p - b)] = k;
+    const int64_t at = (int64_t)atomicAdd((unsigned long long *)(cursor + k), 1ull);
+    if (at < out_offsets[k + 1])
+      out[at] = (int32_t)i;
+  }
 }
 
 } // namespace
@@ -1095,6 +1208,9 @@ static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq,
   const int n = tv.n;
   const int W = std::min(n, std::max(4 * k, 16));
   const int cap = 8 * k > 64 ? 8 * k : 64;
+  // radius factor after an overflowing round (cap + 1 leaves within R):
+  // aim at 2m leaves, m = min(k, n)
+  const float shrink = std::cbrt((float)(2 * std::min(k, n)) / (float)(cap + 1));
   const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t(1) << 31) / (cap * 8)));
   uint2 *cand = (uint2 *)dalloc((size_t)batch * cap * sizeof(uint2), s);
   uint32_t *ovf = (uint32_t *)dalloc((size_t)batch * sizeof(uint32_t), s);
@@ -1107,7 +1223,7 @@ static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq,
     if (e != hipSuccess)
       break;
     AXB_LAUNCH((k_knn_collect<F, G, K>), dim3(grid_of(tn)), dim3(256), 0, s, tv, pts, t0, tn, k,
-               W, order, cand, cap, ovf, ovf_n, oi, od);
+               W, order, cand, cap, shrink, ovf, ovf_n, oi, od);
     AXB_LAUNCH((k_knn_fallback<F, G, K>), dim3((unsigned)std::min<int64_t>(grid_of(tn), 512)),
                dim3(256), 0, s, tv, pts, k, order, ovf, ovf_n, oi, od);
     e = hipGetLastError();
@@ -1185,17 +1301,74 @@ hipError_t launch_compact(const int32_t *buf, int64_t nq, int32_t buffer,
 {
   if (nq <= 0)
     return hipSuccess;
-  AXB_LAUNCH(k_compact, dim3((unsigned)div_up(nq, 4)), dim3(256), 0, s, buf, nq, buffer,
-                     offsets, out);
+  AXB_LAUNCH(k_compact, dim3((unsigned)div_up(nq, 16)), dim3(256), 0, s, buf, nq, buffer,
+             offsets, out);
   return hipGetLastError();
 }
 
-hipError_t launch_max_count(const int64_t *counts, int64_t nq, int32_t buffer, uint32_t *flag,
-                            hipStream_t s)
+hipError_t launch_overflow_list(const int64_t *offsets, const uint32_t *order, int64_t nq,
+                                int32_t buffer, bool emit_index, uint32_t *ovf,
+                                uint32_t *ovf_n, hipStream_t s)
 {
   if (nq <= 0)
     return hipSuccess;
-  AXB_LAUNCH(k_max_count, dim3(grid_of(nq)), dim3(256), 0, s, counts, nq, buffer, flag);
+  AXB_LAUNCH(k_overflow_list, dim3(grid_of(nq)), dim3(256), 0, s, offsets, order, nq, buffer,
+             emit_index ? 1 : 0, ovf, ovf_n);
+  return hipGetLastError();
+}
+
+template <int F, bool FULL>
+static hipError_t launch_nlist_f(const TreeView &tv, float radius, int64_t nl,
+                                 const uint32_t *list, int64_t *offsets, int32_t *out,
+                                 int32_t buffer, uint32_t *rowof, int mode, hipStream_t s)
+{
+  if (mode == MODE_COUNT)
+    AXB_LAUNCH((k_nlist<F, FULL, MODE_COUNT>), dim3(grid_of(nl)), dim3(256), 0, s, tv, radius,
+               nl, list, offsets, out, buffer, rowof);
+  else if (mode == MODE_FILL)
+    AXB_LAUNCH((k_nlist<F, FULL, MODE_FILL>), dim3(grid_of(nl)), dim3(256), 0, s, tv, radius,
+               nl, list, offsets, out, buffer, rowof);
+  else
+    AXB_LAUNCH((k_nlist<F, FULL, MODE_BUFFER>), dim3(grid_of(nl)), dim3(256), 0, s, tv, radius,
+               nl, list, offsets, out, buffer, rowof);
+  return hipGetLastError();
+}
+
+hipError_t launch_nlist(const TreeView &tv, int node_format, bool full, float radius, int64_t nl,
+                        const uint32_t *list, int64_t *offsets, int32_t *out, int32_t buffer,
+                        uint32_t *rowof, int mode, hipStream_t s)
+{
+  if (nl <= 0)
+    return hipSuccess;
+  if (node_format == NODES_Q16)
+    return full ? launch_nlist_f<NODES_Q16, true>(tv, radius, nl, list, offsets, out, buffer,
+                                                  rowof, mode, s)
+                : launch_nlist_f<NODES_Q16, false>(tv, radius, nl, list, offsets, out, buffer,
+                                                   rowof, mode, s);
+  return full ? launch_nlist_f<NODES_FP32, true>(tv, radius, nl, list, offsets, out, buffer,
+                                                 rowof, mode, s)
+              : launch_nlist_f<NODES_FP32, false>(tv, radius, nl, list, offsets, out, buffer,
+                                                  rowof, mode, s);
+}
+
+hipError_t launch_h2f_count(const int64_t *offsets, const int32_t *indices, int64_t n,
+                            int64_t *cnt, uint32_t *err, hipStream_t s)
+{
+  if (n <= 0)
+    return hipSuccess;
+  AXB_LAUNCH(k_h2f_count, dim3(grid_of(n)), dim3(256), 0, s, offsets, indices, n, cnt, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_h2f_fill(const int64_t *offsets, const int32_t *indices, int64_t n,
+                           const int64_t *out_offsets, int64_t *cursor, int32_t *out,
+                           hipStream_t s)
+{
+  if (n <= 0)
+    return hipSuccess;
+  AXB_LAUNCH(k_h2f_cursor, dim3(grid_of(n)), dim3(256), 0, s, offsets, out_offsets, n, cursor);
+  AXB_LAUNCH(k_h2f_fill, dim3(grid_of(n)), dim3(256), 0, s, offsets, indices, n, out_offsets,
+             cursor, out);
   return hipGetLastError();
 }
 

@@ -35,6 +35,13 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n)
   for (; i < n; i += stride)
     p[i] = v;
 }
+__global__ void k_iota_u32(uint32_t *p, int64_t n)
+{
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride)
+    p[i] = (uint32_t)i;
+}
 __global__ void k_fill_u64(uint64_t *p, uint64_t v, int64_t n)
 {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -496,6 +503,13 @@ hipError_t fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t stream)
     return hipSuccess;
   AXB_LAUNCH(k_fill_u32, dim3(fill_grid(n)), dim3(256), 0, stream, p,
                      v, n);
+  return hipGetLastError();
+}
+hipError_t launch_iota_u32(uint32_t *p, int64_t n, hipStream_t stream)
+{
+  if (n <= 0)
+    return hipSuccess;
+  AXB_LAUNCH(k_iota_u32, dim3(fill_grid(n)), dim3(256), 0, stream, p, n);
   return hipGetLastError();
 }
 hipError_t fill_u64(uint64_t *p, uint64_t v, int64_t n, hipStream_t stream)

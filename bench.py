@@ -47,12 +47,205 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--box-n", type=int, default=5_000_000,
+                    help="data and query boxes of the box-primitive case (0 = skip)")
+    ap.add_argument("--fcc-cells", default="80,80,78",
+                    help="FCC cubic cells of the pair-energy case ('' = skip)")
+    ap.add_argument("--e2e-steps", type=int, default=0,
+                    help="steps of the moving-points loop (0 = skip)")
+    ap.add_argument("--e2e-n", type=int, default=100_000_000)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain device copy of this size first, as a known-size "
+                         "kernel for scaling hardware byte counters")
     ap.add_argument("--json-out", default=None)
     return ap.parse_args()
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def calibrate_radius(ax, p, r, target, sample=100, iters=4):
+    """Radius with about `target` neighbours per point (mean over every
+    `sample`-th point), by the cube-root rule on measured counts (untimed)."""
+    bvh = ax.BoundingVolumeHierarchy(p)
+    q = p[::sample].contiguous()
+    for _ in range(iters):
+        _, off = bvh.query(ax.make_intersects(q, r))
+        mean = float(off[-1].item()) / q.shape[0]
+        r = r * (target / max(mean, 1e-3)) ** (1.0 / 3.0)
+    import numpy as np
+    return float(np.float32(r))
+
+
+def timed(fn, steps, warmup, stream):
+    """Mean ms of fn() over `steps` runs after `warmup`, from HIP events on
+    the current stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record(stream)
+    for s in range(steps):
+        fn()
+        ev[s + 1].record(stream)
+    torch.cuda.synchronize()
+    return ev[0].elapsed_time(ev[-1]) / steps
+
+
+def kernel_ms(ax, fn, match):
+    """Total ms of the library's kernels whose name contains one of `match`
+    during one fn() call (HIP events around each launch)."""
+    with ax.profile(timing=True, visits=False) as prof:
+        fn()
+    return sum(sum(v) for k, v in prof.kernels.items() if any(m in k for m in match))
+
+
+def bench_boxes(ax, args, torch, synthetic, stream):
+    """5M lognormal boxes, 5M query boxes drawn the same way: intersects()
+    CRS with FP32 and Q16 internal nodes; identical results are checked;
+    bytes per query from the node records the traversal reads."""
+    n = args.box_n
+    data = torch.from_numpy(synthetic.lognormal_boxes(n, 4)).cuda()
+    qb = torch.from_numpy(synthetic.lognormal_boxes(n, 5)).cuda()
+    pred = ax.intersects(ax.Boxes(qb))
+    out, ref = {}, None
+    for fmt in ("fp32", "q16"):
+        tb = timed(lambda: ax.BoundingVolumeHierarchy(data, node_format=fmt), args.steps,
+                   args.warmup, stream)
+        bvh = ax.BoundingVolumeHierarchy(data, node_format=fmt)
+        tq = timed(lambda: bvh.query(pred), args.steps, args.warmup, stream)
+        trav = kernel_ms(ax, lambda: bvh.query(pred), ("k_spatial",))
+        idx, off = bvh.query(pred)
+        if ref is None:
+            ref = (idx, off)
+        identical = bool(torch.equal(off, ref[1]) and torch.equal(idx, ref[0]))
+        with ax.profile(timing=False, visits=True) as pv:
+            bvh.query(pred)
+        v = pv.visits
+        s_int = 16 if fmt == "q16" else 32
+        # query box 24 B + order 4 B + count 8 B; node records (box leaf 32 B);
+        # 4 B per result
+        alg = v["queries"] * 36 + v["internal"] * s_int + v["leaf"] * 32 + v["results"] * 4
+        out[fmt] = dict(build_ms=tb, query_ms=tq, traversal_kernel_ms=trav,
+                        query_qps=n / tq * 1e3, hits_per_query=v["results"] / n,
+                        internal_visits_per_query=v["internal"] / n,
+                        leaf_visits_per_query=v["leaf"] / n,
+                        bytes_per_query=alg / n, traversal_GBps=alg / (trav * 1e-3) / 1e9,
+                        identical_to_fp32=identical)
+        log(f"[bench] boxes/{fmt}: build {tb:.2f} ms | query {tq:.2f} ms "
+            f"({n / tq / 1e3:.1f} Mq/s, {v['results'] / n:.3f} hits/q) | traversal "
+            f"{trav:.2f} ms, {alg / n:.0f} B/q, {out[fmt]['traversal_GBps']:.0f} GB/s | "
+            f"identical {identical}")
+    out["q16_traversal_speedup"] = out["fp32"]["traversal_kernel_ms"] / out["q16"]["traversal_kernel_ms"]
+    out["config"] = f"{n} data boxes (seed 4) and {n} query boxes (seed 5): centres U[0,1)^3, " \
+                    "sides lognormal(ln 0.002, 0.5)"
+    return out
+
+
+def bench_energy(ax, args, torch, synthetic, stream):
+    """FCC lattice (nearest-neighbour distance 1, jitter +-0.05), cutoff 2.5:
+    pair energy forward only vs fused forward + backward, and the gradient
+    against FP64 central differences on a 1000-atom subregion."""
+    import numpy as np
+    cells = tuple(int(c) for c in args.fcc_cells.split(","))
+    x_h = synthetic.fcc_lattice(cells, seed=6)
+    n = x_h.shape[0]
+    x = torch.from_numpy(x_h).cuda()
+    rc = 2.5
+    off, idx = ax.energy.neighbour_csr(x, rc)
+    nnz = int(off[-1].item())
+    t_csr = timed(lambda: ax.energy.neighbour_csr(x, rc), args.steps, args.warmup, stream)
+    t_f = timed(lambda: ax.pair_energy(x, off, idx, rc, want_grad=False), args.steps,
+                args.warmup, stream)
+    t_fb = timed(lambda: ax.pair_energy(x, off, idx, rc), args.steps, args.warmup, stream)
+    k_f = kernel_ms(ax, lambda: ax.pair_energy(x, off, idx, rc, want_grad=False),
+                    ("k_energy", "k_reduce"))
+    k_fb = kernel_ms(ax, lambda: ax.pair_energy(x, off, idx, rc), ("k_energy", "k_reduce"))
+
+    def autograd_step():
+        xr = x.detach().requires_grad_(True)
+        e = ax.PairEnergy.apply(xr, rc, off, idx, False)
+        e.backward()
+
+    def autograd_recompute():
+        xr = x.detach().requires_grad_(True)
+        e = ax.PairEnergy.apply(xr, rc, None, None, True)
+        e.backward()
+
+    t_ag = timed(autograd_step, args.steps, args.warmup, stream)
+    t_agr = timed(autograd_recompute, max(1, args.steps // 2), 1, stream)
+    e, g = ax.pair_energy(x, off, idx, rc)
+    # 1000 atoms nearest the lattice centre
+    centre = x_h.mean(axis=0)
+    atoms = np.argsort(((x_h - centre) ** 2).sum(1), kind="stable")[:1000].astype(np.int32)
+    fd = ax.pair_energy_fd(x, off, idx, rc, atoms, h=1e-6).cpu().numpy()
+    ga = g.cpu().numpy().astype(np.float64)[atoms]
+    rel = float(np.linalg.norm(ga - fd) / np.linalg.norm(fd))
+    alg_f = n * 12 + nnz * 16 + (n + 1) * 8
+    out = dict(n_points=n, cells=list(cells), cutoff=rc, mean_neighbours=nnz / n - 1,
+               energy=float(e.item()), csr_ms=t_csr, forward_ms=t_f, fwd_bwd_ms=t_fb,
+               fwd_bwd_over_forward=t_fb / t_f, forward_kernel_ms=k_f, fwd_bwd_kernel_ms=k_fb,
+               forward_GBps=alg_f / (k_f * 1e-3) / 1e9,
+               fwd_bwd_GBps=(alg_f + n * 12) / (k_fb * 1e-3) / 1e9,
+               autograd_step_ms=t_ag, autograd_recompute_csr_step_ms=t_agr,
+               grad_rel_err_vs_fp64_fd=rel, fd_atoms=1000, fd_h=1e-6)
+    log(f"[bench] energy: N={n}, {nnz / n - 1:.1f} nbrs | CSR {t_csr:.2f} ms | fwd {t_f:.3f} ms "
+        f"| fwd+bwd {t_fb:.3f} ms (x{t_fb / t_f:.2f}) | autograd {t_ag:.2f} ms, with CSR "
+        f"recompute {t_agr:.2f} ms | grad rel err vs FP64 FD {rel:.2e}")
+    return out
+
+
+def bench_e2e(ax, args, torch, synthetic, stream, fmt):
+    """Moving-points loop: each step displaces every point by N(0, 1e-4) per
+    axis, rebuilds the tree, runs kNN k=32 and the pair energy forward +
+    backward with the CRS recomputed.  The energy sees the coordinates scaled
+    by 2.5 / r32 (r32 = radius of about 32 neighbours), i.e. LJ with
+    sigma = r32 / 2.5 and cutoff 2.5 sigma, so that pair terms stay finite."""
+    n = args.e2e_n
+    x = torch.from_numpy(synthetic.uniform_points(n, 7)).cuda()
+    r32 = float(synthetic.radius_for_neighbours(n, 32))
+    rc = 2.5
+    scale = rc / r32
+    gen = torch.Generator(device="cuda")
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    base = torch.cuda.memory_allocated()
+    tb = tk = te = 0.0
+    finite = True
+    t0 = time.perf_counter()
+    for s in range(args.e2e_steps):
+        gen.manual_seed(8 + s)
+        x.add_(torch.randn(x.shape, generator=gen, device="cuda") * 1e-4)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record(stream)
+        bvh = ax.BoundingVolumeHierarchy(x, node_format=fmt)
+        ev[1].record(stream)
+        ki, ko = bvh.query(ax.make_nearest(x, 32))
+        del ki, ko
+        ev[2].record(stream)
+        del bvh
+        xr = x.detach().requires_grad_(True)
+        e = ax.PairEnergy.apply(xr * scale, rc, None, None, True)
+        e.backward()
+        ev[3].record(stream)
+        torch.cuda.synchronize()
+        finite = finite and bool(torch.isfinite(e).item())
+        tb += ev[0].elapsed_time(ev[1])
+        tk += ev[1].elapsed_time(ev[2])
+        te += ev[2].elapsed_time(ev[3])
+        del xr, e
+        if s % 10 == 0:
+            log(f"[bench] e2e/{fmt} step {s}: {time.perf_counter() - t0:.1f}s")
+    k = args.e2e_steps
+    peak = torch.cuda.max_memory_allocated() - base
+    out = dict(n_points=n, steps=k, cutoff=r32, build_ms=tb / k, knn_ms=tk / k,
+               energy_fwd_bwd_ms=te / k, step_ms=(tb + tk + te) / k,
+               peak_hbm_GB=peak / 1e9, energy_finite=finite)
+    log(f"[bench] e2e/{fmt}: build {tb / k:.1f} ms | kNN k=32 {tk / k:.1f} ms | energy fwd+bwd "
+        f"(CRS recomputed) {te / k:.1f} ms | peak HBM {peak / 1e9:.1f} GB")
+    return out
 
 
 def main():
@@ -67,6 +260,17 @@ def main():
     import arborx_amd as ax
     from arborx_amd import synthetic
 
+    if args.copy_calibration_mib > 0:
+        words = args.copy_calibration_mib * (1 << 18)
+        src = torch.ones(words, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: {args.copy_calibration_mib} MiB read + "
+            f"{args.copy_calibration_mib} MiB written")
+        del src, dst
+
     N, K = args.n, args.k
     r = float(np.float32(synthetic.radius_for_neighbours(N, args.neighbours)))
     gens = {
@@ -76,20 +280,23 @@ def main():
     datasets = [d for d in args.datasets.split(",") if d]
     formats = [f for f in args.formats.split(",") if f]
     pts = {}
+    radius = {}
     for d in datasets:
         t0 = time.time()
         pts[d] = torch.from_numpy(gens[d]()).cuda()
         log(f"[bench] generated {d} N={N} in {time.time() - t0:.1f}s")
+        radius[d] = r if d == "uniform" else calibrate_radius(ax, pts[d], r, args.neighbours)
+        log(f"[bench] {d}: radius {radius[d]:.7g} for ~{args.neighbours} neighbours")
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
 
-    def step(p, fmt, ev=None):
+    def step(p, fmt, ev=None, rq=r):
         if ev:
             ev[0].record(stream)
         bvh = ax.BoundingVolumeHierarchy(p, node_format=fmt)
         if ev:
             ev[1].record(stream)
-        idx, off = bvh.query(ax.make_intersects(p, r))
+        idx, off = bvh.query(ax.make_intersects(p, rq))
         if ev:
             ev[2].record(stream)
         ki, ko = bvh.query(ax.make_nearest(p, K))
@@ -102,7 +309,7 @@ def main():
         for fmt in formats:
             p = pts[d]
             for _ in range(args.warmup):
-                step(p, fmt)
+                step(p, fmt, rq=radius[d])
             torch.cuda.synchronize()
             evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)]
                    for _ in range(args.steps)]
@@ -110,7 +317,7 @@ def main():
             t0 = time.perf_counter()
             nnz = 0
             for s in range(args.steps):
-                nnz, knn_nnz = step(p, fmt, evs[s])
+                nnz, knn_nnz = step(p, fmt, evs[s], rq=radius[d])
             torch.cuda.synchronize()
             wall = (time.perf_counter() - t0) / args.steps
             tb = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
@@ -120,7 +327,7 @@ def main():
                        build_Mpts_s=N / tb / 1e3, radius_qps=N / tr * 1e3, knn_qps=N / tk * 1e3,
                        knn_plus_radius_qps=N / (tr + tk) * 1e3,
                        step_qps=N / (wall * 1e3) * 1e3, radius_results=nnz,
-                       mean_neighbours=nnz / N)
+                       mean_neighbours=nnz / N, radius=radius[d])
             results[f"{d}/{fmt}"] = res
             log(f"[bench] {d}/{fmt}: step {wall * 1e3:.2f} ms | build {tb:.2f} ms "
                 f"({res['build_Mpts_s']:.0f} Mpts/s) | radius {tr:.2f} ms "
@@ -137,7 +344,7 @@ def main():
         d, fmt = head_key.split("/")
         p = pts[d]
         with ax.profile(timing=True, visits=False) as prof:
-            step(p, fmt)
+            step(p, fmt, rq=radius[d])
         kernels = {k: dict(launches=len(v), total_ms=sum(v)) for k, v in prof.kernels.items()}
         dom = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])
         log("[bench] kernel times (one step): " +
@@ -165,6 +372,11 @@ def main():
                 # query 12 B + order 4 B + count 8 B; node records
                 return (vr["queries"] * 24 + vr["internal"] * s_int + vr["leaf"] * 16,
                         "radius query (count pass)")
+            if "MODE_BUFFER" in name:
+                # query 12 B + order 4 B + count 8 B; node records; 4 B per
+                # result kept in the query's scratch row
+                return (vr["queries"] * 24 + vr["internal"] * s_int + vr["leaf"] * 16
+                        + vr["results"] * 4, "radius query (buffered pass)")
             if "MODE_FILL" in name:
                 # query 12 B + order 4 B + offsets 16 B; node records; 4 B per result
                 return (vr["queries"] * 32 + vr["internal"] * s_int + vr["leaf"] * 16
@@ -215,6 +427,17 @@ def main():
                           "(CPU restatement of ArborX Serial/OpenMP)")
         log(f"[bench] cpu baseline: {cpu}")
 
+    extra = {}
+    if args.box_n > 0:
+        extra["boxes"] = bench_boxes(ax, args, torch, synthetic, stream)
+    if args.fcc_cells:
+        extra["energy"] = bench_energy(ax, args, torch, synthetic, stream)
+    if args.e2e_steps > 0:
+        del pts
+        torch.cuda.empty_cache()
+        extra["e2e"] = {fmt: bench_e2e(ax, args, torch, synthetic, stream, fmt)
+                        for fmt in ("q16", "fp32")}
+
     line = {
         "metric": METRIC,
         "value": round(head["step_qps"], 1),
@@ -231,12 +454,16 @@ def main():
         "config": {"workload": "10M uniform points in [0,1]^3 (and 10M 1000-component "
                                "Gaussian mixture): BVH build + radius (~32 nbrs) + kNN k=16, "
                                "queries = points",
-                   "n_points": N, "radius": r, "k": K, "headline": head_key},
+                   "n_points": N, "radius": radius, "k": K, "headline": head_key},
         "cases": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv)
                       for kk, vv in v.items()} for k, v in results.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    for key, v in extra.items():
+        line[key] = {kk: (round(vv, 4) if isinstance(vv, float) else
+                          ({a: (round(b, 4) if isinstance(b, float) else b) for a, b in vv.items()}
+                           if isinstance(vv, dict) else vv)) for kk, vv in v.items()}
     if kernels:
         line["kernels_one_step_ms"] = {k: round(v["total_ms"], 4) for k, v in kernels.items()}
     s = json.dumps(line)

@@ -24,11 +24,22 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cfloat>
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
+
+// Geometry helpers callable from device code when compiled by hipcc (the
+// callback traversals of ArborX_Callbacks.hpp); plain host functions under
+// a host compiler.
+#if defined(__HIPCC__)
+#define ARBORX_HD __host__ __device__ inline
+#else
+#define ARBORX_HD inline
+#endif
 
 namespace ArborX
 {
@@ -70,27 +81,40 @@ struct HipStream
 struct Point
 {
   float _coords[3];
-  float &operator[](int d) { return _coords[d]; }
-  float const &operator[](int d) const { return _coords[d]; }
+  ARBORX_HD float &operator[](int d) { return _coords[d]; }
+  ARBORX_HD float const &operator[](int d) const { return _coords[d]; }
 };
 struct Box
 {
   Point _min_corner{{FLT_MAX, FLT_MAX, FLT_MAX}};
   Point _max_corner{{-FLT_MAX, -FLT_MAX, -FLT_MAX}};
   Box() = default;
-  Box(Point const &lo, Point const &hi) : _min_corner(lo), _max_corner(hi) {}
-  Point &minCorner() { return _min_corner; }
-  Point const &minCorner() const { return _min_corner; }
-  Point &maxCorner() { return _max_corner; }
-  Point const &maxCorner() const { return _max_corner; }
+  ARBORX_HD Box(Point const &lo, Point const &hi) : _min_corner(lo), _max_corner(hi) {}
+  ARBORX_HD Point &minCorner() { return _min_corner; }
+  ARBORX_HD Point const &minCorner() const { return _min_corner; }
+  ARBORX_HD Point &maxCorner() { return _max_corner; }
+  ARBORX_HD Point const &maxCorner() const { return _max_corner; }
 };
 struct Sphere
 {
   Point _centroid;
   float _radius;
-  Point const &centroid() const { return _centroid; }
-  float radius() const { return _radius; }
+  ARBORX_HD Point const &centroid() const { return _centroid; }
+  ARBORX_HD float radius() const { return _radius; }
 };
+
+// distance(Point, Point) (geometry/algorithms/ArborX_Distance.hpp:52-70):
+// squared differences summed in axis order, then sqrt.
+ARBORX_HD float distance(Point const &a, Point const &b)
+{
+  float s = 0.f;
+  for (int d = 0; d < 3; ++d)
+  {
+    float t = a._coords[d] - b._coords[d];
+    s += t * t;
+  }
+  return std::sqrt(s);
+}
 static_assert(sizeof(Point) == 12 && sizeof(Box) == 24 && sizeof(Sphere) == 16,
               "facade geometry must match the C ABI layout");
 
@@ -247,6 +271,34 @@ inline Experimental::PointsNearestK nearest(DeviceVector<Point> const &p, int k 
   return {p.data(), p.size(), k};
 }
 
+template <class T>
+class DeviceVector;
+template <class T>
+struct is_device_vector : std::false_type
+{
+};
+template <class T>
+struct is_device_vector<DeviceVector<T>> : std::true_type
+{
+};
+
+#if defined(__HIPCC__)
+class BoundingVolumeHierarchy;
+namespace Details
+{
+struct DeviceTree;
+DeviceTree device_tree(axb_bvh const *h);
+template <class Batch, class Callback>
+void callback_query(BoundingVolumeHierarchy const &bvh, HipStream const &space, Batch const &b,
+                    Callback const &cb, Experimental::TraversalPolicy const &policy);
+template <class Batch, class Callback, class T>
+void callback_query_crs(BoundingVolumeHierarchy const &bvh, HipStream const &space,
+                        Batch const &b, Callback const &cb, DeviceVector<T> &values,
+                        DeviceVector<int64_t> &offsets,
+                        Experimental::TraversalPolicy const &policy);
+} // namespace Details
+#endif
+
 enum class NodeFormat
 {
   FP32 = AXB_NODES_FP32,
@@ -361,6 +413,32 @@ public:
 
   axb_bvh const *c_handle() const { return _h; }
 
+#if defined(__HIPCC__)
+  // query(space, predicates, callback[, policy]) (ArborX_LinearBVH.hpp:84-88):
+  // callback(predicate, value) per match; spatial callbacks may return
+  // CallbackTreeTraversalControl::early_exit.  Synchronises the stream.
+  template <class Predicates, class Callback,
+            std::enable_if_t<!is_device_vector<Callback>::value, int> = 0>
+  void query(HipStream const &space, Predicates const &predicates, Callback const &callback,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    Details::callback_query(*this, space, predicates, callback, policy);
+  }
+  // query(space, predicates, callback, values, offsets[, policy])
+  // (ArborX_LinearBVH.hpp:90-110): callback(predicate, value, out) calls
+  // out(x) for each element x of the predicate's row.
+  template <class Predicates, class Callback, class T>
+  void query(HipStream const &space, Predicates const &predicates, Callback const &callback,
+             DeviceVector<T> &values, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    Details::callback_query_crs(*this, space, predicates, callback, values, offsets, policy);
+  }
+  // Device view for query(PerThread, ...) inside the caller's kernels;
+  // valid while the tree lives.
+  Details::DeviceTree device_view() const;
+#endif
+
 private:
   static axb_traversal_policy c_policy(Experimental::TraversalPolicy const &p)
   {
@@ -420,6 +498,58 @@ private:
 
 using BVH = BoundingVolumeHierarchy;
 
+namespace Details
+{
+inline void neighbor_list(HipStream const &space, DeviceVector<Point> const &points,
+                          float radius, int32_t full, DeviceVector<int64_t> &offsets,
+                          DeviceVector<int> &indices)
+{
+  if (offsets.size() != points.size() + 1)
+    offsets.resize(points.size() + 1);
+  int32_t *ind = nullptr;
+  int64_t nnz = 0;
+  check(axb_find_neighbor_list((float const *)points.data(), (int64_t)points.size(), radius,
+                               full, offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
+  indices.adopt(ind, (size_t)nnz, space.stream);
+}
+
+// detail/ArborX_ExpandHalfToFull.hpp:23-68: replaces a half list by the
+// full one (offsets and indices are reallocated, as there).
+inline void expandHalfToFull(HipStream const &space, DeviceVector<int64_t> &offsets,
+                             DeviceVector<int> &indices)
+{
+  if (offsets.size() == 0)
+    return;
+  DeviceVector<int64_t> out(offsets.size());
+  int32_t *ind = nullptr;
+  int64_t nnz = 0;
+  check(axb_expand_half_to_full(offsets.data(), indices.data(), (int64_t)offsets.size() - 1,
+                                out.data(), &ind, &nnz, (axb_stream_t)space.stream));
+  offsets = std::move(out);
+  indices.adopt(ind, (size_t)nnz, space.stream);
+}
+} // namespace Details
+
+namespace Experimental
+{
+// detail/ArborX_NeighborList.hpp:105-188: row i lists every j != i within
+// `radius` of point i.
+inline void findFullNeighborList(HipStream const &space, DeviceVector<Point> const &points,
+                                 float radius, DeviceVector<int64_t> &offsets,
+                                 DeviceVector<int> &indices)
+{
+  Details::neighbor_list(space, points, radius, 1, offsets, indices);
+}
+// detail/ArborX_NeighborList.hpp:49-103: each pair once, in the row of the
+// point whose leaf comes later in the tree.
+inline void findHalfNeighborList(HipStream const &space, DeviceVector<Point> const &points,
+                                 float radius, DeviceVector<int64_t> &offsets,
+                                 DeviceVector<int> &indices)
+{
+  Details::neighbor_list(space, points, radius, 0, offsets, indices);
+}
+} // namespace Experimental
+
 // Free function ArborX::query(tree, space, predicates, indices, offsets
 // [, policy]) (src/spatial/ArborX_CrsGraphWrapper.hpp:22-34).
 template <class Predicates, class... Args>
@@ -430,3 +560,12 @@ void query(BoundingVolumeHierarchy const &tree, HipStream const &space,
 }
 
 } // namespace ArborX
+
+#if defined(__HIPCC__)
+#include "ArborX_Callbacks.hpp"
+inline ArborX::Details::DeviceTree ArborX::BoundingVolumeHierarchy::device_view() const
+{
+  return Details::device_tree(_h);
+}
+#endif
+

@@ -1,0 +1,484 @@
+// ArborX_Callbacks.hpp -- header-only callback traversals of the MI355X BVH
+// (compiled by the caller with hipcc for gfx950; included by ArborX.hpp
+// when __HIPCC__ is defined).
+//
+// Mirrors the generic-callback half of ArborX's query interface:
+//   bvh.query(space, predicates, callback[, policy])
+//       (src/spatial/ArborX_LinearBVH.hpp:84-88,258-313;
+//        detail/ArborX_TreeTraversal.hpp:34-336)
+//   bvh.query(space, predicates, callback, values, offsets[, policy])
+//       (ArborX_LinearBVH.hpp:90-110; detail/ArborX_CrsGraphWrapperImpl.hpp)
+//   bvh.query(PerThread, predicate, callback)  -- device code
+//       (ArborX_LinearBVH.hpp:112-122)
+// Spatial callbacks run inside the library's stackless rope walk
+// (arborx_mi355x_device.hpp) and may return CallbackTreeTraversalControl
+// to stop early (detail/ArborX_Callbacks.hpp:96-118).  Nearest callbacks see
+// the k nearest values of each predicate in (distance, index) order, taken
+// from the library's kNN (TreeTraversal.hpp:314-334 calls them the same way,
+// after sorting its heap).
+//
+// The predicate object handed to a callback carries its geometry and its
+// position in the batch: getGeometry(p) and getData(p) (= the position, the
+// reference's attach_indices form, detail/ArborX_AttachIndices.hpp:30-35).
+// The value is the tree's value: the index of the point or box.
+#pragma once
+
+#include "ArborX.hpp"
+#include "arborx_mi355x_device.hpp"
+
+#include <type_traits>
+
+namespace ArborX
+{
+
+// detail/ArborX_Callbacks.hpp:22-26
+enum class CallbackTreeTraversalControl
+{
+  early_exit,
+  normal_continuation
+};
+
+namespace Experimental
+{
+// ArborX_LinearBVH.hpp:112-122 tag of the per-thread query.
+struct PerThread
+{
+};
+} // namespace Experimental
+
+// Predicate objects seen by callbacks (detail/ArborX_Predicates.hpp:57-146).
+template <class Geometry>
+struct Intersects
+{
+  Geometry geometry;
+  int index; // position in the batch
+};
+template <class Geometry>
+struct Nearest
+{
+  Geometry geometry;
+  int k;
+  int index;
+};
+template <class P>
+ARBORX_HD auto const &getGeometry(P const &p)
+{
+  return p.geometry;
+}
+template <class P>
+ARBORX_HD int getData(P const &p)
+{
+  return p.index;
+}
+template <class G>
+ARBORX_HD int getK(Nearest<G> const &p)
+{
+  return p.k;
+}
+
+namespace Details
+{
+
+// Device view of a tree (the layout of arborx_mi355x_device.hpp); the
+// member names are the ones rope_walk reads.
+struct DeviceTree
+{
+  const uint4 *slots = nullptr;
+  uint32_t total = 0;
+  int32_t n = 0;
+  axb::Box6 scene;
+  float ss[3];
+  uint32_t *err = nullptr;
+  int32_t geometry = 0;
+  int32_t format = 0;
+
+  // bvh.query(PerThread, predicate, callback) (ArborX_LinearBVH.hpp:112-122)
+  template <class Geometry, class Callback>
+  __device__ void query(Experimental::PerThread, Intersects<Geometry> const &predicate,
+                        Callback const &callback) const;
+};
+
+inline DeviceTree device_tree(axb_bvh const *h)
+{
+  DeviceTree t;
+  if (!h)
+    return t;
+  axb_tree_view v;
+  check(axb_bvh_view(h, &v));
+  t.slots = (const uint4 *)v.slots;
+  t.total = (uint32_t)v.end_slot;
+  t.n = (int32_t)v.n;
+  for (int d = 0; d < 3; ++d)
+  {
+    t.scene.lo[d] = v.scene[d];
+    t.scene.hi[d] = v.scene[3 + d];
+    t.ss[d] = v.q16_scale[d];
+  }
+  t.geometry = v.geometry;
+  t.format = v.node_format;
+  return t;
+}
+
+// Calls the callback; false means "stop this predicate's traversal"
+// (detail/ArborX_Callbacks.hpp:96-118).
+template <class Callback, class Predicate>
+__device__ __forceinline__ bool invoke_continue(Callback const &cb, Predicate const &p, int v)
+{
+  using Result = decltype(cb(p, v)); // evaluated in device context
+  if constexpr (std::is_same_v<Result, CallbackTreeTraversalControl>)
+    return cb(p, v) != CallbackTreeTraversalControl::early_exit;
+  else
+  {
+    cb(p, v);
+    return true;
+  }
+}
+
+template <class Predicate, class Callback>
+struct CallbackSink
+{
+  Predicate const &p;
+  Callback const &cb;
+  __device__ __forceinline__ bool hit(int32_t v) { return invoke_continue(cb, p, v); }
+};
+
+__device__ __forceinline__ axb::SpatialPred spatial_of(Sphere const &s)
+{
+  return axb::sphere_pred(s._centroid._coords[0], s._centroid._coords[1],
+                          s._centroid._coords[2], s._radius);
+}
+__device__ __forceinline__ axb::SpatialPred spatial_of(Box const &b)
+{
+  return axb::box_pred(axb::Box6{
+      {b._min_corner._coords[0], b._min_corner._coords[1], b._min_corner._coords[2]},
+      {b._max_corner._coords[0], b._max_corner._coords[1], b._max_corner._coords[2]}});
+}
+template <class Geometry>
+constexpr int pred_kind()
+{
+  return std::is_same_v<Geometry, Box> ? axb::PRED_BOXES : axb::PRED_SPHERES;
+}
+
+// One predicate over the whole tree: the rope walk with the callback as
+// its sink.  The branch on the node format and geometry is uniform.
+template <class Geometry, class Callback>
+__device__ void spatial_walk(DeviceTree const &t, Intersects<Geometry> const &pred,
+                             Callback const &cb)
+{
+  if (t.n <= 0)
+    return;
+  const axb::SpatialPred sp = spatial_of(pred.geometry);
+  CallbackSink<Intersects<Geometry>, Callback> sink{pred, cb};
+  uint32_t vi = 0, vl = 0;
+  constexpr int PK = pred_kind<Geometry>();
+  if (t.format == AXB_NODES_Q16)
+  {
+    if (t.geometry == AXB_GEOMETRY_POINTS)
+      axb::rope_walk<axb::NODES_Q16, axb::GEOM_POINTS, PK>(t, sp, 0u, t.total, ~0u, sink, vi,
+                                                           vl);
+    else
+      axb::rope_walk<axb::NODES_Q16, axb::GEOM_BOXES, PK>(t, sp, 0u, t.total, ~0u, sink, vi,
+                                                          vl);
+  }
+  else
+  {
+    if (t.geometry == AXB_GEOMETRY_POINTS)
+      axb::rope_walk<axb::NODES_FP32, axb::GEOM_POINTS, PK>(t, sp, 0u, t.total, ~0u, sink, vi,
+                                                            vl);
+    else
+      axb::rope_walk<axb::NODES_FP32, axb::GEOM_BOXES, PK>(t, sp, 0u, t.total, ~0u, sink, vi,
+                                                           vl);
+  }
+}
+
+// Predicate i of a batch as the object a callback sees.
+__device__ __forceinline__ Intersects<Sphere> predicate_at(Experimental::PointsWithRadius b,
+                                                           int i)
+{
+  return {Sphere{b.points[i], b.r}, i};
+}
+__device__ __forceinline__ Intersects<Sphere> predicate_at(Experimental::IntersectsSpheres b,
+                                                           int i)
+{
+  return {b.spheres[i], i};
+}
+__device__ __forceinline__ Intersects<Box> predicate_at(Experimental::IntersectsBoxes b, int i)
+{
+  return {b.boxes[i], i};
+}
+__device__ __forceinline__ Nearest<Point> predicate_at(Experimental::PointsNearestK b, int i)
+{
+  return {b.points[i], b.k, i};
+}
+
+inline int32_t c_kind(Experimental::PointsWithRadius const &) { return 2; }
+inline int32_t c_kind(Experimental::IntersectsSpheres const &) { return AXB_PREDICATE_SPHERES; }
+inline int32_t c_kind(Experimental::IntersectsBoxes const &) { return AXB_PREDICATE_BOXES; }
+inline float const *c_data(Experimental::PointsWithRadius const &b)
+{
+  return (float const *)b.points;
+}
+inline float const *c_data(Experimental::IntersectsSpheres const &b)
+{
+  return (float const *)b.spheres;
+}
+inline float const *c_data(Experimental::IntersectsBoxes const &b)
+{
+  return (float const *)b.boxes;
+}
+
+template <class Batch>
+struct is_spatial_batch
+    : std::bool_constant<std::is_same_v<Batch, Experimental::PointsWithRadius> ||
+                         std::is_same_v<Batch, Experimental::IntersectsSpheres> ||
+                         std::is_same_v<Batch, Experimental::IntersectsBoxes>>
+{
+};
+
+inline unsigned blocks_of(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// ---- spatial: callback per match
+template <class Batch, class Callback>
+__global__ __launch_bounds__(256) void k_spatial_callback(DeviceTree t, Batch batch, int64_t nq,
+                                                          const uint32_t *order, Callback cb)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nq)
+    return;
+  const int i = order ? (int)order[g] : (int)g;
+  spatial_walk(t, predicate_at(batch, i), cb);
+}
+
+// Counting / writing output functors of the CRS-with-callback form.
+struct CountOut
+{
+  int64_t *c;
+  template <class T>
+  __device__ void operator()(T const &) const
+  {
+    ++*c;
+  }
+};
+template <class T>
+struct WriteOut
+{
+  T *out;
+  int64_t *c;
+  int64_t end; // the row's end: a callback that inserts more than it counted stops there
+  __device__ void operator()(T const &v) const
+  {
+    if (*c < end)
+      out[(*c)++] = v;
+  }
+};
+
+// ---- spatial: CRS with an inserting callback cb(predicate, value, out)
+template <class Batch, class Callback, class T, bool FILL>
+__global__ __launch_bounds__(256) void k_spatial_insert(DeviceTree t, Batch batch, int64_t nq,
+                                                        const uint32_t *order, Callback cb,
+                                                        int64_t *offsets, T *out)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nq)
+    return;
+  const int i = order ? (int)order[g] : (int)g;
+  int64_t c = FILL ? offsets[i] : 0;
+  auto pred = predicate_at(batch, i);
+  if (FILL)
+  {
+    WriteOut<T> w{out, &c, offsets[i + 1]};
+    spatial_walk(t, pred, [&](auto const &p, int v) { cb(p, v, w); });
+  }
+  else
+  {
+    CountOut w{&c};
+    spatial_walk(t, pred, [&](auto const &p, int v) { cb(p, v, w); });
+    offsets[i] = c;
+  }
+}
+
+// ---- nearest: the rows of the library's kNN, callback per value in order
+template <class Callback>
+__global__ __launch_bounds__(256) void k_nearest_callback(Experimental::PointsNearestK batch,
+                                                          int64_t nq, const int64_t *offsets,
+                                                          const int32_t *indices, Callback cb)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq)
+    return;
+  const auto pred = predicate_at(batch, (int)i);
+  for (int64_t j = offsets[i]; j < offsets[i + 1]; ++j)
+    cb(pred, indices[j]);
+}
+template <class Callback, class T, bool FILL>
+__global__ __launch_bounds__(256) void k_nearest_insert(Experimental::PointsNearestK batch,
+                                                        int64_t nq, const int64_t *knn_off,
+                                                        const int32_t *knn_ind, Callback cb,
+                                                        int64_t *offsets, T *out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq)
+    return;
+  const auto pred = predicate_at(batch, (int)i);
+  int64_t c = FILL ? offsets[i] : 0;
+  for (int64_t j = knn_off[i]; j < knn_off[i + 1]; ++j)
+  {
+    if (FILL)
+      cb(pred, knn_ind[j], WriteOut<T>{out, &c, offsets[i + 1]});
+    else
+      cb(pred, knn_ind[j], CountOut{&c});
+  }
+  if (!FILL)
+    offsets[i] = c;
+}
+
+inline void launch_check(const char *what)
+{
+  hip_check(hipGetLastError(), what);
+}
+
+// Processing order of a spatial batch (Morton-sorted unless the policy
+// turns sorting off); empty when not sorted.
+template <class Batch>
+DeviceVector<uint32_t> processing_order(axb_bvh const *h, HipStream const &space,
+                                        Batch const &b,
+                                        Experimental::TraversalPolicy const &policy)
+{
+  DeviceVector<uint32_t> order;
+  if (policy._sort_predicates && b.n > 1)
+  {
+    order.resize(b.n);
+    check(axb_bvh_predicate_order(h, c_kind(b), c_data(b), (int64_t)b.n, order.data(),
+                                  (axb_stream_t)space.stream));
+  }
+  return order;
+}
+
+template <class Batch, class Callback>
+void callback_query(BoundingVolumeHierarchy const &bvh, HipStream const &space,
+                    Batch const &b, Callback const &cb,
+                    Experimental::TraversalPolicy const &policy)
+{
+  axb_bvh const *h = bvh.c_handle();
+  if (!h || b.n == 0)
+    return;
+  if constexpr (is_spatial_batch<Batch>::value)
+  {
+    DeviceTree t = device_tree(h);
+    DeviceVector<uint32_t> order = processing_order(h, space, b, policy);
+    k_spatial_callback<<<blocks_of(b.n), 256, 0, space.stream>>>(t, b, (int64_t)b.n,
+                                                                  order.data(), cb);
+    launch_check("query(callback)");
+    space.fence();
+  }
+  else
+  {
+    static_assert(std::is_same_v<Batch, Experimental::PointsNearestK>,
+                  "unsupported predicates");
+    DeviceVector<int> ind;
+    DeviceVector<int64_t> off;
+    bvh.query(space, b, ind, off, policy);
+    k_nearest_callback<<<blocks_of(b.n), 256, 0, space.stream>>>(b, (int64_t)b.n, off.data(),
+                                                                  ind.data(), cb);
+    launch_check("query(nearest callback)");
+    space.fence();
+  }
+}
+
+// (a template, so that every translation unit may instantiate it)
+template <class I>
+__global__ void k_zero(I *p, int64_t n)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = 0;
+}
+
+template <class Batch, class Callback, class T>
+void callback_query_crs(BoundingVolumeHierarchy const &bvh, HipStream const &space,
+                        Batch const &b, Callback const &cb, DeviceVector<T> &values,
+                        DeviceVector<int64_t> &offsets,
+                        Experimental::TraversalPolicy const &policy)
+{
+  axb_bvh const *h = bvh.c_handle();
+  offsets.resize(b.n + 1);
+  k_zero<int64_t><<<blocks_of((int64_t)b.n + 1), 256, 0, space.stream>>>(offsets.data(),
+                                                                    (int64_t)b.n + 1);
+  launch_check("query(callback, values, offsets)");
+  if (!h || b.n == 0)
+  {
+    space.fence();
+    values.resize(0);
+    return;
+  }
+  const int64_t nq = (int64_t)b.n;
+  if constexpr (is_spatial_batch<Batch>::value)
+  {
+    DeviceTree t = device_tree(h);
+    DeviceVector<uint32_t> order = processing_order(h, space, b, policy);
+    k_spatial_insert<Batch, Callback, T, false><<<blocks_of(nq), 256, 0, space.stream>>>(
+        t, b, nq, order.data(), cb, offsets.data(), (T *)nullptr);
+    launch_check("query(callback, values, offsets): count");
+    check(axb_exclusive_scan_i64(offsets.data(), offsets.data(), nq + 1,
+                                 (axb_stream_t)space.stream));
+    int64_t nnz = 0;
+    hip_check(hipMemcpyAsync(&nnz, offsets.data() + nq, sizeof(nnz), hipMemcpyDeviceToHost,
+                             space.stream),
+              "hipMemcpyAsync");
+    space.fence();
+    values.resize((size_t)nnz);
+    if (nnz > 0)
+    {
+      k_spatial_insert<Batch, Callback, T, true><<<blocks_of(nq), 256, 0, space.stream>>>(
+          t, b, nq, order.data(), cb, offsets.data(), values.data());
+      launch_check("query(callback, values, offsets): fill");
+    }
+    space.fence();
+  }
+  else
+  {
+    static_assert(std::is_same_v<Batch, Experimental::PointsNearestK>,
+                  "unsupported predicates");
+    DeviceVector<int> ind;
+    DeviceVector<int64_t> off;
+    bvh.query(space, b, ind, off, policy);
+    k_nearest_insert<Callback, T, false><<<blocks_of(nq), 256, 0, space.stream>>>(
+        b, nq, off.data(), ind.data(), cb, offsets.data(), (T *)nullptr);
+    launch_check("query(nearest callback, values, offsets): count");
+    check(axb_exclusive_scan_i64(offsets.data(), offsets.data(), nq + 1,
+                                 (axb_stream_t)space.stream));
+    int64_t nnz = 0;
+    hip_check(hipMemcpyAsync(&nnz, offsets.data() + nq, sizeof(nnz), hipMemcpyDeviceToHost,
+                             space.stream),
+              "hipMemcpyAsync");
+    space.fence();
+    values.resize((size_t)nnz);
+    if (nnz > 0)
+    {
+      k_nearest_insert<Callback, T, true><<<blocks_of(nq), 256, 0, space.stream>>>(
+          b, nq, off.data(), ind.data(), cb, offsets.data(), values.data());
+      launch_check("query(nearest callback, values, offsets): fill");
+    }
+    space.fence();
+  }
+}
+
+template <class Geometry, class Callback>
+__device__ void DeviceTree::query(Experimental::PerThread, Intersects<Geometry> const &predicate,
+                                  Callback const &callback) const
+{
+  spatial_walk(*this, predicate, callback);
+}
+
+} // namespace Details
+
+// query(PerThread, predicate, callback) inside a kernel
+// (ArborX_LinearBVH.hpp:112-122): spatial predicates only, as there.
+template <class Geometry, class Callback>
+__device__ void query(Experimental::PerThread, Details::DeviceTree const &tree,
+                      Intersects<Geometry> const &predicate, Callback const &callback)
+{
+  Details::spatial_walk(tree, predicate, callback);
+}
+
+} // namespace ArborX

@@ -155,6 +155,37 @@ axb_status axb_bvh_query_nearest(const axb_bvh *bvh, const float *points,
                                  int32_t **indices, float **distances,
                                  int64_t *nnz_host, axb_stream_t stream);
 
+/* Neighbour lists of a point set with itself,
+ * Experimental::findFullNeighborList / findHalfNeighborList
+ * (src/spatial/detail/ArborX_NeighborList.hpp:49-188): row i lists every
+ * j != i with distance(x_i, x_j) <= radius (full), or only the j whose leaf
+ * precedes i's leaf in the tree (half; each pair appears once, as in
+ * detail/ArborX_HalfTraversal.hpp:45-72).  axb_bvh_neighbor_list runs on a
+ * points tree built by axb_bvh_create (rows indexed by value); the find_
+ * form builds a Q16 tree over `points` first, as the reference builds its
+ * own BVH.  offsets: caller-allocated n + 1 entries; *indices allocated by
+ * the library (*nnz_host entries), each row in traversal order (the
+ * reference's order within a row is unspecified).  policy as for queries;
+ * radius >= 0. */
+axb_status axb_bvh_neighbor_list(const axb_bvh *bvh, float radius, int32_t full,
+                                 axb_traversal_policy policy, int64_t *offsets,
+                                 int32_t **indices, int64_t *nnz_host,
+                                 axb_stream_t stream);
+axb_status axb_find_neighbor_list(const float *points, int64_t n, float radius,
+                                  int32_t full, int64_t *offsets,
+                                  int32_t **indices, int64_t *nnz_host,
+                                  axb_stream_t stream);
+
+/* Details::expandHalfToFull (detail/ArborX_ExpandHalfToFull.hpp:23-68):
+ * from a half list (n rows) to the full list, row i = its own entries
+ * entries plus the k of every row k that lists i (order within a row
+ * unspecified, as in the reference).  out_offsets: caller-allocated n + 1;
+ * *out_indices allocated by the library, 2 * nnz(half) entries. */
+axb_status axb_expand_half_to_full(const int64_t *offsets,
+                                   const int32_t *indices, int64_t n,
+                                   int64_t *out_offsets, int32_t **out_indices,
+                                   int64_t *nnz_host, axb_stream_t stream);
+
 /* Pair energy over a fixed neighbour CRS (no reference counterpart; pattern of
  * examples/molecular_dynamics/example_molecular_dynamics.cpp:128-194).
  * Shifted Lennard-Jones phi(r) = 4(r^-12 - r^-6) - phi(cutoff);
@@ -173,6 +204,43 @@ axb_status axb_pair_energy_fd(const float *xyz, int64_t n,
                               const int64_t *offsets, const int32_t *indices,
                               float cutoff, const int32_t *atoms, int64_t m,
                               double h, double *fd, axb_stream_t stream);
+
+/* ---- building blocks of the header-only callback traversals ----
+ * (query(space, predicates, callback) and query(PerThread, ...) in
+ * include/ArborX.hpp, compiled by the caller with hipcc; they walk the
+ * tree's node array with the device code of include/arborx_mi355x_device.hpp).
+ *
+ * Read-only view of a tree for device code.  slots: pre-order node array
+ * (uint32 x 4 per slot, walked from slot 0 to end_slot); first_leaf_prefix:
+ * n entries locating each leaf's record; scene: {lo xyz, hi xyz};
+ * q16_scale: per-axis step of the scene frame (Q16 trees).  Valid while the
+ * tree lives. */
+typedef struct axb_tree_view
+{
+  const void *slots;
+  int64_t end_slot;
+  int64_t n;
+  const int32_t *first_leaf_prefix;
+  float scene[6];
+  float q16_scale[3];
+  int32_t geometry;
+  int32_t node_format;
+} axb_tree_view;
+axb_status axb_bvh_view(const axb_bvh *bvh, axb_tree_view *out_host);
+
+/* Processing order of a predicate batch: the permutation that sorts the
+ * predicates' centroids by 32-bit Morton code over the tree's scene
+ * (CrsGraphWrapperImpl.hpp:407-419, the reference's sortPredicates).
+ * predicate_kind: AXB_PREDICATE_SPHERES / _BOXES, or 2 for points (3
+ * floats each).  order: device uint32, nq entries (caller-allocated). */
+axb_status axb_bvh_predicate_order(const axb_bvh *bvh, int32_t predicate_kind,
+                                   const float *predicates, int64_t nq,
+                                   uint32_t *order, axb_stream_t stream);
+
+/* Exclusive prefix sum of n int64 values (device; in place allowed),
+ * the scan of the two-pass CRS (KokkosExt::exclusive_scan). */
+axb_status axb_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n,
+                                  axb_stream_t stream);
 
 /* ---- introspection (parity tests, benchmarks) ---- */
 
@@ -205,8 +273,9 @@ axb_status axb_profile_record(int32_t i, char *name, int32_t name_len,
                               double *ms);
 /* {queries, internal-node records read, leaf records read, results} summed
  * over the traversals run since axb_profile_enable(2 | ...), then the same
- * four counters for the nearest queries that took the exact fallback walk. */
-axb_status axb_profile_visits(int64_t out_host[8]);
+ * four counters for the nearest queries that took the exact fallback walk,
+ * then {collect rounds run, queries needing more than one round, 0, 0}. */
+axb_status axb_profile_visits(int64_t out_host[12]);
 
 /* Host helper used by the query kernels, exported for unit tests: the largest
  * float s with RN(sqrt(s)) <= r, so that `d2 <= t` decides exactly the

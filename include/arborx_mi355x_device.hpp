@@ -1,9 +1,13 @@
-// axb_common.hpp -- geometry, Morton and node-format helpers shared by the
-// gfx950 kernels and the host code of the library.
+// arborx_mi355x_device.hpp -- device-side building blocks of the MI355X
+// BVH, shared by the library's gfx950 kernels and by header-only user code
+// (callback traversals, include/ArborX.hpp under hipcc): geometry and
+// Morton helpers, the pre-order node layout, the Q16 node format and the
+// stackless rope walk.
 //
-// Every float expression here is compiled with -ffp-contract=off and
-// correctly rounded division/sqrt, so it rounds exactly like the
-// reference's CPU path (no FMA contraction, IEEE division and sqrt).
+// Every float expression here is meant to be compiled with
+// -ffp-contract=off and correctly rounded division/sqrt, so it rounds
+// exactly like the reference's CPU path (no FMA contraction, IEEE division
+// and sqrt).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -247,6 +251,161 @@ AXB_HD void q16_decode(uint32_t w0, uint32_t w1, uint32_t w2, const Box6 &f,
   out.hi[0] = q16_hi(f.hi[0], s[0], w1 >> 16);
   out.hi[1] = q16_hi(f.hi[1], s[1], w2 & 0xffffu);
   out.hi[2] = q16_hi(f.hi[2], s[2], w2 >> 16);
+}
+
+// ---------------------------------------------------------------- XCDs
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs,
+// each with its own L2.  Traversal kernels process predicates in Morton
+// order, so the logical block index is remapped to give every XCD one
+// contiguous range of blocks: the predicates in flight on an XCD then share
+// tree nodes in that XCD's L2.  A bijection of [0, nblocks).
+constexpr uint32_t NUM_XCDS = 8;
+AXB_HD uint32_t xcd_block(uint32_t b, uint32_t nblocks)
+{
+  const uint32_t per = nblocks / NUM_XCDS, rem = nblocks % NUM_XCDS;
+  const uint32_t x = b % NUM_XCDS, i = b / NUM_XCDS;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
+// ---------------------------------------------------------------- walk
+// Predicate kinds of the traversals.
+constexpr int PRED_SPHERES = 0;
+constexpr int PRED_BOXES = 1;
+constexpr int PRED_RADIUS = 2; // points + one radius
+
+// ORs an error flag into a device word (nullptr: ignored).
+AXB_D void raise_error(uint32_t *err, uint32_t flag)
+{
+  if (err)
+    __hip_atomic_fetch_or(err, flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Box of the internal record s0/s1.  Q16 records decode against the frame
+// of their parent when reached by descending (desc), else against the scene.
+// TV: any tree view with `Box6 scene` and `float ss[3]`.
+template <int F, class TV>
+AXB_D void node_box(const TV &tv, uint4 s0, uint4 s1, bool desc, const Box6 &par,
+                    const float *ps, Box6 &b)
+{
+  if (F == NODES_FP32)
+  {
+    b.lo[0] = bits_f(s0.x), b.lo[1] = bits_f(s0.y), b.lo[2] = bits_f(s0.z);
+    b.hi[0] = bits_f(s1.x), b.hi[1] = bits_f(s1.y), b.hi[2] = bits_f(s1.z);
+  }
+  else
+  {
+    if (desc)
+      q16_decode(s0.x, s0.y, s0.z, par, ps, b);
+    else
+      q16_decode(s0.x, s0.y, s0.z, tv.scene, tv.ss, b);
+  }
+}
+
+template <int F>
+AXB_D void set_frame(const Box6 &b, Box6 &par, float *ps)
+{
+  if (F == NODES_Q16)
+  {
+    par = b;
+    ps[0] = q16_scale(b.lo[0], b.hi[0]);
+    ps[1] = q16_scale(b.lo[1], b.hi[1]);
+    ps[2] = q16_scale(b.lo[2], b.hi[2]);
+  }
+}
+
+// Spatial predicate of one lane: a sphere (centre, sqrt threshold of its
+// radius) or a box.
+struct SpatialPred
+{
+  float cx, cy, cz, thr;
+  Box6 qb;
+};
+
+AXB_D SpatialPred sphere_pred(float x, float y, float z, float r)
+{
+  SpatialPred p;
+  p.cx = x, p.cy = y, p.cz = z, p.thr = sqrt_threshold(r);
+  p.qb = Box6{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  return p;
+}
+AXB_D SpatialPred box_pred(const Box6 &b)
+{
+  SpatialPred p;
+  p.cx = p.cy = p.cz = p.thr = 0.f;
+  p.qb = b;
+  return p;
+}
+
+// Stackless pre-order walk over slots [p, end) (TreeTraversal.hpp:97-119:
+// leaf -> test, next record; internal -> test ? first child : rope).  Each
+// hit calls sink.hit(value), which returns false to stop the walk (the
+// reference's CallbackTreeTraversalControl::early_exit,
+// Callbacks.hpp:96-118).  The leaf record at `skip` is not reported.  A Q16
+// walk starts at the root or at a record whose frame is the scene box.
+// TV: a tree view with `const uint4 *slots`, `Box6 scene`, `float ss[3]` and
+// `uint32_t *err`.  PK: PRED_BOXES tests the box, otherwise the sphere.
+template <int F, int G, int PK, class TV, class Sink>
+AXB_D void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
+                     uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
+{
+  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  Box6 par;
+  float ps[3];
+  bool desc = false;
+  while (p < end)
+  {
+    const uint4 s0 = tv.slots[p];
+    const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
+    const int32_t w = (int32_t)s0.w;
+    if (w < 0)
+    {
+      ++v_leaf;
+      bool hit;
+      const float x = bits_f(s0.x), y = bits_f(s0.y), z = bits_f(s0.z);
+      if (G == GEOM_POINTS)
+      {
+        if (PK == PRED_BOXES)
+          hit = !(x > pr.qb.hi[0] || x < pr.qb.lo[0] || y > pr.qb.hi[1] || y < pr.qb.lo[1] ||
+                  z > pr.qb.hi[2] || z < pr.qb.lo[2]);
+        else
+          hit = dist2_pp(pr.cx, pr.cy, pr.cz, x, y, z) <= pr.thr;
+      }
+      else
+      {
+        const Box6 lb = {{x, y, z}, {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
+        hit = PK == PRED_BOXES ? intersects_bb(pr.qb, lb)
+                               : dist2_pbox(pr.cx, pr.cy, pr.cz, lb) <= pr.thr;
+      }
+      if (hit && p != skip && !sink.hit(~w))
+        break;
+      p += SL;
+      desc = false;
+    }
+    else
+    {
+      ++v_int;
+      Box6 b;
+      node_box<F>(tv, s0, s1, desc, par, ps, b);
+      const bool pass = PK == PRED_BOXES ? intersects_bb(pr.qb, b)
+                                         : dist2_pbox(pr.cx, pr.cy, pr.cz, b) <= pr.thr;
+      if (pass)
+      {
+        set_frame<F>(b, par, ps);
+        desc = true;
+        p += SI;
+      }
+      else
+      {
+        if (w == 0)
+        {
+          raise_error(tv.err, KERR_LAYOUT);
+          break;
+        }
+        desc = false;
+        p += (uint32_t)w;
+      }
+    }
+  }
 }
 
 } // namespace axb

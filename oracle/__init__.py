@@ -67,6 +67,8 @@ def lib():
             "orc_query_knn": (I64, [P, P, I64, I32, P, P, P, P, P]),
             "orc_bruteforce_knn": (None, [P, I64, I32, P, I64, I32, P, P]),
             "orc_pair_energy_f64": (ctypes.c_double, [P, I64, P, P, ctypes.c_double, P]),
+            "orc_neighbor_list": (I64, [P, I64, ctypes.c_float, I32, P, P]),
+            "orc_expand_half_to_full": (I64, [P, P, I64, P, P]),
             "orc_num_threads": (I32, []),
             "orc_set_num_threads": (None, [I32]),
         }
@@ -252,6 +254,33 @@ def pair_energy_f64(xyz, offsets, indices, rc, with_grad=True):
     g = np.zeros((x.shape[0], 3), np.float64) if with_grad else None
     e = lib().orc_pair_energy_f64(_ptr(x), x.shape[0], _ptr(o), _ptr(i), float(rc), _ptr(g))
     return (e, g) if with_grad else e
+
+
+def neighbor_list(points, r, full):
+    """findFullNeighborList / findHalfNeighborList (NeighborList.hpp:49-188):
+    (offsets int64 (n+1,), indices int32).  Row order is the reference's
+    serial order; compare rows as multisets."""
+    x = _f32(points, 3)
+    n = x.shape[0]
+    off = np.zeros(n + 1, np.int64)
+    nnz = lib().orc_neighbor_list(_ptr(x), n, float(np.float32(r)), 1 if full else 0,
+                                  _ptr(off), None)
+    idx = np.zeros(nnz, np.int32)
+    lib().orc_neighbor_list(_ptr(x), n, float(np.float32(r)), 1 if full else 0, _ptr(off),
+                            _ptr(idx))
+    return off, idx
+
+
+def expand_half_to_full(offsets, indices):
+    """expandHalfToFull (ExpandHalfToFull.hpp:23-68) -> (offsets, indices)."""
+    o = np.ascontiguousarray(offsets, np.int64)
+    i = np.ascontiguousarray(indices, np.int32)
+    n = o.shape[0] - 1
+    out_o = np.zeros(n + 1, np.int64)
+    m = lib().orc_expand_half_to_full(_ptr(o), _ptr(i), n, _ptr(out_o), None)
+    out_i = np.zeros(m, np.int32)
+    lib().orc_expand_half_to_full(_ptr(o), _ptr(i), n, _ptr(out_o), _ptr(out_i))
+    return out_o, out_i
 
 
 def num_threads():

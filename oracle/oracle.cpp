@@ -1112,6 +1112,120 @@ double orc_pair_energy_f64(const float *xyz, int64_t n, const int64_t *offsets,
   return e;
 }
 
+} // extern "C"
+
+// ------------------------------------------------------- neighbour lists
+// HalfTraversal.hpp:45-72: for every leaf i (in leaf order) with the sphere
+// {point_i, r}, walk from rope(i) to the end; emit(value_i, value_node) for
+// each leaf hit.  Pairs are therefore (earlier leaf, later leaf).
+template <class Emit>
+static void half_traversal(const Tree &t, float r, Emit emit)
+{
+  const int64_t N = t.n;
+  if (N <= 1) // HalfTraversal.hpp:33-44: nothing for 0 or 1 values
+    return;
+  for (int64_t i = 0; i < N; ++i)
+  {
+    float q[4] = {t.leaf_lo[3 * i], t.leaf_lo[3 * i + 1], t.leaf_lo[3 * i + 2], r};
+    int32_t node = t.leaf_rope[i];
+    while (node != -1)
+    {
+      if (node < N)
+      {
+        if (pred_leaf(0, q, &t.leaf_lo[3 * node], &t.leaf_hi[3 * node], 0))
+          emit((int32_t)t.perm[i], (int32_t)t.perm[node]);
+        node = t.leaf_rope[node];
+      }
+      else
+      {
+        int64_t k = node - N;
+        node = pred_node(0, q, &t.int_lo[3 * k], &t.int_hi[3 * k]) ? t.left_child[k]
+                                                                   : t.rope[k];
+      }
+    }
+  }
+}
+
+extern "C" {
+
+// NeighborList.hpp:49-103 (findHalfNeighborList) and :105-188
+// (findFullNeighborList), serial.  offsets has n + 1 entries; indices is
+// written when non-null (call twice: first with nullptr to size).  Row j of
+// the half list holds the values i whose leaf comes before j's leaf; the
+// full list holds both directions.  Returns nnz.
+int64_t orc_neighbor_list(const float *points, int64_t n, float r, int full,
+                          int64_t *offsets, int32_t *indices)
+{
+  std::fill(offsets, offsets + n + 1, 0);
+  if (n == 0)
+    return 0;
+  Tree *t = (Tree *)build_impl(points, n, 0, nullptr);
+  half_traversal(*t, r, [&](int32_t v1, int32_t v2) {
+    if (full)
+      ++offsets[v1];
+    ++offsets[v2];
+  });
+  int64_t run = 0;
+  for (int64_t i = 0; i <= n; ++i)
+  {
+    int64_t c = offsets[i];
+    offsets[i] = run;
+    run += c;
+  }
+  if (indices)
+  {
+    std::vector<int64_t> counts(offsets, offsets + n);
+    half_traversal(*t, r, [&](int32_t v1, int32_t v2) { indices[counts[v2]++] = v1; });
+    if (full)
+    {
+      // NeighborList.hpp:160-183: mirror each (row i, entry k) into row k
+      std::vector<int64_t> last(counts);
+      for (int64_t i = 0; i < n; ++i)
+        for (int64_t p = offsets[i]; p < last[i]; ++p)
+        {
+          int32_t k = indices[p];
+          indices[counts[k]++] = (int32_t)i;
+        }
+    }
+  }
+  delete t;
+  return run;
+}
+
+// ExpandHalfToFull.hpp:23-68, serial: rows of the full list are the row's
+// own entries plus the mirrored ones.  out_offsets n + 1; out_indices
+// 2 * nnz(half) entries when non-null.
+int64_t orc_expand_half_to_full(const int64_t *offsets, const int32_t *indices, int64_t n,
+                                int64_t *out_offsets, int32_t *out_indices)
+{
+  std::fill(out_offsets, out_offsets + n + 1, 0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t p = offsets[i]; p < offsets[i + 1]; ++p)
+    {
+      ++out_offsets[i];
+      ++out_offsets[indices[p]];
+    }
+  int64_t run = 0;
+  for (int64_t i = 0; i <= n; ++i)
+  {
+    int64_t c = out_offsets[i];
+    out_offsets[i] = run;
+    run += c;
+  }
+  if (out_indices)
+  {
+    std::vector<int64_t> counts(out_offsets, out_offsets + n);
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t p = offsets[i]; p < offsets[i + 1]; ++p)
+      {
+        int32_t k = indices[p];
+        out_indices[counts[i]++] = k;
+        out_indices[counts[k]++] = (int32_t)i;
+      }
+  }
+  return run;
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

@@ -1,0 +1,278 @@
+// test_callbacks.hip -- the header-only callback traversals
+// (include/ArborX_Callbacks.hpp through include/ArborX.hpp) exercised like
+// the reference's test/tstQueryTreeCallbacks.cpp on the GPU:
+//   callback_spatial_predicate / callback_nearest_predicate (:95-179):
+//     inline callbacks inserting {index, distance to origin};
+//   callback_early_exit (:181-229): counts(i)++ < i, else early_exit;
+//   callback_with_attachment_* (:264-380): data added to each insert;
+// plus void callbacks and query(PerThread, ...) in a user kernel against the
+// CRS query on random points, for both node formats.
+#include <ArborX.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <set>
+#include <utility>
+#include <vector>
+
+static int g_fail = 0, g_pass = 0;
+#define CHECK(cond, what)                                                                   \
+  do                                                                                        \
+  {                                                                                         \
+    if (cond)                                                                               \
+      ++g_pass;                                                                             \
+    else                                                                                    \
+    {                                                                                       \
+      ++g_fail;                                                                             \
+      std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, what);                             \
+    }                                                                                       \
+  } while (0)
+
+using namespace ArborX;
+
+struct PairIndexDistance
+{
+  int index;
+  float distance;
+  bool operator<(PairIndexDistance const &o) const
+  {
+    return index < o.index || (index == o.index && distance < o.distance);
+  }
+  bool operator==(PairIndexDistance const &o) const
+  {
+    return index == o.index && distance == o.distance;
+  }
+};
+
+// CustomInlineCallback(WithAttachment) of tstQueryTreeCallbacks.cpp:27-42,
+// 231-247: the attached value is looked up by the predicate's position.
+struct InlineCallback
+{
+  Point const *points;
+  float const *attached; // may be null
+  template <class Predicate, class Insert>
+  __device__ void operator()(Predicate const &p, int index, Insert const &insert) const
+  {
+    Point const origin{{0.f, 0.f, 0.f}};
+    float d = distance(points[index], origin);
+    float data = attached ? attached[getData(p)] : 0.f;
+    insert(PairIndexDistance{index, data + d});
+  }
+};
+
+static std::vector<std::multiset<PairIndexDistance>>
+rows_of(DeviceVector<PairIndexDistance> const &v, DeviceVector<int64_t> const &o)
+{
+  auto hv = v.to_host();
+  auto ho = o.to_host();
+  std::vector<std::multiset<PairIndexDistance>> r;
+  for (size_t q = 0; q + 1 < ho.size(); ++q)
+    r.emplace_back(hv.begin() + ho[q], hv.begin() + ho[q + 1]);
+  return r;
+}
+
+static void reference_cases(HipStream const &space, NodeFormat fmt)
+{
+  int const n = 10;
+  std::vector<Point> hp(n);
+  for (int i = 0; i < n; ++i)
+    hp[i] = {{(float)i, (float)i, (float)i}};
+  DeviceVector<Point> points(hp);
+  BoundingVolumeHierarchy tree(space, points, fmt);
+
+  for (float delta : {0.f, 5.f})
+  {
+    std::multiset<PairIndexDistance> want;
+    for (int i = 0; i < n; ++i)
+    {
+      Point const origin{{0.f, 0.f, 0.f}};
+      want.insert({i, delta + distance(hp[i], origin)});
+    }
+    DeviceVector<float> att(std::vector<float>{delta});
+    InlineCallback cb{points.data(), att.data()};
+
+    DeviceVector<Box> qb(std::vector<Box>{tree.bounds()});
+    DeviceVector<PairIndexDistance> out;
+    DeviceVector<int64_t> off;
+    tree.query(space, intersects(qb), cb, out, off);
+    auto r = rows_of(out, off);
+    CHECK(r.size() == 1 && r[0] == want, "callback_spatial_predicate");
+
+    DeviceVector<Point> origin(std::vector<Point>{{{0.f, 0.f, 0.f}}});
+    tree.query(space, nearest(origin, n), cb, out, off);
+    r = rows_of(out, off);
+    CHECK(r.size() == 1 && r[0] == want, "callback_nearest_predicate");
+    // nearest callbacks arrive in (distance, index) order
+    auto hv = out.to_host();
+    bool sorted = true;
+    for (int i = 0; i < n; ++i)
+      sorted = sorted && hv[i].index == i;
+    CHECK(sorted, "nearest callbacks in distance order");
+  }
+}
+
+// Experimental_CustomCallbackEarlyExit (tstQueryTreeCallbacks.cpp:181-229)
+struct EarlyExit
+{
+  int *counts;
+  template <class Predicate>
+  __device__ CallbackTreeTraversalControl operator()(Predicate const &p, int) const
+  {
+    int i = getData(p);
+    if (atomicAdd(&counts[i], 1) < i)
+      return CallbackTreeTraversalControl::normal_continuation;
+    return CallbackTreeTraversalControl::early_exit;
+  }
+};
+
+static void early_exit(HipStream const &space, NodeFormat fmt)
+{
+  std::vector<Box> hb;
+  for (int i = 0; i < 4; ++i)
+  {
+    float f = (float)i;
+    hb.push_back(Box({{f, f, f}}, {{f, f, f}}));
+  }
+  DeviceVector<Box> boxes(hb);
+  BoundingVolumeHierarchy tree(space, boxes, fmt);
+  Box b = tree.bounds();
+  DeviceVector<Box> q(std::vector<Box>{b, b, b, b});
+  DeviceVector<int> counts(std::vector<int>{0, 0, 0, 0});
+  tree.query(space, intersects(q), EarlyExit{counts.data()});
+  CHECK((counts.to_host() == std::vector<int>{1, 2, 3, 4}), "callback_early_exit");
+}
+
+struct CountHits
+{
+  int *counts;
+  template <class Predicate>
+  __device__ void operator()(Predicate const &p, int) const
+  {
+    atomicAdd(&counts[getData(p)], 1);
+  }
+};
+
+// query(PerThread, ...) from a user kernel: counts and index sums per query.
+__global__ void per_thread_kernel(Details::DeviceTree tree, Sphere const *spheres, int nq,
+                                  int *counts, long long *sums)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq)
+    return;
+  int c = 0;
+  long long s = 0;
+  tree.query(Experimental::PerThread{}, Intersects<Sphere>{spheres[i], i},
+             [&](Intersects<Sphere> const &, int v) {
+               ++c;
+               s += v;
+             });
+  counts[i] = c;
+  sums[i] = s;
+}
+
+static void random_points(HipStream const &space, NodeFormat fmt)
+{
+  int const n = 20000, nq = 5000;
+  std::mt19937 gen(7);
+  std::uniform_real_distribution<float> U(0.f, 1.f);
+  std::vector<Point> hp(n);
+  for (auto &p : hp)
+    p = {{U(gen), U(gen), U(gen)}};
+  std::vector<Sphere> hs(nq);
+  for (auto &s : hs)
+    s = Sphere{{{U(gen), U(gen), U(gen)}}, 0.03f * U(gen)};
+  DeviceVector<Point> pts(hp);
+  DeviceVector<Sphere> sph(hs);
+  BoundingVolumeHierarchy tree(space, pts, fmt);
+
+  DeviceVector<int> I;
+  DeviceVector<int64_t> O;
+  tree.query(space, intersects(sph), I, O);
+  auto ho = O.to_host();
+  auto hi = I.to_host();
+
+  for (bool sorted : {true, false})
+  {
+    DeviceVector<int> counts(std::vector<int>(nq, 0));
+    tree.query(space, intersects(sph), CountHits{counts.data()},
+               Experimental::TraversalPolicy().setPredicateSorting(sorted));
+    auto hc = counts.to_host();
+    bool ok = true;
+    for (int q = 0; q < nq; ++q)
+      ok = ok && hc[q] == ho[q + 1] - ho[q];
+    CHECK(ok, "void callback counts equal the CRS row lengths");
+  }
+
+  DeviceVector<int> c2(nq);
+  DeviceVector<long long> s2(nq);
+  per_thread_kernel<<<(nq + 255) / 256, 256, 0, space.stream>>>(tree.device_view(), sph.data(),
+                                                                nq, c2.data(), s2.data());
+  Details::hip_check(hipStreamSynchronize(space.stream), "per_thread_kernel");
+  auto hc = c2.to_host();
+  auto hsum = s2.to_host();
+  bool ok = true;
+  for (int q = 0; q < nq; ++q)
+  {
+    long long s = 0;
+    for (int64_t j = ho[q]; j < ho[q + 1]; ++j)
+      s += hi[j];
+    ok = ok && hc[q] == ho[q + 1] - ho[q] && hsum[q] == s;
+  }
+  CHECK(ok, "query(PerThread) equals the CRS query");
+
+  // CRS through an inserting callback equals the library's CRS, row order
+  // included (same rope walk)
+  DeviceVector<int> vals;
+  DeviceVector<int64_t> off2;
+  tree.query(
+      space, intersects(sph),
+      [] __device__(Intersects<Sphere> const &, int v, auto const &out) { out(v); }, vals, off2);
+  CHECK(off2.to_host() == ho && vals.to_host() == hi, "default-callback CRS equals query");
+}
+
+// The XCD block remap of the traversal kernels must be a bijection of
+// [0, nblocks) (host-side check of the device helper).
+static void xcd_remap()
+{
+  bool ok = true;
+  for (uint32_t nb : {1u, 2u, 7u, 8u, 9u, 15u, 16u, 17u, 100u, 1001u, 39063u})
+  {
+    std::vector<int> seen(nb, 0);
+    for (uint32_t b = 0; b < nb; ++b)
+    {
+      uint32_t l = axb::xcd_block(b, nb);
+      if (l >= nb)
+        ok = false;
+      else
+        ++seen[l];
+    }
+    for (int s : seen)
+      ok = ok && s == 1;
+    // every XCD (b % 8) gets one contiguous range
+    for (uint32_t b = 8; b < nb; ++b)
+      ok = ok && axb::xcd_block(b, nb) == axb::xcd_block(b - 8, nb) + 1;
+  }
+  CHECK(ok, "xcd_block is a bijection with contiguous ranges per XCD");
+}
+
+int main()
+{
+  HipStream space;
+  xcd_remap();
+  for (NodeFormat fmt : {NodeFormat::FP32, NodeFormat::Q16})
+  {
+    reference_cases(space, fmt);
+    early_exit(space, fmt);
+    random_points(space, fmt);
+  }
+  // empty tree: callbacks are never called
+  BoundingVolumeHierarchy empty;
+  DeviceVector<Box> q(std::vector<Box>{Box({{0.f, 0.f, 0.f}}, {{1.f, 1.f, 1.f}})});
+  DeviceVector<int> counts(std::vector<int>{0});
+  empty.query(space, intersects(q), CountHits{counts.data()});
+  CHECK(counts.to_host()[0] == 0, "empty tree: no callback");
+  std::printf("%d passed, %d failed\n", g_pass, g_fail);
+  return g_fail == 0 ? 0 : 1;
+}

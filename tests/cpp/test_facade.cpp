@@ -2,8 +2,9 @@
 // reference's own Boost.Test suites, on the GPU:
 //   tstQueryTreeDegenerate.cpp (empty / one leaf / two leaves / duplicates /
 //   4096-leaf capacity), tstQueryTreeCallbacks.cpp:138-179 (collinear kNN),
-//   tstQueryTreeManufacturedSolution.cpp (11^3 grid), and
+//   tstQueryTreeManufacturedSolution.cpp (11^3 grid),
 //   tstQueryTreeTraversalPolicy.cpp:101-130 (buffer policy, SearchException),
+//   tstNeighborList.cpp:116-176 (full / half neighbour lists),
 // plus a brute-force comparison on random points.  Rows are compared as
 // multisets (test/boost_ext/CompressedStorageComparison.hpp:21-61).
 #include <ArborX.hpp>
@@ -275,6 +276,42 @@ static void brute_force(HipStream const &space, NodeFormat fmt)
   CHECK(ok_k, "kNN equals brute force (distance, index)");
 }
 
+// tstNeighborList.cpp:116-176: four collinear points, full list and half
+// list expanded to full.
+static void neighbor_lists(HipStream const &space)
+{
+  DeviceVector<Point> none;
+  DeviceVector<int64_t> O;
+  DeviceVector<int> I;
+  Experimental::findFullNeighborList(space, none, INFINITY, O, I);
+  CHECK(O.to_host() == std::vector<int64_t>{0} && I.size() == 0, "no point: offsets {0}");
+  DeviceVector<Point> one(std::vector<Point>{{{0.f, 0.f, 0.f}}});
+  Experimental::findHalfNeighborList(space, one, INFINITY, O, I);
+  CHECK((O.to_host() == std::vector<int64_t>{0, 0}) && I.size() == 0, "one point: no pair");
+
+  DeviceVector<Point> four(std::vector<Point>{
+      {{0.f, 0.f, 0.f}}, {{1.f, 1.f, 1.f}}, {{2.f, 2.f, 2.f}}, {{3.f, 3.f, 3.f}}});
+  struct Case
+  {
+    float r;
+    std::vector<std::vector<int>> want;
+  };
+  std::vector<Case> cases = {
+      {1.f, {{}, {}, {}, {}}},
+      {2.f, {{1}, {0, 2}, {1, 3}, {2}}},
+      {4.f, {{1, 2}, {0, 2, 3}, {0, 1, 3}, {1, 2}}},
+      {6.f, {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}}},
+  };
+  for (auto &c : cases)
+  {
+    Experimental::findFullNeighborList(space, four, c.r, O, I);
+    CHECK(rows_of(I, O) == rows(c.want), "full neighbour list");
+    Experimental::findHalfNeighborList(space, four, c.r, O, I);
+    Details::expandHalfToFull(space, O, I);
+    CHECK(rows_of(I, O) == rows(c.want), "half list expanded to full");
+  }
+}
+
 int main()
 {
   HipStream space;
@@ -286,6 +323,7 @@ int main()
     brute_force(space, fmt);
   }
   policy(space);
+  neighbor_lists(space);
   std::printf("%d passed, %d failed\n", g_pass, g_fail);
   return g_fail == 0 ? 0 : 1;
 }

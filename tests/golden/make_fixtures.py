@@ -158,6 +158,25 @@ def main():
         "source": "test/tstQueryTreeTraversalPolicy.cpp:101-130",
         "note": "hard buffer overflow raises SearchException",
     }
+    # tstNeighborList.cpp:116-176: full lists (also half lists expanded to
+    # full) of four collinear points; rows compare as multisets.  radius
+    # "inf" stands for the test's infinity<float>.
+    kat["neighbor_list"] = {
+        "source": "test/tstNeighborList.cpp:116-176",
+        "degenerate": [
+            {"points": [], "radius": "inf", "offsets": [0], "indices": []},
+            {"points": [[0.0, 0.0, 0.0]], "radius": "inf", "offsets": [0, 0], "indices": []},
+        ],
+        "points": [[0.0, 0.0, 0.0], [1.0, 1.0, 1.0], [2.0, 2.0, 2.0], [3.0, 3.0, 3.0]],
+        "cases": [
+            {"radius": 1.0, "offsets": [0, 0, 0, 0, 0], "indices": []},
+            {"radius": 2.0, "offsets": [0, 1, 3, 5, 6], "indices": [1, 0, 2, 1, 3, 2]},
+            {"radius": 4.0, "offsets": [0, 2, 5, 8, 10],
+             "indices": [1, 2, 0, 2, 3, 0, 1, 3, 1, 2]},
+            {"radius": 6.0, "offsets": [0, 3, 6, 9, 12],
+             "indices": [1, 2, 3, 0, 2, 3, 0, 1, 3, 0, 1, 2]},
+        ],
+    }
     with open(os.path.join(HERE, "reference_kat.json"), "w") as f:
         json.dump(kat, f, indent=1)
     print("wrote", os.path.join(HERE, "reference_kat.json"))

@@ -342,3 +342,53 @@ def test_pair_energy_gradient_fd_f64():
             b[d] -= h
             fd = (e_row(i, a) - e_row(i, b)) / (2 * h)
             assert abs(fd - g[i, d]) <= 1e-6 * max(1.0, abs(g[i, d])), (i, d, fd, g[i, d])
+
+
+# ------------------------------------------------------------ neighbour lists
+def _kat_radius(r):
+    return np.float32(np.inf) if r == "inf" else np.float32(r)
+
+
+def test_neighbor_list_golden(kat):
+    nl = kat["neighbor_list"]
+    for case in nl["degenerate"]:
+        pts = np.array(case["points"], np.float32).reshape(-1, 3)
+        for full in (True, False):
+            o, i = oracle.neighbor_list(pts, _kat_radius(case["radius"]), full)
+            assert list(o) == case["offsets"] and list(i) == case["indices"]
+    pts = np.array(nl["points"], np.float32)
+    for case in nl["cases"]:
+        r = _kat_radius(case["radius"])
+        o, i = oracle.neighbor_list(pts, r, True)
+        assert_crs_equal_as_sets(o, i, case["offsets"], case["indices"])
+        oh, ih = oracle.neighbor_list(pts, r, False)
+        o2, i2 = oracle.expand_half_to_full(oh, ih)
+        assert_crs_equal_as_sets(o2, i2, case["offsets"], case["indices"])
+
+
+@pytest.mark.parametrize("n,seed,dup", [(300, 11, False), (2000, 12, False), (1500, 13, True)])
+def test_neighbor_list_vs_brute_force(n, seed, dup):
+    """Full list = brute-force sphere rows minus the point itself; half list =
+    the entries of the full row whose leaf precedes the row's leaf (leaf
+    order from the oracle's own tree export)."""
+    pts = synthetic.uniform_points(n, seed)
+    if dup:  # exact duplicates: they are neighbours of each other
+        pts[n // 2:] = pts[: n - n // 2]
+    r = np.float32(synthetic.radius_for_neighbours(n, 12))
+    ob, ib = oracle.bruteforce_spatial(pts, 0, 0, spheres_from_points(pts, r))
+    full_rows = [row[row != q] for q, row in enumerate(rows(ob, ib))]
+    off_f = np.concatenate([[0], np.cumsum([len(x) for x in full_rows])])
+    ind_f = np.concatenate(full_rows).astype(np.int32)
+    o, i = oracle.neighbor_list(pts, r, True)
+    assert_crs_equal_as_sets(o, i, off_f, ind_f)
+
+    pos = np.empty(n, np.int64)
+    pos[oracle.Tree(pts).export()["perm"]] = np.arange(n)
+    half_rows = [row[pos[row] < pos[q]] for q, row in enumerate(full_rows)]
+    off_h = np.concatenate([[0], np.cumsum([len(x) for x in half_rows])])
+    ind_h = np.concatenate(half_rows).astype(np.int32)
+    oh, ih = oracle.neighbor_list(pts, r, False)
+    assert_crs_equal_as_sets(oh, ih, off_h, ind_h)
+    assert oh[-1] * 2 == o[-1]
+    o2, i2 = oracle.expand_half_to_full(oh, ih)
+    assert_crs_equal_as_sets(o2, i2, o, i)
