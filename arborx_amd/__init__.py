@@ -39,19 +39,22 @@ GEOMETRY_BOXES = 1
 NODES_FP32 = 0
 NODES_Q16 = 1
 BUILD_KEEP_TOPOLOGY = 1
+# internal nodes over at most this many leaves are not stored (AXB_FLAT_SUBTREE_MAX)
+FLAT_SUBTREE_MAX = 4
 PREDICATE_SPHERES = 0
 PREDICATE_BOXES = 1
 
 # Symbols of include/arborx_mi355x.h (checked by the CPU test suite).
 EXPORTED_SYMBOLS = (
-    "axb_last_error", "axb_version", "axb_set_output_allocator", "axb_free",
+    "axb_last_error", "axb_version", "axb_set_output_allocator", "axb_free", "axb_trim_memory",
     "axb_bvh_create", "axb_bvh_destroy", "axb_bvh_size", "axb_bvh_bounds",
     "axb_bvh_query_intersects", "axb_bvh_query_intersects_radius",
     "axb_bvh_query_nearest", "axb_bvh_neighbor_list", "axb_find_neighbor_list",
     "axb_expand_half_to_full", "axb_pair_energy", "axb_pair_energy_fd",
     "axb_bvh_view", "axb_bvh_predicate_order", "axb_exclusive_scan_i64",
     "axb_bvh_num_slots", "axb_bvh_slots", "axb_bvh_export_topology",
-    "axb_sqrt_threshold", "axb_morton64_host", "axb_profile_enable", "axb_profile_clear",
+    "axb_sqrt_threshold", "axb_morton64_host", "axb_q16_encode_host", "axb_profile_enable",
+    "axb_profile_clear",
     "axb_profile_num_records", "axb_profile_record", "axb_profile_visits",
 )
 
@@ -102,6 +105,7 @@ def lib():
             "axb_version": (ctypes.c_char_p, []),
             "axb_set_output_allocator": (_I32, [_ALLOC_FN, _FREE_FN, _P]),
             "axb_free": (_I32, [_P, _P]),
+            "axb_trim_memory": (_I32, []),
             "axb_bvh_create": (_I32, [_P, _I64, _I32, _I32, _I32, _P, ctypes.POINTER(_P)]),
             "axb_bvh_destroy": (None, [_P]),
             "axb_bvh_size": (_I64, [_P]),
@@ -128,6 +132,7 @@ def lib():
             "axb_bvh_export_topology": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P]),
             "axb_sqrt_threshold": (ctypes.c_float, [ctypes.c_float]),
             "axb_morton64_host": (ctypes.c_uint64, [_P, _P]),
+            "axb_q16_encode_host": (None, [_P, _P, _P]),
             "axb_profile_enable": (_I32, [_I32]),
             "axb_profile_clear": (_I32, []),
             "axb_profile_num_records": (_I32, []),
@@ -505,6 +510,17 @@ class profile:
                                fallback_leaf=v[6], rounds=v[8], multi_round_queries=v[9])
         _check(L.axb_profile_enable(0))
         return False
+
+
+def q16_encode_host(frame6, box6):
+    """Q16 steps {qlo xyz, qhi xyz} of a box in a frame (the build's encoder,
+    run on the host)."""
+    import numpy as np
+    f = np.ascontiguousarray(frame6, np.float32)
+    b = np.ascontiguousarray(box6, np.float32)
+    q = np.zeros(6, np.uint16)
+    lib().axb_q16_encode_host(f.ctypes.data_as(_P), b.ctypes.data_as(_P), q.ctypes.data_as(_P))
+    return q
 
 
 def sqrt_threshold(r):

@@ -11,12 +11,15 @@
 // is the next record and its rope is `slot + skip`.
 //
 // Pre-order position of a node (no top-down pass needed): with c[j] = number
-// of internal nodes whose leaf range starts at leaf j and C its inclusive
-// prefix sum, an internal node X with first leaf f and left-spine height h
-// (internal nodes on the path from X down to leaf f, X included) is preceded
-// by C[f] - h internal nodes and f leaves; leaf i by C[i] internal nodes and
-// i leaves.  c[] is written by the climb: the top of every left spine is the
-// root or a right child, and its height is the spine length.
+// of stored internal nodes whose first leaf is j and C its inclusive prefix
+// sum, a stored internal node X with first leaf f, last leaf l and left-spine
+// height h (stored nodes on the path from X down to leaf f, X included) is
+// preceded by C[f] - h internal nodes and f leaves, and its subtree takes
+// SL * (l - f + 1) + SI * (C[l] - C[f] + h) slots; leaf i is preceded by C[i]
+// internal nodes and i leaves.  Nodes over at most FLAT_MAX leaves are not
+// stored (h = 0); stored ones are the top part of every left spine.  c[] is
+// written by the climb: the top of every left spine is the root or a right
+// child, and its height is the spine length.
 #include "axb_internal.hpp"
 
 namespace axb
@@ -72,16 +75,27 @@ __global__ __launch_bounds__(256) void k_scene_partial(const float *values,
   }
 }
 
-__global__ void k_scene_final(const float *partials, int nblocks,
-                              float *scene)
+// One block of 256 threads: strided partial reductions into LDS, then one
+// thread per coordinate over the 256 values (min / max are exact in any
+// order).
+__global__ __launch_bounds__(256) void k_scene_final(const float *partials, int nblocks,
+                                                     float *scene)
 {
-  if (threadIdx.x < 6)
+  __shared__ float s[6][256];
+  const int t = threadIdx.x;
+  float r[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int b = t; b < nblocks; b += 256)
+    for (int d = 0; d < 6; ++d)
+      r[d] = d < 3 ? kmin(r[d], partials[b * 6 + d]) : kmax(r[d], partials[b * 6 + d]);
+  for (int d = 0; d < 6; ++d)
+    s[d][t] = r[d];
+  __syncthreads();
+  if (t < 6)
   {
-    const int d = threadIdx.x;
-    float r = d < 3 ? FLT_MAX : -FLT_MAX;
-    for (int b = 0; b < nblocks; ++b)
-      r = d < 3 ? kmin(r, partials[b * 6 + d]) : kmax(r, partials[b * 6 + d]);
-    scene[d] = r;
+    float v = s[t][0];
+    for (int j = 1; j < 256; ++j)
+      v = t < 3 ? kmin(v, s[t][j]) : kmax(v, s[t][j]);
+    scene[t] = v;
   }
 }
 
@@ -301,7 +315,10 @@ __global__ __launch_bounds__(256) void k_climb(
       bv.hi[d] = kmax(bv.hi[d], sb.hi[d]);
     }
     const int kp = dr < dl ? range_right : range_left;
-    const int hp = left_child < n ? 1 : h_left + 1;
+    // spine height among stored nodes; 0 = not stored (at most FLAT_MAX
+    // leaves); stored nodes are the top part of every left spine
+    const int hp =
+        range_right - range_left + 1 > FLAT_MAX ? (left_child < n ? 0 : h_left) + 1 : 0;
     const int32_t rope = ROPES ? apetrei_rope(keys, nint, n, range_right, dr) : -1;
     uint64_t *w = (uint64_t *)(inodes + kp);
     st_coh64(w, pack2(f_bits(bv.lo[0]), f_bits(bv.lo[1])));
@@ -339,9 +356,11 @@ __global__ __launch_bounds__(256) void k_lin_fp32(const INode *__restrict__ inod
   if (k >= nint)
     return;
   const INode r = inodes[k];
+  if (r.h == 0) // not stored: its leaves follow each other
+    return;
   const uint32_t pos = SI * (uint32_t)(C[r.first] - r.h) + SL * (uint32_t)r.first;
   const uint32_t len = (uint32_t)(r.last - r.first + 1);
-  const uint32_t skip = SI * (len - 1) + SL * len;
+  const uint32_t skip = SL * len + SI * (uint32_t)(C[r.last] - C[r.first] + r.h);
   if (pos + SI > total || pos + skip > total)
   {
     raise_error(err, KERR_LAYOUT);
@@ -375,6 +394,8 @@ __global__ __launch_bounds__(256) void k_lin_q16(const INode *__restrict__ inode
   while (true)
   {
     const INode r = inodes[k];
+    if (r.h == 0) // the rest of the spine is not stored
+      break;
     uint32_t q[6];
     Box6 dec;
     for (int d = 0; d < 3; ++d)
@@ -388,7 +409,7 @@ __global__ __launch_bounds__(256) void k_lin_q16(const INode *__restrict__ inode
     }
     const uint32_t pos = SI * (uint32_t)(C[r.first] - r.h) + SL * (uint32_t)r.first;
     const uint32_t len = (uint32_t)(r.last - r.first + 1);
-    const uint32_t skip = SI * (len - 1) + SL * len;
+    const uint32_t skip = SL * len + SI * (uint32_t)(C[r.last] - C[r.first] + r.h);
     if (pos + SI > total || pos + skip > total)
     {
       raise_error(err, KERR_LAYOUT);
@@ -443,6 +464,18 @@ __global__ void k_single_leaf(const float *values, uint4 *slots, float *scene)
     scene[d] = lo[d], scene[3 + d] = hi[d];
 }
 
+// Zeroes the SLOT_PAD slots after the last stored record (the read-ahead of
+// the walks' leaf groups sees records of neither kind there).
+template <int SL, int SI>
+__global__ void k_pad_tail(uint4 *slots, const int32_t *C, int n)
+{
+  if (threadIdx.x < SLOT_PAD)
+  {
+    const uint32_t total = SL * (uint32_t)n + SI * (uint32_t)C[n - 1];
+    slots[total + threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
 __global__ void k_export_topology(const INode *inodes, int nint,
                                   int32_t *left_child, int32_t *rope,
                                   float *boxes)
@@ -469,6 +502,8 @@ inline unsigned grid_for(int64_t n, int block = 256)
 
 // ------------------------------------------------------------ host driver
 
+// Upper bound of the node array (every internal node stored); the actual
+// size is SL * n + SI * C[n - 1] (BuildOut::total_slots after the build).
 int64_t total_slots_for(int64_t n, int geometry, int node_format)
 {
   if (n == 0)
@@ -485,13 +520,17 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
                       hipStream_t stream, BuildOut &out)
 {
   out = BuildOut{};
-  out.total_slots = total_slots_for(n, geometry, node_format);
+  // allocated for every internal node stored; the caller sets total_slots to
+  // SL * n + SI * C[n - 1] once C is on the host
+  out.max_slots = total_slots_for(n, geometry, node_format);
+  out.total_slots = out.max_slots;
   out.scene = (float *)dalloc(6 * sizeof(float), stream);
-  out.slots = (uint4 *)dalloc((size_t)(out.total_slots + 1) * sizeof(uint4), stream);
+  out.slots = (uint4 *)dalloc((size_t)(out.max_slots + SLOT_PAD) * sizeof(uint4), stream);
   if (!out.scene || !out.slots)
     return hipErrorOutOfMemory;
-  // padding slot after the last record (kernels may read slot p + 1)
-  hipError_t e0 = fill_u32((uint32_t *)(out.slots + out.total_slots), 0u, 4, stream);
+  // zero slots after the last record (kernels read ahead up to SLOT_PAD
+  // slots and check the record kind)
+  hipError_t e0 = fill_u32((uint32_t *)(out.slots + out.max_slots), 0u, 4 * SLOT_PAD, stream);
   if (e0 != hipSuccess)
     return e0;
   if (n == 0)
@@ -545,8 +584,8 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
   else
     AXB_LAUNCH(k_scene_partial<GEOM_BOXES>, dim3(partial_blocks), dim3(256), 0,
                        stream, values, n, partials);
-  AXB_LAUNCH(k_scene_final, dim3(1), dim3(64), 0, stream, partials, partial_blocks,
-                     out.scene);
+  AXB_LAUNCH(k_scene_final, dim3(1), dim3(256), 0, stream, partials, partial_blocks,
+             out.scene);
 
   // ---- Morton codes + digit histograms
   e0 = fill_u32(hist, 0u, 8 * 256, stream);
@@ -606,16 +645,16 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
       if (geometry == GEOM_POINTS)
       {
         AXB_LAUNCH(k_lin_fp32<GEOM_POINTS>, dim3(gi), dim3(256), 0, stream, inodes,
-                           nint, out.C, out.slots, (uint32_t)out.total_slots, err);
+                           nint, out.C, out.slots, (uint32_t)out.max_slots, err);
         AXB_LAUNCH((k_lin_leaves<GEOM_POINTS, 2>), dim3(gl), dim3(256), 0, stream,
-                           leafrec, ni, out.C, out.slots, (uint32_t)out.total_slots, err);
+                           leafrec, ni, out.C, out.slots, (uint32_t)out.max_slots, err);
       }
       else
       {
         AXB_LAUNCH(k_lin_fp32<GEOM_BOXES>, dim3(gi), dim3(256), 0, stream, inodes,
-                           nint, out.C, out.slots, (uint32_t)out.total_slots, err);
+                           nint, out.C, out.slots, (uint32_t)out.max_slots, err);
         AXB_LAUNCH((k_lin_leaves<GEOM_BOXES, 2>), dim3(gl), dim3(256), 0, stream,
-                           leafrec, ni, out.C, out.slots, (uint32_t)out.total_slots, err);
+                           leafrec, ni, out.C, out.slots, (uint32_t)out.max_slots, err);
       }
     }
     else
@@ -623,18 +662,32 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
       if (geometry == GEOM_POINTS)
       {
         AXB_LAUNCH(k_lin_q16<GEOM_POINTS>, dim3(gl), dim3(256), 0, stream, inodes, ni,
-                           c, top, out.C, out.scene, out.slots, (uint32_t)out.total_slots, err);
+                           c, top, out.C, out.scene, out.slots, (uint32_t)out.max_slots, err);
         AXB_LAUNCH((k_lin_leaves<GEOM_POINTS, 1>), dim3(gl), dim3(256), 0, stream,
-                           leafrec, ni, out.C, out.slots, (uint32_t)out.total_slots, err);
+                           leafrec, ni, out.C, out.slots, (uint32_t)out.max_slots, err);
       }
       else
       {
         AXB_LAUNCH(k_lin_q16<GEOM_BOXES>, dim3(gl), dim3(256), 0, stream, inodes, ni,
-                           c, top, out.C, out.scene, out.slots, (uint32_t)out.total_slots, err);
+                           c, top, out.C, out.scene, out.slots, (uint32_t)out.max_slots, err);
         AXB_LAUNCH((k_lin_leaves<GEOM_BOXES, 1>), dim3(gl), dim3(256), 0, stream,
-                           leafrec, ni, out.C, out.slots, (uint32_t)out.total_slots, err);
+                           leafrec, ni, out.C, out.slots, (uint32_t)out.max_slots, err);
       }
     }
+  }
+  if (geometry == GEOM_POINTS)
+  {
+    if (node_format == NODES_FP32)
+      AXB_LAUNCH((k_pad_tail<1, 2>), dim3(1), dim3(64), 0, stream, out.slots, out.C, ni);
+    else
+      AXB_LAUNCH((k_pad_tail<1, 1>), dim3(1), dim3(64), 0, stream, out.slots, out.C, ni);
+  }
+  else
+  {
+    if (node_format == NODES_FP32)
+      AXB_LAUNCH((k_pad_tail<2, 2>), dim3(1), dim3(64), 0, stream, out.slots, out.C, ni);
+    else
+      AXB_LAUNCH((k_pad_tail<2, 1>), dim3(1), dim3(64), 0, stream, out.slots, out.C, ni);
   }
   e = hipGetLastError();
 

@@ -63,8 +63,40 @@ ProfScope::~ProfScope()
 }
 unsigned long long *visit_counters() { return g_prof_visits.load() ? g_visits : nullptr; }
 
+// Temporaries come from the device's stream-ordered pool.  By default the
+// pool returns freed memory to the driver at every synchronisation, which
+// costs milliseconds per call at these sizes; the library keeps it cached
+// instead (once per device, on first use).
+static void keep_pool_memory()
+{
+  static std::mutex m;
+  static std::vector<bool> done;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return;
+  }
+  std::lock_guard<std::mutex> lock(m);
+  if ((int)done.size() <= dev)
+    done.resize(dev + 1, false);
+  if (done[dev])
+    return;
+  done[dev] = true;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+  {
+    uint64_t keep = UINT64_MAX;
+    if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess)
+      (void)hipGetLastError();
+  }
+  else
+    (void)hipGetLastError();
+}
+
 void *dalloc(size_t bytes, hipStream_t stream)
 {
+  keep_pool_memory();
   void *p = nullptr;
   if (bytes == 0)
     bytes = 16;
@@ -177,6 +209,27 @@ static TreeView view_of(const axb_bvh *t)
 extern "C" {
 
 const char *axb_last_error(void) { return g_error.c_str(); }
+
+void axb_q16_encode_host(const float frame[6], const float box[6], uint16_t q[6])
+{
+  for (int d = 0; d < 3; ++d)
+  {
+    uint32_t lo = 0, hi = 0;
+    q16_encode_axis(frame[d], frame[3 + d], box[d], box[3 + d], lo, hi);
+    q[d] = (uint16_t)lo;
+    q[3 + d] = (uint16_t)hi;
+  }
+}
+
+axb_status axb_trim_memory(void)
+{
+  int dev = 0;
+  AXB_HIP_TRY(hipGetDevice(&dev));
+  hipMemPool_t pool;
+  AXB_HIP_TRY(hipDeviceGetDefaultMemPool(&pool, dev));
+  AXB_HIP_TRY(hipMemPoolTrimTo(pool, 0));
+  return AXB_OK;
+}
 const char *axb_version(void) { return "arborx-mi355x 0.1 (gfx950)"; }
 
 axb_status axb_set_output_allocator(axb_alloc_fn alloc, axb_free_fn free_fn, void *ctx)
@@ -244,10 +297,23 @@ axb_status axb_bvh_create(const float *values, int64_t n, int32_t geometry, int3
     AXB_FAIL(AXB_ERR_HIP, "axb_bvh_create: kernel launch failed");
   }
   hipError_t e = build_tree(values, n, geometry, node_format, t->keep_topology, t->d_err, s, t->b);
+  int32_t c_last = 0; // stored internal nodes (C[n - 1])
   if (e == hipSuccess)
     e = hipMemcpyAsync(t->bounds, t->b.scene, sizeof(t->bounds), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && n > 0)
+    e = hipMemcpyAsync(&c_last, t->b.C + (n - 1), sizeof(c_last), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess)
     e = hipStreamSynchronize(s);
+  if (e == hipSuccess && n > 0)
+  {
+    const int64_t sl = geometry == AXB_GEOMETRY_POINTS ? 1 : 2;
+    const int64_t si = node_format == AXB_NODES_FP32 ? 2 : 1;
+    const int64_t total = sl * n + si * (int64_t)c_last;
+    if (c_last < 0 || total > t->b.max_slots)
+      e = hipErrorUnknown;
+    else
+      t->b.total_slots = total;
+  }
   if (e != hipSuccess)
   {
     axb_status st = hip_fail(e, "axb_bvh_create");

@@ -106,8 +106,9 @@ struct alignas(16) INode
 
 struct BuildOut
 {
-  uint4 *slots;        // total_slots + 1 (one padding slot)
-  int64_t total_slots;
+  uint4 *slots;        // max_slots + SLOT_PAD (zeroed padding after max_slots)
+  int64_t max_slots;   // every internal node stored
+  int64_t total_slots; // records actually stored (SL * n + SI * C[n - 1])
   float *scene;        // 6 floats (device)
   int32_t *C;          // n: inclusive prefix of left-spine counts
   // reference-form topology (kept on request)
@@ -133,6 +134,7 @@ constexpr int MODE_COUNT = 0;
 constexpr int MODE_FILL = 1;
 constexpr int MODE_BUFFER = 2;
 
+constexpr int SLOT_PAD = 4; // zeroed slots after the array (read-ahead of leaf groups)
 int64_t total_slots_for(int64_t n, int geometry, int node_format);
 hipError_t build_tree(const float *values, int64_t n, int geometry, int node_format,
                       bool keep_topology, uint32_t *err, hipStream_t stream, BuildOut &out);

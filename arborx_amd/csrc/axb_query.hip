@@ -9,14 +9,18 @@
 // reference's `sqrt(d2) <= r` (Intersects.hpp:82-114).
 //
 // Nearest: result = the k smallest (distance, index) pairs, distance =
-// RN(sqrt(d2)) as in Distance.hpp:52-80.  Two phases per lane, no stack:
-//   1. greedy descent to the leaf nearest in tree order, then the k leaves
-//      around it in Morton order (contiguous in pre-order) seed a sorted
-//      list, giving an upper bound R on the k-th distance;
-//   2. a rope walk that skips every node farther than the current k-th
-//      distance (ties kept: pruning is `d2 > T(R)`), inserting closer leaves.
-// The list lives in registers (k <= 64, unrolled insertion, static
-// indexing only); larger k use a binary max-heap in the output row
+// RN(sqrt(d2)) as in Distance.hpp:52-80.  k_knn_collect, one lane per query:
+//   1. greedy descent to the leaf nearest in tree order; the k-th distance
+//      R of the W = 4k leaves around it in Morton order (contiguous in
+//      pre-order) bounds the true k-th distance from above (they are real
+//      points);
+//   2. a fixed-radius rope walk appending (d2, index) of every leaf within R
+//      to a row of `cap` candidates, stopping at cap + 1; the radius adapts
+//      over a few rounds until the row holds between k and cap leaves;
+//   3. exact top-k of the row in a register list of packed
+//      (distance bits, index) keys.
+// Queries left unresolved go to k_knn_fallback (exact adaptive walk with an
+// LDS candidate buffer); k > 64 uses a binary max-heap in the output row
 // (the reference's NearestBufferProvider scheme, NearestBufferProvider.hpp:
 // 23-73).
 #include "axb_internal.hpp"
@@ -409,11 +413,153 @@ AXB_D void knn_flush(List &L, float &thr, CandBuf &B)
   B.cnt = 0;
 }
 
+// Greedy descent (closer child first, ties left) from the root to a leaf
+// record near the query; stops early when a left child is a leaf (the
+// start of a run of leaves, whose right sibling is not located without a
+// walk).  Returns the slot of that leaf.  The deepest stored node on the
+// path with at least `min_slots` slots is returned in X (slot, slots, box).
+struct Subtree
+{
+  uint32_t p, slots;
+  Box6 box;
+};
+template <int F, int G>
+AXB_D uint32_t greedy_descent(const TreeView &tv, float qx, float qy, float qz,
+                              uint32_t min_slots, Subtree &X, uint32_t &v_int,
+                              uint32_t &v_leaf)
+{
+  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  const uint32_t end = tv.total;
+  uint32_t p = 0;
+  Box6 cur = tv.scene;
+  float cs[3] = {tv.ss[0], tv.ss[1], tv.ss[2]};
+  X.p = 0;
+  X.slots = end;
+  {
+    const uint4 s0 = tv.slots[0], s1 = tv.slots[1];
+    if ((int32_t)s0.w >= 0)
+    {
+      ++v_int;
+      node_box<F>(tv, s0, s1, false, cur, cs, cur);
+      set_frame<F>(cur, cur, cs);
+    }
+    X.box = cur;
+  }
+  while (true)
+  {
+    const uint4 s0 = tv.slots[p];
+    if ((int32_t)s0.w < 0)
+      return p;
+    const uint32_t lp = p + SI;
+    const uint4 l0 = tv.slots[lp], l1 = tv.slots[lp + 1];
+    const int32_t lw = (int32_t)l0.w;
+    if (lw < 0)
+      return lp;
+    const uint32_t rp = lp + (uint32_t)lw;
+    if (rp >= end)
+    {
+      raise_error(tv.err, KERR_LAYOUT);
+      return lp;
+    }
+    const uint4 r0 = tv.slots[rp], r1 = tv.slots[rp + 1];
+    const int32_t rw = (int32_t)r0.w;
+    Box6 lb, rb;
+    ++v_int;
+    node_box<F>(tv, l0, l1, true, cur, cs, lb);
+    const float dl = dist2_pbox(qx, qy, qz, lb);
+    float dr;
+    if (rw < 0)
+    {
+      ++v_leaf;
+      dr = leaf_d2<G>(qx, qy, qz, r0, r1);
+    }
+    else
+    {
+      ++v_int;
+      node_box<F>(tv, r0, r1, false, cur, cs, rb);
+      dr = dist2_pbox(qx, qy, qz, rb);
+    }
+    if (dl <= dr)
+    {
+      p = lp;
+      set_frame<F>(lb, cur, cs);
+      if ((uint32_t)lw >= min_slots)
+        X = Subtree{lp, (uint32_t)lw, lb};
+    }
+    else
+    {
+      p = rp;
+      if (rw < 0)
+        return rp;
+      set_frame<F>(rb, cur, cs);
+      if ((uint32_t)rw >= min_slots)
+        X = Subtree{rp, (uint32_t)rw, rb};
+    }
+  }
+}
+
+// The W leaves nearest to slot p in slot order (about half before it),
+// fn(s0, s1) for each; [ws, we) is the slot range they span (no other leaf
+// lies in it).  W must not exceed the number of leaves.
+template <int G, class Fn>
+AXB_D void slot_window(const TreeView &tv, uint32_t p, int W, uint32_t &ws, uint32_t &we,
+                       Fn &&fn, uint32_t &v_int, uint32_t &v_leaf)
+{
+  constexpr uint32_t SL = leaf_slots<G>();
+  const uint32_t end = tv.total;
+  int got = 0;
+  uint32_t s = p, e = p;
+  while (got < W / 2 && s > 0)
+  {
+    --s;
+    const uint4 s0 = tv.slots[s];
+    if ((int32_t)s0.w < 0)
+    {
+      ++v_leaf;
+      fn(s0, SL == 2 ? tv.slots[s + 1] : s0);
+      ++got;
+    }
+    else
+      ++v_int;
+  }
+  while (got < W && e < end)
+  {
+    const uint4 s0 = tv.slots[e];
+    if ((int32_t)s0.w < 0)
+    {
+      ++v_leaf;
+      fn(s0, SL == 2 ? tv.slots[e + 1] : s0);
+      ++got;
+      e += SL;
+    }
+    else
+    {
+      ++v_int;
+      ++e;
+    }
+  }
+  while (got < W && s > 0) // near the end of the array: more before p
+  {
+    --s;
+    const uint4 s0 = tv.slots[s];
+    if ((int32_t)s0.w < 0)
+    {
+      ++v_leaf;
+      fn(s0, SL == 2 ? tv.slots[s + 1] : s0);
+      ++got;
+    }
+    else
+      ++v_int;
+  }
+  ws = s;
+  we = e;
+}
+
 template <int F, int G, class List>
 AXB_D void knn_walk(const TreeView &tv, float qx, float qy, float qz, List &L, float &thr,
-                    CandBuf &B, uint32_t p, uint32_t pend, int rank, bool desc, Box6 par,
-                    float ps[3], uint32_t skip_at, uint32_t skip_slots, int skip_leaves, int j0,
-                    int m, uint32_t &v_int, uint32_t &v_leaf)
+                    CandBuf &B, uint32_t p, uint32_t pend, bool desc, Box6 par, float ps[3],
+                    uint32_t skip_at, uint32_t skip_slots, uint32_t ws, uint32_t we,
+                    uint32_t &v_int, uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
   uint4 s0 = make_uint4(0, 0, 0, 0), s1 = s0;
@@ -430,7 +576,6 @@ AXB_D void knn_walk(const TreeView &tv, float qx, float qy, float qz, List &L, f
       if (p == skip_at)
       {
         p += skip_slots;
-        rank += skip_leaves;
         desc = false;
         continue;
       }
@@ -458,13 +603,12 @@ AXB_D void knn_walk(const TreeView &tv, float qx, float qy, float qz, List &L, f
         }
         desc = false;
         p += (uint32_t)w;
-        rank += (int)(((uint32_t)w + SI) / (SI + SL));
       }
     }
     while (p < pend && (int32_t)s0.w < 0 && B.cnt < KNN_BUF)
     {
       ++v_leaf;
-      if (rank < j0 || rank >= j0 + m)
+      if (p < ws || p >= we) // the seed window's leaves are in the list already
       {
         const float d2 = leaf_d2<G>(qx, qy, qz, s0, s1);
         if (d2 <= thr)
@@ -474,7 +618,6 @@ AXB_D void knn_walk(const TreeView &tv, float qx, float qy, float qz, List &L, f
           ++B.cnt;
         }
       }
-      ++rank;
       p += SL;
       desc = false;
       if (p >= pend || p == skip_at)
@@ -493,132 +636,38 @@ AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
   const uint32_t end = tv.total;
   const int n = tv.n;
-  // ---- phase 1: greedy descent (closer child first, ties left), tracking
-  // the Morton rank of the first leaf of the current node and the deepest
-  // node X on the path that still holds >= lmin leaves.
+  // ---- phase 1: greedy descent to a leaf near the query, keeping the
+  // deepest node X on the path with about >= lmin leaves (by its slots)
   const int lmin = 4 * m > 32 ? 4 * m : 32;
-  int first = 0;
-  uint32_t p = 0;
-  Box6 cur;
-  float cs[3];
-  uint32_t pX = 0, sX = end;
-  int fX = 0, lenX = n;
-  Box6 DX;
-  {
-    const uint4 s0 = tv.slots[p], s1 = tv.slots[p + 1];
-    if ((int32_t)s0.w >= 0)
-    {
-      ++v_int;
-      node_box<F>(tv, s0, s1, false, cur, cs, cur);
-      set_frame<F>(cur, cur, cs);
-      DX = cur;
-    }
-  }
-  while (true)
-  {
-    const uint4 s0 = tv.slots[p];
-    if ((int32_t)s0.w < 0)
-      break;
-    const uint32_t lp = p + SI;
-    const uint4 l0 = tv.slots[lp], l1 = tv.slots[lp + 1];
-    const int32_t lw = (int32_t)l0.w;
-    const uint32_t lskip = lw < 0 ? SL : (uint32_t)lw;
-    const int llen = lw < 0 ? 1 : (int)((lskip + SI) / (SI + SL));
-    const uint32_t rp = lp + lskip;
-    if (rp >= end)
-    {
-      raise_error(tv.err, KERR_LAYOUT);
-      return;
-    }
-    const uint4 r0 = tv.slots[rp], r1 = tv.slots[rp + 1];
-    const int32_t rw = (int32_t)r0.w;
-    Box6 lb, rb;
-    float dl, dr;
-    v_leaf += (lw < 0) + (rw < 0);
-    v_int += (lw >= 0) + (rw >= 0);
-    if (lw < 0)
-      dl = leaf_d2<G>(qx, qy, qz, l0, l1);
-    else
-    {
-      node_box<F>(tv, l0, l1, true, cur, cs, lb);
-      dl = dist2_pbox(qx, qy, qz, lb);
-    }
-    if (rw < 0)
-      dr = leaf_d2<G>(qx, qy, qz, r0, r1);
-    else
-    {
-      node_box<F>(tv, r0, r1, false, cur, cs, rb);
-      dr = dist2_pbox(qx, qy, qz, rb);
-    }
-    if (dl <= dr)
-    {
-      p = lp;
-      if (lw >= 0)
-      {
-        set_frame<F>(lb, cur, cs);
-        if (llen >= lmin)
-          pX = lp, sX = lskip, fX = first, lenX = llen, DX = lb;
-      }
-    }
-    else
-    {
-      p = rp;
-      first += llen;
-      if (rw >= 0)
-      {
-        set_frame<F>(rb, cur, cs);
-        const int rlen = (int)(((uint32_t)rw + SI) / (SI + SL));
-        if (rlen >= lmin)
-          pX = rp, sX = (uint32_t)rw, fX = first, lenX = rlen, DX = rb;
-      }
-    }
-  }
-  // ---- seed: the W >= m leaves of ranks [j0, j0 + W) around `first`
-  int j0 = first - (W - 1) / 2;
-  j0 = j0 < 0 ? 0 : (j0 > n - W ? n - W : j0);
-  {
-    uint32_t s = SL * (uint32_t)j0 + SI * (uint32_t)tv.C[j0];
-    int got = 0;
-    while (got < W)
-    {
-      if (s >= end)
-      {
-        raise_error(tv.err, KERR_LAYOUT);
-        return;
-      }
-      const uint4 s0 = tv.slots[s];
-      if ((int32_t)s0.w < 0)
-      {
-        ++v_leaf;
-        const uint4 s1 = SL == 2 ? tv.slots[s + 1] : s0;
+  Subtree X;
+  const uint32_t p = greedy_descent<F, G>(tv, qx, qy, qz, (uint32_t)lmin * (SL + SI), X, v_int,
+                                          v_leaf);
+  // ---- seed: the W >= m leaves around it in slot order
+  uint32_t ws = 0, we = 0;
+  slot_window<G>(
+      tv, p, W, ws, we,
+      [&](uint4 s0, uint4 s1) {
         const float dist = sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1));
         const int32_t idx = ~(int32_t)s0.w;
         if (L.accepts(dist, idx))
           L.insert(dist, idx);
-        ++got;
-        s += SL;
-      }
-      else
-      {
-        ++v_int;
-        s += SI;
-      }
-    }
-  }
+      },
+      v_int, v_leaf);
   if (W >= n)
     return; // every leaf is already in the list
   float thr = L.full() ? sqrt_threshold(L.kth_d) : INFINITY;
   float ps[3];
+  Box6 DX = X.box;
   set_frame<F>(DX, DX, ps);
   // ---- phase 2: the local subtree X first (tightens the k-th distance),
   // then the whole tree with X jumped over.  One walk per segment.
-  const bool local = lenX < n;
+  const bool local = X.slots < end;
   for (int seg = local ? 0 : 1; seg < 2; ++seg)
   {
     const bool first_seg = seg == 0;
-    knn_walk<F, G>(tv, qx, qy, qz, L, thr, B, first_seg ? pX + SI : 0u,
-                   first_seg ? pX + sX : end, first_seg ? fX : 0, first_seg, DX, ps,
-                   (local && !first_seg) ? pX : UINT32_MAX, sX, lenX, j0, W, v_int, v_leaf);
+    knn_walk<F, G>(tv, qx, qy, qz, L, thr, B, first_seg ? X.p + SI : 0u,
+                   first_seg ? X.p + X.slots : end, first_seg, DX, ps,
+                   (local && !first_seg) ? X.p : UINT32_MAX, X.slots, ws, we, v_int, v_leaf);
   }
 }
 
@@ -636,193 +685,96 @@ AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
 // of the spatial traversal.
 // Local subtree of a query: the deepest node on its greedy descent path that
 // still holds at least W leaves (slot, slots, decoded box).
-struct LocalSub
-{
-  uint32_t p, slots;
-  Box6 box;
-  bool valid;
-};
-
 template <int F, int G, int K>
 AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, int W,
-                     LocalSub &X, uint32_t &v_int, uint32_t &v_leaf)
+                     uint32_t &v_int, uint32_t &v_leaf)
 {
-  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
-  const uint32_t end = tv.total;
-  const int n = tv.n;
-  int first = 0;
-  uint32_t p = 0;
-  Box6 cur;
-  float cs[3];
-  X.valid = false;
-  X.p = 0;
-  X.slots = 0;
-  {
-    const uint4 s0 = tv.slots[0], s1 = tv.slots[1];
-    if ((int32_t)s0.w >= 0)
-    {
-      ++v_int;
-      node_box<F>(tv, s0, s1, false, cur, cs, cur);
-      set_frame<F>(cur, cur, cs);
-      X.box = cur;
-    }
-  }
-  while (true)
-  {
-    const uint4 s0 = tv.slots[p];
-    if ((int32_t)s0.w < 0)
-      break;
-    const uint32_t lp = p + SI;
-    const uint4 l0 = tv.slots[lp], l1 = tv.slots[lp + 1];
-    const int32_t lw = (int32_t)l0.w;
-    const uint32_t lskip = lw < 0 ? SL : (uint32_t)lw;
-    const uint32_t rp = lp + lskip;
-    if (rp >= end)
-    {
-      raise_error(tv.err, KERR_LAYOUT);
-      return INFINITY;
-    }
-    const uint4 r0 = tv.slots[rp], r1 = tv.slots[rp + 1];
-    const int32_t rw = (int32_t)r0.w;
-    Box6 lb, rb;
-    float dl, dr;
-    v_leaf += (lw < 0) + (rw < 0);
-    v_int += (lw >= 0) + (rw >= 0);
-    if (lw < 0)
-      dl = leaf_d2<G>(qx, qy, qz, l0, l1);
-    else
-    {
-      node_box<F>(tv, l0, l1, true, cur, cs, lb);
-      dl = dist2_pbox(qx, qy, qz, lb);
-    }
-    if (rw < 0)
-      dr = leaf_d2<G>(qx, qy, qz, r0, r1);
-    else
-    {
-      node_box<F>(tv, r0, r1, false, cur, cs, rb);
-      dr = dist2_pbox(qx, qy, qz, rb);
-    }
-    if (dl <= dr)
-    {
-      p = lp;
-      if (lw >= 0)
-      {
-        set_frame<F>(lb, cur, cs);
-        if ((int)((lskip + SI) / (SI + SL)) >= W)
-          X = LocalSub{lp, lskip, lb, true};
-      }
-    }
-    else
-    {
-      p = rp;
-      first += lw < 0 ? 1 : (int)((lskip + SI) / (SI + SL));
-      if (rw >= 0)
-      {
-        set_frame<F>(rb, cur, cs);
-        if ((int)(((uint32_t)rw + SI) / (SI + SL)) >= W)
-          X = LocalSub{rp, (uint32_t)rw, rb, true};
-      }
-    }
-  }
-  int j0 = first - (W - 1) / 2;
-  j0 = j0 < 0 ? 0 : (j0 > n - W ? n - W : j0);
+  Subtree X;
+  const uint32_t p = greedy_descent<F, G>(tv, qx, qy, qz, UINT32_MAX, X, v_int, v_leaf);
   DistList<K> L;
   L.init();
-  uint32_t s = SL * (uint32_t)j0 + SI * (uint32_t)tv.C[j0];
-  int got = 0;
-  while (got < W)
-  {
-    if (s >= end)
-    {
-      raise_error(tv.err, KERR_LAYOUT);
-      return INFINITY;
-    }
-    const uint4 s0 = tv.slots[s];
-    if ((int32_t)s0.w < 0)
-    {
-      ++v_leaf;
-      const uint4 s1 = SL == 2 ? tv.slots[s + 1] : s0;
-      L.insert(sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1)));
-      ++got;
-      s += SL;
-    }
-    else
-    {
-      ++v_int;
-      s += SI;
-    }
-  }
+  uint32_t ws, we;
+  slot_window<G>(
+      tv, p, W, ws, we,
+      [&](uint4 s0, uint4 s1) { L.insert(sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1))); }, v_int,
+      v_leaf);
   return L.kth(k);
 }
 
-// Fixed-radius rope walk appending every leaf with d2 <= thr to `row`:
-// the local subtree X first, then the rest of the tree with X jumped over,
-// so that the first entries of the row are the query's near leaves.  Stops
-// as soon as more than `cap` leaves qualify; returns the count (cap + 1 on
-// overflow).
+// Fixed-radius rope walk appending (d2, index) of every leaf with
+// d2 <= thr to `row`.  Stops as soon as more than `cap` leaves qualify;
+// returns the count (cap + 1 on overflow).
 template <int F, int G>
 AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, float thr,
-                           const LocalSub &X, uint2 *row, int cap, uint32_t &v_int,
-                           uint32_t &v_leaf)
+                           uint2 *row, int cap, uint32_t &v_int, uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
   int c = 0;
-  for (int seg = X.valid ? 0 : 1; seg < 2 && c <= cap; ++seg)
+  uint32_t p = 0;
+  const uint32_t end = tv.total;
+  Box6 par;
+  float ps[3];
+  bool desc = false;
+  while (p < end)
   {
-    const bool local = seg == 0;
-    uint32_t p = local ? X.p + SI : 0u;
-    const uint32_t end = local ? X.p + X.slots : tv.total;
-    const uint32_t skip_at = (!local && X.valid) ? X.p : UINT32_MAX;
-    Box6 par = X.box;
-    float ps[3];
-    set_frame<F>(X.box, par, ps);
-    bool desc = local;
-    while (p < end)
+    const uint4 s0 = tv.slots[p];
+    const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
+    const int32_t w = (int32_t)s0.w;
+    if (w < 0)
     {
-      if (p == skip_at)
+      // this leaf and the leaf records right after it
+      constexpr int LG = leaf_group<G>();
+      uint4 a[LG], b[LG];
+      a[0] = s0, b[0] = s1;
+#pragma unroll
+      for (int j = 1; j < LG; ++j)
       {
-        p += X.slots;
-        desc = false;
-        continue;
+        a[j] = tv.slots[p + j * SL];
+        b[j] = G == GEOM_BOXES ? tv.slots[p + j * SL + 1] : a[j];
       }
-      const uint4 s0 = tv.slots[p];
-      const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
-      const int32_t w = (int32_t)s0.w;
-      if (w < 0)
+      // p advances with each leaf handled, so it is record j's slot here
+      bool run = true;
+#pragma unroll
+      for (int j = 0; j < LG; ++j)
       {
-        ++v_leaf;
-        const float d2 = leaf_d2<G>(qx, qy, qz, s0, s1);
-        if (d2 <= thr)
+        run = run && c <= cap && p < end && (int32_t)a[j].w < 0;
+        if (run)
         {
-          if (c < cap)
-            row[c] = make_uint2(f_bits(d2), (uint32_t)~w);
-          if (++c > cap)
-            break;
+          ++v_leaf;
+          const float d2 = leaf_d2<G>(qx, qy, qz, a[j], b[j]);
+          if (d2 <= thr)
+          {
+            if (c < cap)
+              row[c] = make_uint2(f_bits(d2), (uint32_t)~(int32_t)a[j].w);
+            ++c;
+          }
+          p += SL;
         }
-        p += SL;
-        desc = false;
+      }
+      if (c > cap)
+        break;
+      desc = false;
+    }
+    else
+    {
+      ++v_int;
+      Box6 b;
+      node_box<F>(tv, s0, s1, desc, par, ps, b);
+      if (dist2_pbox(qx, qy, qz, b) <= thr)
+      {
+        set_frame<F>(b, par, ps);
+        desc = true;
+        p += SI;
       }
       else
       {
-        ++v_int;
-        Box6 b;
-        node_box<F>(tv, s0, s1, desc, par, ps, b);
-        if (dist2_pbox(qx, qy, qz, b) <= thr)
+        if (w == 0)
         {
-          set_frame<F>(b, par, ps);
-          desc = true;
-          p += SI;
+          raise_error(tv.err, KERR_LAYOUT);
+          return c;
         }
-        else
-        {
-          if (w == 0)
-          {
-            raise_error(tv.err, KERR_LAYOUT);
-            return c;
-          }
-          desc = false;
-          p += (uint32_t)w;
-        }
+        desc = false;
+        p += (uint32_t)w;
       }
     }
   }
@@ -854,8 +806,7 @@ __global__ __launch_bounds__(256) void k_knn_collect(
   const float qx = pts[3 * q], qy = pts[3 * q + 1], qz = pts[3 * q + 2];
   const int m = k < tv.n ? k : tv.n;
   uint32_t v_int = 0, v_leaf = 0;
-  LocalSub X;
-  float R = knn_seed<F, G, K>(tv, qx, qy, qz, k, W, X, v_int, v_leaf);
+  float R = knn_seed<F, G, K>(tv, qx, qy, qz, k, W, v_int, v_leaf);
   uint2 *row = cand + g * cap;
   int c = 0;
   bool done = false;
@@ -865,7 +816,7 @@ __global__ __launch_bounds__(256) void k_knn_collect(
   for (int round = 0; round < KNN_ROUNDS; ++round)
   {
     ++rounds;
-    c = knn_collect_walk<F, G>(tv, qx, qy, qz, sqrt_threshold(R), X, row, cap, v_int, v_leaf);
+    c = knn_collect_walk<F, G>(tv, qx, qy, qz, sqrt_threshold(R), row, cap, v_int, v_leaf);
     if (c >= m && c <= cap)
     {
       done = true;

@@ -13,8 +13,7 @@ import ctypes
 
 import torch
 
-from . import (BoundingVolumeHierarchy, TraversalPolicy, _check, _dev_f32, _ptr, _stream,
-               lib, make_intersects)
+from . import BoundingVolumeHierarchy, _check, _dev_f32, _ptr, _stream, lib
 
 
 def pair_energy(xyz, offsets, indices, cutoff, want_energy=True, want_grad=True, stream=None):
@@ -41,11 +40,11 @@ def pair_energy_fd(xyz, offsets, indices, cutoff, atoms, h=1e-6, stream=None):
     return fd
 
 
-def neighbour_csr(xyz, cutoff, node_format="fp32"):
-    """Full radius-neighbour CRS of the points with themselves (self included)."""
+def neighbour_csr(xyz, cutoff, node_format="q16"):
+    """Full neighbour list of the points within `cutoff` (row i: every j != i,
+    findFullNeighborList semantics) over a tree built from xyz."""
     bvh = BoundingVolumeHierarchy(xyz, node_format=node_format)
-    indices, offsets = bvh.query(make_intersects(xyz, cutoff), TraversalPolicy())
-    return offsets, indices
+    return bvh.neighbor_list(cutoff, full=True)
 
 
 class PairEnergy(torch.autograd.Function):

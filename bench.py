@@ -78,6 +78,24 @@ def calibrate_radius(ax, p, r, target, sample=100, iters=4):
     return float(np.float32(r))
 
 
+PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_uniform10M_q16.json")
+
+
+def measured_traffic(kernel, case):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC runs
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy kernel), for
+    the workload they were taken on; None otherwise."""
+    if case != "uniform/q16" or not os.path.exists(PMC_TRAFFIC):
+        return None
+    with open(PMC_TRAFFIC) as f:
+        data = json.load(f)
+    base = kernel.strip("()").split("<")[0]
+    for name, v in data["kernels"].items():
+        if name.split("<")[0] == base:
+            return v["hbm_bytes_per_dispatch"]
+    return None
+
+
 def timed(fn, steps, warmup, stream):
     """Mean ms of fn() over `steps` runs after `warmup`, from HIP events on
     the current stream."""
@@ -184,14 +202,14 @@ def bench_energy(ax, args, torch, synthetic, stream):
     ga = g.cpu().numpy().astype(np.float64)[atoms]
     rel = float(np.linalg.norm(ga - fd) / np.linalg.norm(fd))
     alg_f = n * 12 + nnz * 16 + (n + 1) * 8
-    out = dict(n_points=n, cells=list(cells), cutoff=rc, mean_neighbours=nnz / n - 1,
+    out = dict(n_points=n, cells=list(cells), cutoff=rc, mean_neighbours=nnz / n,
                energy=float(e.item()), csr_ms=t_csr, forward_ms=t_f, fwd_bwd_ms=t_fb,
                fwd_bwd_over_forward=t_fb / t_f, forward_kernel_ms=k_f, fwd_bwd_kernel_ms=k_fb,
                forward_GBps=alg_f / (k_f * 1e-3) / 1e9,
                fwd_bwd_GBps=(alg_f + n * 12) / (k_fb * 1e-3) / 1e9,
                autograd_step_ms=t_ag, autograd_recompute_csr_step_ms=t_agr,
                grad_rel_err_vs_fp64_fd=rel, fd_atoms=1000, fd_h=1e-6)
-    log(f"[bench] energy: N={n}, {nnz / n - 1:.1f} nbrs | CSR {t_csr:.2f} ms | fwd {t_f:.3f} ms "
+    log(f"[bench] energy: N={n}, {nnz / n:.1f} nbrs | CSR {t_csr:.2f} ms | fwd {t_f:.3f} ms "
         f"| fwd+bwd {t_fb:.3f} ms (x{t_fb / t_f:.2f}) | autograd {t_ag:.2f} ms, with CSR "
         f"recompute {t_agr:.2f} ms | grad rel err vs FP64 FD {rel:.2e}")
     return out
@@ -393,8 +411,9 @@ def main():
             tot_ms = v["total_ms"]
         launches = kernels[name]["launches"]
         achieved = alg / (tot_ms * 1e-3) / 1e9
+        traffic = measured_traffic(name, head_key)
         roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                        unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                        unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                         kernel=name, launches_per_call=launches,
                         avg_launch_ms=round(tot_ms / launches, 4),
                         alg_bytes_per_launch=int(alg / launches),

@@ -101,6 +101,17 @@ axb_status axb_set_output_allocator(axb_alloc_fn alloc, axb_free_fn free_fn,
                                     void *ctx);
 /* Releases an array returned by a query when the default allocator is in use. */
 axb_status axb_free(void *device_ptr, axb_stream_t stream);
+/* The library's temporaries come from the current device's default
+ * stream-ordered memory pool, which the library sets to keep freed memory
+ * cached (returning it to the driver at every synchronisation costs
+ * milliseconds per call).  This returns the cached memory of the current
+ * device's default pool to the driver. */
+axb_status axb_trim_memory(void);
+
+/* Node array layout: an internal node whose subtree holds at most this many
+ * leaves is not stored; its leaves follow each other in the pre-order array
+ * (the walk tests each of them, so results are unchanged). */
+#define AXB_FLAT_SUBTREE_MAX 4
 
 /* Build: BoundingVolumeHierarchy(space, values[, indexable_getter, Morton64])
  * (src/spatial/ArborX_LinearBVH.hpp:68-73,167-256).  Values are points or
@@ -259,6 +270,11 @@ axb_status axb_bvh_export_topology(const axb_bvh *bvh, uint64_t *keys,
                                    uint32_t *perm, int32_t *leaf_rope,
                                    int32_t *left_child, int32_t *rope,
                                    float *boxes, axb_stream_t stream);
+
+/* Q16 encoding of one box against a frame (host; the function the build's
+ * encoder runs): q = {qlo xyz, qhi xyz}, the tightest outward-rounded steps
+ * (see the node layout).  For tests. */
+void axb_q16_encode_host(const float frame[6], const float box[6], uint16_t q[6]);
 
 /* ---- profiling (replaces the Kokkos Tools labels the reference puts on
  * every region and kernel, spatial/ArborX_LinearBVH.hpp:201,217-255) ----

@@ -10,6 +10,8 @@
 // and sqrt).
 #pragma once
 
+#include "arborx_mi355x.h"
+
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -32,7 +34,11 @@ namespace axb
 //   point leaf    : 1 slot  {x y z ~index}
 //   box leaf      : 2 slots {lo.x lo.y lo.z ~index} {hi.x hi.y hi.z 0}
 // The left child of an internal node is the next record; its right child is
-// the rope of its left child (HappyTreeFriends.hpp:73-78).
+// the rope of its left child (HappyTreeFriends.hpp:73-78).  An internal node
+// whose subtree holds at most FLAT_MAX leaves is not stored: its leaves
+// follow each other in the array (a walk reaching the subtree tests them
+// all, so results and their order are unchanged).
+constexpr int FLAT_MAX = AXB_FLAT_SUBTREE_MAX;
 constexpr int GEOM_POINTS = 0;
 constexpr int GEOM_BOXES = 1;
 constexpr int NODES_FP32 = 0;
@@ -210,6 +216,50 @@ AXB_HD float q16_hi(float fhi, float s, uint32_t q)
 {
   return __builtin_fmaf(-(float)(65535u - q), s, fhi);
 }
+// Largest q in [0, 65535] with lo(q) <= v (0 when there is none).  lo(q)
+// is non-decreasing in q; the search starts from the window around the
+// estimate (v - flo) / s and widens to the full range when the window does
+// not bracket the answer, so the result is the plain binary search's.
+AXB_HD uint32_t q16_lo_index(float flo, float s, float v)
+{
+  const float est = (v - flo) / s;
+  const int q = est >= 0.f ? (est <= 65535.f ? (int)est : 65535) : 0; // NaN -> 0
+  int a = q - 2 < 0 ? 0 : q - 2, b = q + 2 > 65535 ? 65535 : q + 2;
+  if (!((a == 0 || q16_lo(flo, s, (uint32_t)a) <= v) &&
+        (b == 65535 || q16_lo(flo, s, (uint32_t)(b + 1)) > v)))
+    a = 0, b = 65535;
+  while (a < b)
+  {
+    const int m = (a + b + 1) >> 1;
+    if (q16_lo(flo, s, (uint32_t)m) <= v)
+      a = m;
+    else
+      b = m - 1;
+  }
+  return (uint32_t)a;
+}
+
+// Smallest q in [0, 65535] with hi(q) >= v (65535 when there is none);
+// hi(q) is non-decreasing in q.  Same bracketing as q16_lo_index.
+AXB_HD uint32_t q16_hi_index(float fhi, float s, float v)
+{
+  const float est = (fhi - v) / s; // steps below fhi
+  const int q = est >= 0.f ? (est <= 65535.f ? 65535 - (int)est : 0) : 65535;
+  int a = q - 2 < 0 ? 0 : q - 2, b = q + 2 > 65535 ? 65535 : q + 2;
+  if (!((b == 65535 || q16_hi(fhi, s, (uint32_t)b) >= v) &&
+        (a == 0 || q16_hi(fhi, s, (uint32_t)(a - 1)) < v)))
+    a = 0, b = 65535;
+  while (a < b)
+  {
+    const int m = (a + b) >> 1;
+    if (q16_hi(fhi, s, (uint32_t)m) >= v)
+      b = m;
+    else
+      a = m + 1;
+  }
+  return (uint32_t)a;
+}
+
 AXB_HD void q16_encode_axis(float flo, float fhi, float vlo, float vhi,
                             uint32_t &qlo, uint32_t &qhi)
 {
@@ -219,26 +269,8 @@ AXB_HD void q16_encode_axis(float flo, float fhi, float vlo, float vhi,
     qlo = 0, qhi = 65535;
     return;
   }
-  int a = 0, b = 65535;
-  while (a < b)
-  {
-    int m = (a + b + 1) >> 1;
-    if (q16_lo(flo, s, (uint32_t)m) <= vlo)
-      a = m;
-    else
-      b = m - 1;
-  }
-  qlo = (uint32_t)a;
-  a = 0, b = 65535;
-  while (a < b)
-  {
-    int m = (a + b) >> 1;
-    if (q16_hi(fhi, s, (uint32_t)m) >= vhi)
-      b = m;
-    else
-      a = m + 1;
-  }
-  qhi = (uint32_t)a;
+  qlo = q16_lo_index(flo, s, vlo);
+  qhi = q16_hi_index(fhi, s, vhi);
 }
 
 // Decodes a Q16 record (words x, y, z) against frame (flo, fhi, scale).
@@ -344,11 +376,45 @@ AXB_D SpatialPred box_pred(const Box6 &b)
 // walk starts at the root or at a record whose frame is the scene box.
 // TV: a tree view with `const uint4 *slots`, `Box6 scene`, `float ss[3]` and
 // `uint32_t *err`.  PK: PRED_BOXES tests the box, otherwise the sphere.
+template <int G, int PK>
+AXB_D bool leaf_hit(const SpatialPred &pr, uint4 s0, uint4 s1)
+{
+  const float x = bits_f(s0.x), y = bits_f(s0.y), z = bits_f(s0.z);
+  if (G == GEOM_POINTS)
+  {
+    if (PK == PRED_BOXES)
+      return !(x > pr.qb.hi[0] || x < pr.qb.lo[0] || y > pr.qb.hi[1] || y < pr.qb.lo[1] ||
+               z > pr.qb.hi[2] || z < pr.qb.lo[2]);
+    return dist2_pp(pr.cx, pr.cy, pr.cz, x, y, z) <= pr.thr;
+  }
+  const Box6 lb = {{x, y, z}, {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
+  return PK == PRED_BOXES ? intersects_bb(pr.qb, lb)
+                          : dist2_pbox(pr.cx, pr.cy, pr.cz, lb) <= pr.thr;
+}
+
+// Leaf records handled per step of a walk: a run of consecutive leaves is
+// read together (64 bytes).  The node array keeps LEAF_GROUP * SL - 1 zeroed
+// slots after its last record for this read-ahead.
+template <int G>
+constexpr int leaf_group()
+{
+  return G == GEOM_POINTS ? 4 : 2;
+}
+
+// Stackless pre-order walk over slots [p, end) (TreeTraversal.hpp:97-119:
+// leaf -> test, next record; internal -> test ? first child : rope).  Each
+// hit calls sink.hit(value), which returns false to stop the walk (the
+// reference's CallbackTreeTraversalControl::early_exit,
+// Callbacks.hpp:96-118).  The leaf record at `skip` is not reported.  A Q16
+// walk starts at the root or at a record whose frame is the scene box.
+// TV: a tree view with `const uint4 *slots`, `Box6 scene`, `float ss[3]` and
+// `uint32_t *err`.  PK: PRED_BOXES tests the box, otherwise the sphere.
 template <int F, int G, int PK, class TV, class Sink>
 AXB_D void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
                      uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  constexpr int LG = leaf_group<G>();
   Box6 par;
   float ps[3];
   bool desc = false;
@@ -359,38 +425,43 @@ AXB_D void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t e
     const int32_t w = (int32_t)s0.w;
     if (w < 0)
     {
-      ++v_leaf;
-      bool hit;
-      const float x = bits_f(s0.x), y = bits_f(s0.y), z = bits_f(s0.z);
-      if (G == GEOM_POINTS)
+      // this leaf and the leaf records right after it, in order
+      uint4 a[LG], b[LG];
+      a[0] = s0, b[0] = s1;
+#pragma unroll
+      for (int j = 1; j < LG; ++j)
       {
-        if (PK == PRED_BOXES)
-          hit = !(x > pr.qb.hi[0] || x < pr.qb.lo[0] || y > pr.qb.hi[1] || y < pr.qb.lo[1] ||
-                  z > pr.qb.hi[2] || z < pr.qb.lo[2]);
-        else
-          hit = dist2_pp(pr.cx, pr.cy, pr.cz, x, y, z) <= pr.thr;
+        a[j] = tv.slots[p + j * SL];
+        b[j] = G == GEOM_BOXES ? tv.slots[p + j * SL + 1] : a[j];
       }
-      else
+      // p advances with each leaf handled, so it is record j's slot here
+      bool run = true, stop = false;
+#pragma unroll
+      for (int j = 0; j < LG; ++j)
       {
-        const Box6 lb = {{x, y, z}, {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
-        hit = PK == PRED_BOXES ? intersects_bb(pr.qb, lb)
-                               : dist2_pbox(pr.cx, pr.cy, pr.cz, lb) <= pr.thr;
+        run = run && !stop && p < end && (int32_t)a[j].w < 0;
+        if (run)
+        {
+          ++v_leaf;
+          if (p != skip && leaf_hit<G, PK>(pr, a[j], b[j]) && !sink.hit(~(int32_t)a[j].w))
+            stop = true;
+          p += SL;
+        }
       }
-      if (hit && p != skip && !sink.hit(~w))
+      if (stop)
         break;
-      p += SL;
       desc = false;
     }
     else
     {
       ++v_int;
-      Box6 b;
-      node_box<F>(tv, s0, s1, desc, par, ps, b);
-      const bool pass = PK == PRED_BOXES ? intersects_bb(pr.qb, b)
-                                         : dist2_pbox(pr.cx, pr.cy, pr.cz, b) <= pr.thr;
+      Box6 bx;
+      node_box<F>(tv, s0, s1, desc, par, ps, bx);
+      const bool pass = PK == PRED_BOXES ? intersects_bb(pr.qb, bx)
+                                         : dist2_pbox(pr.cx, pr.cy, pr.cz, bx) <= pr.thr;
       if (pass)
       {
-        set_frame<F>(b, par, ps);
+        set_frame<F>(bx, par, ps);
         desc = true;
         p += SI;
       }

@@ -61,7 +61,7 @@ def lib():
             "orc_bounds": (None, [P, P]),
             "orc_export": (None, [P, P, P, P, P, P, P]),
             "orc_q16_encode_box": (None, [P, P, P, P]),
-            "orc_linearize": (I64, [P, I32, P]),
+            "orc_linearize": (I64, [P, I32, I32, P]),
             "orc_query_spatial": (I64, [P, I32, P, I64, P, P, P, P]),
             "orc_bruteforce_spatial": (I64, [P, I64, I32, I32, P, I64, P, P]),
             "orc_query_knn": (I64, [P, P, I64, I32, P, P, P, P, P]),
@@ -173,14 +173,16 @@ class Tree:
         return dict(keys=keys, perm=perm, leaf_rope=leaf_rope,
                     left_child=left_child, rope=rope, boxes=boxes)
 
-    def linearize(self, node_format=0):
-        """Pre-order slot array (uint32, shape (slots, 4))."""
+    def linearize(self, node_format=0, flat_max=1):
+        """Pre-order slot array (uint32, shape (slots, 4)).  Subtrees of at
+        most `flat_max` leaves are stored as their leaves only (1: every
+        internal node is stored)."""
         n = self.n
         s_leaf = 1 if self.geometry == 0 else 2
         s_int = 2 if node_format == 0 else 1
         cap = max(n * s_leaf + max(n - 1, 0) * s_int, 1)
         words = np.zeros((cap, 4), np.uint32)
-        slots = lib().orc_linearize(self._h, node_format, _ptr(words))
+        slots = lib().orc_linearize(self._h, node_format, int(flat_max), _ptr(words))
         return words[:slots]
 
     def query_spatial(self, kind, preds, with_visits=False):

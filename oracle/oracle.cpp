@@ -631,7 +631,7 @@ void orc_q16_encode_box(const float *frame6, const float *box6, uint16_t *q6,
 
 // Writes the pre-order slot array (uint32 words, 4 per slot).  Returns the
 // number of slots.  node_format 0 = FP32 internal nodes, 1 = Q16.
-int64_t orc_linearize(void *h, int node_format, uint32_t *words)
+int64_t orc_linearize(void *h, int node_format, int flat_max, uint32_t *words)
 {
   Tree *t = (Tree *)h;
   int64_t n = t->n;
@@ -663,7 +663,7 @@ int64_t orc_linearize(void *h, int node_format, uint32_t *words)
   // explicit DFS: (node, frame for Q16, subtree leaf count unknown -> compute)
   // subtree leaf counts via a post-order pass
   const int64_t N = n;
-  std::vector<int64_t> len(n - 1, 0);
+  std::vector<int64_t> len(n - 1, 0), first(n - 1, 0), nslots(n - 1, 0);
   auto right_of = [&](int32_t node) -> int32_t {
     int32_t lc = t->left_child[node - N];
     return lc < N ? t->leaf_rope[lc] : t->rope[lc - N];
@@ -690,7 +690,14 @@ int64_t orc_linearize(void *h, int node_format, uint32_t *words)
       {
         int32_t lc = t->left_child[node - N];
         int32_t rc = right_of(node);
-        len[node - N] = (lc < N ? 1 : len[lc - N]) + (rc < N ? 1 : len[rc - N]);
+        const int64_t k = node - N;
+        len[k] = (lc < N ? 1 : len[lc - N]) + (rc < N ? 1 : len[rc - N]);
+        first[k] = lc < N ? lc : first[lc - N];
+        // a subtree of at most flat_max leaves is stored as its leaves only
+        if (len[k] <= flat_max)
+          nslots[k] = s_leaf * len[k];
+        else
+          nslots[k] = s_int + (lc < N ? s_leaf : nslots[lc - N]) + (rc < N ? s_leaf : nslots[rc - N]);
         st.pop_back();
       }
     }
@@ -717,8 +724,16 @@ int64_t orc_linearize(void *h, int node_format, uint32_t *words)
       continue;
     }
     int64_t k = it.node - N;
-    int64_t L = len[k];
-    uint32_t skip = (uint32_t)(s_int * (L - 1) + s_leaf * L);
+    if (len[k] <= flat_max)
+    {
+      for (int64_t i = first[k]; i < first[k] + len[k]; ++i)
+      {
+        write_leaf(slot, i);
+        slot += s_leaf;
+      }
+      continue;
+    }
+    uint32_t skip = (uint32_t)nslots[k];
     int64_t w = slot * 4;
     float box[6];
     for (int d = 0; d < 3; ++d)

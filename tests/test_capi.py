@@ -81,3 +81,33 @@ def test_policy_and_predicates_mirror_reference_defaults():
     assert p.setBufferSize(-1).setPredicateSorting(False)._c().buffer_size == -1
     with pytest.raises(ValueError):
         arborx_amd.make_intersects(np.zeros((1, 3)), 0.0)  # PredicateHelpers.hpp:102
+
+
+def test_flat_subtree_constant_matches_header():
+    import re
+    with open(os.path.join(ROOT, "include", "arborx_mi355x.h")) as f:
+        m = re.search(r"#define AXB_FLAT_SUBTREE_MAX (\d+)", f.read())
+    assert m and int(m.group(1)) == arborx_amd.FLAT_SUBTREE_MAX
+
+
+def test_q16_encoder_matches_oracle():
+    """The build's Q16 encoder (bracketed search from an estimate) against the
+    oracle's plain binary search, on random, tiny, huge and degenerate
+    frames and on boxes touching the frame."""
+    import oracle
+    rng = np.random.default_rng(11)
+    for t in range(4000):
+        scale = [1e-30, 1e-6, 1.0, 1e6, 1e30][t % 5]
+        f = np.sort(rng.uniform(-1, 1, (2, 3)) * scale, axis=0).astype(np.float32)
+        if t % 97 == 0:
+            f[1, 0] = f[0, 0]  # flat axis
+        frame = np.concatenate([f[0], f[1]])
+        u = np.sort(rng.uniform(0, 1, (2, 3)), axis=0)
+        if t % 7 == 0:
+            u[0, :] = 0.0
+            u[1, :] = 1.0
+        box = np.concatenate([f[0] + (f[1] - f[0]) * u[0], f[0] + (f[1] - f[0]) * u[1]])
+        box = box.astype(np.float32)
+        box = np.concatenate([np.maximum(box[:3], frame[:3]), np.minimum(box[3:], frame[3:])])
+        q_ref, _ = oracle.q16_encode_box(frame, box)
+        np.testing.assert_array_equal(arborx_amd.q16_encode_host(frame, box), q_ref, err_msg=str(t))

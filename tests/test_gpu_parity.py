@@ -79,7 +79,7 @@ def test_preorder_nodes_bit_exact(node_format, kind, n):
     v, g, t = oracle_tree(kind, n)
     bvh = ax.BoundingVolumeHierarchy(dev(v), geometry="points" if g == 0 else "boxes",
                                      node_format=node_format)
-    ref = t.linearize(0 if node_format == "fp32" else 1)
+    ref = t.linearize(0 if node_format == "fp32" else 1, flat_max=ax.FLAT_SUBTREE_MAX)
     got = host(bvh.slots()).view(np.uint32)
     assert got.shape == ref.shape
     np.testing.assert_array_equal(got, ref)

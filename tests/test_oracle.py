@@ -194,6 +194,74 @@ def test_linearized_layout(node_format):
     np.testing.assert_array_equal(leaves, e["perm"])
 
 
+def _rope_walk_rows(w, s_int, s_leaf, spheres):
+    """Rows of a sphere query by the stackless walk over a pre-order slot
+    array (leaf: test, next record; internal: FP32 box test, left child =
+    next record, else jump by the stored skip)."""
+    lo = w[:, :3].view(np.float32)
+    hi = np.roll(w, -1, axis=0)[:, :3].view(np.float32)
+    word = w[:, 3].astype(np.int64)
+    word = np.where(word >= (1 << 31), word - (1 << 32), word)
+    rows = []
+    for c in spheres:
+        r2 = np.float32(c[3]) ** 2
+        row, p = [], 0
+        while p < w.shape[0]:
+            if word[p] < 0:
+                d = lo[p] - c[:3]
+                if np.sqrt(np.float32(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])) <= c[3]:
+                    row.append(~int(word[p]))
+                p += s_leaf
+            else:
+                q = np.clip(c[:3], lo[p], hi[p]) - c[:3]
+                if np.float32(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) <= r2 * 1.0001:
+                    p += s_int
+                else:
+                    p += int(word[p])
+        rows.append(row)
+    return rows
+
+
+@pytest.mark.parametrize("flat_max", [1, 2, 4, 7])
+def test_flattened_layout(flat_max):
+    """Subtrees of at most flat_max leaves stored as their leaves only: the
+    slot count is n + 2 * (internal nodes over more leaves), leaves keep the
+    Morton order, and the rope walk over the array returns the reference
+    rows in the reference's order."""
+    n = 3000
+    v = synthetic.uniform_points(n, 15)
+    t = oracle.Tree(v)
+    e = t.export()
+    w = t.linearize(0, flat_max=flat_max)
+    # subtree sizes from the reference-form topology
+    lc, rope, lrope = e["left_child"], e["rope"], e["leaf_rope"]
+    size = {}
+
+    def right(k):
+        c = lc[k]
+        return lrope[c] if c < n else rope[c - n]
+
+    order, st = [], [n]
+    while st:
+        x = st.pop()
+        if x >= n:
+            order.append(x)
+            st += [lc[x - n], right(x - n)]
+    for x in reversed(order):
+        size[x] = sum(1 if c < n else size[c] for c in (lc[x - n], right(x - n)))
+    stored = sum(1 for x in order if size[x] > flat_max)
+    assert w.shape[0] == n + 2 * stored
+    word = w[:, 3].astype(np.int64)
+    leaves = [~int(x - (1 << 32)) for x in word if x >= (1 << 31)]
+    np.testing.assert_array_equal(leaves, e["perm"])
+    r = np.float32(synthetic.radius_for_neighbours(n, 20))
+    sph = spheres_from_points(v[::150], r)
+    o_ref, i_ref = t.query_spatial(0, sph)
+    got = _rope_walk_rows(w, 2, 1, sph)
+    for q, row in enumerate(got):
+        assert row == list(i_ref[o_ref[q]:o_ref[q + 1]])
+
+
 def test_q16_encoder_is_tight_and_conservative():
     rng = np.random.default_rng(7)
     for _ in range(2000):
