@@ -164,24 +164,55 @@ AXB_D int32_t apetrei_rope(const uint64_t *__restrict__ keys, int nint, int n,
 }
 
 // Hand-off between the two threads that meet at a parent.  The data one
-// thread leaves for the other (its node's box and spine height, or its leaf)
-// is written and read with relaxed agent-scope atomics, which are coherent
-// across the XCDs' L2s on their own; the writer waits for those stores to
-// complete before its CAS.  This gives the release/acquire pairing of
-// TreeConstruction.hpp:241-265 (CAS + load_fence) for exactly the words that
-// cross threads, without the whole-cache writeback/invalidate that an
-// acq_rel CAS at agent scope costs on gfx950.
-AXB_D void st_coh64(void *p, uint64_t v)
+// thread leaves for the other (its node's record with box and spine height,
+// or its leaf record) is written and read with 16-byte accesses that go
+// through the XCD's L2 to memory (the sc1 policy that agent-scope atomics
+// use), so they are coherent across the XCDs' L2s on their own; the writer
+// waits for those stores to complete before its CAS.  This gives the
+// release/acquire pairing of TreeConstruction.hpp:241-265 (CAS + load_fence)
+// for exactly the words that cross threads, without the whole-cache
+// writeback/invalidate that an acq_rel CAS at agent scope costs on gfx950.
+// The record loads are issued together and waited for once.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+AXB_D uint4 to_uint4(u32x4 v) { return make_uint4(v.x, v.y, v.z, v.w); }
+AXB_D void st_coh128(void *p, uint4 v)
 {
-  __hip_atomic_store((uint64_t *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const u32x4 x = {v.x, v.y, v.z, v.w};
+  asm volatile("global_store_dwordx4 %0, %1, off sc1" : : "v"(p), "v"(x) : "memory");
 }
-AXB_D uint64_t ld_coh64(const void *p)
+AXB_D uint4 ld_coh128(const void *p)
 {
-  return __hip_atomic_load((const uint64_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  u32x4 a;
+  asm volatile("global_load_dwordx4 %0, %1, off sc1\n\t"
+               "s_waitcnt vmcnt(0)"
+               : "=&v"(a)
+               : "v"(p)
+               : "memory");
+  return to_uint4(a);
 }
-AXB_D uint64_t pack2(uint32_t a, uint32_t b) { return (uint64_t)a | ((uint64_t)b << 32); }
-AXB_D uint32_t lo32(uint64_t w) { return (uint32_t)w; }
-AXB_D uint32_t hi32(uint64_t w) { return (uint32_t)(w >> 32); }
+AXB_D void ld_coh128x2(const void *p, uint4 &a, uint4 &b)
+{
+  u32x4 x, y;
+  asm volatile("global_load_dwordx4 %0, %2, off sc1\n\t"
+               "global_load_dwordx4 %1, %2, off offset:16 sc1\n\t"
+               "s_waitcnt vmcnt(0)"
+               : "=&v"(x), "=&v"(y)
+               : "v"(p)
+               : "memory");
+  a = to_uint4(x), b = to_uint4(y);
+}
+AXB_D void ld_coh128x3(const void *p, uint4 &a, uint4 &b, uint4 &c)
+{
+  u32x4 x, y, z;
+  asm volatile("global_load_dwordx4 %0, %3, off sc1\n\t"
+               "global_load_dwordx4 %1, %3, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %2, %3, off offset:32 sc1\n\t"
+               "s_waitcnt vmcnt(0)"
+               : "=&v"(x), "=&v"(y), "=&v"(z)
+               : "v"(p)
+               : "memory");
+  a = to_uint4(x), b = to_uint4(y), c = to_uint4(z);
+}
 
 AXB_D int cas_handoff(int32_t *p, int32_t expected, int32_t desired)
 {
@@ -201,28 +232,29 @@ AXB_D void read_sibling(bool is_leaf, int j, const float4 *leafrec, const INode 
 {
   if (is_leaf)
   {
-    const uint64_t *w = (const uint64_t *)(leafrec + (G == GEOM_POINTS ? j : 2 * j));
-    const uint64_t w0 = ld_coh64(w), w1 = ld_coh64(w + 1);
-    b.lo[0] = bits_f(lo32(w0)), b.lo[1] = bits_f(hi32(w0)), b.lo[2] = bits_f(lo32(w1));
     if (G == GEOM_POINTS)
     {
-      b.hi[0] = b.lo[0], b.hi[1] = b.lo[1], b.hi[2] = b.lo[2];
+      const uint4 a = ld_coh128(leafrec + j);
+      b.lo[0] = b.hi[0] = bits_f(a.x), b.lo[1] = b.hi[1] = bits_f(a.y);
+      b.lo[2] = b.hi[2] = bits_f(a.z);
     }
     else
     {
-      const uint64_t w2 = ld_coh64(w + 2), w3 = ld_coh64(w + 3);
-      b.hi[0] = bits_f(lo32(w2)), b.hi[1] = bits_f(hi32(w2)), b.hi[2] = bits_f(lo32(w3));
+      uint4 a, c;
+      ld_coh128x2(leafrec + 2 * j, a, c);
+      b.lo[0] = bits_f(a.x), b.lo[1] = bits_f(a.y), b.lo[2] = bits_f(a.z);
+      b.hi[0] = bits_f(c.x), b.hi[1] = bits_f(c.y), b.hi[2] = bits_f(c.z);
     }
     h = 0;
   }
   else
   {
-    const uint64_t *w = (const uint64_t *)(inodes + j);
-    const uint64_t w0 = ld_coh64(w), w1 = ld_coh64(w + 1), w2 = ld_coh64(w + 2),
-                   w3 = ld_coh64(w + 3), w5 = ld_coh64(w + 5);
-    b.lo[0] = bits_f(lo32(w0)), b.lo[1] = bits_f(hi32(w0)), b.lo[2] = bits_f(lo32(w1));
-    b.hi[0] = bits_f(lo32(w2)), b.hi[1] = bits_f(hi32(w2)), b.hi[2] = bits_f(lo32(w3));
-    h = (int)lo32(w5);
+    // INode: {lo xyz, left_child} {hi xyz, rope} {first, last, h, pad}
+    uint4 a, c, e;
+    ld_coh128x3(inodes + j, a, c, e);
+    b.lo[0] = bits_f(a.x), b.lo[1] = bits_f(a.y), b.lo[2] = bits_f(a.z);
+    b.hi[0] = bits_f(c.x), b.hi[1] = bits_f(c.y), b.hi[2] = bits_f(c.z);
+    h = (int)e.z;
   }
 }
 
@@ -252,20 +284,17 @@ __global__ __launch_bounds__(256) void k_climb(
     bv.lo[2] = kmin(FLT_MAX, z);
     bv.hi[0] = kmax(-FLT_MAX, x), bv.hi[1] = kmax(-FLT_MAX, y),
     bv.hi[2] = kmax(-FLT_MAX, z);
-    uint64_t *w = (uint64_t *)(leafrec + i0);
-    st_coh64(w, pack2(f_bits(x), f_bits(y)));
-    st_coh64(w + 1, pack2(f_bits(z), ~orig));
+    st_coh128(leafrec + i0, make_uint4(f_bits(x), f_bits(y), f_bits(z), ~orig));
   }
   else
   {
     const float *v = values + 6 * (int64_t)orig;
     for (int d = 0; d < 3; ++d)
       bv.lo[d] = kmin(FLT_MAX, v[d]), bv.hi[d] = kmax(-FLT_MAX, v[3 + d]);
-    uint64_t *w = (uint64_t *)(leafrec + 2 * i0);
-    st_coh64(w, pack2(f_bits(bv.lo[0]), f_bits(bv.lo[1])));
-    st_coh64(w + 1, pack2(f_bits(bv.lo[2]), ~orig));
-    st_coh64(w + 2, pack2(f_bits(bv.hi[0]), f_bits(bv.hi[1])));
-    st_coh64(w + 3, pack2(f_bits(bv.hi[2]), 0u));
+    st_coh128(leafrec + 2 * i0,
+              make_uint4(f_bits(bv.lo[0]), f_bits(bv.lo[1]), f_bits(bv.lo[2]), ~orig));
+    st_coh128(leafrec + 2 * i0 + 1,
+              make_uint4(f_bits(bv.hi[0]), f_bits(bv.hi[1]), f_bits(bv.hi[2]), 0u));
   }
   int range_left = i0, range_right = i0;
   int64_t dl = apetrei_delta(keys, nint, range_left - 1);
@@ -320,13 +349,12 @@ __global__ __launch_bounds__(256) void k_climb(
     const int hp =
         range_right - range_left + 1 > FLAT_MAX ? (left_child < n ? 0 : h_left) + 1 : 0;
     const int32_t rope = ROPES ? apetrei_rope(keys, nint, n, range_right, dr) : -1;
-    uint64_t *w = (uint64_t *)(inodes + kp);
-    st_coh64(w, pack2(f_bits(bv.lo[0]), f_bits(bv.lo[1])));
-    st_coh64(w + 1, pack2(f_bits(bv.lo[2]), (uint32_t)left_child));
-    st_coh64(w + 2, pack2(f_bits(bv.hi[0]), f_bits(bv.hi[1])));
-    st_coh64(w + 3, pack2(f_bits(bv.hi[2]), (uint32_t)rope));
-    st_coh64(w + 4, pack2((uint32_t)range_left, (uint32_t)range_right));
-    st_coh64(w + 5, pack2((uint32_t)hp, 0u));
+    uint4 *w = (uint4 *)(inodes + kp);
+    st_coh128(w, make_uint4(f_bits(bv.lo[0]), f_bits(bv.lo[1]), f_bits(bv.lo[2]),
+                            (uint32_t)left_child));
+    st_coh128(w + 1, make_uint4(f_bits(bv.hi[0]), f_bits(bv.hi[1]), f_bits(bv.hi[2]),
+                                (uint32_t)rope));
+    st_coh128(w + 2, make_uint4((uint32_t)range_left, (uint32_t)range_right, (uint32_t)hp, 0u));
     if (right_node >= n)
     {
       // the right child tops a left spine starting at leaf ap + 1

@@ -11,6 +11,9 @@
 #include "axb_internal.hpp"
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <chrono>
 #include <atomic>
 #include <vector>
 #include <cstring>
@@ -37,6 +40,7 @@ struct ProfRec
 {
   std::string name;
   hipEvent_t a, b;
+  double host_ms; // >= 0: a host-side interval (no events)
 };
 std::atomic<bool> g_prof_timing{false};
 std::atomic<bool> g_prof_visits{false};
@@ -58,8 +62,30 @@ ProfScope::~ProfScope()
   if (on_ && hipEventRecord(b_, s_) == hipSuccess)
   {
     std::lock_guard<std::mutex> lock(g_prof_mutex);
-    g_prof_recs.push_back({name_, a_, b_});
+    g_prof_recs.push_back({name_, a_, b_, -1.0});
   }
+}
+
+HostScope::HostScope(const char *name) : name_(name), on_(g_prof_timing.load())
+{
+  if (on_)
+    t0_ = std::chrono::steady_clock::now();
+}
+HostScope::~HostScope()
+{
+  if (on_)
+  {
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
+    std::lock_guard<std::mutex> lock(g_prof_mutex);
+    g_prof_recs.push_back({name_, nullptr, nullptr, ms});
+  }
+}
+
+hipError_t sync_stream(hipStream_t s)
+{
+  HostScope h("host:stream_sync");
+  return hipStreamSynchronize(s);
 }
 unsigned long long *visit_counters() { return g_prof_visits.load() ? g_visits : nullptr; }
 
@@ -96,6 +122,7 @@ static void keep_pool_memory()
 
 void *dalloc(size_t bytes, hipStream_t stream)
 {
+  HostScope h("host:dalloc");
   keep_pool_memory();
   void *p = nullptr;
   if (bytes == 0)
@@ -109,13 +136,72 @@ void *dalloc(size_t bytes, hipStream_t stream)
 }
 void dfree(void *p, hipStream_t stream)
 {
+  HostScope h("host:dfree");
   if (p)
     (void)hipFreeAsync(p, stream);
+}
+
+// Grow-only workspaces per (device, stream, slot) for the large per-call
+// scratch buffers (scratch rows of the radius CRS, kNN candidate rows):
+// freeing buffers of gigabytes costs milliseconds per call, so they are
+// kept for the next call on the same stream (stream order keeps the uses
+// apart).  axb_trim_memory releases them.
+namespace
+{
+struct WsKey
+{
+  int dev;
+  hipStream_t s;
+  bool operator<(const WsKey &o) const { return dev < o.dev || (dev == o.dev && s < o.s); }
+};
+struct WsBlock
+{
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+std::map<WsKey, std::array<WsBlock, WS_NUM>> g_ws;
+} // namespace
+
+void *ws_get(hipStream_t s, int slot, size_t bytes)
+{
+  HostScope h("host:ws_get");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  WsBlock &b = g_ws[WsKey{dev, s}][slot];
+  if (b.bytes >= bytes && b.p)
+    return b.p;
+  if (b.p)
+    dfree(b.p, s);
+  b.p = dalloc(bytes, s);
+  b.bytes = b.p ? bytes : 0;
+  return b.p;
+}
+
+// Releases every cached workspace (after the device is idle).
+static void ws_release_all()
+{
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  for (auto &kv : g_ws)
+    for (WsBlock &b : kv.second)
+      if (b.p)
+      {
+        (void)hipSetDevice(kv.first.dev);
+        (void)hipFree(b.p);
+        b = WsBlock{};
+      }
+  g_ws.clear();
 }
 
 // Result arrays the caller owns.
 static void *out_alloc(size_t bytes, hipStream_t stream)
 {
+  HostScope h("host:out_alloc");
   std::lock_guard<std::mutex> lock(g_alloc_mutex);
   if (g_alloc)
     return g_alloc(bytes == 0 ? 16 : bytes, g_alloc_ctx);
@@ -167,7 +253,7 @@ static axb_status check_kernel_error(uint32_t *d_err, hipStream_t s, const char 
   uint32_t h = 0;
   hipError_t e = hipMemcpyAsync(&h, d_err, sizeof(h), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess)
-    e = hipStreamSynchronize(s);
+    e = sync_stream(s);
   if (e != hipSuccess)
     return hip_fail(e, where);
   if (h)
@@ -225,6 +311,9 @@ axb_status axb_trim_memory(void)
 {
   int dev = 0;
   AXB_HIP_TRY(hipGetDevice(&dev));
+  AXB_HIP_TRY(hipDeviceSynchronize());
+  ws_release_all();
+  AXB_HIP_TRY(hipSetDevice(dev));
   hipMemPool_t pool;
   AXB_HIP_TRY(hipDeviceGetDefaultMemPool(&pool, dev));
   AXB_HIP_TRY(hipMemPoolTrimTo(pool, 0));
@@ -303,7 +392,7 @@ axb_status axb_bvh_create(const float *values, int64_t n, int32_t geometry, int3
   if (e == hipSuccess && n > 0)
     e = hipMemcpyAsync(&c_last, t->b.C + (n - 1), sizeof(c_last), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess)
-    e = hipStreamSynchronize(s);
+    e = sync_stream(s);
   if (e == hipSuccess && n > 0)
   {
     const int64_t sl = geometry == AXB_GEOMETRY_POINTS ? 1 : 2;
@@ -462,7 +551,7 @@ static axb_status crs_query(const axb_bvh *t, int64_t nq, axb_traversal_policy p
   uint32_t n_ovf = 0;
   if (e == hipSuccess && B > 0)
   {
-    buf = (int32_t *)dalloc((size_t)nq * B * sizeof(int32_t), s);
+    buf = (int32_t *)ws_get(s, WS_ROWS, (size_t)nq * B * sizeof(int32_t));
     ovf = (uint32_t *)dalloc((size_t)nq * sizeof(uint32_t), s);
     ovf_n = (uint32_t *)dalloc(sizeof(uint32_t), s);
     e = (buf && ovf && ovf_n) ? hipSuccess : hipErrorOutOfMemory;
@@ -488,7 +577,7 @@ static axb_status crs_query(const axb_bvh *t, int64_t nq, axb_traversal_policy p
   if (e == hipSuccess)
     e = hipMemcpyAsync(&nnz, offsets + nq, sizeof(nnz), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess)
-    e = hipStreamSynchronize(s);
+    e = sync_stream(s);
   if (e == hipSuccess && hard && n_ovf > 0)
   {
     set_error(std::string(what) + ": a row has more results than the hard buffer size "
@@ -514,9 +603,8 @@ static axb_status crs_query(const axb_bvh *t, int64_t nq, axb_traversal_policy p
     }
   }
   if (e == hipSuccess)
-    e = hipStreamSynchronize(s);
+    e = sync_stream(s);
   dfree(scan_ws, s);
-  dfree(buf, s);
   dfree(ovf, s);
   dfree(ovf_n, s);
   if (e != hipSuccess)
@@ -557,7 +645,7 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
   {
     AXB_HIP_TRY(fill_u64((uint64_t *)offsets, 0ull, nq + 1, s));
     *indices = (int32_t *)out_alloc(0, s);
-    AXB_HIP_TRY(hipStreamSynchronize(s));
+    AXB_HIP_TRY(sync_stream(s));
     return AXB_OK;
   }
   const TreeView tv = view_of(t);
@@ -702,7 +790,7 @@ axb_status axb_bvh_neighbor_list(const axb_bvh *t, float radius, int32_t full,
   {
     AXB_HIP_TRY(fill_u64((uint64_t *)offsets, 0ull, n + 1, s));
     *indices = (int32_t *)out_alloc(0, s);
-    AXB_HIP_TRY(hipStreamSynchronize(s));
+    AXB_HIP_TRY(sync_stream(s));
     return AXB_OK;
   }
   const TreeView tv = view_of(t);
@@ -756,7 +844,7 @@ axb_status axb_expand_half_to_full(const int64_t *offsets, const int32_t *indice
   *nnz_host = 0;
   int64_t half = 0;
   AXB_HIP_TRY(hipMemcpyAsync(&half, offsets + n, sizeof(half), hipMemcpyDeviceToHost, s));
-  AXB_HIP_TRY(hipStreamSynchronize(s));
+  AXB_HIP_TRY(sync_stream(s));
   if (half > 0 && !indices)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_expand_half_to_full: indices is NULL");
   int64_t *cursor = (int64_t *)dalloc((size_t)(n + 1) * sizeof(int64_t), s);
@@ -775,7 +863,7 @@ axb_status axb_expand_half_to_full(const int64_t *offsets, const int32_t *indice
   if (e == hipSuccess)
     e = hipMemcpyAsync(&m, out_offsets + n, sizeof(m), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess)
-    e = hipStreamSynchronize(s);
+    e = sync_stream(s);
   if (e == hipSuccess)
   {
     *out_indices = (int32_t *)out_alloc((size_t)m * sizeof(int32_t), s);
@@ -785,7 +873,7 @@ axb_status axb_expand_half_to_full(const int64_t *offsets, const int32_t *indice
   if (e == hipSuccess && m > 0)
     e = launch_h2f_fill(offsets, indices, n, out_offsets, cursor, *out_indices, s);
   if (e == hipSuccess)
-    e = hipStreamSynchronize(s);
+    e = sync_stream(s);
   axb_status st = AXB_OK;
   if (e == hipSuccess)
     st = check_kernel_error(err, s, "axb_expand_half_to_full");
@@ -860,7 +948,7 @@ axb_status axb_bvh_query_nearest(const axb_bvh *t, const float *points, int64_t 
                      dist ? dist : tmp_dist, s);
   }
   if (e == hipSuccess)
-    e = hipStreamSynchronize(s);
+    e = sync_stream(s);
   dfree(order_blk, s);
   dfree(tmp_dist, s);
   axb_status st = e == hipSuccess ? check_kernel_error(t->d_err, s, "axb_bvh_query_nearest")
@@ -905,6 +993,8 @@ axb_status axb_profile_clear(void)
   std::lock_guard<std::mutex> lock(g_prof_mutex);
   for (auto &r : g_prof_recs)
   {
+    if (r.host_ms >= 0.0)
+      continue;
     (void)hipEventSynchronize(r.b);
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
@@ -925,13 +1015,18 @@ axb_status axb_profile_record(int32_t i, char *name, int32_t name_len, double *m
   if (i < 0 || i >= (int32_t)g_prof_recs.size() || !ms)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_profile_record: bad index");
   const ProfRec &r = g_prof_recs[i];
-  hipError_t e = hipEventSynchronize(r.b);
-  float t = 0.f;
-  if (e == hipSuccess)
-    e = hipEventElapsedTime(&t, r.a, r.b);
-  if (e != hipSuccess)
-    return hip_fail(e, "axb_profile_record");
-  *ms = t;
+  if (r.host_ms >= 0.0)
+    *ms = r.host_ms;
+  else
+  {
+    hipError_t e = hipEventSynchronize(r.b);
+    float t = 0.f;
+    if (e == hipSuccess)
+      e = hipEventElapsedTime(&t, r.a, r.b);
+    if (e != hipSuccess)
+      return hip_fail(e, "axb_profile_record");
+    *ms = t;
+  }
   if (name && name_len > 0)
   {
     std::strncpy(name, r.name.c_str(), name_len - 1);

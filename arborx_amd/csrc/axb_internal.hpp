@@ -3,6 +3,8 @@
 
 #include "arborx_mi355x_device.hpp"
 
+#include <chrono>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -42,6 +44,27 @@ struct ProfScope
   bool on_;
   hipEvent_t a_, b_;
 };
+// Host-side interval recorded with the kernel timings (name "host:...").
+struct HostScope
+{
+  explicit HostScope(const char *name);
+  ~HostScope();
+  const char *name_;
+  bool on_;
+  std::chrono::steady_clock::time_point t0_;
+};
+// Cached scratch buffer of at least `bytes` for calls on stream s (grow-only,
+// kept between calls; released by axb_trim_memory).  nullptr when out of
+// memory.
+enum WsSlot
+{
+  WS_ROWS = 0,     // scratch rows of the radius CRS
+  WS_KNN_CAND = 1, // kNN candidate rows
+  WS_NUM = 2
+};
+void *ws_get(hipStream_t s, int slot, size_t bytes);
+// hipStreamSynchronize, timed as "host:stream_sync" while profiling.
+hipError_t sync_stream(hipStream_t s);
 #define AXB_LAUNCH(kernel, grid, block, shmem, stream, ...)                    \
   do                                                                           \
   {                                                                            \

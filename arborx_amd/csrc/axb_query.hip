@@ -414,10 +414,13 @@ AXB_D void knn_flush(List &L, float &thr, CandBuf &B)
 }
 
 // Greedy descent (closer child first, ties left) from the root to a leaf
-// record near the query; stops early when a left child is a leaf (the
-// start of a run of leaves, whose right sibling is not located without a
-// walk).  Returns the slot of that leaf.  The deepest stored node on the
-// path with at least `min_slots` slots is returned in X (slot, slots, box).
+// record near the query.  A left child that is a leaf starts a run of at
+// most FLAT_MAX leaves (a subtree that is not stored); the run is scanned
+// to find the right child, the first internal record inside the parent's
+// subtree after it (if the right child is a run as well, the parent holds
+// only leaves and the nearest of them ends the descent).  Returns the slot
+// of the leaf reached.  The deepest stored node on the path with at least
+// `min_slots` slots is returned in X (slot, slots, box).
 struct Subtree
 {
   uint32_t p, slots;
@@ -448,25 +451,51 @@ AXB_D uint32_t greedy_descent(const TreeView &tv, float qx, float qy, float qz,
   while (true)
   {
     const uint4 s0 = tv.slots[p];
-    if ((int32_t)s0.w < 0)
+    const int32_t w = (int32_t)s0.w;
+    if (w < 0)
       return p;
+    const uint32_t sub_end = p + (uint32_t)w; // end of this node's subtree
     const uint32_t lp = p + SI;
     const uint4 l0 = tv.slots[lp], l1 = tv.slots[lp + 1];
     const int32_t lw = (int32_t)l0.w;
+    Box6 lb;
+    float dl;
+    uint32_t rp, lbest = lp;
     if (lw < 0)
-      return lp;
-    const uint32_t rp = lp + (uint32_t)lw;
+    {
+      // the left child is a run of leaves: nearest of them, and the slot
+      // after the run
+      dl = INFINITY;
+      rp = lp;
+      for (int j = 0; j < 2 * FLAT_MAX && rp < sub_end; ++j)
+      {
+        const uint4 a0 = tv.slots[rp];
+        if ((int32_t)a0.w >= 0)
+          break;
+        ++v_leaf;
+        const float d = leaf_d2<G>(qx, qy, qz, a0, SL == 2 ? tv.slots[rp + 1] : a0);
+        if (d < dl)
+          dl = d, lbest = rp;
+        rp += SL;
+      }
+      if (rp >= sub_end || (int32_t)tv.slots[rp].w < 0)
+        return lbest; // only leaves below this node
+    }
+    else
+    {
+      ++v_int;
+      node_box<F>(tv, l0, l1, true, cur, cs, lb);
+      dl = dist2_pbox(qx, qy, qz, lb);
+      rp = lp + (uint32_t)lw;
+    }
     if (rp >= end)
     {
       raise_error(tv.err, KERR_LAYOUT);
-      return lp;
+      return lbest;
     }
     const uint4 r0 = tv.slots[rp], r1 = tv.slots[rp + 1];
     const int32_t rw = (int32_t)r0.w;
-    Box6 lb, rb;
-    ++v_int;
-    node_box<F>(tv, l0, l1, true, cur, cs, lb);
-    const float dl = dist2_pbox(qx, qy, qz, lb);
+    Box6 rb;
     float dr;
     if (rw < 0)
     {
@@ -481,6 +510,8 @@ AXB_D uint32_t greedy_descent(const TreeView &tv, float qx, float qy, float qz,
     }
     if (dl <= dr)
     {
+      if (lw < 0)
+        return lbest;
       p = lp;
       set_frame<F>(lb, cur, cs);
       if ((uint32_t)lw >= min_slots)
@@ -1163,7 +1194,7 @@ static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq,
   // aim at 2m leaves, m = min(k, n)
   const float shrink = std::cbrt((float)(2 * std::min(k, n)) / (float)(cap + 1));
   const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t(1) << 31) / (cap * 8)));
-  uint2 *cand = (uint2 *)dalloc((size_t)batch * cap * sizeof(uint2), s);
+  uint2 *cand = (uint2 *)ws_get(s, WS_KNN_CAND, (size_t)batch * cap * sizeof(uint2));
   uint32_t *ovf = (uint32_t *)dalloc((size_t)batch * sizeof(uint32_t), s);
   uint32_t *ovf_n = (uint32_t *)dalloc(sizeof(uint32_t), s);
   hipError_t e = (cand && ovf && ovf_n) ? hipSuccess : hipErrorOutOfMemory;
@@ -1179,7 +1210,6 @@ static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq,
                dim3(256), 0, s, tv, pts, k, order, ovf, ovf_n, oi, od);
     e = hipGetLastError();
   }
-  dfree(cand, s);
   dfree(ovf, s);
   dfree(ovf_n, s);
   return e;

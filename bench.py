@@ -363,7 +363,12 @@ def main():
         p = pts[d]
         with ax.profile(timing=True, visits=False) as prof:
             step(p, fmt, rq=radius[d])
-        kernels = {k: dict(launches=len(v), total_ms=sum(v)) for k, v in prof.kernels.items()}
+        kernels = {k: dict(launches=len(v), total_ms=sum(v)) for k, v in prof.kernels.items()
+                   if not k.startswith("host:")}
+        host = {k: dict(calls=len(v), total_ms=sum(v)) for k, v in prof.kernels.items()
+                if k.startswith("host:")}
+        log("[bench] host time (one step): " +
+            ", ".join(f"{k}: {v['total_ms']:.3f} ms x{v['calls']}" for k, v in host.items()))
         dom = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])
         log("[bench] kernel times (one step): " +
             ", ".join(f"{k}: {v['total_ms']:.3f} ms x{v['launches']}"

@@ -306,7 +306,7 @@ constexpr int PRED_BOXES = 1;
 constexpr int PRED_RADIUS = 2; // points + one radius
 
 // ORs an error flag into a device word (nullptr: ignored).
-AXB_D void raise_error(uint32_t *err, uint32_t flag)
+AXB_HD void raise_error(uint32_t *err, uint32_t flag)
 {
   if (err)
     __hip_atomic_fetch_or(err, flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -316,7 +316,7 @@ AXB_D void raise_error(uint32_t *err, uint32_t flag)
 // of their parent when reached by descending (desc), else against the scene.
 // TV: any tree view with `Box6 scene` and `float ss[3]`.
 template <int F, class TV>
-AXB_D void node_box(const TV &tv, uint4 s0, uint4 s1, bool desc, const Box6 &par,
+AXB_HD void node_box(const TV &tv, uint4 s0, uint4 s1, bool desc, const Box6 &par,
                     const float *ps, Box6 &b)
 {
   if (F == NODES_FP32)
@@ -334,7 +334,7 @@ AXB_D void node_box(const TV &tv, uint4 s0, uint4 s1, bool desc, const Box6 &par
 }
 
 template <int F>
-AXB_D void set_frame(const Box6 &b, Box6 &par, float *ps)
+AXB_HD void set_frame(const Box6 &b, Box6 &par, float *ps)
 {
   if (F == NODES_Q16)
   {
@@ -353,14 +353,14 @@ struct SpatialPred
   Box6 qb;
 };
 
-AXB_D SpatialPred sphere_pred(float x, float y, float z, float r)
+AXB_HD SpatialPred sphere_pred(float x, float y, float z, float r)
 {
   SpatialPred p;
   p.cx = x, p.cy = y, p.cz = z, p.thr = sqrt_threshold(r);
   p.qb = Box6{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   return p;
 }
-AXB_D SpatialPred box_pred(const Box6 &b)
+AXB_HD SpatialPred box_pred(const Box6 &b)
 {
   SpatialPred p;
   p.cx = p.cy = p.cz = p.thr = 0.f;
@@ -368,6 +368,8 @@ AXB_D SpatialPred box_pred(const Box6 &b)
   return p;
 }
 
+// (Host-callable as well, so the CPU tests run this very walk over the
+// oracle's node arrays.)
 // Stackless pre-order walk over slots [p, end) (TreeTraversal.hpp:97-119:
 // leaf -> test, next record; internal -> test ? first child : rope).  Each
 // hit calls sink.hit(value), which returns false to stop the walk (the
@@ -377,7 +379,7 @@ AXB_D SpatialPred box_pred(const Box6 &b)
 // TV: a tree view with `const uint4 *slots`, `Box6 scene`, `float ss[3]` and
 // `uint32_t *err`.  PK: PRED_BOXES tests the box, otherwise the sphere.
 template <int G, int PK>
-AXB_D bool leaf_hit(const SpatialPred &pr, uint4 s0, uint4 s1)
+AXB_HD bool leaf_hit(const SpatialPred &pr, uint4 s0, uint4 s1)
 {
   const float x = bits_f(s0.x), y = bits_f(s0.y), z = bits_f(s0.z);
   if (G == GEOM_POINTS)
@@ -401,6 +403,8 @@ constexpr int leaf_group()
   return G == GEOM_POINTS ? 4 : 2;
 }
 
+// (Host-callable as well, so the CPU tests run this very walk over the
+// oracle's node arrays.)
 // Stackless pre-order walk over slots [p, end) (TreeTraversal.hpp:97-119:
 // leaf -> test, next record; internal -> test ? first child : rope).  Each
 // hit calls sink.hit(value), which returns false to stop the walk (the
@@ -410,7 +414,7 @@ constexpr int leaf_group()
 // TV: a tree view with `const uint4 *slots`, `Box6 scene`, `float ss[3]` and
 // `uint32_t *err`.  PK: PRED_BOXES tests the box, otherwise the sphere.
 template <int F, int G, int PK, class TV, class Sink>
-AXB_D void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
+AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
                      uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();

@@ -1,0 +1,200 @@
+// test_walk_host.cpp -- the library's traversal code (rope_walk of
+// include/arborx_mi355x_device.hpp, the function the GPU kernels run) run on
+// the CPU over node arrays written by the oracle (oracle/oracle.cpp), and
+// compared with the oracle's reference-form rope traversal
+// (TreeTraversal.hpp:97-119): identical rows in identical order, for FP32
+// and Q16 nodes, flattened small subtrees, leaf groups, a skipped leaf and
+// a shortened walk (the neighbour-list forms).  Host code only (hipcc host
+// pass); no GPU is used.
+#include "arborx_mi355x_device.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+extern "C" {
+void *orc_build(const float *values, int64_t n, int geometry);
+void orc_free(void *h);
+int64_t orc_linearize(void *h, int node_format, int flat_max, uint32_t *words);
+int64_t orc_query_spatial(void *h, int kind, const float *preds, int64_t nq, int64_t *offsets,
+                          int32_t *indices, int64_t *visits_int, int64_t *visits_leaf);
+void orc_bounds(void *h, float *out6);
+}
+
+static int g_fail = 0, g_pass = 0;
+#define CHECK(cond, what)                                                                   \
+  do                                                                                        \
+  {                                                                                         \
+    if (cond)                                                                               \
+      ++g_pass;                                                                             \
+    else                                                                                    \
+    {                                                                                       \
+      ++g_fail;                                                                             \
+      std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, what);                             \
+    }                                                                                       \
+  } while (0)
+
+struct HostTree
+{
+  const uint4 *slots;
+  axb::Box6 scene;
+  float ss[3];
+  uint32_t *err;
+};
+
+struct RowSink
+{
+  std::vector<int32_t> *row;
+  bool hit(int32_t v)
+  {
+    row->push_back(v);
+    return true;
+  }
+};
+
+// One case: n points (geometry 0) or boxes (1), spheres around the first
+// nq values of radius r.
+static void run_case(int geometry, int64_t n, unsigned seed, float r, int node_format)
+{
+  std::mt19937 gen(seed);
+  std::uniform_real_distribution<float> U(0.f, 1.f);
+  const int vs = geometry == 0 ? 3 : 6;
+  std::vector<float> v(n * vs);
+  for (int64_t i = 0; i < n; ++i)
+  {
+    float x = U(gen), y = U(gen), z = U(gen);
+    if (i % 17 == 5 && i > 0) // exact duplicates
+      x = v[vs * (i - 1)], y = v[vs * (i - 1) + 1], z = v[vs * (i - 1) + 2];
+    v[vs * i] = x, v[vs * i + 1] = y, v[vs * i + 2] = z;
+    if (geometry == 1)
+    {
+      float e = 0.004f * U(gen);
+      v[vs * i + 3] = x + e, v[vs * i + 4] = y + e, v[vs * i + 5] = z + e;
+    }
+  }
+  void *t = orc_build(v.data(), n, geometry);
+  std::vector<uint32_t> words((size_t)(n * 2 + n * 2 + 16) * 4, 0u);
+  const int64_t total = orc_linearize(t, node_format, AXB_FLAT_SUBTREE_MAX, words.data());
+  // the library keeps zeroed slots after the array for the leaf-group reads
+  std::vector<uint4> slots((size_t)total + 8, make_uint4(0u, 0u, 0u, 0u));
+  std::memcpy(slots.data(), words.data(), (size_t)total * sizeof(uint4));
+  HostTree tv;
+  tv.slots = slots.data();
+  float b[6];
+  orc_bounds(t, b);
+  for (int d = 0; d < 3; ++d)
+  {
+    tv.scene.lo[d] = b[d], tv.scene.hi[d] = b[3 + d];
+    tv.ss[d] = axb::q16_scale(b[d], b[3 + d]);
+  }
+  uint32_t err = 0;
+  tv.err = &err;
+
+  const int64_t nq = n < 400 ? n : 400;
+  std::vector<float> sph(nq * 4);
+  for (int64_t q = 0; q < nq; ++q)
+  {
+    for (int d = 0; d < 3; ++d)
+      sph[4 * q + d] = v[vs * q + d];
+    sph[4 * q + 3] = r;
+  }
+  std::vector<int64_t> off(nq + 1);
+  const int64_t nnz = orc_query_spatial(t, 0, sph.data(), nq, off.data(), nullptr, nullptr,
+                                        nullptr);
+  std::vector<int32_t> ref(nnz > 0 ? nnz : 1);
+  orc_query_spatial(t, 0, sph.data(), nq, off.data(), ref.data(), nullptr, nullptr);
+
+  bool ok = true, ok_skip = true;
+  for (int64_t q = 0; q < nq && ok; ++q)
+  {
+    const axb::SpatialPred pr =
+        axb::sphere_pred(sph[4 * q], sph[4 * q + 1], sph[4 * q + 2], sph[4 * q + 3]);
+    std::vector<int32_t> row;
+    RowSink sink{&row};
+    uint32_t vi = 0, vl = 0;
+    if (geometry == 0)
+    {
+      if (node_format == 0)
+        axb::rope_walk<axb::NODES_FP32, axb::GEOM_POINTS, axb::PRED_SPHERES>(
+            tv, pr, 0u, (uint32_t)total, ~0u, sink, vi, vl);
+      else
+        axb::rope_walk<axb::NODES_Q16, axb::GEOM_POINTS, axb::PRED_SPHERES>(
+            tv, pr, 0u, (uint32_t)total, ~0u, sink, vi, vl);
+    }
+    else
+    {
+      if (node_format == 0)
+        axb::rope_walk<axb::NODES_FP32, axb::GEOM_BOXES, axb::PRED_SPHERES>(
+            tv, pr, 0u, (uint32_t)total, ~0u, sink, vi, vl);
+      else
+        axb::rope_walk<axb::NODES_Q16, axb::GEOM_BOXES, axb::PRED_SPHERES>(
+            tv, pr, 0u, (uint32_t)total, ~0u, sink, vi, vl);
+    }
+    ok = row.size() == (size_t)(off[q + 1] - off[q]) &&
+         std::equal(row.begin(), row.end(), ref.begin() + off[q]);
+    if (geometry == 0 && ok)
+    {
+      // the neighbour-list forms: skip this query's own leaf record, or
+      // stop at it (the half list)
+      uint32_t self = ~0u;
+      for (int64_t p = 0; p < total; ++p)
+        if ((int32_t)slots[p].w == ~(int32_t)q)
+          self = (uint32_t)p;
+      std::vector<int32_t> full, half;
+      RowSink sf{&full}, sh{&half};
+      if (node_format == 0)
+      {
+        axb::rope_walk<axb::NODES_FP32, axb::GEOM_POINTS, axb::PRED_SPHERES>(
+            tv, pr, 0u, (uint32_t)total, self, sf, vi, vl);
+        axb::rope_walk<axb::NODES_FP32, axb::GEOM_POINTS, axb::PRED_SPHERES>(
+            tv, pr, 0u, self, self, sh, vi, vl);
+      }
+      else
+      {
+        axb::rope_walk<axb::NODES_Q16, axb::GEOM_POINTS, axb::PRED_SPHERES>(
+            tv, pr, 0u, (uint32_t)total, self, sf, vi, vl);
+        axb::rope_walk<axb::NODES_Q16, axb::GEOM_POINTS, axb::PRED_SPHERES>(
+            tv, pr, 0u, self, self, sh, vi, vl);
+      }
+      std::vector<int32_t> want_full, want_half;
+      bool before = true;
+      for (int64_t j = off[q]; j < off[q + 1]; ++j)
+      {
+        if (ref[j] == (int32_t)q)
+        {
+          before = false;
+          continue;
+        }
+        want_full.push_back(ref[j]);
+        if (before)
+          want_half.push_back(ref[j]);
+      }
+      ok_skip = ok_skip && full == want_full && half == want_half;
+    }
+  }
+  char what[160];
+  std::snprintf(what, sizeof what, "rows: geometry %d n %lld format %d", geometry, (long long)n,
+                node_format);
+  CHECK(ok && err == 0, what);
+  if (geometry == 0)
+  {
+    std::snprintf(what, sizeof what, "skip/stop forms: n %lld format %d", (long long)n,
+                  node_format);
+    CHECK(ok_skip, what);
+  }
+  orc_free(t);
+}
+
+int main()
+{
+  for (int fmt : {0, 1})
+  {
+    for (int64_t n : {1, 2, 3, 4, 5, 6, 9, 17, 100, 1000, 20000})
+      run_case(0, n, 7u + (unsigned)n, n < 100 ? 0.5f : 0.05f, fmt);
+    for (int64_t n : {2, 5, 300, 5000})
+      run_case(1, n, 11u + (unsigned)n, 0.03f, fmt);
+  }
+  std::printf("%d passed, %d failed\n", g_pass, g_fail);
+  return g_fail == 0 ? 0 : 1;
+}

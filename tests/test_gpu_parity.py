@@ -85,6 +85,21 @@ def test_preorder_nodes_bit_exact(node_format, kind, n):
     np.testing.assert_array_equal(got, ref)
 
 
+def test_trim_memory_between_queries():
+    """Cached workspaces and pool memory handed back between queries: the
+    next queries allocate afresh and give the same rows."""
+    pts = dev(synthetic.uniform_points(50_000, 31))
+    r = float(np.float32(synthetic.radius_for_neighbours(50_000, 16)))
+    bvh = ax.BoundingVolumeHierarchy(pts, node_format="q16")
+    i1, o1 = bvh.query(ax.make_intersects(pts, r))
+    k1, _ = bvh.query(ax.make_nearest(pts, 8))
+    torch.cuda.synchronize()
+    assert ax.lib().axb_trim_memory() == 0
+    i2, o2 = bvh.query(ax.make_intersects(pts, r))
+    k2, _ = bvh.query(ax.make_nearest(pts, 8))
+    assert torch.equal(o1, o2) and torch.equal(i1, i2) and torch.equal(k1, k2)
+
+
 def test_empty_tree():
     bvh = ax.BoundingVolumeHierarchy(torch.empty((0, 3), device="cuda"))
     assert bvh.empty() and bvh.size() == 0
