@@ -7,7 +7,7 @@
 // happens where the reference has one: after the build (bounds(),
 // TreeConstruction.hpp:108-113) and after the counting pass of a spatial
 // query (lastElement, CrsGraphWrapperImpl.hpp:248).
-#include "../../include/arborx_mi355x.h"
+#include "arborx_mi355x.h"
 #include "axb_internal.hpp"
 
 #include <algorithm>

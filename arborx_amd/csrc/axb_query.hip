@@ -367,7 +367,7 @@ AXB_D float leaf_d2(float qx, float qy, float qz, uint4 s0, uint4 s1)
   if (G == GEOM_POINTS)
     return dist2_pp(qx, qy, qz, x, y, z);
   const Box6 lb = {{x, y, z}, {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
-  return dist2_pbox(qx, qy, qz, lb);
+  return dist2_pnode(qx, qy, qz, lb);
 }
 
 // Rope walk over [p, pend) for nearest search: a node farther than the
@@ -485,7 +485,7 @@ AXB_D uint32_t greedy_descent(const TreeView &tv, float qx, float qy, float qz,
     {
       ++v_int;
       node_box<F>(tv, l0, l1, true, cur, cs, lb);
-      dl = dist2_pbox(qx, qy, qz, lb);
+      dl = dist2_pnode(qx, qy, qz, lb);
       rp = lp + (uint32_t)lw;
     }
     if (rp >= end)
@@ -506,7 +506,7 @@ AXB_D uint32_t greedy_descent(const TreeView &tv, float qx, float qy, float qz,
     {
       ++v_int;
       node_box<F>(tv, r0, r1, false, cur, cs, rb);
-      dr = dist2_pbox(qx, qy, qz, rb);
+      dr = dist2_pnode(qx, qy, qz, rb);
     }
     if (dl <= dr)
     {
@@ -531,7 +531,9 @@ AXB_D uint32_t greedy_descent(const TreeView &tv, float qx, float qy, float qz,
 
 // The W leaves nearest to slot p in slot order (about half before it),
 // fn(s0, s1) for each; [ws, we) is the slot range they span (no other leaf
-// lies in it).  W must not exceed the number of leaves.
+// lies in it).  W must not exceed the number of leaves.  (One slot per
+// step: fn is inlined at three sites only, which keeps a sorted list of
+// up to 64 entries in registers.)
 template <int G, class Fn>
 AXB_D void slot_window(const TreeView &tv, uint32_t p, int W, uint32_t &ws, uint32_t &we,
                        Fn &&fn, uint32_t &v_int, uint32_t &v_leaf)
@@ -618,7 +620,7 @@ AXB_D void knn_walk(const TreeView &tv, float qx, float qy, float qz, List &L, f
       ++v_int;
       Box6 b;
       node_box<F>(tv, s0, s1, desc, par, ps, b);
-      if (dist2_pbox(qx, qy, qz, b) <= thr)
+      if (dist2_pnode(qx, qy, qz, b) <= thr)
       {
         set_frame<F>(b, par, ps);
         desc = true;
@@ -677,7 +679,7 @@ AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
   uint32_t ws = 0, we = 0;
   slot_window<G>(
       tv, p, W, ws, we,
-      [&](uint4 s0, uint4 s1) {
+      [&](uint4 s0, uint4 s1) __attribute__((always_inline)) {
         const float dist = sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1));
         const int32_t idx = ~(int32_t)s0.w;
         if (L.accepts(dist, idx))
@@ -727,7 +729,7 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
   uint32_t ws, we;
   slot_window<G>(
       tv, p, W, ws, we,
-      [&](uint4 s0, uint4 s1) { L.insert(sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1))); }, v_int,
+      [&](uint4 s0, uint4 s1) __attribute__((always_inline)) { L.insert(sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1))); }, v_int,
       v_leaf);
   return L.kth(k);
 }
@@ -740,6 +742,65 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
                            uint2 *row, int cap, uint32_t &v_int, uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  if constexpr (G == GEOM_POINTS)
+  {
+    // one record per step, leaf and internal node through the same code
+    // (the closest point of a leaf is the point), as in rope_walk_points
+    int c = 0;
+    uint32_t p = 0;
+    const uint32_t end = tv.total;
+    Box6 par = tv.scene;
+    float ps[3] = {tv.ss[0], tv.ss[1], tv.ss[2]};
+    bool desc = false;
+    while (p < end)
+    {
+      const uint4 s0 = tv.slots[p];
+      const uint4 s1 = F == NODES_FP32 ? tv.slots[p + 1] : s0;
+      const int32_t w = (int32_t)s0.w;
+      const bool leaf = w < 0;
+      Box6 bx;
+      node_box<F>(tv, s0, s1, desc, par, ps, bx);
+      const float cx = leaf ? bits_f(s0.x) : fminf(fmaxf(qx, bx.lo[0]), bx.hi[0]);
+      const float cy = leaf ? bits_f(s0.y) : fminf(fmaxf(qy, bx.lo[1]), bx.hi[1]);
+      const float cz = leaf ? bits_f(s0.z) : fminf(fmaxf(qz, bx.lo[2]), bx.hi[2]);
+      const float d2 = dist2_pp(qx, qy, qz, cx, cy, cz);
+      const bool pass = d2 <= thr;
+      if (leaf)
+      {
+        ++v_leaf;
+        if (pass)
+        {
+          if (c < cap)
+            row[c] = make_uint2(f_bits(d2), (uint32_t)~w);
+          if (++c > cap)
+            break;
+        }
+        p += 1;
+        desc = false;
+      }
+      else
+      {
+        ++v_int;
+        if (pass)
+        {
+          set_frame<F>(bx, par, ps);
+          desc = true;
+          p += SI;
+        }
+        else
+        {
+          if (w == 0)
+          {
+            raise_error(tv.err, KERR_LAYOUT);
+            return c;
+          }
+          desc = false;
+          p += (uint32_t)w;
+        }
+      }
+    }
+    return c;
+  }
   int c = 0;
   uint32_t p = 0;
   const uint32_t end = tv.total;
@@ -791,7 +852,7 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
       ++v_int;
       Box6 b;
       node_box<F>(tv, s0, s1, desc, par, ps, b);
-      if (dist2_pbox(qx, qy, qz, b) <= thr)
+      if (dist2_pnode(qx, qy, qz, b) <= thr)
       {
         set_frame<F>(b, par, ps);
         desc = true;
@@ -823,7 +884,7 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
 constexpr int KNN_ROUNDS = 6;
 
 template <int F, int G, int K>
-__global__ __launch_bounds__(256) void k_knn_collect(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 && G == GEOM_POINTS) ? 8 : 1))) void k_knn_collect(
     TreeView tv, const float *__restrict__ pts, int64_t t0, int64_t tn, int k, int W,
     const uint32_t *__restrict__ order, uint2 *__restrict__ cand, int cap, float shrink,
     uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_n, int32_t *__restrict__ oi,

@@ -147,6 +147,18 @@ AXB_HD float dist2_pbox(float px, float py, float pz, const Box6 &b)
                   closest(py, b.lo[1], b.hi[1]),
                   closest(pz, b.lo[2], b.hi[2]));
 }
+// The same squared distance to an internal node's box, clamping with
+// min / max (two instructions per axis instead of two compares and two
+// selects).  For a box with lo <= hi -- every node box -- both clamps give
+// the same d^2 for all inputs, NaN included (min / max return the non-NaN
+// operand, and d = c - p is NaN whenever p is; the sign of a zero does not
+// change its square).  Leaf boxes keep the reference's comparisons, since
+// user data may hold inverted boxes.
+AXB_HD float dist2_pnode(float px, float py, float pz, const Box6 &b)
+{
+  return dist2_pp(px, py, pz, fminf(fmaxf(px, b.lo[0]), b.hi[0]),
+                  fminf(fmaxf(py, b.lo[1]), b.hi[1]), fminf(fmaxf(pz, b.lo[2]), b.hi[2]));
+}
 // ArborX_Intersects.hpp:51-65.
 AXB_HD bool intersects_bb(const Box6 &a, const Box6 &b)
 {
@@ -326,10 +338,18 @@ AXB_HD void node_box(const TV &tv, uint4 s0, uint4 s1, bool desc, const Box6 &pa
   }
   else
   {
-    if (desc)
-      q16_decode(s0.x, s0.y, s0.z, par, ps, b);
-    else
-      q16_decode(s0.x, s0.y, s0.z, tv.scene, tv.ss, b);
+    // frame chosen by selects rather than a branch: lanes of a wavefront
+    // disagree on `desc` all the time
+    Box6 f;
+    float fs[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      f.lo[d] = desc ? par.lo[d] : tv.scene.lo[d];
+      f.hi[d] = desc ? par.hi[d] : tv.scene.hi[d];
+      fs[d] = desc ? ps[d] : tv.ss[d];
+    }
+    q16_decode(s0.x, s0.y, s0.z, f, fs, b);
   }
 }
 
@@ -413,10 +433,72 @@ constexpr int leaf_group()
 // walk starts at the root or at a record whose frame is the scene box.
 // TV: a tree view with `const uint4 *slots`, `Box6 scene`, `float ss[3]` and
 // `uint32_t *err`.  PK: PRED_BOXES tests the box, otherwise the sphere.
+// Point leaves with sphere predicates: one record per step, leaf and
+// internal node through the same straight-line code.  The point test
+// d2(c, p) and the node test d2(c, clamp(c, box)) are one computation once
+// the closest point is chosen (the point itself for a leaf), so the lanes
+// of a wavefront, which disagree on the record kind in most steps, share
+// the decode and distance instructions instead of running both paths.
+template <int F, class TV, class Sink>
+AXB_HD void rope_walk_points(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
+                             uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
+{
+  constexpr uint32_t SI = internal_slots<F>();
+  Box6 par = tv.scene;
+  float ps[3] = {tv.ss[0], tv.ss[1], tv.ss[2]};
+  bool desc = false;
+  while (p < end)
+  {
+    const uint4 s0 = tv.slots[p];
+    const uint4 s1 = F == NODES_FP32 ? tv.slots[p + 1] : s0;
+    const int32_t w = (int32_t)s0.w;
+    const bool leaf = w < 0;
+    Box6 bx; // meaningless for a leaf record, and unused then
+    node_box<F>(tv, s0, s1, desc, par, ps, bx);
+    const float cx = leaf ? bits_f(s0.x) : fminf(fmaxf(pr.cx, bx.lo[0]), bx.hi[0]);
+    const float cy = leaf ? bits_f(s0.y) : fminf(fmaxf(pr.cy, bx.lo[1]), bx.hi[1]);
+    const float cz = leaf ? bits_f(s0.z) : fminf(fmaxf(pr.cz, bx.lo[2]), bx.hi[2]);
+    const bool pass = dist2_pp(pr.cx, pr.cy, pr.cz, cx, cy, cz) <= pr.thr;
+    if (leaf)
+    {
+      ++v_leaf;
+      if (pass && p != skip && !sink.hit(~w))
+        break;
+      p += 1;
+      desc = false;
+    }
+    else
+    {
+      ++v_int;
+      if (pass)
+      {
+        set_frame<F>(bx, par, ps);
+        desc = true;
+        p += SI;
+      }
+      else
+      {
+        if (w == 0)
+        {
+          raise_error(tv.err, KERR_LAYOUT);
+          break;
+        }
+        desc = false;
+        p += (uint32_t)w;
+      }
+    }
+  }
+}
+
 template <int F, int G, int PK, class TV, class Sink>
 AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
                      uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
 {
+  if constexpr (G == GEOM_POINTS && PK != PRED_BOXES)
+  {
+    rope_walk_points<F>(tv, pr, p, end, skip, sink, v_int, v_leaf);
+    return;
+  }
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
   constexpr int LG = leaf_group<G>();
   Box6 par;
@@ -462,7 +544,7 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
       Box6 bx;
       node_box<F>(tv, s0, s1, desc, par, ps, bx);
       const bool pass = PK == PRED_BOXES ? intersects_bb(pr.qb, bx)
-                                         : dist2_pbox(pr.cx, pr.cy, pr.cz, bx) <= pr.thr;
+                                         : dist2_pnode(pr.cx, pr.cy, pr.cz, bx) <= pr.thr;
       if (pass)
       {
         set_frame<F>(bx, par, ps);
