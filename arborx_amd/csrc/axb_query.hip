@@ -17,8 +17,8 @@
 //   2. a fixed-radius rope walk appending (d2, index) of every leaf within R
 //      to a row of `cap` candidates, stopping at cap + 1; the radius adapts
 //      over a few rounds until the row holds between k and cap leaves;
-//   3. exact top-k of the row in a register list of packed
-//      (distance bits, index) keys.
+//   3. exact top-k of the row, (d2, index) keys sorted and merged by
+//      bitonic networks in registers.
 // Queries left unresolved go to k_knn_fallback (exact adaptive walk with an
 // LDS candidate buffer); k > 64 uses a binary max-heap in the output row
 // (the reference's NearestBufferProvider scheme, NearestBufferProvider.hpp:
@@ -177,73 +177,10 @@ AXB_D bool pair_less(float da, int32_t ia, float db, int32_t ib)
   return da < db || (da == db && ia < ib);
 }
 
-// (distance, index) packed into one 64-bit key: distances are >= 0, whose
-// float bit patterns order like the values, so the key order is exactly the
-// lexicographic (distance, index) order.
-AXB_D uint64_t pack_key(float d, int32_t i)
-{
-  return ((uint64_t)f_bits(d) << 32) | (uint32_t)i;
-}
-
-// Sorted list of K keys in registers; only the first k (k <= K) entries are
-// meaningful.  Insertion is a branch-free network a'[j] = min(a[j],
-// max(a[j-1], x)) with static indexing only.
-template <int K>
-struct RegList
-{
-  uint64_t a[K];
-  int k, count;
-  float kth_d;
-  uint64_t kth;
-  AXB_D void init(int k_)
-  {
-    k = k_;
-    count = 0;
-#pragma unroll
-    for (int j = 0; j < K; ++j)
-      a[j] = ~0ull;
-    kth_d = INFINITY;
-    kth = ~0ull;
-  }
-  AXB_D bool full() const { return count >= k; }
-  AXB_D bool accepts(float dn, int32_t in) const { return pack_key(dn, in) < kth; }
-  AXB_D void insert(float dn, int32_t in)
-  {
-    const uint64_t x = pack_key(dn, in);
-#pragma unroll
-    for (int j = K - 1; j > 0; --j)
-    {
-      const uint64_t hi = a[j - 1] < x ? x : a[j - 1];
-      a[j] = hi < a[j] ? hi : a[j];
-    }
-    a[0] = x < a[0] ? x : a[0];
-    ++count;
-    uint64_t kk = a[0];
-#pragma unroll
-    for (int j = 1; j < K; ++j)
-      if (j == k - 1)
-        kk = a[j];
-    if (count >= k)
-    {
-      kth = kk;
-      kth_d = bits_f((uint32_t)(kk >> 32));
-    }
-  }
-  AXB_D void write(int32_t *oi, float *od, int per) const
-  {
-#pragma unroll
-    for (int j = 0; j < K; ++j)
-      if (j < per)
-      {
-        oi[j] = (int32_t)(uint32_t)a[j];
-        if (od)
-          od[j] = bits_f((uint32_t)(a[j] >> 32));
-      }
-  }
-};
-
-// The K smallest distances seen (float bits, ascending; no indices): the
-// seed only needs the k-th distance, in half the registers of RegList.
+// The K smallest values seen (float bits, ascending; no indices): the seed
+// only needs the k-th distance, in half the registers of RegList.  It is fed
+// squared distances: RN(sqrt) is monotone, so the k-th smallest distance is
+// RN(sqrt) of the k-th smallest squared distance, one square root per query.
 template <int K>
 struct DistList
 {
@@ -265,7 +202,8 @@ struct DistList
     }
     a[0] = x < a[0] ? x : a[0];
   }
-  // k-th smallest (1 <= k <= K); +inf bits while fewer than k were inserted
+  // RN(sqrt) of the k-th smallest (1 <= k <= K); +inf while fewer than k
+  // were inserted
   AXB_D float kth(int k) const
   {
     uint32_t v = a[0];
@@ -273,7 +211,7 @@ struct DistList
     for (int j = 1; j < K; ++j)
       if (j == k - 1)
         v = a[j];
-    return v == ~0u ? INFINITY : bits_f(v);
+    return v == ~0u ? INFINITY : sqrt_rn(bits_f(v));
   }
 };
 
@@ -729,17 +667,19 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
   uint32_t ws, we;
   slot_window<G>(
       tv, p, W, ws, we,
-      [&](uint4 s0, uint4 s1) __attribute__((always_inline)) { L.insert(sqrt_rn(leaf_d2<G>(qx, qy, qz, s0, s1))); }, v_int,
+      [&](uint4 s0, uint4 s1) __attribute__((always_inline)) { L.insert(leaf_d2<G>(qx, qy, qz, s0, s1)); }, v_int,
       v_leaf);
   return L.kth(k);
 }
 
-// Fixed-radius rope walk appending (d2, index) of every leaf with
-// d2 <= thr to `row`.  Stops as soon as more than `cap` leaves qualify;
+// Fixed-radius rope walk appending every leaf with d2 <= thr to `row` as
+// {index, d2 bits}: read as one little-endian 64-bit word that is the
+// (d2, index) key, d2 bits high (d2 >= 0, so bit order is value order).  Stops as soon as more than `cap` leaves qualify;
 // returns the count (cap + 1 on overflow).
 template <int F, int G>
 AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, float thr,
-                           uint2 *row, int cap, uint32_t &v_int, uint32_t &v_leaf)
+                           uint2 *__restrict__ cand, uint32_t row_at, int cap,
+                           uint32_t &v_int, uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
   if constexpr (G == GEOM_POINTS)
@@ -760,44 +700,28 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
       const bool leaf = w < 0;
       Box6 bx;
       node_box<F>(tv, s0, s1, desc, par, ps, bx);
-      const float cx = leaf ? bits_f(s0.x) : fminf(fmaxf(qx, bx.lo[0]), bx.hi[0]);
-      const float cy = leaf ? bits_f(s0.y) : fminf(fmaxf(qy, bx.lo[1]), bx.hi[1]);
-      const float cz = leaf ? bits_f(s0.z) : fminf(fmaxf(qz, bx.lo[2]), bx.hi[2]);
+      const float cx = leaf ? bits_f(s0.x) : clamp_box(qx, bx.lo[0], bx.hi[0]);
+      const float cy = leaf ? bits_f(s0.y) : clamp_box(qy, bx.lo[1], bx.hi[1]);
+      const float cz = leaf ? bits_f(s0.z) : clamp_box(qz, bx.lo[2], bx.hi[2]);
       const float d2 = dist2_pp(qx, qy, qz, cx, cy, cz);
       const bool pass = d2 <= thr;
-      if (leaf)
+      v_leaf += leaf;
+      v_int += !leaf;
+      if (leaf && pass)
       {
-        ++v_leaf;
-        if (pass)
-        {
-          if (c < cap)
-            row[c] = make_uint2(f_bits(d2), (uint32_t)~w);
-          if (++c > cap)
-            break;
-        }
-        p += 1;
-        desc = false;
+        if (c < cap)
+          cand[row_at + c] = make_uint2((uint32_t)~w, f_bits(d2));
+        if (++c > cap)
+          break;
       }
-      else
+      if (!leaf && !pass && w == 0)
       {
-        ++v_int;
-        if (pass)
-        {
-          set_frame<F>(bx, par, ps);
-          desc = true;
-          p += SI;
-        }
-        else
-        {
-          if (w == 0)
-          {
-            raise_error(tv.err, KERR_LAYOUT);
-            return c;
-          }
-          desc = false;
-          p += (uint32_t)w;
-        }
+        raise_error(tv.err, KERR_LAYOUT);
+        return c;
       }
+      set_frame<F>(bx, par, ps); // read only when the next record descends
+      desc = !leaf && pass;
+      p += leaf ? 1u : (pass ? SI : (uint32_t)w);
     }
     return c;
   }
@@ -837,7 +761,7 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
           if (d2 <= thr)
           {
             if (c < cap)
-              row[c] = make_uint2(f_bits(d2), (uint32_t)~(int32_t)a[j].w);
+              cand[row_at + c] = make_uint2((uint32_t)~(int32_t)a[j].w, f_bits(d2));
             ++c;
           }
           p += SL;
@@ -875,12 +799,14 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
 
 // One lane per query: seed radius, then up to KNN_ROUNDS collect walks until
 // one finds between k and cap leaves within the radius (then every point
-// within the true k-th distance is in the row), followed by the exact
-// top-k selection of that row.  On overflow (the walk stops at cap + 1) the
-// radius is an upper bound and the next one shrinks by the volume ratio
-// (2k / (cap + 1))^(1/3); with fewer than k leaves the radius was below the
-// k-th distance and the next one bisects towards the upper bound.  Queries
-// still unresolved go to the exact fallback.
+// within the true k-th distance is in the row), followed by the exact top-k
+// selection of that row while it is still in L2.  On overflow (the walk
+// stops at cap + 1) the radius is an upper bound and the next one shrinks
+// by the volume ratio (2k / (cap + 1))^(1/3); with fewer than k leaves the
+// radius was below the k-th distance and the next one bisects towards the
+// upper bound.  Queries still unresolved go to the exact fallback.  The row
+// is addressed as cand + row_at (a scalar base and a 32-bit offset): the
+// walk then fits the 64 registers of full occupancy without spilling.
 constexpr int KNN_ROUNDS = 6;
 
 template <int F, int G, int K>
@@ -899,7 +825,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 &&
   const int m = k < tv.n ? k : tv.n;
   uint32_t v_int = 0, v_leaf = 0;
   float R = knn_seed<F, G, K>(tv, qx, qy, qz, k, W, v_int, v_leaf);
-  uint2 *row = cand + g * cap;
+  const uint32_t row_at = (uint32_t)(g * cap); // batch * cap < 2^28
   int c = 0;
   bool done = false;
   int rounds = 0;
@@ -908,7 +834,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 &&
   for (int round = 0; round < KNN_ROUNDS; ++round)
   {
     ++rounds;
-    c = knn_collect_walk<F, G>(tv, qx, qy, qz, sqrt_threshold(R), row, cap, v_int, v_leaf);
+    c = knn_collect_walk<F, G>(tv, qx, qy, qz, sqrt_threshold(R), cand, row_at, cap, v_int,
+                               v_leaf);
     if (c >= m && c <= cap)
     {
       done = true;
@@ -929,22 +856,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 &&
       R = next > R_lo ? next : 0.5f * (R_lo + R_hi);
     }
   }
-  if (done)
-  {
-    // exact top-k of the row (every leaf within the k-th distance is in it)
-    RegList<K> L;
-    L.init(k);
-    for (int j = 0; j < c; ++j)
-    {
-      const uint2 e = row[j];
-      const float dist = sqrt_rn(bits_f(e.x));
-      const int32_t idx = (int32_t)e.y;
-      if (L.accepts(dist, idx))
-        L.insert(dist, idx);
-    }
-    L.write(oi + q * m, od ? od + q * m : nullptr, m);
-  }
-  if (!done)
+  if (done) // exact top-k of the row (every leaf within the k-th distance is in it)
+    select_row<K>(cand, row_at, c, m, oi, od, q * m);
+  else
     ovf[atomicAdd(ovf_n, 1u)] = (uint32_t)t;
   if (tv.visits)
   {
@@ -1191,20 +1105,21 @@ __global__ void k_h2f_fill(const int64_t *__restrict__ offsets,
 // ------------------------------------------------------------ host launchers
 inline unsigned grid_of(int64_t n) { return (unsigned)div_up(n, 256); }
 
+
 template <int F, int G, int PK>
 static hipError_t launch_spatial_fg(const TreeView &tv, const float *preds, int64_t nq,
                                     float radius, const uint32_t *order, int64_t *offsets,
                                     int32_t *out, int32_t buffer, int mode, hipStream_t s)
 {
   if (mode == MODE_COUNT)
-    AXB_LAUNCH((k_spatial<F, G, PK, MODE_COUNT>), dim3(grid_of(nq)), dim3(256), 0, s,
-                       tv, preds, nq, radius, order, offsets, out, buffer);
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_COUNT>), dim3(grid_of(nq)), dim3(256), 0, s, tv, preds,
+               nq, radius, order, offsets, out, buffer);
   else if (mode == MODE_FILL)
-    AXB_LAUNCH((k_spatial<F, G, PK, MODE_FILL>), dim3(grid_of(nq)), dim3(256), 0, s,
-                       tv, preds, nq, radius, order, offsets, out, buffer);
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_FILL>), dim3(grid_of(nq)), dim3(256), 0, s, tv, preds,
+               nq, radius, order, offsets, out, buffer);
   else
-    AXB_LAUNCH((k_spatial<F, G, PK, MODE_BUFFER>), dim3(grid_of(nq)), dim3(256), 0, s,
-                       tv, preds, nq, radius, order, offsets, out, buffer);
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_BUFFER>), dim3(grid_of(nq)), dim3(256), 0, s, tv, preds,
+               nq, radius, order, offsets, out, buffer);
   return hipGetLastError();
 }
 

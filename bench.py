@@ -388,7 +388,7 @@ def main():
             """Bytes the algorithm must move for all launches of one call."""
             if "k_knn_collect" in name:
                 # query 12 B + order 4 B; node records; each candidate written
-                # and read back (8 B each way); k indices out
+                # to the query's row and read back (8 B each way); k indices out
                 return (vk["queries"] * (16 + 4 * K) + vk["internal"] * s_int
                         + vk["leaf"] * 16 + vk["results"] * 16), "kNN query"
             if "MODE_COUNT" in name:

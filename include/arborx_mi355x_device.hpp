@@ -154,10 +154,21 @@ AXB_HD float dist2_pbox(float px, float py, float pz, const Box6 &b)
 // operand, and d = c - p is NaN whenever p is; the sign of a zero does not
 // change its square).  Leaf boxes keep the reference's comparisons, since
 // user data may hold inverted boxes.
+// min(max(q, lo), hi) for lo <= hi (every node box): one median-of-three
+// instruction on the GPU.  A NaN q gives a NaN distance either way, since
+// the distance is taken from q itself.
+AXB_HD float clamp_box(float q, float lo, float hi)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_fmed3f(q, lo, hi);
+#else
+  return fminf(fmaxf(q, lo), hi);
+#endif
+}
 AXB_HD float dist2_pnode(float px, float py, float pz, const Box6 &b)
 {
-  return dist2_pp(px, py, pz, fminf(fmaxf(px, b.lo[0]), b.hi[0]),
-                  fminf(fmaxf(py, b.lo[1]), b.hi[1]), fminf(fmaxf(pz, b.lo[2]), b.hi[2]));
+  return dist2_pp(px, py, pz, clamp_box(px, b.lo[0], b.hi[0]), clamp_box(py, b.lo[1], b.hi[1]),
+                  clamp_box(pz, b.lo[2], b.hi[2]));
 }
 // ArborX_Intersects.hpp:51-65.
 AXB_HD bool intersects_bb(const Box6 &a, const Box6 &b)
@@ -388,6 +399,61 @@ AXB_HD SpatialPred box_pred(const Box6 &b)
   return p;
 }
 
+// State a point walk carries from one record to the next: the Q16 frame
+// (box and scale of the record just left, read only when the walk descended
+// from it) and whether it descended.
+struct WalkFrame
+{
+  Box6 par;
+  float ps[3];
+  bool desc;
+};
+
+template <class TV>
+AXB_HD void walk_frame_init(const TV &tv, WalkFrame &fr)
+{
+  fr.par = tv.scene;
+  fr.ps[0] = tv.ss[0], fr.ps[1] = tv.ss[1], fr.ps[2] = tv.ss[2];
+  fr.desc = false;
+}
+
+// One record of the point walk at slot p, leaf and internal node through
+// the same code: the closest point of a leaf is the point, that of a node
+// box the clamp of the centre.  Advances p; returns false when the walk
+// stops here (the sink asked to, or a zero rope, reported as a layout
+// error).  A leaf at slot `skip` is not reported.
+template <int F, class TV, class Sink>
+AXB_HD bool point_step(const TV &tv, const SpatialPred &pr, uint32_t &p, uint32_t skip,
+                       WalkFrame &fr, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
+{
+  constexpr uint32_t SI = internal_slots<F>();
+  const uint4 s0 = tv.slots[p];
+  const uint4 s1 = F == NODES_FP32 ? tv.slots[p + 1] : s0;
+  const int32_t w = (int32_t)s0.w;
+  const bool leaf = w < 0;
+  Box6 bx; // meaningless for a leaf record, and unused then
+  node_box<F>(tv, s0, s1, fr.desc, fr.par, fr.ps, bx);
+  const float cx = leaf ? bits_f(s0.x) : clamp_box(pr.cx, bx.lo[0], bx.hi[0]);
+  const float cy = leaf ? bits_f(s0.y) : clamp_box(pr.cy, bx.lo[1], bx.hi[1]);
+  const float cz = leaf ? bits_f(s0.z) : clamp_box(pr.cz, bx.lo[2], bx.hi[2]);
+  const bool pass = dist2_pp(pr.cx, pr.cy, pr.cz, cx, cy, cz) <= pr.thr;
+  v_leaf += leaf;
+  v_int += !leaf;
+  if (leaf && pass && p != skip && !sink.hit(~w))
+    return false;
+  if (!leaf && !pass && w == 0)
+  {
+    raise_error(tv.err, KERR_LAYOUT);
+    return false;
+  }
+  // the frame of the next record is this box only when it descends; it is
+  // stored every step (selects, no branch) and read only then
+  set_frame<F>(bx, fr.par, fr.ps);
+  fr.desc = !leaf && pass;
+  p += leaf ? 1u : (pass ? SI : (uint32_t)w);
+  return true;
+}
+
 // (Host-callable as well, so the CPU tests run this very walk over the
 // oracle's node arrays.)
 // Stackless pre-order walk over slots [p, end) (TreeTraversal.hpp:97-119:
@@ -443,50 +509,10 @@ template <int F, class TV, class Sink>
 AXB_HD void rope_walk_points(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
                              uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
 {
-  constexpr uint32_t SI = internal_slots<F>();
-  Box6 par = tv.scene;
-  float ps[3] = {tv.ss[0], tv.ss[1], tv.ss[2]};
-  bool desc = false;
-  while (p < end)
+  WalkFrame fr;
+  walk_frame_init(tv, fr);
+  while (p < end && point_step<F>(tv, pr, p, skip, fr, sink, v_int, v_leaf))
   {
-    const uint4 s0 = tv.slots[p];
-    const uint4 s1 = F == NODES_FP32 ? tv.slots[p + 1] : s0;
-    const int32_t w = (int32_t)s0.w;
-    const bool leaf = w < 0;
-    Box6 bx; // meaningless for a leaf record, and unused then
-    node_box<F>(tv, s0, s1, desc, par, ps, bx);
-    const float cx = leaf ? bits_f(s0.x) : fminf(fmaxf(pr.cx, bx.lo[0]), bx.hi[0]);
-    const float cy = leaf ? bits_f(s0.y) : fminf(fmaxf(pr.cy, bx.lo[1]), bx.hi[1]);
-    const float cz = leaf ? bits_f(s0.z) : fminf(fmaxf(pr.cz, bx.lo[2]), bx.hi[2]);
-    const bool pass = dist2_pp(pr.cx, pr.cy, pr.cz, cx, cy, cz) <= pr.thr;
-    if (leaf)
-    {
-      ++v_leaf;
-      if (pass && p != skip && !sink.hit(~w))
-        break;
-      p += 1;
-      desc = false;
-    }
-    else
-    {
-      ++v_int;
-      if (pass)
-      {
-        set_frame<F>(bx, par, ps);
-        desc = true;
-        p += SI;
-      }
-      else
-      {
-        if (w == 0)
-        {
-          raise_error(tv.err, KERR_LAYOUT);
-          break;
-        }
-        desc = false;
-        p += (uint32_t)w;
-      }
-    }
   }
 }
 
@@ -563,6 +589,248 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
       }
     }
   }
+}
+
+// ------------------------------------------------------------ k nearest
+// (distance, index) packed into one 64-bit key: distances are >= 0, whose
+// float bit patterns order like the values, so the key order is exactly the
+// lexicographic (distance, index) order.
+AXB_HD uint64_t pack_key(float d, int32_t i)
+{
+  return ((uint64_t)f_bits(d) << 32) | (uint32_t)i;
+}
+
+// Sorted list of K keys in registers; only the first k (k <= K) entries are
+// meaningful.  Insertion is a branch-free network a'[j] = min(a[j],
+// max(a[j-1], x)) with static indexing only.
+template <int K>
+struct RegList
+{
+  uint64_t a[K];
+  int k, count;
+  float kth_d;
+  uint64_t kth;
+  AXB_HD void init(int k_)
+  {
+    k = k_;
+    count = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      a[j] = ~0ull;
+    kth_d = INFINITY;
+    kth = ~0ull;
+  }
+  AXB_HD bool full() const { return count >= k; }
+  AXB_HD bool accepts(float dn, int32_t in) const { return pack_key(dn, in) < kth; }
+  AXB_HD void insert(float dn, int32_t in)
+  {
+    const uint64_t x = pack_key(dn, in);
+#pragma unroll
+    for (int j = K - 1; j > 0; --j)
+    {
+      const uint64_t hi = a[j - 1] < x ? x : a[j - 1];
+      a[j] = hi < a[j] ? hi : a[j];
+    }
+    a[0] = x < a[0] ? x : a[0];
+    ++count;
+    uint64_t kk = a[0];
+#pragma unroll
+    for (int j = 1; j < K; ++j)
+      if (j == k - 1)
+        kk = a[j];
+    if (count >= k)
+    {
+      kth = kk;
+      kth_d = bits_f((uint32_t)(kk >> 32));
+    }
+  }
+  AXB_HD void write(int32_t *oi, float *od, int per) const
+  {
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (j < per)
+      {
+        oi[j] = (int32_t)(uint32_t)a[j];
+        if (od)
+          od[j] = bits_f((uint32_t)(a[j] >> 32));
+      }
+  }
+};
+
+// Compare-exchange of two keys into ascending order.
+AXB_HD void cas_u64(uint64_t &x, uint64_t &y)
+{
+  const bool lt = x < y;
+  const uint64_t lo = lt ? x : y;
+  y = lt ? y : x;
+  x = lo;
+}
+
+// Bitonic sorting network, ascending (N a power of two; static indices only,
+// so the array stays in registers).
+template <int N>
+AXB_HD void bitonic_sort(uint64_t (&a)[N])
+{
+#pragma unroll
+  for (int size = 2; size <= N; size <<= 1)
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+      {
+        const int j = i ^ stride;
+        if (j > i)
+        {
+          if ((i & size) == 0)
+            cas_u64(a[i], a[j]);
+          else
+            cas_u64(a[j], a[i]);
+        }
+      }
+}
+
+// Half-cleaner cascade: sorts a bitonic sequence ascending.
+template <int N>
+AXB_HD void bitonic_merge(uint64_t (&a)[N])
+{
+#pragma unroll
+  for (int stride = N >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+    {
+      const int j = i ^ stride;
+      if (j > i)
+        cas_u64(a[i], a[j]);
+    }
+}
+
+// Exact top-m (m <= K) of a collected row of c (d2, index) keys (the row at
+// cand + row_at), written as (distance, index) ascending, distance =
+// RN(sqrt(d2)), to oi_base + out_at (and od_base + out_at unless null).
+//
+// The K smallest keys are kept sorted in registers: the first K entries are
+// sorted by a bitonic network, then each further batch of NB entries is
+// sorted and merged in (the elementwise minimum of the ascending list and
+// the reversed batch is a bitonic sequence holding the K smallest; a
+// half-cleaner cascade sorts it).  About 13 compare-exchanges per entry for
+// K = 16, against a K-step insertion network per entry.
+//
+// The reference orders by (distance, index) with distance = RN(sqrt(d2)).
+// Sorting by (d2, index) gives the same order unless two different d2 round
+// to the same distance among the outputs, or an entry left out has the last
+// output's distance with another d2; in that case (rare: the d2 are within a
+// few ulps) the row is selected again with the distance keys themselves.
+//
+// The row holds room for at least K entries and for c rounded up to a
+// multiple of 8, and starts on a 16-byte boundary: entries past c are read
+// and ignored.
+// Keys row[b, b + N) with entries from c on replaced by ~0.  The whole range
+// lies inside the row's capacity (the caller's contract), so every load is
+// issued unconditionally, 16 bytes at a time, all in flight together.
+template <int N>
+AXB_HD void load_keys(const uint2 *__restrict__ row, int b, int c, uint64_t (&a)[N])
+{
+  if constexpr (N % 2 == 0)
+  {
+    const uint4 *r4 = reinterpret_cast<const uint4 *>(row + b);
+#pragma unroll
+    for (int j = 0; j < N / 2; ++j)
+    {
+      const uint4 v = r4[j];
+      a[2 * j] = b + 2 * j < c ? ((uint64_t)v.y << 32) | v.x : ~0ull;
+      a[2 * j + 1] = b + 2 * j + 1 < c ? ((uint64_t)v.w << 32) | v.z : ~0ull;
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+    {
+      const uint2 v = row[b + j];
+      a[j] = b + j < c ? ((uint64_t)v.y << 32) | v.x : ~0ull;
+    }
+  }
+}
+
+template <int K>
+AXB_HD void select_row(const uint2 *__restrict__ cand, uint32_t row_at, int c, int m,
+                      int32_t *__restrict__ oi_base, float *__restrict__ od_base, int64_t out_at)
+{
+  // (base, offset) pairs rather than pointers: the bases stay scalar, which
+  // keeps the caller's walk within its register budget
+  const uint2 *__restrict__ row = cand + row_at;
+  int32_t *__restrict__ oi = oi_base + out_at;
+  float *__restrict__ od = od_base ? od_base + out_at : nullptr;
+  constexpr int NB = K < 8 ? K : 8;
+  uint64_t A[K];
+  load_keys<K>(row, 0, c, A);
+  bitonic_sort<K>(A);
+  uint64_t left = ~0ull; // smallest key dropped by a merge
+  for (int b = K; b < c; b += NB)
+  {
+    uint64_t B[NB];
+    load_keys<NB>(row, b, c, B);
+    bitonic_sort<NB>(B);
+#pragma unroll
+    for (int j = K - NB; j < K; ++j)
+    {
+      const uint64_t x = A[j], y = B[K - 1 - j];
+      const bool lt = x < y;
+      A[j] = lt ? x : y;
+      const uint64_t drop = lt ? y : x;
+      left = drop < left ? drop : left;
+    }
+    bitonic_merge<K>(A);
+  }
+  // outputs written as the checks go; a tie rewrites them below
+  bool tie = false;
+  uint64_t last = A[0], next = left;
+  float d_prev = 0.f, d_last = 0.f;
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+  {
+    if (j < m)
+    {
+      const float dj = sqrt_rn(bits_f((uint32_t)(A[j] >> 32)));
+      if (j > 0)
+        tie = tie || (dj == d_prev && (uint32_t)(A[j] >> 32) != (uint32_t)(A[j - 1] >> 32));
+      oi[j] = (int32_t)(uint32_t)A[j];
+      if (od)
+        od[j] = dj;
+      d_prev = dj;
+      if (j == m - 1)
+        last = A[j], d_last = dj;
+    }
+    if (j == m)
+      next = A[j];
+  }
+  // a left-out entry with the last distance but another d2 (an exact repeat
+  // of the last d2 is ordered by index already; one behind it may not be)
+  const uint32_t h_last = (uint32_t)(last >> 32);
+  if (!tie && next != ~0ull && sqrt_rn(bits_f((uint32_t)(next >> 32))) == d_last)
+  {
+    if ((uint32_t)(next >> 32) != h_last)
+      tie = true;
+    else
+      for (int j = 0; j < c && !tie; ++j)
+      {
+        const uint32_t h = row[j].y;
+        tie = h != h_last && sqrt_rn(bits_f(h)) == d_last;
+      }
+  }
+  if (!tie)
+    return;
+  RegList<K> L;
+  L.init(m);
+  for (int j = 0; j < c; ++j)
+  {
+    const uint2 e = row[j];
+    const float dist = sqrt_rn(bits_f(e.y));
+    const int32_t idx = (int32_t)e.x;
+    if (L.accepts(dist, idx))
+      L.insert(dist, idx);
+  }
+  L.write(oi, od, m);
 }
 
 } // namespace axb
