@@ -51,6 +51,9 @@ EXPORTED_SYMBOLS = (
     "axb_bvh_query_intersects", "axb_bvh_query_intersects_radius",
     "axb_bvh_query_nearest", "axb_bvh_neighbor_list", "axb_find_neighbor_list",
     "axb_expand_half_to_full", "axb_pair_energy", "axb_pair_energy_fd",
+    "axb_bruteforce_create", "axb_bruteforce_destroy", "axb_bruteforce_size",
+    "axb_bruteforce_bounds", "axb_bruteforce_query_intersects",
+    "axb_bruteforce_query_intersects_radius", "axb_bruteforce_query_nearest",
     "axb_bvh_view", "axb_bvh_predicate_order", "axb_exclusive_scan_i64",
     "axb_bvh_num_slots", "axb_bvh_slots", "axb_bvh_export_topology",
     "axb_sqrt_threshold", "axb_morton64_host", "axb_q16_encode_host", "axb_profile_enable",
@@ -124,6 +127,19 @@ def lib():
                                               ctypes.POINTER(_P), ctypes.POINTER(_I64), _P]),
             "axb_expand_half_to_full": (_I32, [_P, _P, _I64, _P, ctypes.POINTER(_P),
                                                ctypes.POINTER(_I64), _P]),
+            "axb_bruteforce_create": (_I32, [_P, _I64, _I32, _P, ctypes.POINTER(_P)]),
+            "axb_bruteforce_destroy": (None, [_P]),
+            "axb_bruteforce_size": (_I64, [_P]),
+            "axb_bruteforce_bounds": (_I32, [_P, _P]),
+            "axb_bruteforce_query_intersects": (_I32, [_P, _I32, _P, _I64, _Policy, _P,
+                                                       ctypes.POINTER(_P), ctypes.POINTER(_I64),
+                                                       _P]),
+            "axb_bruteforce_query_intersects_radius": (_I32, [_P, _P, _I64, ctypes.c_float,
+                                                              _Policy, _P, ctypes.POINTER(_P),
+                                                              ctypes.POINTER(_I64), _P]),
+            "axb_bruteforce_query_nearest": (_I32, [_P, _P, _I64, _I32, _Policy, _P,
+                                                    ctypes.POINTER(_P), ctypes.POINTER(_P),
+                                                    ctypes.POINTER(_I64), _P]),
             "axb_pair_energy": (_I32, [_P, _I64, _P, _P, ctypes.c_float, _P, _P, _P]),
             "axb_pair_energy_fd": (_I32, [_P, _I64, _P, _P, ctypes.c_float, _P, _I64,
                                           ctypes.c_double, _P, _P]),
@@ -354,42 +370,7 @@ class BoundingVolumeHierarchy:
         """query(space, predicates, indices, offsets[, policy]) -> (indices, offsets)
         (src/spatial/ArborX_LinearBVH.hpp:90-110).  Nearest predicates with
         return_distances=True also return the float distances."""
-        policy = policy or TraversalPolicy()
-        s = _stream(stream)
-        ind = ctypes.c_void_p()
-        nnz = ctypes.c_int64()
-        L = lib()
-        if isinstance(predicates, Nearest):
-            pts = predicates.points
-            nq = pts.shape[0]
-            offsets = torch.empty(nq + 1, dtype=torch.int64, device="cuda")
-            dist = ctypes.c_void_p()
-            _check(L.axb_bvh_query_nearest(self._h, _ptr(pts), nq, predicates.k, policy._c(),
-                                           _ptr(offsets), ctypes.byref(ind),
-                                           ctypes.byref(dist) if return_distances else None,
-                                           ctypes.byref(nnz), s))
-            indices = _take_output(ind.value, nnz.value, torch.int32)
-            if return_distances:
-                return indices, offsets, _take_output(dist.value, nnz.value, torch.float32)
-            return indices, offsets
-        if isinstance(predicates, _PointsWithRadius):
-            pts = predicates.points
-            nq = pts.shape[0]
-            offsets = torch.empty(nq + 1, dtype=torch.int64, device="cuda")
-            _check(L.axb_bvh_query_intersects_radius(self._h, _ptr(pts), nq, predicates.r,
-                                                     policy._c(), _ptr(offsets),
-                                                     ctypes.byref(ind), ctypes.byref(nnz), s))
-            return _take_output(ind.value, nnz.value, torch.int32), offsets
-        if isinstance(predicates, Intersects):
-            g = predicates.geometry
-            kind = PREDICATE_SPHERES if isinstance(g, Spheres) else PREDICATE_BOXES
-            nq = g.data.shape[0]
-            offsets = torch.empty(nq + 1, dtype=torch.int64, device="cuda")
-            _check(L.axb_bvh_query_intersects(self._h, kind, _ptr(g.data), nq, policy._c(),
-                                              _ptr(offsets), ctypes.byref(ind),
-                                              ctypes.byref(nnz), s))
-            return _take_output(ind.value, nnz.value, torch.int32), offsets
-        raise TypeError("unsupported predicates")
+        return _crs_query("axb_bvh", self._h, predicates, policy, return_distances, stream)
 
 
     def neighbor_list(self, radius, full=True, policy=None, stream=None):
@@ -405,6 +386,92 @@ class BoundingVolumeHierarchy:
                                            policy._c(), _ptr(offsets), ctypes.byref(ind),
                                            ctypes.byref(nnz), _stream(stream)))
         return offsets, _take_output(ind.value, nnz.value, torch.int32)
+
+
+def _crs_query(prefix, handle, predicates, policy, return_distances, stream):
+    """The CRS query of an index whose C entry points start with `prefix`."""
+    policy = policy or TraversalPolicy()
+    s = _stream(stream)
+    ind = ctypes.c_void_p()
+    nnz = ctypes.c_int64()
+    L = lib()
+    if isinstance(predicates, Nearest):
+        pts = predicates.points
+        nq = pts.shape[0]
+        offsets = torch.empty(nq + 1, dtype=torch.int64, device="cuda")
+        dist = ctypes.c_void_p()
+        _check(getattr(L, prefix + "_query_nearest")(
+            handle, _ptr(pts), nq, predicates.k, policy._c(), _ptr(offsets), ctypes.byref(ind),
+            ctypes.byref(dist) if return_distances else None, ctypes.byref(nnz), s))
+        indices = _take_output(ind.value, nnz.value, torch.int32)
+        if return_distances:
+            return indices, offsets, _take_output(dist.value, nnz.value, torch.float32)
+        return indices, offsets
+    if isinstance(predicates, _PointsWithRadius):
+        pts = predicates.points
+        nq = pts.shape[0]
+        offsets = torch.empty(nq + 1, dtype=torch.int64, device="cuda")
+        _check(getattr(L, prefix + "_query_intersects_radius")(
+            handle, _ptr(pts), nq, predicates.r, policy._c(), _ptr(offsets), ctypes.byref(ind),
+            ctypes.byref(nnz), s))
+        return _take_output(ind.value, nnz.value, torch.int32), offsets
+    if isinstance(predicates, Intersects):
+        g = predicates.geometry
+        kind = PREDICATE_SPHERES if isinstance(g, Spheres) else PREDICATE_BOXES
+        nq = g.data.shape[0]
+        offsets = torch.empty(nq + 1, dtype=torch.int64, device="cuda")
+        _check(getattr(L, prefix + "_query_intersects")(
+            handle, kind, _ptr(g.data), nq, policy._c(), _ptr(offsets), ctypes.byref(ind),
+            ctypes.byref(nnz), s))
+        return _take_output(ind.value, nnz.value, torch.int32), offsets
+    raise TypeError("unsupported predicates")
+
+
+class BruteForce:
+    """ArborX::BruteForce over points (N, 3) or boxes (N, 6)
+    (src/spatial/ArborX_BruteForce.hpp): the BVH's construction and query
+    interface, answered by testing every predicate against every value.
+    Spatial rows list values in increasing index order; nearest rows are
+    ordered by (distance, index)."""
+
+    def __init__(self, values=None, geometry=None, stream=None):
+        self._h = None
+        _use_torch_allocator()
+        if values is None:
+            values = torch.empty(0, 3)
+            geometry = "points"
+        v = torch.as_tensor(values)
+        if geometry is None:
+            geometry = "boxes" if (v.dim() == 2 and v.shape[1] == 6) else "points"
+        self.geometry = geometry
+        g = GEOMETRY_POINTS if geometry == "points" else GEOMETRY_BOXES
+        v = _dev_f32(v, 3 if g == GEOMETRY_POINTS else 6)
+        h = ctypes.c_void_p()
+        _check(lib().axb_bruteforce_create(_ptr(v), v.shape[0], g, _stream(stream),
+                                           ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.axb_bruteforce_destroy(self._h)
+            self._h = None
+
+    def size(self):
+        return int(lib().axb_bruteforce_size(self._h))
+
+    def empty(self):
+        return self.size() == 0
+
+    def bounds(self):
+        out = (ctypes.c_float * 6)()
+        _check(lib().axb_bruteforce_bounds(self._h, ctypes.cast(out, _P)))
+        return [float(x) for x in out]
+
+    def query(self, predicates, policy=None, return_distances=False, stream=None):
+        """query(space, predicates, indices, offsets[, policy]) -> (indices, offsets)
+        (ArborX_BruteForce.hpp:66-91); as BoundingVolumeHierarchy.query."""
+        return _crs_query("axb_bruteforce", self._h, predicates, policy, return_distances,
+                          stream)
 
 
 BVH = BoundingVolumeHierarchy

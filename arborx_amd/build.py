@@ -21,7 +21,8 @@ LIB = os.path.join(LIB_DIR, "libarborx_mi355x.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["axb_sort.hip", "axb_build.hip", "axb_query.hip", "axb_energy.hip", "axb_capi.hip"]
+SOURCES = ["axb_sort.hip", "axb_build.hip", "axb_query.hip", "axb_bruteforce.hip", "axb_energy.hip",
+           "axb_capi.hip"]
 
 CFLAGS = [
     f"--offload-arch={ARCH}",

@@ -543,6 +543,25 @@ int64_t total_slots_for(int64_t n, int geometry, int node_format)
 
 // Builds the tree; all device buffers in `out` are allocated here with
 // dalloc on `stream`.  keep_topology keeps keys/perm/leaf_rope/inodes.
+hipError_t scene_box(const float *values, int64_t n, int geometry, float *scene,
+                     hipStream_t stream)
+{
+  const int partial_blocks = (int)std::min<int64_t>(1024, div_up(n > 0 ? n : 1, 256));
+  float *partials = (float *)dalloc(partial_blocks * 6 * sizeof(float), stream);
+  if (!partials)
+    return hipErrorOutOfMemory;
+  if (geometry == GEOM_POINTS)
+    AXB_LAUNCH(k_scene_partial<GEOM_POINTS>, dim3(partial_blocks), dim3(256), 0, stream, values,
+               n, partials);
+  else
+    AXB_LAUNCH(k_scene_partial<GEOM_BOXES>, dim3(partial_blocks), dim3(256), 0, stream, values,
+               n, partials);
+  AXB_LAUNCH(k_scene_final, dim3(1), dim3(256), 0, stream, partials, partial_blocks, scene);
+  const hipError_t e = hipGetLastError();
+  dfree(partials, stream);
+  return e;
+}
+
 hipError_t build_tree(const float *values, int64_t n, int geometry,
                       int node_format, bool keep_topology, uint32_t *err,
                       hipStream_t stream, BuildOut &out)

@@ -530,7 +530,7 @@ static axb_status predicate_order(const axb_bvh *t, int pred_kind, const float *
 // < 0 -> that many, and a row that overflows is an error
 // (AXB_ERR_BUFFER_OVERFLOW, the reference's SearchException).
 template <class First, class Fill>
-static axb_status crs_query(const axb_bvh *t, int64_t nq, axb_traversal_policy policy,
+static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy policy,
                             const uint32_t *ovf_order, bool ovf_emit_index, First first,
                             Fill fill, int64_t *offsets, int32_t **indices, int64_t *nnz_host,
                             hipStream_t s, const char *what)
@@ -564,7 +564,7 @@ static axb_status crs_query(const axb_bvh *t, int64_t nq, axb_traversal_policy p
   if (e == hipSuccess)
     e = fill_u64((uint64_t *)(offsets + nq), 0ull, 1, s);
   if (e == hipSuccess)
-    e = exclusive_scan_i64(offsets, offsets, nq + 1, scan_ws, t->d_err, s);
+    e = exclusive_scan_i64(offsets, offsets, nq + 1, scan_ws, d_err, s);
   if (e == hipSuccess && B > 0)
   {
     e = fill_u32(ovf_n, 0u, 1, s);
@@ -615,7 +615,7 @@ static axb_status crs_query(const axb_bvh *t, int64_t nq, axb_traversal_policy p
   }
   if (st != AXB_OK)
     return st;
-  st = check_kernel_error(t->d_err, s, what);
+  st = check_kernel_error(d_err, s, what);
   if (st != AXB_OK)
   {
     out_free(*indices, s);
@@ -666,8 +666,8 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
     return launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, n_list, radius,
                           list ? list : order, offsets, out, 0, MODE_FILL, s);
   };
-  axb_status st = crs_query(t, nq, policy, order, false, first, fill, offsets, indices, nnz_host,
-                            s, "query");
+  axb_status st = crs_query(t->d_err, nq, policy, order, false, first, fill, offsets, indices,
+                            nnz_host, s, "query");
   dfree(order_blk, s);
   return st;
 }
@@ -808,8 +808,8 @@ axb_status axb_bvh_neighbor_list(const axb_bvh *t, float radius, int32_t full,
     return launch_nlist(tv, t->node_format, f, radius, n_list, list, offsets, out, 0, nullptr,
                         MODE_FILL, s);
   };
-  axb_status st = crs_query(t, n, policy, rowof, true, first, fill, offsets, indices, nnz_host,
-                            s, "axb_bvh_neighbor_list");
+  axb_status st = crs_query(t->d_err, n, policy, rowof, true, first, fill, offsets, indices,
+                            nnz_host, s, "axb_bvh_neighbor_list");
   dfree(rowof, s);
   return st;
 }
@@ -1080,6 +1080,207 @@ axb_status axb_pair_energy_fd(const float *xyz, int64_t n, const int64_t *offset
       pair_energy_fd(xyz, offsets, indices, cutoff, atoms, m, h, fd, (hipStream_t)stream);
   if (e != hipSuccess)
     return hip_fail(e, "axb_pair_energy_fd");
+  return AXB_OK;
+}
+
+
+// ------------------------------------------------------------ brute force
+// ArborX::BruteForce (src/spatial/ArborX_BruteForce.hpp): the values (a
+// device copy), their bounds and the kernels' error word.
+struct axb_bruteforce
+{
+  int64_t n = 0;
+  int geometry = 0;
+  float *values = nullptr;
+  float bounds[6];
+  hipStream_t stream = nullptr;
+  uint32_t *d_err = nullptr;
+};
+
+axb_status axb_bruteforce_create(const float *values, int64_t n, int32_t geometry,
+                                 axb_stream_t stream_, axb_bruteforce **out)
+{
+  if (!out)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_create: out is NULL");
+  *out = nullptr;
+  if (n < 0 || n >= (int64_t(1) << 31))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_create: n must be in [0, 2^31)");
+  if (geometry != AXB_GEOMETRY_POINTS && geometry != AXB_GEOMETRY_BOXES)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_create: unknown geometry");
+  if (n > 0 && !values)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_create: values is NULL");
+  hipStream_t s = (hipStream_t)stream_;
+  const size_t bytes = (size_t)n * (geometry == AXB_GEOMETRY_POINTS ? 3 : 6) * sizeof(float);
+  axb_bruteforce *bf = new axb_bruteforce;
+  bf->n = n;
+  bf->geometry = geometry;
+  bf->stream = s;
+  bf->d_err = (uint32_t *)dalloc(sizeof(uint32_t), s);
+  bf->values = n > 0 ? (float *)dalloc(bytes, s) : nullptr;
+  float *scene = (float *)dalloc(6 * sizeof(float), s);
+  hipError_t e = (bf->d_err && scene && (n == 0 || bf->values)) ? hipSuccess : hipErrorOutOfMemory;
+  if (e == hipSuccess)
+    e = fill_u32(bf->d_err, 0u, 1, s);
+  if (e == hipSuccess && n > 0)
+    e = hipMemcpyAsync(bf->values, values, bytes, hipMemcpyDeviceToDevice, s);
+  // bounds: the reduction over the values (BruteForceImpl.hpp:34-50)
+  if (e == hipSuccess)
+    e = scene_box(bf->values, n, geometry, scene, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(bf->bounds, scene, sizeof(bf->bounds), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = sync_stream(s);
+  dfree(scene, s);
+  if (e != hipSuccess)
+  {
+    axb_status st = hip_fail(e, "axb_bruteforce_create");
+    axb_bruteforce_destroy(bf);
+    return st;
+  }
+  *out = bf;
+  return AXB_OK;
+}
+
+void axb_bruteforce_destroy(axb_bruteforce *bf)
+{
+  if (!bf)
+    return;
+  dfree(bf->values, bf->stream);
+  dfree(bf->d_err, bf->stream);
+  delete bf;
+}
+
+int64_t axb_bruteforce_size(const axb_bruteforce *bf) { return bf ? bf->n : 0; }
+
+axb_status axb_bruteforce_bounds(const axb_bruteforce *bf, float out_host[6])
+{
+  if (!bf || !out_host)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_bounds: NULL argument");
+  std::memcpy(out_host, bf->bounds, sizeof(bf->bounds));
+  return AXB_OK;
+}
+
+static axb_status bf_query_spatial(const axb_bruteforce *bf, int pred_kind, const float *preds,
+                                   int64_t nq, float radius, axb_traversal_policy policy,
+                                   int64_t *offsets, int32_t **indices, int64_t *nnz_host,
+                                   axb_stream_t stream_)
+{
+  if (!bf || !offsets || !indices || !nnz_host)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "bruteforce query: NULL argument");
+  if (nq < 0 || nq >= (int64_t(1) << 31))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "bruteforce query: nq must be in [0, 2^31)");
+  if (nq > 0 && !preds)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "bruteforce query: predicates is NULL");
+  hipStream_t s = (hipStream_t)stream_;
+  *indices = nullptr;
+  *nnz_host = 0;
+  if (nq == 0 || bf->n == 0)
+  {
+    AXB_HIP_TRY(fill_u64((uint64_t *)offsets, 0ull, nq + 1, s));
+    *indices = (int32_t *)out_alloc(0, s);
+    AXB_HIP_TRY(sync_stream(s));
+    return AXB_OK;
+  }
+  // predicates in the caller's order: the values are read in the same order
+  // by every lane, so there is no locality to gain from sorting them
+  auto first = [&](int mode, int32_t *out, int32_t B) {
+    return launch_bf_spatial(bf->values, bf->n, bf->geometry, pred_kind, preds, nq, radius,
+                             nullptr, offsets, out, B, mode, bf->d_err, s);
+  };
+  auto fill = [&](const uint32_t *list, int64_t n_list, int32_t *out) {
+    return launch_bf_spatial(bf->values, bf->n, bf->geometry, pred_kind, preds, n_list, radius,
+                             list, offsets, out, 0, MODE_FILL, bf->d_err, s);
+  };
+  return crs_query(bf->d_err, nq, policy, nullptr, false, first, fill, offsets, indices,
+                   nnz_host, s, "bruteforce query");
+}
+
+axb_status axb_bruteforce_query_intersects(const axb_bruteforce *bf, int32_t predicate_kind,
+                                          const float *predicates, int64_t nq,
+                                          axb_traversal_policy policy, int64_t *offsets,
+                                          int32_t **indices, int64_t *nnz_host,
+                                          axb_stream_t stream)
+{
+  if (predicate_kind != AXB_PREDICATE_SPHERES && predicate_kind != AXB_PREDICATE_BOXES)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT,
+             "axb_bruteforce_query_intersects: unknown predicate kind");
+  return bf_query_spatial(bf, predicate_kind, predicates, nq, 0.f, policy, offsets, indices,
+                          nnz_host, stream);
+}
+
+axb_status axb_bruteforce_query_intersects_radius(const axb_bruteforce *bf, const float *points,
+                                                 int64_t nq, float radius,
+                                                 axb_traversal_policy policy, int64_t *offsets,
+                                                 int32_t **indices, int64_t *nnz_host,
+                                                 axb_stream_t stream)
+{
+  // PredicateHelpers.hpp:102: KOKKOS_ASSERT(r > 0)
+  if (!(radius > 0.f))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT,
+             "axb_bruteforce_query_intersects_radius: radius must be > 0");
+  return bf_query_spatial(bf, PRED_RADIUS, points, nq, radius, policy, offsets, indices,
+                          nnz_host, stream);
+}
+
+axb_status axb_bruteforce_query_nearest(const axb_bruteforce *bf, const float *points,
+                                       int64_t nq, int32_t k, axb_traversal_policy policy,
+                                       int64_t *offsets, int32_t **indices, float **distances,
+                                       int64_t *nnz_host, axb_stream_t stream_)
+{
+  (void)policy; // no traversal order to choose: every value is visited
+  if (!bf || !offsets || !indices || !nnz_host)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_query_nearest: NULL argument");
+  if (nq < 0 || nq >= (int64_t(1) << 31))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_query_nearest: nq must be in [0, 2^31)");
+  if (nq > 0 && !points)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_query_nearest: points is NULL");
+  hipStream_t s = (hipStream_t)stream_;
+  const int64_t per = (k < 1 || bf->n == 0) ? 0 : std::min<int64_t>(k, bf->n);
+  const int64_t nnz = nq * per;
+  if (nnz >= (int64_t(1) << 40))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_query_nearest: result too large");
+  *indices = nullptr;
+  if (distances)
+    *distances = nullptr;
+  AXB_HIP_TRY(launch_knn_offsets(offsets, nq, per, s));
+  *indices = (int32_t *)out_alloc((size_t)nnz * sizeof(int32_t), s);
+  float *dist = distances ? (float *)out_alloc((size_t)nnz * sizeof(float), s) : nullptr;
+  if (!*indices || (distances && !dist))
+  {
+    out_free(*indices, s);
+    out_free(dist, s);
+    *indices = nullptr;
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bruteforce_query_nearest: out of device memory");
+  }
+  hipError_t e = hipSuccess;
+  float *tmp_dist = nullptr;
+  if (nnz > 0)
+  {
+    if (k > 64 && !dist) // the heap keeps its distances in the output row
+    {
+      tmp_dist = (float *)dalloc((size_t)nnz * sizeof(float), s);
+      if (!tmp_dist)
+        e = hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess)
+      e = launch_bf_nearest(bf->values, bf->n, bf->geometry, points, nq, k, *indices,
+                            dist ? dist : tmp_dist, s);
+  }
+  if (e == hipSuccess)
+    e = sync_stream(s);
+  dfree(tmp_dist, s);
+  axb_status st = e == hipSuccess ? check_kernel_error(bf->d_err, s, "axb_bruteforce_query_nearest")
+                                  : hip_fail(e, "axb_bruteforce_query_nearest");
+  if (st != AXB_OK)
+  {
+    out_free(*indices, s);
+    out_free(dist, s);
+    *indices = nullptr;
+    return st;
+  }
+  if (distances)
+    *distances = dist;
+  *nnz_host = nnz;
   return AXB_OK;
 }
 

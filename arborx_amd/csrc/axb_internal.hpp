@@ -157,8 +157,83 @@ constexpr int MODE_COUNT = 0;
 constexpr int MODE_FILL = 1;
 constexpr int MODE_BUFFER = 2;
 
+// ------------------------------------------------------------ CRS rows
+// int32 x 4 with 4-byte alignment (gfx9 global stores take dword-aligned
+// multi-dword accesses).
+typedef int32_t i32x4_u __attribute__((ext_vector_type(4), aligned(4)));
+
+// Result sink of one row.  MODE_COUNT only counts; MODE_BUFFER keeps the
+// first `buffer` hits in the row's scratch slot; MODE_FILL writes the row of
+// the CRS.  Hits leave in 16-byte groups (one store per 4 hits).
+template <int MODE>
+struct RowWriter
+{
+  int32_t *out;
+  int64_t base, limit; // FILL: row start and length; BUFFER: limit = buffer
+  int64_t cnt = 0;
+  int32_t h0 = 0, h1 = 0, h2 = 0;
+
+  AXB_D bool hit(int32_t v)
+  {
+    if (MODE != MODE_COUNT && cnt < limit)
+    {
+      const int j = (int)(cnt & 3);
+      if (j == 3)
+        *(i32x4_u *)(out + base + cnt - 3) = (i32x4_u){h0, h1, h2, v};
+      else
+      {
+        h0 = j == 0 ? v : h0;
+        h1 = j == 1 ? v : h1;
+        h2 = j == 2 ? v : h2;
+      }
+    }
+    ++cnt;
+    return true;
+  }
+  // Writes the last partial group of the kept hits.
+  AXB_D void finish()
+  {
+    if (MODE == MODE_COUNT)
+      return;
+    const int64_t kept = cnt < limit ? cnt : limit;
+    const int rem = (int)(kept & 3);
+    if (rem > 0)
+    {
+      const int64_t at = base + kept - rem;
+      out[at] = h0;
+      if (rem > 1)
+        out[at + 1] = h1;
+      if (rem > 2)
+        out[at + 2] = h2;
+    }
+  }
+};
+
+template <int MODE>
+AXB_D void row_epilogue(const TreeView &tv, RowWriter<MODE> &rw, int64_t *offsets, int64_t q,
+                        uint32_t v_int, uint32_t v_leaf)
+{
+  rw.finish();
+  if (MODE != MODE_FILL)
+    offsets[q] = rw.cnt;
+  else if (rw.cnt != rw.limit)
+    raise_error(tv.err, KERR_LAYOUT);
+  if (tv.visits && MODE != MODE_FILL)
+  {
+    atomicAdd(tv.visits + 0, 1ull);
+    atomicAdd(tv.visits + 1, (unsigned long long)v_int);
+    atomicAdd(tv.visits + 2, (unsigned long long)v_leaf);
+    atomicAdd(tv.visits + 3, (unsigned long long)rw.cnt);
+  }
+}
+
+
 constexpr int SLOT_PAD = 4; // zeroed slots after the array (read-ahead of leaf groups)
 int64_t total_slots_for(int64_t n, int geometry, int node_format);
+// Scene box {lo xyz, hi xyz} of n points or boxes (TreeConstruction.hpp:26-39
+// semantics; the empty box {+FLT_MAX, -FLT_MAX} for n = 0) into device scene.
+hipError_t scene_box(const float *values, int64_t n, int geometry, float *scene,
+                     hipStream_t stream);
 hipError_t build_tree(const float *values, int64_t n, int geometry, int node_format,
                       bool keep_topology, uint32_t *err, hipStream_t stream, BuildOut &out);
 hipError_t export_topology(const INode *inodes, int64_t nint, int32_t *left_child, int32_t *rope,
@@ -192,6 +267,17 @@ hipError_t launch_h2f_count(const int64_t *offsets, const int32_t *indices, int6
 hipError_t launch_h2f_fill(const int64_t *offsets, const int32_t *indices, int64_t n,
                            const int64_t *out_offsets, int64_t *cursor, int32_t *out,
                            hipStream_t s);
+// Brute-force index (every predicate against every value): spatial rows in
+// value order through the RowWriter modes of the tree queries (lane t takes
+// predicate list ? list[t] : t, nq of them); nearest rows of min(k, n)
+// (distance, index) pairs, for k <= 64 by the register selection, above by
+// a heap in the output row.
+hipError_t launch_bf_spatial(const float *values, int64_t n, int geometry, int pred_kind,
+                             const float *preds, int64_t nq, float radius, const uint32_t *list,
+                             int64_t *offsets, int32_t *out, int32_t buffer, int mode,
+                             uint32_t *err, hipStream_t s);
+hipError_t launch_bf_nearest(const float *values, int64_t n, int geometry, const float *pts,
+                             int64_t nq, int k, int32_t *oi, float *od, hipStream_t s);
 hipError_t pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
                        const int32_t *indices, float cutoff, double *energy, float *grad,
                        hipStream_t s);

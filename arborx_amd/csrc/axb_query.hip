@@ -33,76 +33,6 @@ namespace axb
 namespace
 {
 
-// int32 x 4 with 4-byte alignment (gfx9 global stores take dword-aligned
-// multi-dword accesses).
-typedef int32_t i32x4_u __attribute__((ext_vector_type(4), aligned(4)));
-
-// ------------------------------------------------------------ spatial
-// Result sink of one row.  MODE_COUNT only counts; MODE_BUFFER keeps the
-// first `buffer` hits in the row's scratch slot; MODE_FILL writes the row of
-// the CRS.  Hits leave in 16-byte groups (one store per 4 hits).
-template <int MODE>
-struct RowWriter
-{
-  int32_t *out;
-  int64_t base, limit; // FILL: row start and length; BUFFER: limit = buffer
-  int64_t cnt = 0;
-  int32_t h0 = 0, h1 = 0, h2 = 0;
-
-  AXB_D bool hit(int32_t v)
-  {
-    if (MODE != MODE_COUNT && cnt < limit)
-    {
-      const int j = (int)(cnt & 3);
-      if (j == 3)
-        *(i32x4_u *)(out + base + cnt - 3) = (i32x4_u){h0, h1, h2, v};
-      else
-      {
-        h0 = j == 0 ? v : h0;
-        h1 = j == 1 ? v : h1;
-        h2 = j == 2 ? v : h2;
-      }
-    }
-    ++cnt;
-    return true;
-  }
-  // Writes the last partial group of the kept hits.
-  AXB_D void finish()
-  {
-    if (MODE == MODE_COUNT)
-      return;
-    const int64_t kept = cnt < limit ? cnt : limit;
-    const int rem = (int)(kept & 3);
-    if (rem > 0)
-    {
-      const int64_t at = base + kept - rem;
-      out[at] = h0;
-      if (rem > 1)
-        out[at + 1] = h1;
-      if (rem > 2)
-        out[at + 2] = h2;
-    }
-  }
-};
-
-template <int MODE>
-AXB_D void row_epilogue(const TreeView &tv, RowWriter<MODE> &rw, int64_t *offsets, int64_t q,
-                        uint32_t v_int, uint32_t v_leaf)
-{
-  rw.finish();
-  if (MODE != MODE_FILL)
-    offsets[q] = rw.cnt;
-  else if (rw.cnt != rw.limit)
-    raise_error(tv.err, KERR_LAYOUT);
-  if (tv.visits && MODE != MODE_FILL)
-  {
-    atomicAdd(tv.visits + 0, 1ull);
-    atomicAdd(tv.visits + 1, (unsigned long long)v_int);
-    atomicAdd(tv.visits + 2, (unsigned long long)v_leaf);
-    atomicAdd(tv.visits + 3, (unsigned long long)rw.cnt);
-  }
-}
-
 // One lane per predicate (in `order` when given): rope traversal of the
 // whole tree; row q of the CRS (counts, scratch rows or the final rows).
 template <int F, int G, int PK, int MODE>
@@ -172,10 +102,6 @@ __global__ __launch_bounds__(256) void k_nlist(TreeView tv, float radius, int64_
 }
 
 // ------------------------------------------------------------ nearest
-AXB_D bool pair_less(float da, int32_t ia, float db, int32_t ib)
-{
-  return da < db || (da == db && ia < ib);
-}
 
 // The K smallest values seen (float bits, ascending; no indices): the seed
 // only needs the k-th distance, in half the registers of RegList.  It is fed
@@ -215,88 +141,6 @@ struct DistList
   }
 };
 
-// Binary max-heap on (distance, index) stored in the output row (global).
-struct RowHeap
-{
-  int32_t *hi;
-  float *hd;
-  int k, count;
-  float kth_d;
-  int32_t kth_i;
-  AXB_D void init(int k_, int32_t *oi, float *od)
-  {
-    hi = oi, hd = od, k = k_, count = 0;
-    kth_d = INFINITY, kth_i = INT_MAX;
-  }
-  AXB_D bool full() const { return count >= k; }
-  AXB_D bool accepts(float dn, int32_t in) const
-  {
-    return count < k || pair_less(dn, in, kth_d, kth_i);
-  }
-  AXB_D void insert(float dn, int32_t in)
-  {
-    if (count < k)
-    {
-      // push: sift up
-      int j = count++;
-      while (j > 0)
-      {
-        int par = (j - 1) >> 1;
-        if (!pair_less(hd[par], hi[par], dn, in))
-          break;
-        hd[j] = hd[par], hi[j] = hi[par];
-        j = par;
-      }
-      hd[j] = dn, hi[j] = in;
-    }
-    else
-    {
-      // pop-push: replace the top, sift down (PriorityQueue.hpp:87-93)
-      int j = 0;
-      while (true)
-      {
-        int l = 2 * j + 1;
-        if (l >= k)
-          break;
-        int r = l + 1;
-        int big = (r < k && pair_less(hd[l], hi[l], hd[r], hi[r])) ? r : l;
-        if (!pair_less(dn, in, hd[big], hi[big]))
-          break;
-        hd[j] = hd[big], hi[j] = hi[big];
-        j = big;
-      }
-      hd[j] = dn, hi[j] = in;
-    }
-    if (count >= k)
-      kth_d = hd[0], kth_i = hi[0];
-  }
-  // heap sort ascending in place (Heap.hpp sortHeap)
-  AXB_D void finish(int per)
-  {
-    for (int end = per - 1; end > 0; --end)
-    {
-      float td = hd[0];
-      int32_t ti = hi[0];
-      float xd = hd[end];
-      int32_t xi = hi[end];
-      hd[end] = td, hi[end] = ti;
-      int j = 0;
-      while (true)
-      {
-        int l = 2 * j + 1;
-        if (l >= end)
-          break;
-        int r = l + 1;
-        int big = (r < end && pair_less(hd[l], hi[l], hd[r], hi[r])) ? r : l;
-        if (!pair_less(xd, xi, hd[big], hi[big]))
-          break;
-        hd[j] = hd[big], hi[j] = hi[big];
-        j = big;
-      }
-      hd[j] = xd, hi[j] = xi;
-    }
-  }
-};
 
 template <int G>
 AXB_D float leaf_d2(float qx, float qy, float qz, uint4 s0, uint4 s1)

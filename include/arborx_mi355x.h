@@ -197,6 +197,49 @@ axb_status axb_expand_half_to_full(const int64_t *offsets,
                                    int64_t *out_offsets, int32_t **out_indices,
                                    int64_t *nnz_host, axb_stream_t stream);
 
+/* ---- brute-force index ----
+ * ArborX::BruteForce<MemorySpace, Value> (src/spatial/ArborX_BruteForce.hpp:
+ * 40-178, detail/ArborX_BruteForceImpl.hpp:34-229): the same construction
+ * and query interface as the BVH, answered by testing every predicate
+ * against every value -- an alternative index for small sets and an
+ * independent check of the tree.  The index keeps a device copy of the
+ * values.  Spatial rows list the values in increasing index order (the
+ * reference's order inside a row is unspecified); nearest rows are ordered
+ * by (distance, index) as for the BVH.  policy.buffer_size as for the BVH
+ * queries; sort_predicates has no effect. */
+typedef struct axb_bruteforce axb_bruteforce;
+/* BruteForce(space, values) (ArborX_BruteForce.hpp:52-53,117-150); n < 2^31. */
+axb_status axb_bruteforce_create(const float *values, int64_t n,
+                                 int32_t geometry, axb_stream_t stream,
+                                 axb_bruteforce **out);
+void axb_bruteforce_destroy(axb_bruteforce *bf);
+/* size() / bounds() (ArborX_BruteForce.hpp:56-62); bounds of an empty index
+ * are {+FLT_MAX, -FLT_MAX}. */
+int64_t axb_bruteforce_size(const axb_bruteforce *bf);
+axb_status axb_bruteforce_bounds(const axb_bruteforce *bf, float out_host[6]);
+/* query(space, predicates, indices, offsets[, policy])
+ * (ArborX_BruteForce.hpp:66-91,151-178, BruteForceImpl.hpp:53-153): as for
+ * axb_bvh_query_intersects / _radius. */
+axb_status axb_bruteforce_query_intersects(const axb_bruteforce *bf,
+                                           int32_t predicate_kind,
+                                           const float *predicates, int64_t nq,
+                                           axb_traversal_policy policy,
+                                           int64_t *offsets, int32_t **indices,
+                                           int64_t *nnz_host,
+                                           axb_stream_t stream);
+axb_status axb_bruteforce_query_intersects_radius(
+    const axb_bruteforce *bf, const float *points, int64_t nq, float radius,
+    axb_traversal_policy policy, int64_t *offsets, int32_t **indices,
+    int64_t *nnz_host, axb_stream_t stream);
+/* Nearest (BruteForceImpl.hpp:155-229): arguments as for
+ * axb_bvh_query_nearest. */
+axb_status axb_bruteforce_query_nearest(const axb_bruteforce *bf,
+                                        const float *points, int64_t nq,
+                                        int32_t k, axb_traversal_policy policy,
+                                        int64_t *offsets, int32_t **indices,
+                                        float **distances, int64_t *nnz_host,
+                                        axb_stream_t stream);
+
 /* Pair energy over a fixed neighbour CRS (no reference counterpart; pattern of
  * examples/molecular_dynamics/example_molecular_dynamics.cpp:128-194).
  * Shifted Lennard-Jones phi(r) = 4(r^-12 - r^-6) - phi(cutoff);

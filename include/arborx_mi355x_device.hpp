@@ -592,6 +592,94 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
 }
 
 // ------------------------------------------------------------ k nearest
+AXB_HD bool pair_less(float da, int32_t ia, float db, int32_t ib)
+{
+  return da < db || (da == db && ia < ib);
+}
+
+// Binary max-heap on (distance, index) stored in the output row (global).
+struct RowHeap
+{
+  int32_t *hi;
+  float *hd;
+  int k, count;
+  float kth_d;
+  int32_t kth_i;
+  AXB_HD void init(int k_, int32_t *oi, float *od)
+  {
+    hi = oi, hd = od, k = k_, count = 0;
+    kth_d = INFINITY, kth_i = INT_MAX;
+  }
+  AXB_HD bool full() const { return count >= k; }
+  AXB_HD bool accepts(float dn, int32_t in) const
+  {
+    return count < k || pair_less(dn, in, kth_d, kth_i);
+  }
+  AXB_HD void insert(float dn, int32_t in)
+  {
+    if (count < k)
+    {
+      // push: sift up
+      int j = count++;
+      while (j > 0)
+      {
+        int par = (j - 1) >> 1;
+        if (!pair_less(hd[par], hi[par], dn, in))
+          break;
+        hd[j] = hd[par], hi[j] = hi[par];
+        j = par;
+      }
+      hd[j] = dn, hi[j] = in;
+    }
+    else
+    {
+      // pop-push: replace the top, sift down (PriorityQueue.hpp:87-93)
+      int j = 0;
+      while (true)
+      {
+        int l = 2 * j + 1;
+        if (l >= k)
+          break;
+        int r = l + 1;
+        int big = (r < k && pair_less(hd[l], hi[l], hd[r], hi[r])) ? r : l;
+        if (!pair_less(dn, in, hd[big], hi[big]))
+          break;
+        hd[j] = hd[big], hi[j] = hi[big];
+        j = big;
+      }
+      hd[j] = dn, hi[j] = in;
+    }
+    if (count >= k)
+      kth_d = hd[0], kth_i = hi[0];
+  }
+  // heap sort ascending in place (Heap.hpp sortHeap)
+  AXB_HD void finish(int per)
+  {
+    for (int end = per - 1; end > 0; --end)
+    {
+      float td = hd[0];
+      int32_t ti = hi[0];
+      float xd = hd[end];
+      int32_t xi = hi[end];
+      hd[end] = td, hi[end] = ti;
+      int j = 0;
+      while (true)
+      {
+        int l = 2 * j + 1;
+        if (l >= end)
+          break;
+        int r = l + 1;
+        int big = (r < end && pair_less(hd[l], hi[l], hd[r], hi[r])) ? r : l;
+        if (!pair_less(xd, xi, hd[big], hi[big]))
+          break;
+        hd[j] = hd[big], hi[j] = hi[big];
+        j = big;
+      }
+      hd[j] = xd, hi[j] = xi;
+    }
+  }
+};
+
 // (distance, index) packed into one 64-bit key: distances are >= 0, whose
 // float bit patterns order like the values, so the key order is exactly the
 // lexicographic (distance, index) order.
@@ -704,26 +792,6 @@ AXB_HD void bitonic_merge(uint64_t (&a)[N])
     }
 }
 
-// Exact top-m (m <= K) of a collected row of c (d2, index) keys (the row at
-// cand + row_at), written as (distance, index) ascending, distance =
-// RN(sqrt(d2)), to oi_base + out_at (and od_base + out_at unless null).
-//
-// The K smallest keys are kept sorted in registers: the first K entries are
-// sorted by a bitonic network, then each further batch of NB entries is
-// sorted and merged in (the elementwise minimum of the ascending list and
-// the reversed batch is a bitonic sequence holding the K smallest; a
-// half-cleaner cascade sorts it).  About 13 compare-exchanges per entry for
-// K = 16, against a K-step insertion network per entry.
-//
-// The reference orders by (distance, index) with distance = RN(sqrt(d2)).
-// Sorting by (d2, index) gives the same order unless two different d2 round
-// to the same distance among the outputs, or an entry left out has the last
-// output's distance with another d2; in that case (rare: the d2 are within a
-// few ulps) the row is selected again with the distance keys themselves.
-//
-// The row holds room for at least K entries and for c rounded up to a
-// multiple of 8, and starts on a 16-byte boundary: entries past c are read
-// and ignored.
 // Keys row[b, b + N) with entries from c on replaced by ~0.  The whole range
 // lies inside the row's capacity (the caller's contract), so every load is
 // issued unconditionally, 16 bytes at a time, all in flight together.
@@ -752,24 +820,55 @@ AXB_HD void load_keys(const uint2 *__restrict__ row, int b, int c, uint64_t (&a)
   }
 }
 
-template <int K>
-AXB_HD void select_row(const uint2 *__restrict__ cand, uint32_t row_at, int c, int m,
-                      int32_t *__restrict__ oi_base, float *__restrict__ od_base, int64_t out_at)
+// Key source over a collected row of {index, d2 bits} entries.  The row
+// holds room for at least K entries and for c rounded up to a multiple of 8,
+// and starts on a 16-byte boundary: entries past c are read and ignored.
+struct RowKeys
 {
-  // (base, offset) pairs rather than pointers: the bases stay scalar, which
-  // keeps the caller's walk within its register budget
-  const uint2 *__restrict__ row = cand + row_at;
-  int32_t *__restrict__ oi = oi_base + out_at;
-  float *__restrict__ od = od_base ? od_base + out_at : nullptr;
+  const uint2 *__restrict__ row;
+  template <int N>
+  AXB_HD void load(int b, int c, uint64_t (&a)[N]) const
+  {
+    load_keys<N>(row, b, c, a);
+  }
+  AXB_HD uint64_t at(int j) const
+  {
+    const uint2 v = row[j];
+    return ((uint64_t)v.y << 32) | v.x;
+  }
+};
+
+// Exact top-m (m <= K) of c candidates given as (d2 bits, index) keys by
+// `src` (load<N>(b, c, a): keys b .. b+N-1, ~0 from c on; at(j): key j),
+// written as (distance, index) ascending, distance = RN(sqrt(d2)), to oi
+// (and od unless null).
+//
+// The K smallest keys are kept sorted in registers: the first K are sorted
+// by a bitonic network, then each further batch of NB is sorted and merged
+// in (the elementwise minimum of the ascending list and the reversed batch
+// is a bitonic sequence holding the K smallest; a half-cleaner cascade
+// sorts it).  About 13 compare-exchanges per candidate for K = 16, against
+// a K-step insertion network per candidate.
+//
+// The reference orders by (distance, index) with distance = RN(sqrt(d2)).
+// Sorting by (d2, index) gives the same order unless two different d2 round
+// to the same distance among the outputs, or a candidate left out has the
+// last output's distance with another d2; in that case (rare: the d2 are
+// within a few ulps) the candidates are selected again with the distance
+// keys themselves.
+template <int K, class Src>
+AXB_HD void select_keys(const Src &src, int c, int m, int32_t *__restrict__ oi,
+                        float *__restrict__ od)
+{
   constexpr int NB = K < 8 ? K : 8;
   uint64_t A[K];
-  load_keys<K>(row, 0, c, A);
+  src.template load<K>(0, c, A);
   bitonic_sort<K>(A);
   uint64_t left = ~0ull; // smallest key dropped by a merge
   for (int b = K; b < c; b += NB)
   {
     uint64_t B[NB];
-    load_keys<NB>(row, b, c, B);
+    src.template load<NB>(b, c, B);
     bitonic_sort<NB>(B);
 #pragma unroll
     for (int j = K - NB; j < K; ++j)
@@ -804,8 +903,9 @@ AXB_HD void select_row(const uint2 *__restrict__ cand, uint32_t row_at, int c, i
     if (j == m)
       next = A[j];
   }
-  // a left-out entry with the last distance but another d2 (an exact repeat
-  // of the last d2 is ordered by index already; one behind it may not be)
+  // a left-out candidate with the last distance but another d2 (an exact
+  // repeat of the last d2 is ordered by index already; one behind it may
+  // not be)
   const uint32_t h_last = (uint32_t)(last >> 32);
   if (!tie && next != ~0ull && sqrt_rn(bits_f((uint32_t)(next >> 32))) == d_last)
   {
@@ -814,7 +914,7 @@ AXB_HD void select_row(const uint2 *__restrict__ cand, uint32_t row_at, int c, i
     else
       for (int j = 0; j < c && !tie; ++j)
       {
-        const uint32_t h = row[j].y;
+        const uint32_t h = (uint32_t)(src.at(j) >> 32);
         tie = h != h_last && sqrt_rn(bits_f(h)) == d_last;
       }
   }
@@ -824,13 +924,26 @@ AXB_HD void select_row(const uint2 *__restrict__ cand, uint32_t row_at, int c, i
   L.init(m);
   for (int j = 0; j < c; ++j)
   {
-    const uint2 e = row[j];
-    const float dist = sqrt_rn(bits_f(e.y));
-    const int32_t idx = (int32_t)e.x;
+    const uint64_t e = src.at(j);
+    const float dist = sqrt_rn(bits_f((uint32_t)(e >> 32)));
+    const int32_t idx = (int32_t)(uint32_t)e;
     if (L.accepts(dist, idx))
       L.insert(dist, idx);
   }
   L.write(oi, od, m);
+}
+
+// select_keys over the row at cand + row_at, written to oi_base + out_at
+// (and od_base + out_at unless null).  (Base, offset) pairs rather than
+// pointers: the bases stay scalar, which keeps the caller's walk within its
+// register budget.
+template <int K>
+AXB_HD void select_row(const uint2 *__restrict__ cand, uint32_t row_at, int c, int m,
+                       int32_t *__restrict__ oi_base, float *__restrict__ od_base,
+                       int64_t out_at)
+{
+  const RowKeys src{cand + row_at};
+  select_keys<K>(src, c, m, oi_base + out_at, od_base ? od_base + out_at : nullptr);
 }
 
 } // namespace axb
