@@ -498,6 +498,164 @@ private:
 
 using BVH = BoundingVolumeHierarchy;
 
+// spatial/ArborX_BruteForce.hpp:40-178: the BVH's construction and CRS query
+// interface over an index that tests every predicate against every value.
+// Spatial rows list values in increasing index order; nearest rows are
+// ordered by (distance, index).
+class BruteForce
+{
+public:
+  using size_type = int64_t;
+  using bounding_volume_type = Box;
+
+  BruteForce() = default; // empty index
+
+  BruteForce(HipStream const &space, DeviceVector<Point> const &values)
+  {
+    build(space, (float const *)values.data(), (int64_t)values.size(), AXB_GEOMETRY_POINTS);
+  }
+  BruteForce(HipStream const &space, DeviceVector<Box> const &values)
+  {
+    build(space, (float const *)values.data(), (int64_t)values.size(), AXB_GEOMETRY_BOXES);
+  }
+  BruteForce(BruteForce const &) = delete;
+  BruteForce &operator=(BruteForce const &) = delete;
+  BruteForce(BruteForce &&o) noexcept { std::swap(_h, o._h); }
+  BruteForce &operator=(BruteForce &&o) noexcept
+  {
+    std::swap(_h, o._h);
+    return *this;
+  }
+  ~BruteForce()
+  {
+    if (_h)
+      axb_bruteforce_destroy(_h);
+  }
+
+  size_type size() const noexcept { return _h ? axb_bruteforce_size(_h) : 0; }
+  bool empty() const noexcept { return size() == 0; }
+  Box bounds() const
+  {
+    Box b;
+    if (_h)
+    {
+      float f[6];
+      Details::check(axb_bruteforce_bounds(_h, f));
+      b = Box({{f[0], f[1], f[2]}}, {{f[3], f[4], f[5]}});
+    }
+    return b;
+  }
+
+  // query(space, predicates, indices, offsets[, policy])
+  // (ArborX_BruteForce.hpp:71-91): indices/offsets are (re)allocated.
+  void query(HipStream const &space, Experimental::PointsWithRadius const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    prepare(offsets, preds.n);
+    int32_t *ind = nullptr;
+    int64_t nnz = 0;
+    if (_h)
+      Details::check(axb_bruteforce_query_intersects_radius(
+          _h, (float const *)preds.points, (int64_t)preds.n, preds.r, c_policy(policy),
+          offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
+    finish(space, indices, ind, nnz, offsets, preds.n);
+  }
+  void query(HipStream const &space, Experimental::IntersectsSpheres const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    prepare(offsets, preds.n);
+    int32_t *ind = nullptr;
+    int64_t nnz = 0;
+    if (_h)
+      Details::check(axb_bruteforce_query_intersects(
+          _h, AXB_PREDICATE_SPHERES, (float const *)preds.spheres, (int64_t)preds.n,
+          c_policy(policy), offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
+    finish(space, indices, ind, nnz, offsets, preds.n);
+  }
+  void query(HipStream const &space, Experimental::IntersectsBoxes const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    prepare(offsets, preds.n);
+    int32_t *ind = nullptr;
+    int64_t nnz = 0;
+    if (_h)
+      Details::check(axb_bruteforce_query_intersects(
+          _h, AXB_PREDICATE_BOXES, (float const *)preds.boxes, (int64_t)preds.n,
+          c_policy(policy), offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
+    finish(space, indices, ind, nnz, offsets, preds.n);
+  }
+  void query(HipStream const &space, Experimental::PointsNearestK const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    nearest_impl(space, preds, indices, offsets, nullptr, policy);
+  }
+  void query(HipStream const &space, Experimental::PointsNearestK const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             DeviceVector<float> &distances,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    nearest_impl(space, preds, indices, offsets, &distances, policy);
+  }
+
+  axb_bruteforce const *c_handle() const { return _h; }
+
+private:
+  static axb_traversal_policy c_policy(Experimental::TraversalPolicy const &p)
+  {
+    return axb_traversal_policy{p._buffer_size, p._sort_predicates ? 1 : 0};
+  }
+  void build(HipStream const &space, float const *v, int64_t n, int32_t geometry)
+  {
+    Details::check(axb_bruteforce_create(v, n, geometry, (axb_stream_t)space.stream, &_h));
+  }
+  static void prepare(DeviceVector<int64_t> &offsets, size_t nq)
+  {
+    if (offsets.size() != nq + 1)
+      offsets.resize(nq + 1);
+  }
+  void finish(HipStream const &space, DeviceVector<int> &indices, int32_t *ind, int64_t nnz,
+              DeviceVector<int64_t> &offsets, size_t nq) const
+  {
+    if (!_h)
+    {
+      std::vector<int64_t> z(nq + 1, 0);
+      offsets.assign(z);
+      indices.resize(0);
+      return;
+    }
+    indices.adopt(ind, (size_t)nnz, space.stream);
+  }
+  void nearest_impl(HipStream const &space, Experimental::PointsNearestK const &preds,
+                    DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+                    DeviceVector<float> *distances,
+                    Experimental::TraversalPolicy const &policy) const
+  {
+    prepare(offsets, preds.n);
+    if (!_h)
+    {
+      finish(space, indices, nullptr, 0, offsets, preds.n);
+      if (distances)
+        distances->resize(0);
+      return;
+    }
+    int32_t *ind = nullptr;
+    float *dist = nullptr;
+    int64_t nnz = 0;
+    Details::check(axb_bruteforce_query_nearest(
+        _h, (float const *)preds.points, (int64_t)preds.n, preds.k, c_policy(policy),
+        offsets.data(), &ind, distances ? &dist : nullptr, &nnz, (axb_stream_t)space.stream));
+    indices.adopt(ind, (size_t)nnz, space.stream);
+    if (distances)
+      distances->adopt(dist, (size_t)nnz, space.stream);
+  }
+
+  axb_bruteforce *_h = nullptr;
+};
+
 namespace Details
 {
 inline void neighbor_list(HipStream const &space, DeviceVector<Point> const &points,
@@ -557,6 +715,12 @@ void query(BoundingVolumeHierarchy const &tree, HipStream const &space,
            Predicates const &predicates, Args &&...args)
 {
   tree.query(space, predicates, std::forward<Args>(args)...);
+}
+template <class Predicates, class... Args>
+void query(BruteForce const &index, HipStream const &space, Predicates const &predicates,
+           Args &&...args)
+{
+  index.query(space, predicates, std::forward<Args>(args)...);
 }
 
 } // namespace ArborX

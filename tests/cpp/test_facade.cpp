@@ -5,7 +5,8 @@
 //   tstQueryTreeManufacturedSolution.cpp (11^3 grid),
 //   tstQueryTreeTraversalPolicy.cpp:101-130 (buffer policy, SearchException),
 //   tstNeighborList.cpp:116-176 (full / half neighbour lists),
-// plus a brute-force comparison on random points.  Rows are compared as
+// plus a brute-force comparison on random points; the same suites over the
+// brute-force index (ArborX::BruteForce).  Rows are compared as
 // multisets (test/boost_ext/CompressedStorageComparison.hpp:21-61).
 #include <ArborX.hpp>
 
@@ -50,14 +51,38 @@ static Rows rows(std::vector<std::vector<int>> const &v)
 }
 static Box pbox(float a) { return Box({{a, a, a}}, {{a, a, a}}); }
 
-static void degenerate(HipStream const &space, NodeFormat fmt)
+// Index factories: the same suites run over the BVH (both node formats)
+// and the brute-force index, as the reference instantiates its query-tree
+// suites for both (test/CMakeLists.txt:87-124).
+struct MakeBVH
+{
+  using index_type = BoundingVolumeHierarchy;
+  NodeFormat fmt;
+  template <class V>
+  index_type operator()(HipStream const &space, DeviceVector<V> const &v) const
+  {
+    return index_type(space, v, fmt);
+  }
+};
+struct MakeBruteForce
+{
+  using index_type = BruteForce;
+  template <class V>
+  index_type operator()(HipStream const &space, DeviceVector<V> const &v) const
+  {
+    return index_type(space, v);
+  }
+};
+
+template <class Make>
+static void degenerate(HipStream const &space, Make make)
 {
   // empty tree (tstQueryTreeDegenerate.cpp:24-120)
   {
-    BoundingVolumeHierarchy bvh;
+    typename Make::index_type bvh;
     CHECK(bvh.empty() && bvh.size() == 0, "default tree is empty");
     DeviceVector<Box> none;
-    BoundingVolumeHierarchy bvh2(space, none, fmt);
+    auto bvh2 = make(space, none);
     CHECK(bvh2.empty(), "tree over no values is empty");
     Box b = bvh2.bounds();
     CHECK(b.minCorner()[0] == FLT_MAX && b.maxCorner()[0] == -FLT_MAX, "empty bounds");
@@ -73,7 +98,7 @@ static void degenerate(HipStream const &space, NodeFormat fmt)
   // single leaf (:123-225)
   {
     DeviceVector<Box> v(std::vector<Box>{Box({{0, 0, 0}}, {{1, 1, 1}})});
-    BoundingVolumeHierarchy bvh(space, v, fmt);
+    auto bvh = make(space, v);
     CHECK(bvh.size() == 1, "one leaf");
     DeviceVector<Box> q(std::vector<Box>{pbox(5), pbox(.5f)});
     DeviceVector<int> ind;
@@ -91,7 +116,7 @@ static void degenerate(HipStream const &space, NodeFormat fmt)
   // two leaves (:227-331)
   {
     DeviceVector<Box> v(std::vector<Box>{pbox(0), pbox(1)});
-    BoundingVolumeHierarchy bvh(space, v, fmt);
+    auto bvh = make(space, v);
     DeviceVector<Box> q(std::vector<Box>{Box(), Box({{0, 0, 0}}, {{1, 1, 1}}),
                                          Box({{.5f, .5f, .5f}}, {{1.5f, 1.5f, 1.5f}}), pbox(0)});
     DeviceVector<int> ind;
@@ -105,7 +130,7 @@ static void degenerate(HipStream const &space, NodeFormat fmt)
   // duplicated leaves (:333-368)
   {
     DeviceVector<Box> v(std::vector<Box>{pbox(0), pbox(1), pbox(1), pbox(1)});
-    BoundingVolumeHierarchy bvh(space, v, fmt);
+    auto bvh = make(space, v);
     DeviceVector<Sphere> s(std::vector<Sphere>{
         {{{0, 0, 0}}, 1.f}, {{{1, 1, 1}}, 1.f}, {{{.5f, .5f, .5f}}, 1.f}});
     DeviceVector<int> ind;
@@ -120,7 +145,7 @@ static void degenerate(HipStream const &space, NodeFormat fmt)
       b.push_back(Box({{(float)i, (float)i, (float)i}},
                       {{(float)i + 1, (float)i + 1, (float)i + 1}}));
     DeviceVector<Box> v(b);
-    BoundingVolumeHierarchy bvh(space, v, fmt);
+    auto bvh = make(space, v);
     DeviceVector<Box> q(std::vector<Box>{Box(), Box({{0, 0, 0}}, {{4096, 4096, 4096}})});
     DeviceVector<int> ind;
     DeviceVector<int64_t> off;
@@ -132,14 +157,15 @@ static void degenerate(HipStream const &space, NodeFormat fmt)
   }
 }
 
-static void callbacks_nearest(HipStream const &space, NodeFormat fmt)
+template <class Make>
+static void callbacks_nearest(HipStream const &space, Make make)
 {
   // tstQueryTreeCallbacks.cpp:138-179: points (i,i,i), nearest(origin, 10)
   std::vector<Point> pts;
   for (int i = 0; i < 10; ++i)
     pts.push_back({{(float)i, (float)i, (float)i}});
   DeviceVector<Point> v(pts);
-  BoundingVolumeHierarchy bvh(space, v, fmt);
+  auto bvh = make(space, v);
   DeviceVector<Point> q(std::vector<Point>{{{0, 0, 0}}});
   DeviceVector<int> ind;
   DeviceVector<int64_t> off;
@@ -158,7 +184,8 @@ static void callbacks_nearest(HipStream const &space, NodeFormat fmt)
   CHECK(ok, "collinear nearest: indices and distances");
 }
 
-static void manufactured(HipStream const &space, NodeFormat fmt)
+template <class Make>
+static void manufactured(HipStream const &space, Make make)
 {
   // tstQueryTreeManufacturedSolution.cpp:29-270 (cases i and ii)
   const int nx = 11, ny = 11, nz = 11, n = nx * ny * nz;
@@ -182,7 +209,7 @@ static void manufactured(HipStream const &space, NodeFormat fmt)
                 ref[ind(i, j, k)].push_back(ind(i + a, j + b, k + c));
       }
   DeviceVector<Box> v(boxes);
-  BoundingVolumeHierarchy bvh(space, v, fmt);
+  auto bvh = make(space, v);
   DeviceVector<int> I;
   DeviceVector<int64_t> O;
   bvh.query(space, intersects(v), I, O);
@@ -195,14 +222,15 @@ static void manufactured(HipStream const &space, NodeFormat fmt)
   CHECK(rows_of(I, O) == rows(ref), "manufactured (ii): first neighbours");
 }
 
-static void policy(HipStream const &space)
+template <class Make>
+static void policy(HipStream const &space, Make make)
 {
   // tstQueryTreeTraversalPolicy.cpp:101-130
   std::vector<Point> pts;
   for (int i = 0; i < 64; ++i)
     pts.push_back({{(float)(i % 4), (float)((i / 4) % 4), (float)(i / 16)}});
   DeviceVector<Point> v(pts);
-  BoundingVolumeHierarchy bvh(space, v);
+  auto bvh = make(space, v);
   DeviceVector<int> I, I0;
   DeviceVector<int64_t> O, O0;
   bvh.query(space, Experimental::make_intersects(v, 1.5f), I0, O0);
@@ -228,7 +256,8 @@ static void policy(HipStream const &space)
   CHECK(rows_of(I, O) == rows_of(I0, O0), "predicate sorting off gives the same rows");
 }
 
-static void brute_force(HipStream const &space, NodeFormat fmt)
+template <class Make>
+static void brute_force(HipStream const &space, Make make)
 {
   std::mt19937 gen(42);
   std::uniform_real_distribution<float> U(0.f, 1.f);
@@ -240,7 +269,7 @@ static void brute_force(HipStream const &space, NodeFormat fmt)
   for (auto &p : qp)
     p = {{U(gen), U(gen), U(gen)}};
   DeviceVector<Point> v(pts), q(qp);
-  BoundingVolumeHierarchy bvh(space, v, fmt);
+  auto bvh = make(space, v);
   DeviceVector<int> I;
   DeviceVector<int64_t> O;
   bvh.query(space, Experimental::make_intersects(q, r), I, O);
@@ -317,12 +346,17 @@ int main()
   HipStream space;
   for (NodeFormat fmt : {NodeFormat::FP32, NodeFormat::Q16})
   {
-    degenerate(space, fmt);
-    callbacks_nearest(space, fmt);
-    manufactured(space, fmt);
-    brute_force(space, fmt);
+    degenerate(space, MakeBVH{fmt});
+    callbacks_nearest(space, MakeBVH{fmt});
+    manufactured(space, MakeBVH{fmt});
+    brute_force(space, MakeBVH{fmt});
   }
-  policy(space);
+  policy(space, MakeBVH{NodeFormat::FP32});
+  degenerate(space, MakeBruteForce{});
+  callbacks_nearest(space, MakeBruteForce{});
+  manufactured(space, MakeBruteForce{});
+  brute_force(space, MakeBruteForce{});
+  policy(space, MakeBruteForce{});
   neighbor_lists(space);
   std::printf("%d passed, %d failed\n", g_pass, g_fail);
   return g_fail == 0 ? 0 : 1;
