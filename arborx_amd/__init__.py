@@ -477,6 +477,12 @@ class BruteForce:
 BVH = BoundingVolumeHierarchy
 
 
+def trim_memory():
+    """Synchronises the device and returns the memory the library keeps cached
+    (its stream-ordered pool and per-stream workspaces) to the driver."""
+    _check(lib().axb_trim_memory())
+
+
 def _neighbor_list(points, radius, full, stream):
     x = _dev_f32(points, 3)
     n = x.shape[0]

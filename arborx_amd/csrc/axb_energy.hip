@@ -28,16 +28,19 @@ template <bool WANT_G>
 __global__ __launch_bounds__(256) void k_energy(const float *__restrict__ xyz, int64_t n,
                                                 const int64_t *__restrict__ offsets,
                                                 const int32_t *__restrict__ indices, float phic,
+                                                const uint32_t *__restrict__ order,
                                                 double *__restrict__ partials,
                                                 float *__restrict__ grad)
 {
-  // rows of neighbouring points share neighbours: one contiguous range of
-  // blocks per XCD (its L2)
+  // rows of neighbouring points share neighbours: rows are taken in Morton
+  // order of their points (`order`), one contiguous range of blocks per XCD
+  // (its L2), so the x_j gathered by a wavefront and by its XCD overlap
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t i = (int64_t)blk * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)blk * blockDim.x + threadIdx.x;
   double e = 0.0;
-  if (i < n)
+  if (t < n)
   {
+    const int64_t i = order ? (int64_t)order[t] : t;
     const float xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
     double gx = 0.0, gy = 0.0, gz = 0.0;
     const int64_t b = offsets[i], end = offsets[i + 1];
@@ -134,8 +137,8 @@ __global__ __launch_bounds__(256) void k_energy_fd(const float *__restrict__ xyz
 } // namespace
 
 hipError_t pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
-                       const int32_t *indices, float cutoff, double *energy, float *grad,
-                       hipStream_t s)
+                       const int32_t *indices, float cutoff, const uint32_t *order,
+                       double *energy, float *grad, hipStream_t s)
 {
   const double irc2 = 1.0 / ((double)cutoff * (double)cutoff);
   const double irc6 = irc2 * irc2 * irc2;
@@ -146,10 +149,10 @@ hipError_t pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
     return hipErrorOutOfMemory;
   if (grad)
     AXB_LAUNCH(k_energy<true>, dim3((unsigned)nb), dim3(256), 0, s, xyz, n, offsets,
-                       indices, phic, partials, grad);
+               indices, phic, order, partials, grad);
   else
     AXB_LAUNCH(k_energy<false>, dim3((unsigned)nb), dim3(256), 0, s, xyz, n, offsets,
-                       indices, phic, partials, grad);
+               indices, phic, order, partials, grad);
   if (energy)
     AXB_LAUNCH(k_reduce_f64, dim3(1), dim3(256), 0, s, partials, nb, energy);
   dfree(partials, s);

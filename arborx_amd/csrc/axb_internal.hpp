@@ -278,9 +278,11 @@ hipError_t launch_bf_spatial(const float *values, int64_t n, int geometry, int p
                              uint32_t *err, hipStream_t s);
 hipError_t launch_bf_nearest(const float *values, int64_t n, int geometry, const float *pts,
                              int64_t nq, int k, int32_t *oi, float *od, hipStream_t s);
+// Rows are processed in `order` (nullptr: 0 .. n-1); E and grad do not
+// depend on it except for the summation order of E.
 hipError_t pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
-                       const int32_t *indices, float cutoff, double *energy, float *grad,
-                       hipStream_t s);
+                       const int32_t *indices, float cutoff, const uint32_t *order,
+                       double *energy, float *grad, hipStream_t s);
 hipError_t pair_energy_fd(const float *xyz, const int64_t *offsets, const int32_t *indices,
                           float cutoff, const int32_t *atoms, int64_t m, double h, double *fd,
                           hipStream_t s);

@@ -111,11 +111,16 @@ template <int K>
 struct DistList
 {
   uint32_t a[K];
-  AXB_D void init()
+  // The first K - k entries hold 0, below every value inserted (all >= 0):
+  // they stay in front, and the k-th smallest value inserted is always
+  // a[K - 1], read at a fixed index.  (Picking a[k - 1] for a run-time k
+  // turns into an indexed read, which keeps the list in scratch memory for
+  // K > 16.)
+  AXB_D void init(int k)
   {
 #pragma unroll
     for (int j = 0; j < K; ++j)
-      a[j] = ~0u;
+      a[j] = j < K - k ? 0u : ~0u;
   }
   AXB_D void insert(float d)
   {
@@ -128,15 +133,11 @@ struct DistList
     }
     a[0] = x < a[0] ? x : a[0];
   }
-  // RN(sqrt) of the k-th smallest (1 <= k <= K); +inf while fewer than k
-  // were inserted
-  AXB_D float kth(int k) const
+  // RN(sqrt) of the k-th smallest (init's k, 1 <= k <= K); +inf while fewer
+  // than k were inserted
+  AXB_D float kth() const
   {
-    uint32_t v = a[0];
-#pragma unroll
-    for (int j = 1; j < K; ++j)
-      if (j == k - 1)
-        v = a[j];
+    const uint32_t v = a[K - 1];
     return v == ~0u ? INFINITY : sqrt_rn(bits_f(v));
   }
 };
@@ -507,13 +508,13 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
   Subtree X;
   const uint32_t p = greedy_descent<F, G>(tv, qx, qy, qz, UINT32_MAX, X, v_int, v_leaf);
   DistList<K> L;
-  L.init();
+  L.init(k);
   uint32_t ws, we;
   slot_window<G>(
       tv, p, W, ws, we,
       [&](uint4 s0, uint4 s1) __attribute__((always_inline)) { L.insert(leaf_d2<G>(qx, qy, qz, s0, s1)); }, v_int,
       v_leaf);
-  return L.kth(k);
+  return L.kth();
 }
 
 // Fixed-radius rope walk appending every leaf with d2 <= thr to `row` as
