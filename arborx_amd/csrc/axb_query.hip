@@ -1016,18 +1016,24 @@ static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq,
   const float shrink = std::cbrt((float)(2 * std::min(k, n)) / (float)(cap + 1));
   const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t(1) << 31) / (cap * 8)));
   uint2 *cand = (uint2 *)ws_get(s, WS_KNN_CAND, (size_t)batch * cap * sizeof(uint2));
-  uint32_t *ovf = (uint32_t *)dalloc((size_t)batch * sizeof(uint32_t), s);
+  // unresolved queries of every batch, for one fallback launch at the end:
+  // each of them is one lane walking alone, so launches of a few lanes per
+  // batch would each cost the longest such walk
+  uint32_t *ovf = (uint32_t *)dalloc((size_t)nq * sizeof(uint32_t), s);
   uint32_t *ovf_n = (uint32_t *)dalloc(sizeof(uint32_t), s);
   hipError_t e = (cand && ovf && ovf_n) ? hipSuccess : hipErrorOutOfMemory;
+  if (e == hipSuccess)
+    e = fill_u32(ovf_n, 0u, 1, s);
   for (int64_t t0 = 0; e == hipSuccess && t0 < nq; t0 += batch)
   {
     const int64_t tn = std::min(batch, nq - t0);
-    e = fill_u32(ovf_n, 0u, 1, s);
-    if (e != hipSuccess)
-      break;
     AXB_LAUNCH((k_knn_collect<F, G, K>), dim3(grid_of(tn)), dim3(256), 0, s, tv, pts, t0, tn, k,
                W, order, cand, cap, shrink, ovf, ovf_n, oi, od);
-    AXB_LAUNCH((k_knn_fallback<F, G, K>), dim3((unsigned)std::min<int64_t>(grid_of(tn), 512)),
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+  {
+    AXB_LAUNCH((k_knn_fallback<F, G, K>), dim3((unsigned)std::min<int64_t>(grid_of(nq), 512)),
                dim3(256), 0, s, tv, pts, k, order, ovf, ovf_n, oi, od);
     e = hipGetLastError();
   }
