@@ -454,16 +454,9 @@ AXB_HD bool point_step(const TV &tv, const SpatialPred &pr, uint32_t &p, uint32_
   return true;
 }
 
-// (Host-callable as well, so the CPU tests run this very walk over the
-// oracle's node arrays.)
-// Stackless pre-order walk over slots [p, end) (TreeTraversal.hpp:97-119:
-// leaf -> test, next record; internal -> test ? first child : rope).  Each
-// hit calls sink.hit(value), which returns false to stop the walk (the
-// reference's CallbackTreeTraversalControl::early_exit,
-// Callbacks.hpp:96-118).  The leaf record at `skip` is not reported.  A Q16
-// walk starts at the root or at a record whose frame is the scene box.
-// TV: a tree view with `const uint4 *slots`, `Box6 scene`, `float ss[3]` and
-// `uint32_t *err`.  PK: PRED_BOXES tests the box, otherwise the sphere.
+// Exact test of a leaf record (s0, and s1 for a box leaf) against a sphere
+// or box predicate: Intersects.hpp:51-114 with the sphere decided by the
+// squared-distance threshold.
 template <int G, int PK>
 AXB_HD bool leaf_hit(const SpatialPred &pr, uint4 s0, uint4 s1)
 {
