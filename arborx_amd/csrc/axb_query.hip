@@ -15,8 +15,9 @@
 //      pre-order) bounds the true k-th distance from above (they are real
 //      points);
 //   2. a fixed-radius rope walk appending (d2, index) of every leaf within R
-//      to a row of `cap` candidates, stopping at cap + 1; the radius adapts
-//      over a few rounds until the row holds between k and cap leaves;
+//      to a row of `cap` candidates; a full row shrinks R in place and the
+//      walk goes on; a few rounds adapt R until the row holds between k and
+//      cap leaves;
 //   3. exact top-k of the row, (d2, index) keys sorted and merged by
 //      bitonic networks in registers.
 // Queries left unresolved go to k_knn_fallback (exact adaptive walk with an
@@ -522,15 +523,22 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
 // (d2, index) key, d2 bits high (d2 >= 0, so bit order is value order).  Stops as soon as more than `cap` leaves qualify;
 // returns the count (cap + 1 on overflow).
 template <int F, int G>
-AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, float thr,
-                           uint2 *__restrict__ cand, uint32_t row_at, int cap,
-                           uint32_t &v_int, uint32_t &v_leaf)
+AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, float &R,
+                           float &R_hi, float R_lo, float shrink, uint2 *__restrict__ cand,
+                           uint32_t row_at, int cap, uint32_t &v_int, uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  float thr = sqrt_threshold(R);
   if constexpr (G == GEOM_POINTS)
   {
     // one record per step, leaf and internal node through the same code
-    // (the closest point of a leaf is the point), as in rope_walk_points
+    // (the closest point of a leaf is the point), as in rope_walk_points.
+    // A full row does not end the walk: the radius shrinks (by `shrink`,
+    // up to 3 times, staying above R_lo, a radius known to hold fewer than
+    // k leaves) and the row keeps the entries within the new radius; every
+    // leaf passed over so far lay outside the old radius, so the row still
+    // holds every leaf seen within the new one, and the walk goes on (R_hi:
+    // the last radius known to hold at least the row's capacity).
     int c = 0;
     uint32_t p = 0;
     const uint32_t end = tv.total;
@@ -554,10 +562,27 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
       v_int += !leaf;
       if (leaf && pass)
       {
-        if (c < cap)
-          cand[row_at + c] = make_uint2((uint32_t)~w, f_bits(d2));
-        if (++c > cap)
-          break;
+        if (c == cap)
+        {
+          for (int t = 0; t < 3 && c == cap && R * shrink > R_lo; ++t)
+          {
+            R_hi = R;
+            R = R * shrink;
+            thr = sqrt_threshold(R);
+            int kept = 0;
+            for (int j = 0; j < c; ++j)
+            {
+              const uint2 e = cand[row_at + j];
+              if (bits_f(e.y) <= thr)
+                cand[row_at + kept++] = e;
+            }
+            c = kept;
+          }
+          if (c == cap)
+            return cap + 1; // still full: left to the next round
+        }
+        if (d2 <= thr)
+          cand[row_at + c++] = make_uint2((uint32_t)~w, f_bits(d2));
       }
       if (!leaf && !pass && w == 0)
       {
@@ -645,11 +670,11 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
 // One lane per query: seed radius, then up to KNN_ROUNDS collect walks until
 // one finds between k and cap leaves within the radius (then every point
 // within the true k-th distance is in the row), followed by the exact top-k
-// selection of that row while it is still in L2.  On overflow (the walk
-// stops at cap + 1) the radius is an upper bound and the next one shrinks
-// by the volume ratio (2k / (cap + 1))^(1/3); with fewer than k leaves the
-// radius was below the k-th distance and the next one bisects towards the
-// upper bound.  Queries still unresolved go to the exact fallback.  The row
+// selection of that row while it is still in L2.  A walk whose row fills up
+// shrinks its radius by the volume ratio (2k / (cap + 1))^(1/3) and goes on
+// (points; boxes stop at cap + 1 and the next round shrinks); with fewer
+// than k leaves the radius was below the k-th distance and the next round
+// bisects towards the last radius known to hold enough.  Queries still unresolved go to the exact fallback.  The row
 // is addressed as cand + row_at (a scalar base and a 32-bit offset): the
 // walk then fits the 64 registers of full occupancy without spilling.
 constexpr int KNN_ROUNDS = 6;
@@ -679,7 +704,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 &&
   for (int round = 0; round < KNN_ROUNDS; ++round)
   {
     ++rounds;
-    c = knn_collect_walk<F, G>(tv, qx, qy, qz, sqrt_threshold(R), cand, row_at, cap, v_int,
+    c = knn_collect_walk<F, G>(tv, qx, qy, qz, R, R_hi, R_lo, shrink, cand, row_at, cap, v_int,
                                v_leaf);
     if (c >= m && c <= cap)
     {
