@@ -1039,7 +1039,10 @@ static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq,
   // 4k 13.2, 6k 13.1, 8k 13.15 -- a tighter seed radius against a longer
   // seed scan)
   const int W = std::min(n, std::max(6 * k, 16));
-  const int cap = 8 * k > 64 ? 8 * k : 64;
+  // row capacity: about 6k, a multiple of 8 (10M points, k = 16: 4k 13.1 ms
+  // uniform / 20.0 ms Gaussian mixture, 6k 12.85 / 20.9, 8k 13.1 / 23.2 --
+  // a full row costs an in-place shrink, a long row a longer selection)
+  const int cap = 6 * k > 64 ? (6 * k + 7) / 8 * 8 : 64;
   // radius factor after an overflowing round (cap + 1 leaves within R):
   // aim at 2m leaves, m = min(k, n)
   const float shrink = std::cbrt((float)(2 * std::min(k, n)) / (float)(cap + 1));
