@@ -509,6 +509,51 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
 {
   Subtree X;
   const uint32_t p = greedy_descent<F, G>(tv, qx, qy, qz, UINT32_MAX, X, v_int, v_leaf);
+  if constexpr (G == GEOM_POINTS)
+  {
+    // The slots [a, a + S) around p, S = 1.5 W rounded up to 8 (about W
+    // leaves), NB at a time: keys = d2 bits of the leaf records, ~0 for the
+    // others; each batch is sorted and merged into the K smallest by
+    // bitonic networks on 32-bit keys (min / max pairs), all loads of a
+    // batch in flight together.  The first K - k entries hold 0 so that the
+    // k-th smallest is A[K - 1].
+    constexpr int NB = K < 8 ? K : 8;
+    const uint32_t total = tv.total;
+    const uint32_t S = (uint32_t)((3 * W / 2 + 7) & ~7);
+    uint32_t a = p > S / 2 ? p - S / 2 : 0u;
+    if (a + S > total)
+      a = total > S ? total - S : 0u;
+    const uint32_t e = a + S < total ? a + S : total;
+    uint32_t A[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      A[j] = j < K - k ? 0u : ~0u;
+    uint32_t leaves = 0;
+    for (uint32_t b = a; b < e; b += NB)
+    {
+      uint32_t B[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+      {
+        const uint32_t slot = b + (uint32_t)j;
+        const uint4 r = tv.slots[slot < e ? slot : e - 1];
+        const bool leaf = slot < e && (int32_t)r.w < 0;
+        B[j] = leaf ? f_bits(dist2_pp(qx, qy, qz, bits_f(r.x), bits_f(r.y), bits_f(r.z)))
+                    : ~0u;
+        leaves += leaf;
+      }
+      bitonic_sort<NB>(B);
+#pragma unroll
+      for (int j = K - NB; j < K; ++j)
+        A[j] = A[j] < B[K - 1 - j] ? A[j] : B[K - 1 - j];
+      bitonic_merge<K>(A);
+    }
+    v_leaf += leaves;
+    v_int += (e - a) - leaves;
+    const int m = k < tv.n ? k : tv.n;
+    if ((int)leaves >= m) // else the leaf-counting window below
+      return A[K - 1] == ~0u ? INFINITY : sqrt_rn(bits_f(A[K - 1]));
+  }
   DistList<K> L;
   L.init(k);
   uint32_t ws, we;

@@ -738,19 +738,21 @@ struct RegList
   }
 };
 
-// Compare-exchange of two keys into ascending order.
-AXB_HD void cas_u64(uint64_t &x, uint64_t &y)
+// Compare-exchange of two keys into ascending order (64-bit keys: compare
+// and selects; 32-bit keys: v_min / v_max).
+template <class T>
+AXB_HD void cas_keys(T &x, T &y)
 {
   const bool lt = x < y;
-  const uint64_t lo = lt ? x : y;
+  const T lo = lt ? x : y;
   y = lt ? y : x;
   x = lo;
 }
 
 // Bitonic sorting network, ascending (N a power of two; static indices only,
 // so the array stays in registers).
-template <int N>
-AXB_HD void bitonic_sort(uint64_t (&a)[N])
+template <int N, class T>
+AXB_HD void bitonic_sort(T (&a)[N])
 {
 #pragma unroll
   for (int size = 2; size <= N; size <<= 1)
@@ -763,16 +765,16 @@ AXB_HD void bitonic_sort(uint64_t (&a)[N])
         if (j > i)
         {
           if ((i & size) == 0)
-            cas_u64(a[i], a[j]);
+            cas_keys(a[i], a[j]);
           else
-            cas_u64(a[j], a[i]);
+            cas_keys(a[j], a[i]);
         }
       }
 }
 
 // Half-cleaner cascade: sorts a bitonic sequence ascending.
-template <int N>
-AXB_HD void bitonic_merge(uint64_t (&a)[N])
+template <int N, class T>
+AXB_HD void bitonic_merge(T (&a)[N])
 {
 #pragma unroll
   for (int stride = N >> 1; stride > 0; stride >>= 1)
@@ -781,7 +783,7 @@ AXB_HD void bitonic_merge(uint64_t (&a)[N])
     {
       const int j = i ^ stride;
       if (j > i)
-        cas_u64(a[i], a[j]);
+        cas_keys(a[i], a[j]);
     }
 }
 
