@@ -529,17 +529,19 @@ static axb_status predicate_order(const axb_bvh *t, int pred_kind, const float *
 // 140M queries), at least 4 GiB.
 static int64_t scratch_rows_budget()
 {
-  static int64_t budget[64] = {0};
+  static std::atomic<int64_t> budget[64] = {};
   int dev = 0;
   (void)hipGetDevice(&dev);
-  int64_t &b = budget[dev & 63];
-  if (b == 0)
+  std::atomic<int64_t> &b = budget[dev & 63];
+  int64_t v = b.load(std::memory_order_relaxed);
+  if (v == 0)
   {
     size_t total = 0;
-    b = hipDeviceTotalMem(&total, dev) == hipSuccess ? (int64_t)(total / 8) : 0;
-    b = std::max<int64_t>(b, int64_t(4) << 30);
+    v = hipDeviceTotalMem(&total, dev) == hipSuccess ? (int64_t)(total / 8) : 0;
+    v = std::max<int64_t>(v, int64_t(4) << 30);
+    b.store(v, std::memory_order_relaxed);
   }
-  return b;
+  return v;
 }
 
 // One CRS query of nq rows (CrsGraphWrapperImpl.hpp:145-374 semantics).
