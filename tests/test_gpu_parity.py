@@ -302,6 +302,48 @@ def test_buffer_policy():
     np.testing.assert_array_equal(host(i), i_ref)
 
 
+def test_concurrent_queries_on_two_streams():
+    """A radius query and a kNN query of one tree from two host threads on two
+    streams at once (each stream has its own workspaces)."""
+    import threading
+
+    v, g, t = oracle_tree("uniform", 100_000)
+    r = np.float32((48.0 / (4.0 * math.pi * 100_000)) ** (1.0 / 3.0))
+    q = v[::3]
+    o_ref, i_ref = t.query_spatial(0, spheres_from_points(v, r))
+    ko_ref, ki_ref, kd_ref = t.query_knn(q, 16)
+    bvh = ax.BoundingVolumeHierarchy(dev(v), node_format="q16")
+    pts, qd = dev(v), dev(q)
+    torch.cuda.synchronize()
+    out = {}
+
+    def radius(s):
+        with torch.cuda.stream(s):
+            for it in range(3):
+                out[("r", it)] = bvh.query(ax.make_intersects(pts, float(r)), stream=s)
+
+    def knn(s):
+        with torch.cuda.stream(s):
+            for it in range(3):
+                out[("k", it)] = bvh.query(ax.make_nearest(qd, 16), return_distances=True,
+                                           stream=s)
+
+    th = [threading.Thread(target=radius, args=(torch.cuda.Stream(),)),
+          threading.Thread(target=knn, args=(torch.cuda.Stream(),))]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    torch.cuda.synchronize()
+    for it in range(3):
+        idx, off = out[("r", it)]
+        np.testing.assert_array_equal(host(off), o_ref)
+        np.testing.assert_array_equal(host(idx), i_ref)
+        ki, ko, kd = out[("k", it)]
+        np.testing.assert_array_equal(host(ki), ki_ref)
+        np.testing.assert_array_equal(host(kd), kd_ref)
+
+
 # ------------------------------------------------------------ energy
 def _fcc_csr(cells, rc=2.5):
     x = synthetic.fcc_lattice(cells, seed=6)
