@@ -1217,6 +1217,16 @@ axb_status axb_bruteforce_bounds(const axb_bruteforce *bf, float out_host[6])
   return AXB_OK;
 }
 
+axb_status axb_bruteforce_view(const axb_bruteforce *bf, axb_values_view *out)
+{
+  if (!bf || !out)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_view: NULL argument");
+  out->values = bf->values;
+  out->n = bf->n;
+  out->geometry = bf->geometry;
+  return AXB_OK;
+}
+
 static axb_status bf_query_spatial(const axb_bruteforce *bf, int pred_kind, const float *preds,
                                    int64_t nq, float radius, axb_traversal_policy policy,
                                    int64_t *offsets, int32_t **indices, int64_t *nnz_host,

@@ -283,17 +283,17 @@ struct is_device_vector<DeviceVector<T>> : std::true_type
 };
 
 #if defined(__HIPCC__)
-class BoundingVolumeHierarchy;
 namespace Details
 {
 struct DeviceTree;
 DeviceTree device_tree(axb_bvh const *h);
-template <class Batch, class Callback>
-void callback_query(BoundingVolumeHierarchy const &bvh, HipStream const &space, Batch const &b,
+// Index: BoundingVolumeHierarchy or BruteForce
+template <class Index, class Batch, class Callback>
+void callback_query(Index const &index, HipStream const &space, Batch const &b,
                     Callback const &cb, Experimental::TraversalPolicy const &policy);
-template <class Batch, class Callback, class T>
-void callback_query_crs(BoundingVolumeHierarchy const &bvh, HipStream const &space,
-                        Batch const &b, Callback const &cb, DeviceVector<T> &values,
+template <class Index, class Batch, class Callback, class T>
+void callback_query_crs(Index const &index, HipStream const &space, Batch const &b,
+                        Callback const &cb, DeviceVector<T> &values,
                         DeviceVector<int64_t> &offsets,
                         Experimental::TraversalPolicy const &policy);
 } // namespace Details
@@ -602,6 +602,28 @@ public:
   }
 
   axb_bruteforce const *c_handle() const { return _h; }
+
+#if defined(__HIPCC__)
+  // query(space, predicates, callback[, policy]) (ArborX_BruteForce.hpp:
+  // 64-67,151-178): callback(predicate, value) per match, values in index
+  // order; nearest callbacks in (distance, index) order.  Synchronises.
+  template <class Predicates, class Callback,
+            std::enable_if_t<!is_device_vector<Callback>::value, int> = 0>
+  void query(HipStream const &space, Predicates const &predicates, Callback const &callback,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    Details::callback_query(*this, space, predicates, callback, policy);
+  }
+  // query(space, predicates, callback, values, offsets[, policy])
+  // (ArborX_BruteForce.hpp:69-91): callback(predicate, value, out).
+  template <class Predicates, class Callback, class T>
+  void query(HipStream const &space, Predicates const &predicates, Callback const &callback,
+             DeviceVector<T> &values, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    Details::callback_query_crs(*this, space, predicates, callback, values, offsets, policy);
+  }
+#endif
 
 private:
   static axb_traversal_policy c_policy(Experimental::TraversalPolicy const &p)

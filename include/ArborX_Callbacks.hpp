@@ -17,10 +17,14 @@
 // from the library's kNN (TreeTraversal.hpp:314-334 calls them the same way,
 // after sorting its heap).
 //
+// The same two host forms exist for the brute-force index
+// (src/spatial/ArborX_BruteForce.hpp:64-91): spatial callbacks see every
+// matching value in index order, nearest callbacks as above.
+//
 // The predicate object handed to a callback carries its geometry and its
 // position in the batch: getGeometry(p) and getData(p) (= the position, the
 // reference's attach_indices form, detail/ArborX_AttachIndices.hpp:30-35).
-// The value is the tree's value: the index of the point or box.
+// The value is the index's value: the index of the point or box.
 #pragma once
 
 #include "ArborX.hpp"
@@ -191,6 +195,60 @@ __device__ void spatial_walk(DeviceTree const &t, Intersects<Geometry> const &pr
   }
 }
 
+// Device view of a brute-force index (axb_bruteforce_view).
+struct BruteForceView
+{
+  const float *values = nullptr;
+  int64_t n = 0;
+  int32_t geometry = 0;
+};
+
+inline BruteForceView bruteforce_view(axb_bruteforce const *h)
+{
+  BruteForceView v;
+  if (!h)
+    return v;
+  axb_values_view c;
+  check(axb_bruteforce_view(h, &c));
+  v.values = c.values;
+  v.n = c.n;
+  v.geometry = c.geometry;
+  return v;
+}
+
+// One predicate against every value of a brute-force index, in index
+// order (BruteForceImpl.hpp:53-153), with the same exact leaf test as the
+// tree's walk; a callback returning early_exit stops the predicate.
+template <class Geometry, class Callback>
+__device__ void spatial_walk(BruteForceView const &t, Intersects<Geometry> const &pred,
+                             Callback const &cb)
+{
+  const axb::SpatialPred sp = spatial_of(pred.geometry);
+  CallbackSink<Intersects<Geometry>, Callback> sink{pred, cb};
+  constexpr int PK = pred_kind<Geometry>();
+  if (t.geometry == AXB_GEOMETRY_POINTS)
+  {
+    for (int64_t j = 0; j < t.n; ++j)
+    {
+      const float *v = t.values + 3 * j;
+      const uint4 s0 = make_uint4(axb::f_bits(v[0]), axb::f_bits(v[1]), axb::f_bits(v[2]), 0u);
+      if (axb::leaf_hit<axb::GEOM_POINTS, PK>(sp, s0, s0) && !sink.hit((int32_t)j))
+        return;
+    }
+  }
+  else
+  {
+    for (int64_t j = 0; j < t.n; ++j)
+    {
+      const float *v = t.values + 6 * j;
+      const uint4 s0 = make_uint4(axb::f_bits(v[0]), axb::f_bits(v[1]), axb::f_bits(v[2]), 0u);
+      const uint4 s1 = make_uint4(axb::f_bits(v[3]), axb::f_bits(v[4]), axb::f_bits(v[5]), 0u);
+      if (axb::leaf_hit<axb::GEOM_BOXES, PK>(sp, s0, s1) && !sink.hit((int32_t)j))
+        return;
+    }
+  }
+}
+
 // Predicate i of a batch as the object a callback sees.
 __device__ __forceinline__ Intersects<Sphere> predicate_at(Experimental::PointsWithRadius b,
                                                            int i)
@@ -238,8 +296,8 @@ struct is_spatial_batch
 inline unsigned blocks_of(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 // ---- spatial: callback per match
-template <class Batch, class Callback>
-__global__ __launch_bounds__(256) void k_spatial_callback(DeviceTree t, Batch batch, int64_t nq,
+template <class View, class Batch, class Callback>
+__global__ __launch_bounds__(256) void k_spatial_callback(View t, Batch batch, int64_t nq,
                                                           const uint32_t *order, Callback cb)
 {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -273,8 +331,8 @@ struct WriteOut
 };
 
 // ---- spatial: CRS with an inserting callback cb(predicate, value, out)
-template <class Batch, class Callback, class T, bool FILL>
-__global__ __launch_bounds__(256) void k_spatial_insert(DeviceTree t, Batch batch, int64_t nq,
+template <class View, class Batch, class Callback, class T, bool FILL>
+__global__ __launch_bounds__(256) void k_spatial_insert(View t, Batch batch, int64_t nq,
                                                         const uint32_t *order, Callback cb,
                                                         int64_t *offsets, T *out)
 {
@@ -354,18 +412,37 @@ DeviceVector<uint32_t> processing_order(axb_bvh const *h, HipStream const &space
   return order;
 }
 
-template <class Batch, class Callback>
-void callback_query(BoundingVolumeHierarchy const &bvh, HipStream const &space,
-                    Batch const &b, Callback const &cb,
-                    Experimental::TraversalPolicy const &policy)
+// The device view and the predicate processing order of an index: the
+// tree's walk in Morton order of the predicates (unless the policy turns
+// sorting off); the brute-force index in the caller's order.
+inline DeviceTree index_view(BoundingVolumeHierarchy const &i)
 {
-  axb_bvh const *h = bvh.c_handle();
-  if (!h || b.n == 0)
+  return device_tree(i.c_handle());
+}
+inline BruteForceView index_view(BruteForce const &i) { return bruteforce_view(i.c_handle()); }
+template <class Batch>
+DeviceVector<uint32_t> index_order(BoundingVolumeHierarchy const &i, HipStream const &space,
+                                   Batch const &b, Experimental::TraversalPolicy const &policy)
+{
+  return processing_order(i.c_handle(), space, b, policy);
+}
+template <class Batch>
+DeviceVector<uint32_t> index_order(BruteForce const &, HipStream const &, Batch const &,
+                                   Experimental::TraversalPolicy const &)
+{
+  return {};
+}
+
+template <class Index, class Batch, class Callback>
+void callback_query(Index const &index, HipStream const &space, Batch const &b,
+                    Callback const &cb, Experimental::TraversalPolicy const &policy)
+{
+  if (!index.c_handle() || b.n == 0)
     return;
   if constexpr (is_spatial_batch<Batch>::value)
   {
-    DeviceTree t = device_tree(h);
-    DeviceVector<uint32_t> order = processing_order(h, space, b, policy);
+    auto t = index_view(index);
+    DeviceVector<uint32_t> order = index_order(index, space, b, policy);
     k_spatial_callback<<<blocks_of(b.n), 256, 0, space.stream>>>(t, b, (int64_t)b.n,
                                                                   order.data(), cb);
     launch_check("query(callback)");
@@ -377,7 +454,7 @@ void callback_query(BoundingVolumeHierarchy const &bvh, HipStream const &space,
                   "unsupported predicates");
     DeviceVector<int> ind;
     DeviceVector<int64_t> off;
-    bvh.query(space, b, ind, off, policy);
+    index.query(space, b, ind, off, policy);
     k_nearest_callback<<<blocks_of(b.n), 256, 0, space.stream>>>(b, (int64_t)b.n, off.data(),
                                                                   ind.data(), cb);
     launch_check("query(nearest callback)");
@@ -394,13 +471,13 @@ __global__ void k_zero(I *p, int64_t n)
     p[i] = 0;
 }
 
-template <class Batch, class Callback, class T>
-void callback_query_crs(BoundingVolumeHierarchy const &bvh, HipStream const &space,
-                        Batch const &b, Callback const &cb, DeviceVector<T> &values,
+template <class Index, class Batch, class Callback, class T>
+void callback_query_crs(Index const &index, HipStream const &space, Batch const &b,
+                        Callback const &cb, DeviceVector<T> &values,
                         DeviceVector<int64_t> &offsets,
                         Experimental::TraversalPolicy const &policy)
 {
-  axb_bvh const *h = bvh.c_handle();
+  auto const *h = index.c_handle();
   offsets.resize(b.n + 1);
   k_zero<int64_t><<<blocks_of((int64_t)b.n + 1), 256, 0, space.stream>>>(offsets.data(),
                                                                     (int64_t)b.n + 1);
@@ -414,10 +491,11 @@ void callback_query_crs(BoundingVolumeHierarchy const &bvh, HipStream const &spa
   const int64_t nq = (int64_t)b.n;
   if constexpr (is_spatial_batch<Batch>::value)
   {
-    DeviceTree t = device_tree(h);
-    DeviceVector<uint32_t> order = processing_order(h, space, b, policy);
-    k_spatial_insert<Batch, Callback, T, false><<<blocks_of(nq), 256, 0, space.stream>>>(
-        t, b, nq, order.data(), cb, offsets.data(), (T *)nullptr);
+    auto t = index_view(index);
+    DeviceVector<uint32_t> order = index_order(index, space, b, policy);
+    k_spatial_insert<decltype(t), Batch, Callback, T, false>
+        <<<blocks_of(nq), 256, 0, space.stream>>>(t, b, nq, order.data(), cb, offsets.data(),
+                                                   (T *)nullptr);
     launch_check("query(callback, values, offsets): count");
     check(axb_exclusive_scan_i64(offsets.data(), offsets.data(), nq + 1,
                                  (axb_stream_t)space.stream));
@@ -429,8 +507,9 @@ void callback_query_crs(BoundingVolumeHierarchy const &bvh, HipStream const &spa
     values.resize((size_t)nnz);
     if (nnz > 0)
     {
-      k_spatial_insert<Batch, Callback, T, true><<<blocks_of(nq), 256, 0, space.stream>>>(
-          t, b, nq, order.data(), cb, offsets.data(), values.data());
+      k_spatial_insert<decltype(t), Batch, Callback, T, true>
+          <<<blocks_of(nq), 256, 0, space.stream>>>(t, b, nq, order.data(), cb, offsets.data(),
+                                                     values.data());
       launch_check("query(callback, values, offsets): fill");
     }
     space.fence();
@@ -441,7 +520,7 @@ void callback_query_crs(BoundingVolumeHierarchy const &bvh, HipStream const &spa
                   "unsupported predicates");
     DeviceVector<int> ind;
     DeviceVector<int64_t> off;
-    bvh.query(space, b, ind, off, policy);
+    index.query(space, b, ind, off, policy);
     k_nearest_insert<Callback, T, false><<<blocks_of(nq), 256, 0, space.stream>>>(
         b, nq, off.data(), ind.data(), cb, offsets.data(), (T *)nullptr);
     launch_check("query(nearest callback, values, offsets): count");

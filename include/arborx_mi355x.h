@@ -240,6 +240,18 @@ axb_status axb_bruteforce_query_nearest(const axb_bruteforce *bf,
                                         float **distances, int64_t *nnz_host,
                                         axb_stream_t stream);
 
+/* Read-only view of a brute-force index for device code (the callback
+ * forms of query over it, include/ArborX_Callbacks.hpp): the index's device
+ * copy of its values (n x 3 or n x 6 floats) and their geometry.  Valid
+ * while the index lives. */
+typedef struct axb_values_view
+{
+  const float *values;
+  int64_t n;
+  int32_t geometry;
+} axb_values_view;
+axb_status axb_bruteforce_view(const axb_bruteforce *bf, axb_values_view *out_host);
+
 /* Pair energy over a fixed neighbour CRS (no reference counterpart; pattern of
  * examples/molecular_dynamics/example_molecular_dynamics.cpp:128-194).
  * Shifted Lennard-Jones phi(r) = 4(r^-12 - r^-6) - phi(cutoff);

@@ -6,7 +6,10 @@
 //   callback_early_exit (:181-229): counts(i)++ < i, else early_exit;
 //   callback_with_attachment_* (:264-380): data added to each insert;
 // plus void callbacks and query(PerThread, ...) in a user kernel against the
-// CRS query on random points, for both node formats.
+// CRS query on random points, for both node formats; the reference cases,
+// early exit and the CRS-through-callbacks check also over the brute-force
+// index (the reference runs its callback suite over BruteForce too,
+// test/CMakeLists.txt:87-124).
 #include <ArborX.hpp>
 
 #include <algorithm>
@@ -73,14 +76,34 @@ rows_of(DeviceVector<PairIndexDistance> const &v, DeviceVector<int64_t> const &o
   return r;
 }
 
-static void reference_cases(HipStream const &space, NodeFormat fmt)
+// Index factories: the tree in either node format, or the brute-force index.
+struct MakeBVH
+{
+  NodeFormat fmt;
+  template <class V>
+  BoundingVolumeHierarchy operator()(HipStream const &space, DeviceVector<V> const &v) const
+  {
+    return BoundingVolumeHierarchy(space, v, fmt);
+  }
+};
+struct MakeBruteForce
+{
+  template <class V>
+  BruteForce operator()(HipStream const &space, DeviceVector<V> const &v) const
+  {
+    return BruteForce(space, v);
+  }
+};
+
+template <class Make>
+static void reference_cases(HipStream const &space, Make make)
 {
   int const n = 10;
   std::vector<Point> hp(n);
   for (int i = 0; i < n; ++i)
     hp[i] = {{(float)i, (float)i, (float)i}};
   DeviceVector<Point> points(hp);
-  BoundingVolumeHierarchy tree(space, points, fmt);
+  auto tree = make(space, points);
 
   for (float delta : {0.f, 5.f})
   {
@@ -127,7 +150,8 @@ struct EarlyExit
   }
 };
 
-static void early_exit(HipStream const &space, NodeFormat fmt)
+template <class Make>
+static void early_exit(HipStream const &space, Make make)
 {
   std::vector<Box> hb;
   for (int i = 0; i < 4; ++i)
@@ -136,7 +160,7 @@ static void early_exit(HipStream const &space, NodeFormat fmt)
     hb.push_back(Box({{f, f, f}}, {{f, f, f}}));
   }
   DeviceVector<Box> boxes(hb);
-  BoundingVolumeHierarchy tree(space, boxes, fmt);
+  auto tree = make(space, boxes);
   Box b = tree.bounds();
   DeviceVector<Box> q(std::vector<Box>{b, b, b, b});
   DeviceVector<int> counts(std::vector<int>{0, 0, 0, 0});
@@ -232,6 +256,47 @@ static void random_points(HipStream const &space, NodeFormat fmt)
   CHECK(off2.to_host() == ho && vals.to_host() == hi, "default-callback CRS equals query");
 }
 
+// The brute-force index's callbacks against its own CRS query: counts per
+// predicate, and the CRS built through an inserting callback equals it,
+// row order included (values in index order on both sides).
+static void bruteforce_random(HipStream const &space)
+{
+  int const n = 5000, nq = 2000;
+  std::mt19937 gen(9);
+  std::uniform_real_distribution<float> U(0.f, 1.f);
+  std::vector<Box> hb(n);
+  for (auto &b : hb)
+  {
+    float x = U(gen), y = U(gen), z = U(gen), e = 0.02f * U(gen);
+    b = Box({{x, y, z}}, {{x + e, y + e, z + e}});
+  }
+  std::vector<Sphere> hs(nq);
+  for (auto &s : hs)
+    s = Sphere{{{U(gen), U(gen), U(gen)}}, 0.05f * U(gen)};
+  DeviceVector<Box> boxes(hb);
+  DeviceVector<Sphere> sph(hs);
+  BruteForce bf(space, boxes);
+  DeviceVector<int> I;
+  DeviceVector<int64_t> O;
+  bf.query(space, intersects(sph), I, O);
+  auto ho = O.to_host();
+  auto hi = I.to_host();
+  DeviceVector<int> counts(std::vector<int>(nq, 0));
+  bf.query(space, intersects(sph), CountHits{counts.data()});
+  auto hc = counts.to_host();
+  bool ok = ho.back() > 0;
+  for (int q = 0; q < nq; ++q)
+    ok = ok && hc[q] == ho[q + 1] - ho[q];
+  CHECK(ok, "brute force: void callback counts equal the CRS row lengths");
+  DeviceVector<int> vals;
+  DeviceVector<int64_t> off2;
+  bf.query(
+      space, intersects(sph),
+      [] __device__(Intersects<Sphere> const &, int v, auto const &out) { out(v); }, vals, off2);
+  CHECK(off2.to_host() == ho && vals.to_host() == hi,
+        "brute force: default-callback CRS equals query");
+}
+
 // The XCD block remap of the traversal kernels must be a bijection of
 // [0, nblocks) (host-side check of the device helper).
 static void xcd_remap()
@@ -263,10 +328,13 @@ int main()
   xcd_remap();
   for (NodeFormat fmt : {NodeFormat::FP32, NodeFormat::Q16})
   {
-    reference_cases(space, fmt);
-    early_exit(space, fmt);
+    reference_cases(space, MakeBVH{fmt});
+    early_exit(space, MakeBVH{fmt});
     random_points(space, fmt);
   }
+  reference_cases(space, MakeBruteForce{});
+  early_exit(space, MakeBruteForce{});
+  bruteforce_random(space);
   // empty tree: callbacks are never called
   BoundingVolumeHierarchy empty;
   DeviceVector<Box> q(std::vector<Box>{Box({{0.f, 0.f, 0.f}}, {{1.f, 1.f, 1.f}})});
