@@ -43,13 +43,16 @@ BUILD_KEEP_TOPOLOGY = 1
 FLAT_SUBTREE_MAX = 4
 PREDICATE_SPHERES = 0
 PREDICATE_BOXES = 1
+PREDICATE_RAYS = 3
+PREDICATE_ORDERED_RAYS = 4
 
 # Symbols of include/arborx_mi355x.h (checked by the CPU test suite).
 EXPORTED_SYMBOLS = (
     "axb_last_error", "axb_version", "axb_set_output_allocator", "axb_free", "axb_trim_memory",
     "axb_bvh_create", "axb_bvh_destroy", "axb_bvh_size", "axb_bvh_bounds",
     "axb_bvh_query_intersects", "axb_bvh_query_intersects_radius",
-    "axb_bvh_query_nearest", "axb_bvh_neighbor_list", "axb_find_neighbor_list",
+    "axb_bvh_query_nearest", "axb_bvh_query_nearest_rays", "axb_make_rays",
+    "axb_bvh_neighbor_list", "axb_find_neighbor_list",
     "axb_expand_half_to_full", "axb_pair_energy", "axb_pair_energy_fd",
     "axb_bruteforce_create", "axb_bruteforce_destroy", "axb_bruteforce_size",
     "axb_bruteforce_bounds", "axb_bruteforce_query_intersects",
@@ -122,6 +125,10 @@ def lib():
             "axb_bvh_query_nearest": (_I32, [_P, _P, _I64, _I32, _Policy, _P,
                                              ctypes.POINTER(_P), ctypes.POINTER(_P),
                                              ctypes.POINTER(_I64), _P]),
+            "axb_bvh_query_nearest_rays": (_I32, [_P, _P, _I64, _I32, _Policy, _P,
+                                                  ctypes.POINTER(_P), ctypes.POINTER(_I64),
+                                                  _P]),
+            "axb_make_rays": (_I32, [_P, _P, _I64, _P, _P]),
             "axb_bvh_neighbor_list": (_I32, [_P, ctypes.c_float, _I32, _Policy, _P,
                                              ctypes.POINTER(_P), ctypes.POINTER(_I64), _P]),
             "axb_find_neighbor_list": (_I32, [_P, _I64, ctypes.c_float, _I32, _P,
@@ -224,13 +231,50 @@ class Boxes:
         self.data = _dev_f32(lohi, 6)
 
 
+class Rays:
+    """nq rays, Experimental::Ray(origin, direction) over a batch
+    (src/geometry/ArborX_Ray.hpp:28-66): `data` is a (nq, 6) tensor
+    {origin, unit direction}.  The directions are normalised on the device
+    as the reference's constructor does (norm and division in double); a
+    zero direction raises ValueError.  Rays.from_normalized(t) takes (nq, 6)
+    data as is."""
+
+    def __init__(self, origins, directions, stream=None):
+        o = _dev_f32(origins, 3)
+        d = _dev_f32(directions, 3)
+        if o.shape != d.shape:
+            raise ValueError("Rays: origins and directions differ in shape")
+        self.data = torch.empty((o.shape[0], 6), dtype=torch.float32, device=o.device)
+        st = lib().axb_make_rays(_ptr(o), _ptr(d), o.shape[0], _ptr(self.data), _stream(stream))
+        if st == AXB_ERR_INVALID_ARGUMENT:
+            raise ValueError(lib().axb_last_error().decode())
+        _check(st)
+
+    @classmethod
+    def from_normalized(cls, data):
+        r = cls.__new__(cls)
+        r.data = _dev_f32(data, 6)
+        return r
+
+
 class Intersects:
     """ArborX::Intersects<Geometry> over a batch (detail/ArborX_Predicates.hpp:83-101)."""
 
     def __init__(self, geometry):
-        if not isinstance(geometry, (Spheres, Boxes)):
-            raise TypeError("intersects() takes Spheres or Boxes")
+        if not isinstance(geometry, (Spheres, Boxes, Rays)):
+            raise TypeError("intersects() takes Spheres, Boxes or Rays")
         self.geometry = geometry
+
+
+class OrderedIntersects:
+    """Experimental::OrderedSpatial<Ray> over a batch
+    (detail/ArborX_Predicates.hpp:104-155): rows list the boxes each ray
+    hits in increasing (distance, index) order."""
+
+    def __init__(self, rays):
+        if not isinstance(rays, Rays):
+            raise TypeError("ordered_intersects() takes Rays")
+        self.geometry = rays
 
 
 class _PointsWithRadius:
@@ -247,14 +291,35 @@ class Nearest:
         self.k = int(k)
 
 
+class NearestRays:
+    """ArborX::Nearest<Ray> over a batch with one k: the first min(k, hits)
+    boxes of each ray by (distance, index); rows may be shorter than k."""
+
+    def __init__(self, rays, k=1):
+        self.rays = rays
+        self.k = int(k)
+
+
 def intersects(geometry):
     """ArborX::intersects (detail/ArborX_Predicates.hpp:135-140)."""
     return Intersects(geometry)
 
 
 def nearest(points, k=1):
-    """ArborX::nearest (detail/ArborX_Predicates.hpp:129-133)."""
+    """ArborX::nearest (detail/ArborX_Predicates.hpp:129-133); points or Rays."""
+    if isinstance(points, Rays):
+        return NearestRays(points, k)
     return Nearest(points, k)
+
+
+def ordered_intersects(rays):
+    """Experimental::ordered_intersects (detail/ArborX_Predicates.hpp:148-155)."""
+    return OrderedIntersects(rays)
+
+
+def make_ordered_intersects(rays):
+    """Experimental::make_ordered_intersects(rays) over a batch of Rays."""
+    return OrderedIntersects(rays)
 
 
 def make_intersects(points, r):
@@ -266,8 +331,9 @@ def make_intersects(points, r):
 
 
 def make_nearest(points, k):
-    """Experimental::make_nearest(points, k) (PredicateHelpers.hpp:114-119)."""
-    return Nearest(points, k)
+    """Experimental::make_nearest(points, k) (PredicateHelpers.hpp:114-119);
+    points or Rays."""
+    return nearest(points, k)
 
 
 class TraversalPolicy:
@@ -416,9 +482,24 @@ def _crs_query(prefix, handle, predicates, policy, return_distances, stream):
             handle, _ptr(pts), nq, predicates.r, policy._c(), _ptr(offsets), ctypes.byref(ind),
             ctypes.byref(nnz), s))
         return _take_output(ind.value, nnz.value, torch.int32), offsets
-    if isinstance(predicates, Intersects):
+    if isinstance(predicates, NearestRays):
+        if prefix != "axb_bvh":
+            raise TypeError("nearest(Rays) needs a BoundingVolumeHierarchy")
+        r = predicates.rays.data
+        nq = r.shape[0]
+        offsets = torch.empty(nq + 1, dtype=torch.int64, device="cuda")
+        _check(L.axb_bvh_query_nearest_rays(handle, _ptr(r), nq, predicates.k, policy._c(),
+                                            _ptr(offsets), ctypes.byref(ind), ctypes.byref(nnz),
+                                            s))
+        return _take_output(ind.value, nnz.value, torch.int32), offsets
+    if isinstance(predicates, (Intersects, OrderedIntersects)):
         g = predicates.geometry
-        kind = PREDICATE_SPHERES if isinstance(g, Spheres) else PREDICATE_BOXES
+        if isinstance(predicates, OrderedIntersects):
+            kind = PREDICATE_ORDERED_RAYS
+        elif isinstance(g, Rays):
+            kind = PREDICATE_RAYS
+        else:
+            kind = PREDICATE_SPHERES if isinstance(g, Spheres) else PREDICATE_BOXES
         nq = g.data.shape[0]
         offsets = torch.empty(nq + 1, dtype=torch.int64, device="cuda")
         _check(getattr(L, prefix + "_query_intersects")(

@@ -42,7 +42,9 @@ __global__ __launch_bounds__(256) void k_bf_spatial(const float *__restrict__ va
     return;
   const int64_t q = list ? (int64_t)list[t] : t;
   SpatialPred pr;
-  if (PK == PRED_BOXES)
+  if (PK == PRED_RAYS)
+    pr = ray_spatial_pred(preds + 6 * q);
+  else if (PK == PRED_BOXES)
   {
     const float *p = preds + 6 * q;
     pr = box_pred(Box6{{p[0], p[1], p[2]}, {p[3], p[4], p[5]}});
@@ -202,6 +204,9 @@ hipError_t launch_bf_spatial(const float *values, int64_t n, int geometry, int p
 {
   if (nq <= 0)
     return hipSuccess;
+  if (pred_kind == PRED_RAYS) // boxes only (the caller checks)
+    return bf_spatial_gp<GEOM_BOXES, PRED_RAYS>(values, n, preds, nq, radius, list, offsets, out,
+                                                buffer, mode, err, s);
   if (geometry == GEOM_POINTS)
     return bf_spatial_g<GEOM_POINTS>(values, n, pred_kind, preds, nq, radius, list, offsets, out,
                                      buffer, mode, err, s);

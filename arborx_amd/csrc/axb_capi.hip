@@ -656,12 +656,18 @@ static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy po
 
 extern "C" {
 
+// Spatial CRS queries; pred_kind AXB_PREDICATE_ORDERED_RAYS takes the
+// best-first walk, rows cut at `limit` results (Nearest<Ray>: k).
 static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *preds, int64_t nq,
                                 float radius, axb_traversal_policy policy, int64_t *offsets,
-                                int32_t **indices, int64_t *nnz_host, axb_stream_t stream_)
+                                int32_t **indices, int64_t *nnz_host, axb_stream_t stream_,
+                                int32_t limit = INT32_MAX)
 {
   if (!t || !offsets || !indices || !nnz_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: NULL argument");
+  const bool ordered = pred_kind == AXB_PREDICATE_ORDERED_RAYS;
+  if ((ordered || pred_kind == PRED_RAYS) && t->geometry != GEOM_BOXES && t->n > 0)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: ray predicates need a tree of boxes");
   if (nq < 0 || nq >= (int64_t(1) << 31))
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: nq must be in [0, 2^31)");
   if (nq > 0 && !preds)
@@ -681,16 +687,23 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
   void *order_blk = nullptr;
   if (policy.sort_predicates)
   {
-    axb_status st = predicate_order(t, pred_kind, preds, nq, s, &order, &order_blk);
+    axb_status st = predicate_order(t, ordered ? PRED_RAYS : pred_kind, preds, nq, s, &order,
+                                    &order_blk);
     if (st != AXB_OK)
       return st;
   }
   auto first = [&](int mode, int32_t *out, int32_t B) {
+    if (ordered)
+      return launch_ordered(tv, t->node_format, preds, nq, limit, order, offsets, out, B, mode,
+                            s);
     return launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, nq, radius, order,
                           offsets, out, B, mode, s);
   };
   // the overflow list holds rows; the fill pass takes them as its order
   auto fill = [&](const uint32_t *list, int64_t n_list, int32_t *out) {
+    if (ordered)
+      return launch_ordered(tv, t->node_format, preds, n_list, limit, list ? list : order,
+                            offsets, out, 0, MODE_FILL, s);
     return launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, n_list, radius,
                           list ? list : order, offsets, out, 0, MODE_FILL, s);
   };
@@ -705,10 +718,48 @@ axb_status axb_bvh_query_intersects(const axb_bvh *t, int32_t predicate_kind,
                                     axb_traversal_policy policy, int64_t *offsets,
                                     int32_t **indices, int64_t *nnz_host, axb_stream_t stream)
 {
-  if (predicate_kind != AXB_PREDICATE_SPHERES && predicate_kind != AXB_PREDICATE_BOXES)
+  if (predicate_kind != AXB_PREDICATE_SPHERES && predicate_kind != AXB_PREDICATE_BOXES &&
+      predicate_kind != AXB_PREDICATE_RAYS && predicate_kind != AXB_PREDICATE_ORDERED_RAYS)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_intersects: unknown predicate kind");
   return query_spatial(t, predicate_kind, predicates, nq, 0.f, policy, offsets, indices,
                        nnz_host, stream);
+}
+
+axb_status axb_bvh_query_nearest_rays(const axb_bvh *t, const float *rays, int64_t nq,
+                                      int32_t k, axb_traversal_policy policy,
+                                      int64_t *offsets, int32_t **indices, int64_t *nnz_host,
+                                      axb_stream_t stream)
+{
+  // TreeTraversal.hpp:143-178: k < 1 returns nothing
+  return query_spatial(t, AXB_PREDICATE_ORDERED_RAYS, rays, nq, 0.f, policy, offsets, indices,
+                       nnz_host, stream, k > 0 ? k : 0);
+}
+
+axb_status axb_make_rays(const float *origins, const float *directions, int64_t n,
+                         float *rays, axb_stream_t stream_)
+{
+  if (n < 0 || (n > 0 && (!origins || !directions || !rays)))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_make_rays: NULL argument or n < 0");
+  if (n == 0)
+    return AXB_OK;
+  hipStream_t s = (hipStream_t)stream_;
+  uint32_t *err = (uint32_t *)dalloc(sizeof(uint32_t), s);
+  if (!err)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_make_rays: out of device memory");
+  hipError_t e = fill_u32(err, 0u, 1, s);
+  if (e == hipSuccess)
+    e = launch_make_rays(origins, directions, n, rays, err, s);
+  uint32_t flags = 0;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&flags, err, sizeof(flags), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = sync_stream(s);
+  dfree(err, s);
+  if (e != hipSuccess)
+    return hip_fail(e, "axb_make_rays");
+  if (flags)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_make_rays: a direction is the zero vector");
+  return AXB_OK;
 }
 
 axb_status axb_bvh_query_intersects_radius(const axb_bvh *t, const float *points, int64_t nq,
@@ -750,8 +801,11 @@ axb_status axb_bvh_predicate_order(const axb_bvh *t, int32_t predicate_kind,
   if (!t || (nq > 0 && (!preds || !order)))
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: NULL argument");
   if (predicate_kind != AXB_PREDICATE_SPHERES && predicate_kind != AXB_PREDICATE_BOXES &&
-      predicate_kind != PRED_RADIUS)
+      predicate_kind != PRED_RADIUS && predicate_kind != AXB_PREDICATE_RAYS &&
+      predicate_kind != AXB_PREDICATE_ORDERED_RAYS)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: unknown predicate kind");
+  if (predicate_kind == AXB_PREDICATE_ORDERED_RAYS)
+    predicate_kind = PRED_RAYS;
   if (nq < 0 || nq >= (int64_t(1) << 31))
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: nq must be in [0, 2^31)");
   hipStream_t s = (hipStream_t)stream_;
@@ -1268,9 +1322,13 @@ axb_status axb_bruteforce_query_intersects(const axb_bruteforce *bf, int32_t pre
                                           int32_t **indices, int64_t *nnz_host,
                                           axb_stream_t stream)
 {
-  if (predicate_kind != AXB_PREDICATE_SPHERES && predicate_kind != AXB_PREDICATE_BOXES)
+  if (predicate_kind != AXB_PREDICATE_SPHERES && predicate_kind != AXB_PREDICATE_BOXES &&
+      predicate_kind != AXB_PREDICATE_RAYS)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT,
              "axb_bruteforce_query_intersects: unknown predicate kind");
+  if (predicate_kind == AXB_PREDICATE_RAYS && bf && bf->geometry != GEOM_BOXES && bf->n > 0)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT,
+             "axb_bruteforce_query_intersects: ray predicates need boxes");
   return bf_query_spatial(bf, predicate_kind, predicates, nq, 0.f, policy, offsets, indices,
                           nnz_host, stream);
 }

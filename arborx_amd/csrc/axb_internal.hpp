@@ -242,6 +242,16 @@ hipError_t launch_spatial(const TreeView &tv, int node_format, int geometry, int
                           const float *preds, int64_t nq, float radius, const uint32_t *order,
                           int64_t *offsets, int32_t *out, int32_t buffer, int mode,
                           hipStream_t s);
+// Best-first ray queries over a box tree (rays nq x 6 {origin, unit
+// direction}): row q holds the leaves ray q hits in (distance, index) order,
+// at most `limit` of them.
+hipError_t launch_ordered(const TreeView &tv, int node_format, const float *rays, int64_t nq,
+                          int32_t limit, const uint32_t *order, int64_t *offsets, int32_t *out,
+                          int32_t buffer, int mode, hipStream_t s);
+// Ray construction: rays[i] = {origins[i], directions[i] / |directions[i]|}
+// with the norm in double; a zero direction raises KERR_LAYOUT in err.
+hipError_t launch_make_rays(const float *origins, const float *directions, int64_t n,
+                            float *rays, uint32_t *err, hipStream_t s);
 hipError_t launch_knn(const TreeView &tv, int node_format, int geometry, const float *pts,
                       int64_t nq, int k, const uint32_t *order, int32_t *oi, float *od,
                       hipStream_t s);

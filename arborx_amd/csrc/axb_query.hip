@@ -47,7 +47,9 @@ __global__ __launch_bounds__(256) void k_spatial(
     return;
   const int64_t q = order ? (int64_t)order[t] : t;
   SpatialPred pr;
-  if (PK == PRED_BOXES)
+  if (PK == PRED_RAYS)
+    pr = ray_spatial_pred(preds + 6 * q);
+  else if (PK == PRED_BOXES)
   {
     const float *p = preds + 6 * q;
     pr = box_pred(Box6{{p[0], p[1], p[2]}, {p[3], p[4], p[5]}});
@@ -64,6 +66,77 @@ __global__ __launch_bounds__(256) void k_spatial(
   uint32_t v_int = 0, v_leaf = 0;
   rope_walk<F, G, PK>(tv, pr, 0u, tv.total, ~0u, rw, v_int, v_leaf);
   row_epilogue(tv, rw, offsets, q, v_int, v_leaf);
+}
+
+// Row writer that stops the walk after `limit` hits.
+template <int MODE>
+struct LimitedRow
+{
+  RowWriter<MODE> *rw;
+  int64_t limit;
+  AXB_D bool hit(int32_t v)
+  {
+    rw->hit(v);
+    return rw->cnt < limit;
+  }
+};
+
+// Ordered ray queries (OrderedSpatial<Ray>, and Nearest<Ray> with limit =
+// k): one lane per ray, the best-first walk, row q in (distance, index)
+// order.  The walk's heap lives in private (scratch) memory.
+template <int F, int MODE>
+__global__ __launch_bounds__(256) void k_ordered(TreeView tv, const float *__restrict__ rays,
+                                                 int64_t nq, int32_t limit,
+                                                 const uint32_t *__restrict__ order,
+                                                 int64_t *__restrict__ offsets,
+                                                 int32_t *__restrict__ out, int32_t buffer)
+{
+  const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= nq)
+    return;
+  const int64_t q = order ? (int64_t)order[t] : t;
+  const RayPred ray = ray_pred(rays + 6 * q);
+  RowWriter<MODE> rw;
+  rw.out = out;
+  rw.base = MODE == MODE_FILL ? offsets[q] : (MODE == MODE_BUFFER ? q * (int64_t)buffer : 0);
+  rw.limit = MODE == MODE_FILL ? offsets[q + 1] - rw.base : (int64_t)buffer;
+  LimitedRow<MODE> sink{&rw, (int64_t)limit};
+  uint32_t v_int = 0, v_leaf = 0;
+  if (limit > 0)
+    ordered_walk<F>(tv, ray, sink, v_int, v_leaf);
+  row_epilogue(tv, rw, offsets, q, v_int, v_leaf);
+}
+
+// Ray construction (ArborX_Ray.hpp:42-51, misc/ArborX_Vector.hpp:41-55 and
+// :77-87): the norm is the square root of the double sum r = 0;
+// r += v_d * v_d, each component v_d / norm in double rounded to float, a
+// zero component stored as +0.
+__global__ __launch_bounds__(256) void k_make_rays(const float *__restrict__ origins,
+                                                   const float *__restrict__ dirs, int64_t n,
+                                                   float *__restrict__ rays,
+                                                   uint32_t *__restrict__ err)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  double r = 0.0;
+  double v[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    v[d] = (double)dirs[3 * i + d];
+    r = r + v[d] * v[d];
+  }
+  const double mag = __builtin_sqrt(r);
+  if (!(mag > 0.0))
+    raise_error(err, KERR_LAYOUT);
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    rays[6 * i + d] = origins[3 * i + d];
+    const float w = (float)(v[d] / mag);
+    rays[6 * i + 3 + d] = w == 0.f ? 0.f : w;
+  }
 }
 
 // Neighbour lists over a point tree (NeighborList.hpp:49-188 semantics):
@@ -879,7 +952,12 @@ __global__ __launch_bounds__(256) void k_pred_morton32(
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += stride)
   {
     float cx, cy, cz;
-    if (PK == PRED_BOXES)
+    if (PK == PRED_RAYS) // a ray's centroid is its origin (ArborX_Ray.hpp:76-80)
+    {
+      const float *v = preds + 6 * i;
+      cx = v[0], cy = v[1], cz = v[2];
+    }
+    else if (PK == PRED_BOXES)
     {
       const float *v = preds + 6 * i;
       cx = (v[0] + v[3]) / 2, cy = (v[1] + v[4]) / 2, cz = (v[2] + v[5]) / 2;
@@ -1065,6 +1143,15 @@ hipError_t launch_spatial(const TreeView &tv, int node_format, int geometry, int
 {
   if (nq <= 0)
     return hipSuccess;
+  if (pred_kind == PRED_RAYS)
+  {
+    // box trees only (the caller checks)
+    if (node_format == NODES_FP32)
+      return launch_spatial_fg<NODES_FP32, GEOM_BOXES, PRED_RAYS>(tv, preds, nq, radius, order,
+                                                                  offsets, out, buffer, mode, s);
+    return launch_spatial_fg<NODES_Q16, GEOM_BOXES, PRED_RAYS>(tv, preds, nq, radius, order,
+                                                               offsets, out, buffer, mode, s);
+  }
   if (pred_kind == PRED_SPHERES)
     return launch_spatial_p<PRED_SPHERES>(tv, node_format, geometry, preds, nq, radius, order,
                                           offsets, out, buffer, mode, s);
@@ -1158,6 +1245,44 @@ hipError_t launch_knn(const TreeView &tv, int node_format, int geometry, const f
   return launch_knn_fg<NODES_Q16, GEOM_BOXES>(tv, pts, nq, k, order, oi, od, s);
 }
 
+template <int F>
+static hipError_t launch_ordered_f(const TreeView &tv, const float *rays, int64_t nq,
+                                   int32_t limit, const uint32_t *order, int64_t *offsets,
+                                   int32_t *out, int32_t buffer, int mode, hipStream_t s)
+{
+  if (mode == MODE_COUNT)
+    AXB_LAUNCH((k_ordered<F, MODE_COUNT>), dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq,
+               limit, order, offsets, out, buffer);
+  else if (mode == MODE_FILL)
+    AXB_LAUNCH((k_ordered<F, MODE_FILL>), dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq,
+               limit, order, offsets, out, buffer);
+  else
+    AXB_LAUNCH((k_ordered<F, MODE_BUFFER>), dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq,
+               limit, order, offsets, out, buffer);
+  return hipGetLastError();
+}
+
+hipError_t launch_ordered(const TreeView &tv, int node_format, const float *rays, int64_t nq,
+                          int32_t limit, const uint32_t *order, int64_t *offsets, int32_t *out,
+                          int32_t buffer, int mode, hipStream_t s)
+{
+  if (nq <= 0)
+    return hipSuccess;
+  if (node_format == NODES_FP32)
+    return launch_ordered_f<NODES_FP32>(tv, rays, nq, limit, order, offsets, out, buffer, mode,
+                                        s);
+  return launch_ordered_f<NODES_Q16>(tv, rays, nq, limit, order, offsets, out, buffer, mode, s);
+}
+
+hipError_t launch_make_rays(const float *origins, const float *directions, int64_t n,
+                            float *rays, uint32_t *err, hipStream_t s)
+{
+  if (n <= 0)
+    return hipSuccess;
+  AXB_LAUNCH(k_make_rays, dim3(grid_of(n)), dim3(256), 0, s, origins, directions, n, rays, err);
+  return hipGetLastError();
+}
+
 hipError_t launch_knn_offsets(int64_t *offsets, int64_t nq, int64_t per, hipStream_t s)
 {
   AXB_LAUNCH(k_knn_offsets, dim3(grid_of(nq + 1)), dim3(256), 0, s, offsets, nq, per);
@@ -1169,7 +1294,10 @@ hipError_t launch_pred_morton32(int pred_kind, const float *preds, int64_t nq,
                                 hipStream_t s)
 {
   const unsigned g = (unsigned)std::min<int64_t>(2048, div_up(nq, 256));
-  if (pred_kind == PRED_SPHERES)
+  if (pred_kind == PRED_RAYS)
+    AXB_LAUNCH(k_pred_morton32<PRED_RAYS>, dim3(g), dim3(256), 0, s, preds, nq, scene, keys,
+               hist);
+  else if (pred_kind == PRED_SPHERES)
     AXB_LAUNCH(k_pred_morton32<PRED_SPHERES>, dim3(g), dim3(256), 0, s, preds, nq,
                        scene, keys, hist);
   else if (pred_kind == PRED_BOXES)

@@ -72,7 +72,18 @@ enum
 enum
 {
   AXB_PREDICATE_SPHERES = 0, /* Intersects<Sphere>: nq x 4 {x,y,z,r} */
-  AXB_PREDICATE_BOXES = 1    /* Intersects<Box>:    nq x 6 {lo, hi} */
+  AXB_PREDICATE_BOXES = 1,   /* Intersects<Box>:    nq x 6 {lo, hi} */
+  /* Rays, nq x 6 {origin xyz, unit direction xyz} (Experimental::Ray,
+   * src/geometry/ArborX_Ray.hpp:28-66; axb_make_rays builds them), box
+   * trees only.  Intersects<Ray>: the boxes the ray's half-line meets
+   * (ArborX_Ray.hpp:156-167). */
+  AXB_PREDICATE_RAYS = 3,
+  /* OrderedSpatial<Ray> (Experimental::ordered_intersects,
+   * detail/ArborX_Predicates.hpp:104-155, TreeTraversal.hpp:338-489): the
+   * same boxes, each row in increasing (distance, index) order, distance =
+   * the ray parameter where it enters the box (ArborX_Ray.hpp:429-444).  The
+   * reference leaves equal distances in the order of its traversal heap. */
+  AXB_PREDICATE_ORDERED_RAYS = 4
 };
 
 /* ArborX::Experimental::TraversalPolicy
@@ -143,6 +154,26 @@ axb_status axb_bvh_query_intersects(const axb_bvh *bvh, int32_t predicate_kind,
                                     axb_traversal_policy policy,
                                     int64_t *offsets, int32_t **indices,
                                     int64_t *nnz_host, axb_stream_t stream);
+
+/* Ray construction, Experimental::Ray(origin, direction)
+ * (src/geometry/ArborX_Ray.hpp:42-51): rays (device, n x 6) receive
+ * {origins[i], directions[i] / |directions[i]|}, the norm and the division
+ * in double (misc/ArborX_Vector.hpp:41-55,77-87).  A zero component is
+ * stored as +0.  A zero direction is an error (the reference asserts
+ * norm > 0).  origins, directions: device n x 3. */
+axb_status axb_make_rays(const float *origins, const float *directions, int64_t n,
+                         float *rays, axb_stream_t stream);
+
+/* Nearest<Ray> to CRS: nearest(ray, k) over a box tree
+ * (detail/ArborX_TreeTraversal.hpp:122-336 with distance(Ray, Box),
+ * ArborX_Ray.hpp:429-444): row i holds the first min(k, hits) boxes of ray
+ * i in (distance, index) order -- boxes the ray misses are at +inf and never
+ * qualify, so rows may be shorter than k.  Arguments as for
+ * axb_bvh_query_intersects. */
+axb_status axb_bvh_query_nearest_rays(const axb_bvh *bvh, const float *rays, int64_t nq,
+                                      int32_t k, axb_traversal_policy policy,
+                                      int64_t *offsets, int32_t **indices,
+                                      int64_t *nnz_host, axb_stream_t stream);
 
 /* make_intersects(points, r) (detail/ArborX_PredicateHelpers.hpp:99-105,
  * 163-186): spheres of one radius r (stored as float) around nq points. */
@@ -219,7 +250,9 @@ int64_t axb_bruteforce_size(const axb_bruteforce *bf);
 axb_status axb_bruteforce_bounds(const axb_bruteforce *bf, float out_host[6]);
 /* query(space, predicates, indices, offsets[, policy])
  * (ArborX_BruteForce.hpp:66-91,151-178, BruteForceImpl.hpp:53-153): as for
- * axb_bvh_query_intersects / _radius. */
+ * axb_bvh_query_intersects / _radius; rows in increasing value index.
+ * AXB_PREDICATE_RAYS over boxes is accepted; AXB_PREDICATE_ORDERED_RAYS is
+ * not (the reference's brute force has no ordered traversal). */
 axb_status axb_bruteforce_query_intersects(const axb_bruteforce *bf,
                                            int32_t predicate_kind,
                                            const float *predicates, int64_t nq,
@@ -297,8 +330,10 @@ axb_status axb_bvh_view(const axb_bvh *bvh, axb_tree_view *out_host);
 /* Processing order of a predicate batch: the permutation that sorts the
  * predicates' centroids by 32-bit Morton code over the tree's scene
  * (CrsGraphWrapperImpl.hpp:407-419, the reference's sortPredicates).
- * predicate_kind: AXB_PREDICATE_SPHERES / _BOXES, or 2 for points (3
- * floats each).  order: device uint32, nq entries (caller-allocated). */
+ * predicate_kind: AXB_PREDICATE_SPHERES / _BOXES, 2 for points (3
+ * floats each), or AXB_PREDICATE_RAYS / _ORDERED_RAYS (a ray's centroid is
+ * its origin, ArborX_Ray.hpp:76-80).  order: device uint32, nq entries
+ * (caller-allocated). */
 axb_status axb_bvh_predicate_order(const axb_bvh *bvh, int32_t predicate_kind,
                                    const float *predicates, int64_t nq,
                                    uint32_t *order, axb_stream_t stream);

@@ -327,6 +327,7 @@ AXB_HD uint32_t xcd_block(uint32_t b, uint32_t nblocks)
 constexpr int PRED_SPHERES = 0;
 constexpr int PRED_BOXES = 1;
 constexpr int PRED_RADIUS = 2; // points + one radius
+constexpr int PRED_RAYS = 3;   // rays {origin, unit direction}, box trees
 
 // ORs an error flag into a device word (nullptr: ignored).
 AXB_HD void raise_error(uint32_t *err, uint32_t flag)
@@ -376,12 +377,95 @@ AXB_HD void set_frame(const Box6 &b, Box6 &par, float *ps)
   }
 }
 
+// ---------------------------------------------------------------- rays
+// A ray: origin and unit direction (ArborX_Ray.hpp:28-66).
+struct RayPred
+{
+  float o[3];
+  float d[3];
+};
+
+// A direction component equal to zero is taken as +0: the slab rule below
+// treats -0 as a ray leaving every slab that strictly contains its origin,
+// which would make a hit depend on the boxes of the nodes above the leaf.
+AXB_HD RayPred ray_pred(const float *p6)
+{
+  RayPred r;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+  {
+    r.o[a] = p6[a];
+    r.d[a] = p6[3 + a] == 0.f ? 0.f : p6[3 + a];
+  }
+  return r;
+}
+
+// Parameter interval [tmin, tmax] of the ray inside box b by slabs
+// (ArborX_Ray.hpp:107-153): per axis the parameters (face - origin) /
+// direction of the two faces, entry first; an axis the ray runs parallel to
+// gives (-inf, +inf) when the origin lies in the slab and an empty interval
+// otherwise (the sign of direction * (face - origin) decides), and nothing
+// at all when the origin is on the slab's lower face.  Non-empty iff
+// tmin <= tmax.
+AXB_HD bool ray_interval(const RayPred &r, const Box6 &b, float &tmin, float &tmax)
+{
+  tmin = -INFINITY;
+  tmax = INFINITY;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+  {
+    const float lo_o = b.lo[a] - r.o[a], hi_o = b.hi[a] - r.o[a];
+    const float da = r.d[a];
+    float t0, t1;
+    if (da == 0.f)
+    {
+      const bool on_face = lo_o == 0.f;
+      t0 = on_face ? -INFINITY : ((f_bits(da * hi_o) >> 31) ? INFINITY : -INFINITY);
+      t1 = on_face ? INFINITY : ((f_bits(da * lo_o) >> 31) ? INFINITY : -INFINITY);
+    }
+    else
+    {
+      const float ta = lo_o / da, tb = hi_o / da;
+      t0 = da > 0.f ? ta : tb;
+      t1 = da > 0.f ? tb : ta;
+    }
+    if (tmin < t0)
+      tmin = t0;
+    if (tmax > t1)
+      tmax = t1;
+  }
+  return tmin <= tmax;
+}
+
+// intersects(ray, box): the box meets the ray's half-line (ArborX_Ray.hpp:
+// 156-167).
+AXB_HD bool ray_hits(const RayPred &r, const Box6 &b)
+{
+  float t0, t1;
+  return ray_interval(r, b, t0, t1) && t1 >= 0.f;
+}
+
+// distance(ray, box): the first parameter t >= 0 at which the ray is in the
+// box, +inf if it never is (ArborX_Ray.hpp:429-444); -0 is returned as +0.
+// A box containing another gets a distance no larger (the slab parameters
+// are monotone in the faces), so node distances bound leaf distances from
+// below, Q16 boxes (rounded outwards) included.
+AXB_HD float ray_distance(const RayPred &r, const Box6 &b)
+{
+  float t0, t1;
+  if (!(ray_interval(r, b, t0, t1) && t1 >= 0.f))
+    return INFINITY;
+  const float t = kmax(t0, 0.f);
+  return t == 0.f ? 0.f : t;
+}
+
 // Spatial predicate of one lane: a sphere (centre, sqrt threshold of its
-// radius) or a box.
+// radius), a box or a ray.
 struct SpatialPred
 {
   float cx, cy, cz, thr;
   Box6 qb;
+  RayPred ray;
 };
 
 AXB_HD SpatialPred sphere_pred(float x, float y, float z, float r)
@@ -389,6 +473,7 @@ AXB_HD SpatialPred sphere_pred(float x, float y, float z, float r)
   SpatialPred p;
   p.cx = x, p.cy = y, p.cz = z, p.thr = sqrt_threshold(r);
   p.qb = Box6{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  p.ray = RayPred{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   return p;
 }
 AXB_HD SpatialPred box_pred(const Box6 &b)
@@ -396,6 +481,16 @@ AXB_HD SpatialPred box_pred(const Box6 &b)
   SpatialPred p;
   p.cx = p.cy = p.cz = p.thr = 0.f;
   p.qb = b;
+  p.ray = RayPred{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  return p;
+}
+// p6 = {origin, direction}
+AXB_HD SpatialPred ray_spatial_pred(const float *p6)
+{
+  SpatialPred p;
+  p.cx = p6[0], p.cy = p6[1], p.cz = p6[2], p.thr = 0.f;
+  p.qb = Box6{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  p.ray = ray_pred(p6);
   return p;
 }
 
@@ -460,6 +555,7 @@ AXB_HD bool point_step(const TV &tv, const SpatialPred &pr, uint32_t &p, uint32_
 template <int G, int PK>
 AXB_HD bool leaf_hit(const SpatialPred &pr, uint4 s0, uint4 s1)
 {
+  static_assert(!(G == GEOM_POINTS && PK == PRED_RAYS), "rays are tested against boxes");
   const float x = bits_f(s0.x), y = bits_f(s0.y), z = bits_f(s0.z);
   if (G == GEOM_POINTS)
   {
@@ -469,6 +565,8 @@ AXB_HD bool leaf_hit(const SpatialPred &pr, uint4 s0, uint4 s1)
     return dist2_pp(pr.cx, pr.cy, pr.cz, x, y, z) <= pr.thr;
   }
   const Box6 lb = {{x, y, z}, {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
+  if (PK == PRED_RAYS)
+    return ray_hits(pr.ray, lb);
   return PK == PRED_BOXES ? intersects_bb(pr.qb, lb)
                           : dist2_pbox(pr.cx, pr.cy, pr.cz, lb) <= pr.thr;
 }
@@ -562,8 +660,9 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
       ++v_int;
       Box6 bx;
       node_box<F>(tv, s0, s1, desc, par, ps, bx);
-      const bool pass = PK == PRED_BOXES ? intersects_bb(pr.qb, bx)
-                                         : dist2_pnode(pr.cx, pr.cy, pr.cz, bx) <= pr.thr;
+      const bool pass = PK == PRED_RAYS    ? ray_hits(pr.ray, bx)
+                        : PK == PRED_BOXES ? intersects_bb(pr.qb, bx)
+                                           : dist2_pnode(pr.cx, pr.cy, pr.cz, bx) <= pr.thr;
       if (pass)
       {
         set_frame<F>(bx, par, ps);
@@ -580,6 +679,275 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
         desc = false;
         p += (uint32_t)w;
       }
+    }
+  }
+}
+
+// ------------------------------------------------------- ordered (rays)
+// Binary min-heap of (key, payload) pairs in private arrays, for the
+// best-first walk below.
+template <int CAP>
+struct OrderedHeap
+{
+  uint64_t key[CAP];
+  uint64_t pay[CAP];
+  int size = 0;
+
+  AXB_HD bool push(uint64_t k, uint64_t p)
+  {
+    if (size >= CAP)
+      return false;
+    int j = size++;
+    while (j > 0)
+    {
+      const int up = (j - 1) >> 1;
+      if (key[up] <= k)
+        break;
+      key[j] = key[up], pay[j] = pay[up];
+      j = up;
+    }
+    key[j] = k, pay[j] = p;
+    return true;
+  }
+  // Puts (k, p) at position j and moves it down to its place.
+  AXB_HD void sift_down(int j, uint64_t k, uint64_t p)
+  {
+    while (true)
+    {
+      int c = 2 * j + 1;
+      if (c >= size)
+        break;
+      if (c + 1 < size && key[c + 1] < key[c])
+        ++c;
+      if (k <= key[c])
+        break;
+      key[j] = key[c], pay[j] = pay[c];
+      j = c;
+    }
+    key[j] = k, pay[j] = p;
+  }
+  AXB_HD void pop(uint64_t &k, uint64_t &p)
+  {
+    k = key[0], p = pay[0];
+    --size;
+    if (size > 0)
+      sift_down(0, key[size], pay[size]);
+  }
+};
+
+// Heap key of a walk entry: distance bits (distances are >= +0, so their
+// bits order like the values), then 0 for a node and 2^31 | index for a
+// leaf.  At equal distance every node is opened before any leaf is
+// reported, and leaves come out by index.
+constexpr uint32_t ORDERED_LEAF = 0x80000000u;
+AXB_HD uint64_t ordered_key(float d, uint32_t tie)
+{
+  return ((uint64_t)f_bits(d) << 32) | tie;
+}
+
+// Best-first walk of a box tree for one ray, the reference's ordered
+// spatial traversal (TreeTraversal.hpp:338-489): the leaves the ray hits
+// are reported in increasing (distance, index) order, distance =
+// ray_distance; sink.hit(value) returns false to stop (early exit,
+// Callbacks.hpp:96-118).  The reference reports equal distances in the
+// order of its heap; here they come by index.  Node distances bound those
+// of the leaves below, so taking entries smallest key first reports leaves
+// sorted.  An expanded node's children go to the heap except the smallest,
+// which is taken next unless the heap holds a smaller key (the reference's
+// closer-child step).  A node's children are its left child, the record
+// after it, and the right child: the record at the left child's rope, or,
+// when the left child is a run of leaves (a subtree that is not stored), the
+// first internal record after the run inside the node's subtree.
+// Q16: a left child decodes against its parent's box, so a node entry keeps
+// (slot, slot of the first record of its left spine), that first record
+// decoding against the scene, and is decoded down that spine when taken
+// from the heap.  Returns false when the heap is full (KERR_STACK; the walk
+// stops).  TV: a tree view with `slots`, `total`, `scene`, `ss`, `err`.
+template <int F, int CAP = 64, class TV, class Sink>
+AXB_HD bool ordered_walk(const TV &tv, const RayPred &ray, Sink &sink, uint32_t &v_int,
+                         uint32_t &v_leaf)
+{
+  constexpr uint32_t SL = 2, SI = internal_slots<F>();
+  auto leaf_box = [](uint4 a, uint4 b) {
+    return Box6{{bits_f(a.x), bits_f(a.y), bits_f(a.z)}, {bits_f(b.x), bits_f(b.y), bits_f(b.z)}};
+  };
+  OrderedHeap<CAP> h;
+  const uint4 r0 = tv.slots[0], r1 = tv.slots[1];
+  if ((int32_t)r0.w < 0)
+  {
+    // no stored node: at most FLAT_MAX leaves, followed by zeroed slots
+    uint32_t q = 0;
+    for (int j = 0; j < FLAT_MAX; ++j, q += SL)
+    {
+      const uint4 a0 = tv.slots[q];
+      if ((int32_t)a0.w >= 0)
+        break;
+      ++v_leaf;
+      const float d = ray_distance(ray, leaf_box(a0, tv.slots[q + 1]));
+      if (d < INFINITY)
+        h.push(ordered_key(d, ORDERED_LEAF | ~(uint32_t)a0.w), 0);
+    }
+    while (h.size > 0)
+    {
+      uint64_t k, pl;
+      h.pop(k, pl);
+      if (!sink.hit((int32_t)((uint32_t)k & ~ORDERED_LEAF)))
+        break;
+    }
+    return true;
+  }
+  // current entry; for a node, cb is its box when have_box
+  uint64_t ck = 0, cp = 0;
+  Box6 cb;
+  node_box<F>(tv, r0, r1, false, tv.scene, tv.ss, cb);
+  ++v_int;
+  if (!(ray_distance(ray, cb) < INFINITY))
+    return true;
+  bool have_box = true;
+  while (true)
+  {
+    if ((uint32_t)ck & ORDERED_LEAF)
+    {
+      if (!sink.hit((int32_t)((uint32_t)ck & ~ORDERED_LEAF)) || h.size == 0)
+        return true;
+      h.pop(ck, cp);
+      have_box = false;
+      continue;
+    }
+    const uint32_t p = (uint32_t)cp, anchor = (uint32_t)(cp >> 32);
+    if (!have_box)
+    {
+      if (F == NODES_FP32)
+        node_box<F>(tv, tv.slots[p], tv.slots[p + 1], false, tv.scene, tv.ss, cb);
+      else
+      {
+        // down the left spine from the anchor (consecutive records)
+        Box6 f = tv.scene;
+        float fs[3] = {tv.ss[0], tv.ss[1], tv.ss[2]};
+        for (uint32_t s = anchor;; ++s)
+        {
+          const uint4 q0 = tv.slots[s];
+          node_box<F>(tv, q0, q0, s != anchor, f, fs, cb);
+          if (s >= p)
+            break;
+          set_frame<F>(cb, f, fs);
+        }
+      }
+    }
+    Box6 frame = cb;
+    float fs[3] = {0.f, 0.f, 0.f};
+    set_frame<F>(cb, frame, fs);
+    const uint32_t sub_end = p + (uint32_t)tv.slots[p].w;
+    // the best child so far (bk, bp, bb) and the others pushed
+    bool have_best = false, best_box = false;
+    uint64_t bk = 0, bp = 0;
+    Box6 bb;
+    bool full = false;
+    auto offer = [&](uint64_t k, uint64_t pl, bool is_node, const Box6 &b) {
+      if (!have_best || k < bk)
+      {
+        if (have_best)
+          full |= !h.push(bk, bp);
+        have_best = true, bk = k, bp = pl, best_box = is_node, bb = b;
+      }
+      else
+        full |= !h.push(k, pl);
+    };
+    auto leaf_child = [&](uint32_t q, uint4 a0) {
+      ++v_leaf;
+      const Box6 lb = leaf_box(a0, tv.slots[q + 1]);
+      const float d = ray_distance(ray, lb);
+      if (d < INFINITY)
+        offer(ordered_key(d, ORDERED_LEAF | ~(uint32_t)a0.w), 0, false, lb);
+    };
+    auto node_child = [&](uint32_t q, uint4 a0, bool left) {
+      ++v_int;
+      Box6 nb;
+      node_box<F>(tv, a0, F == NODES_FP32 ? tv.slots[q + 1] : a0, left, frame, fs, nb);
+      const float d = ray_distance(ray, nb);
+      if (d < INFINITY)
+        offer(ordered_key(d, 0u), (uint64_t)q | ((uint64_t)(left ? anchor : q) << 32), true, nb);
+    };
+    const uint32_t lp = p + SI;
+    const uint4 l0 = tv.slots[lp];
+    uint32_t q = lp;
+    if ((int32_t)l0.w < 0)
+    {
+      // leaves of the left child, and of the right one too when it is not
+      // stored either (at most 2 * FLAT_MAX leaves)
+      for (int j = 0; j < 2 * FLAT_MAX && q < sub_end; ++j)
+      {
+        const uint4 a0 = tv.slots[q];
+        if ((int32_t)a0.w >= 0)
+          break;
+        leaf_child(q, a0);
+        q += SL;
+      }
+      if (q < sub_end)
+      {
+        const uint4 a0 = tv.slots[q];
+        if ((int32_t)a0.w < 0)
+        {
+          raise_error(tv.err, KERR_LAYOUT);
+          return false;
+        }
+        node_child(q, a0, false);
+      }
+    }
+    else
+    {
+      node_child(lp, l0, true);
+      q = lp + (uint32_t)l0.w;
+      if (q >= sub_end)
+      {
+        raise_error(tv.err, KERR_LAYOUT);
+        return false;
+      }
+      const uint4 a0 = tv.slots[q];
+      if ((int32_t)a0.w >= 0)
+        node_child(q, a0, false);
+      else
+      {
+        // the right child's leaves fill the rest of the subtree
+        for (int j = 0; j < FLAT_MAX && q < sub_end; ++j)
+        {
+          const uint4 b0 = tv.slots[q];
+          if ((int32_t)b0.w >= 0)
+            break;
+          leaf_child(q, b0);
+          q += SL;
+        }
+        if (q != sub_end)
+        {
+          raise_error(tv.err, KERR_LAYOUT);
+          return false;
+        }
+      }
+    }
+    if (full)
+    {
+      raise_error(tv.err, KERR_STACK);
+      return false;
+    }
+    if (!have_best)
+    {
+      if (h.size == 0)
+        return true;
+      h.pop(ck, cp);
+      have_box = false;
+    }
+    else if (h.size > 0 && h.key[0] < bk)
+    {
+      // the heap's top comes first: it becomes current, the best child
+      // takes its place
+      ck = h.key[0], cp = h.pay[0];
+      h.sift_down(0, bk, bp);
+      have_box = false;
+    }
+    else
+    {
+      ck = bk, cp = bp, cb = bb;
+      have_box = best_box;
     }
   }
 }

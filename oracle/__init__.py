@@ -66,6 +66,9 @@ def lib():
             "orc_bruteforce_spatial": (I64, [P, I64, I32, I32, P, I64, P, P]),
             "orc_query_knn": (I64, [P, P, I64, I32, P, P, P, P, P]),
             "orc_bruteforce_knn": (None, [P, I64, I32, P, I64, I32, P, P]),
+            "orc_make_rays": (I64, [P, P, I64, P]),
+            "orc_bruteforce_rays_ordered": (I64, [P, I64, P, I64, I64, P, P]),
+            "orc_query_ordered": (I64, [P, P, I64, P, P, P]),
             "orc_pair_energy_f64": (ctypes.c_double, [P, I64, P, P, ctypes.c_double, P]),
             "orc_neighbor_list": (I64, [P, I64, ctypes.c_float, I32, P, P]),
             "orc_expand_half_to_full": (I64, [P, P, I64, P, P]),
@@ -185,8 +188,23 @@ class Tree:
         slots = lib().orc_linearize(self._h, node_format, int(flat_max), _ptr(words))
         return words[:slots]
 
+    def query_ordered(self, rays):
+        """The reference's ordered (best-first) ray traversal -> (offsets,
+        indices, distances), rows in traversal order."""
+        r = _f32(rays, 6)
+        nq = r.shape[0]
+        offsets = np.zeros(nq + 1, np.int64)
+        nnz = lib().orc_query_ordered(self._h, _ptr(r), nq, _ptr(offsets), None, None)
+        if nnz < 0:
+            raise RuntimeError("ordered traversal: heap of 64 entries exceeded")
+        idx = np.zeros(max(nnz, 1), np.int32)
+        dist = np.zeros(max(nnz, 1), np.float32)
+        lib().orc_query_ordered(self._h, _ptr(r), nq, _ptr(offsets), _ptr(idx), _ptr(dist))
+        return offsets, idx[:nnz], dist[:nnz]
+
     def query_spatial(self, kind, preds, with_visits=False):
-        """kind 0: spheres (nq,4) {x,y,z,r}; kind 1: boxes (nq,6)."""
+        """kind 0: spheres (nq,4) {x,y,z,r}; kind 1: boxes (nq,6); kind 3:
+        rays (nq,6) {origin, unit direction}."""
         p = _f32(preds, 4 if kind == 0 else 6)
         nq = p.shape[0]
         offsets = np.zeros(nq + 1, np.int64)
@@ -238,6 +256,33 @@ def bruteforce_knn(values, geometry, points, k):
     dist = np.zeros(max(nq * per, 1), np.float32)
     lib().orc_bruteforce_knn(_ptr(v), n, geometry, _ptr(p), nq, k, _ptr(idx), _ptr(dist))
     return idx[: nq * per], dist[: nq * per]
+
+
+def make_rays(origins, directions):
+    """Rays {origin, unit direction} as the reference's Ray constructor makes
+    them (double-precision normalisation); zero components stored as +0."""
+    o = _f32(origins, 3)
+    d = _f32(directions, 3)
+    out = np.zeros((o.shape[0], 6), np.float32)
+    bad = lib().orc_make_rays(_ptr(o), _ptr(d), o.shape[0], _ptr(out))
+    if bad:
+        raise ValueError("make_rays: zero direction")
+    return out
+
+
+def bruteforce_rays_ordered(boxes, rays, limit=-1):
+    """Rows of the boxes each ray hits, in (distance, index) order, at most
+    `limit` per row (-1: all) -> (offsets, indices)."""
+    b = _f32(boxes, 6)
+    r = _f32(rays, 6)
+    nq = r.shape[0]
+    offsets = np.zeros(nq + 1, np.int64)
+    nnz = lib().orc_bruteforce_rays_ordered(_ptr(b), b.shape[0], _ptr(r), nq, limit,
+                                            _ptr(offsets), None)
+    idx = np.zeros(max(nnz, 1), np.int32)
+    lib().orc_bruteforce_rays_ordered(_ptr(b), b.shape[0], _ptr(r), nq, limit, _ptr(offsets),
+                                      _ptr(idx))
+    return offsets, idx[:nnz]
 
 
 def q16_encode_box(frame6, box6):

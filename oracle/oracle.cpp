@@ -17,6 +17,11 @@
 //   * predicate processing order (Morton32 sort of queries,
 //     CrsGraphWrapperImpl.hpp:407-419) does not change any result row, so the
 //     oracle processes queries in their original order.
+//   * rays: a zero direction component is stored as +0 (orc_make_rays), and
+//     the ordered (best-first) traversal stops when both children of the
+//     node it stands on are missed and its heap is empty, where the
+//     reference descends into the closer child anyway
+//     (TreeTraversal.hpp:468-478) and so reports a leaf the ray misses.
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
@@ -103,6 +108,61 @@ static inline bool intersects_bb(const float *alo, const float *ahi,
     if (alo[d] > bhi[d] || ahi[d] < blo[d])
       return false;
   return true;
+}
+
+// ---------------------------------------------------------------- rays
+// A ray is 6 floats {origin, unit direction} (ArborX_Ray.hpp:28-66).
+// Slab intersection, ArborX_Ray.hpp:107-153.
+static inline bool ray_interval(const float *ray, const float *lo,
+                                const float *hi, float &tmin, float &tmax)
+{
+  const float inf = std::numeric_limits<float>::infinity();
+  const float *orig = ray, *dir = ray + 3;
+  tmin = -inf;
+  tmax = inf;
+  for (int d = 0; d < 3; ++d)
+  {
+    float tdmin, tdmax;
+    if (dir[d] == 0)
+    {
+      const float min_orig = lo[d] - orig[d];
+      if (min_orig == 0)
+        continue;
+      const float max_orig = hi[d] - orig[d];
+      tdmin = std::signbit(dir[d] * max_orig) ? inf : -inf;
+      tdmax = std::signbit(dir[d] * min_orig) ? inf : -inf;
+    }
+    else if (dir[d] > 0)
+    {
+      tdmin = (lo[d] - orig[d]) / dir[d];
+      tdmax = (hi[d] - orig[d]) / dir[d];
+    }
+    else
+    {
+      tdmin = (hi[d] - orig[d]) / dir[d];
+      tdmax = (lo[d] - orig[d]) / dir[d];
+    }
+    if (tmin < tdmin)
+      tmin = tdmin;
+    if (tmax > tdmax)
+      tmax = tdmax;
+  }
+  return tmin <= tmax;
+}
+// ArborX_Ray.hpp:156-167: intersects(ray, box).
+static inline bool ray_intersects(const float *ray, const float *lo,
+                                  const float *hi)
+{
+  float tmin, tmax;
+  return ray_interval(ray, lo, hi, tmin, tmax) && tmax >= 0;
+}
+// ArborX_Ray.hpp:429-444: distance(ray, box).
+static inline float ray_distance(const float *ray, const float *lo,
+                                 const float *hi)
+{
+  float tmin, tmax;
+  const bool hit = ray_interval(ray, lo, hi, tmin, tmax) && tmax >= 0;
+  return hit ? kmax(tmin, 0.f) : std::numeric_limits<float>::infinity();
 }
 
 // ---------------------------------------------------------------- Morton
@@ -770,10 +830,13 @@ int64_t orc_linearize(void *h, int node_format, int flat_max, uint32_t *words)
 
 // ------------------------------------------------------- spatial queries
 // Predicate kinds: 0 sphere {x,y,z,r} (Intersects<Sphere>), 1 box
-// {lo.xyz, hi.xyz} (Intersects<Box>).
+// {lo.xyz, hi.xyz} (Intersects<Box>), 3 ray {origin, direction}
+// (Intersects<Ray>, box leaves only).
 static inline bool pred_leaf(int kind, const float *q, const float *llo,
                              const float *lhi, int geometry)
 {
+  if (kind == 3)
+    return ray_intersects(q, llo, lhi);
   if (kind == 0)
   {
     // Intersects.hpp:105-114 (sphere-point) / :82-91 (sphere-box)
@@ -785,6 +848,8 @@ static inline bool pred_leaf(int kind, const float *q, const float *llo,
 static inline bool pred_node(int kind, const float *q, const float *lo,
                              const float *hi)
 {
+  if (kind == 3)
+    return ray_intersects(q, lo, hi);
   if (kind == 0)
     return dist_pbox(q, lo, hi) <= q[3]; // Intersects.hpp:82-91
   return intersects_bb(q, q + 3, lo, hi);
@@ -917,6 +982,179 @@ struct Cand
 static inline bool cand_less(const Cand &a, const Cand &b)
 {
   return a.d < b.d || (a.d == b.d && a.idx < b.idx);
+}
+
+// ------------------------------------------------------- ordered (rays)
+// Ray construction, ArborX_Ray.hpp:42-51 with misc/ArborX_Vector.hpp:41-55
+// and :77-87: norm = sqrt of a double sum r = 0; r += v_d * v_d; each
+// component v_d / norm in double, rounded to float.  A zero component is
+// stored as +0 (see the header).  Returns the number of zero directions
+// (the reference asserts norm > 0); their rays are left as NaN.
+int64_t orc_make_rays(const float *origins, const float *dirs, int64_t n,
+                      float *out6)
+{
+  int64_t bad = 0;
+  for (int64_t i = 0; i < n; ++i)
+  {
+    const float *v = dirs + 3 * i;
+    double r = 0;
+    for (int d = 0; d < 3; ++d)
+      r += static_cast<double>(v[d]) * static_cast<double>(v[d]);
+    const double mag = std::sqrt(r);
+    bad += !(mag > 0);
+    for (int d = 0; d < 3; ++d)
+    {
+      out6[6 * i + d] = origins[3 * i + d];
+      float w = mag > 0 ? (float)(v[d] / mag) : std::nanf("");
+      out6[6 * i + 3 + d] = w == 0.f ? 0.f : w;
+    }
+  }
+  return bad;
+}
+
+// Ordered rows by brute force: every box the ray hits, sorted by
+// (distance(ray, box), index), at most `limit` per row (limit < 0: all).
+// This is the row of an OrderedSpatial<Ray> query (limit < 0) and of a
+// Nearest<Ray> query with k = limit: the kNN traversal only keeps leaves at
+// a distance below its radius, which starts at +inf
+// (TreeTraversal.hpp:255-275), so boxes the ray misses never qualify.
+int64_t orc_bruteforce_rays_ordered(const float *boxes, int64_t n,
+                                    const float *rays, int64_t nq,
+                                    int64_t limit, int64_t *offsets,
+                                    int32_t *indices)
+{
+  std::vector<std::vector<int32_t>> rows(nq);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < nq; ++i)
+  {
+    std::vector<Cand> hits;
+    for (int64_t j = 0; j < n; ++j)
+    {
+      const float d = ray_distance(rays + 6 * i, boxes + 6 * j, boxes + 6 * j + 3);
+      if (d < std::numeric_limits<float>::infinity())
+        hits.push_back({d, (int32_t)j});
+    }
+    std::sort(hits.begin(), hits.end(), cand_less);
+    const size_t m = limit < 0 ? hits.size() : std::min<size_t>(hits.size(), (size_t)limit);
+    for (size_t j = 0; j < m; ++j)
+      rows[i].push_back(hits[j].idx);
+  }
+  int64_t run = 0;
+  for (int64_t i = 0; i < nq; ++i)
+  {
+    offsets[i] = run;
+    if (indices)
+      std::copy(rows[i].begin(), rows[i].end(), indices + run);
+    run += rows[i].size();
+  }
+  offsets[nq] = run;
+  return run;
+}
+
+// TreeTraversal.hpp:383-489: the ordered spatial traversal over the
+// reference-form tree.  OneLeafTree: the leaf if its distance is finite.
+// FullTree: from the root, a leaf is reported and the heap's top taken; at
+// an internal node the distances of both children are computed, the closer
+// (distance_left < distance_right ? left : right) is taken unless the heap's
+// top is closer (then the top is taken and the closer child pushed), and
+// the further child is pushed, children at +inf never.  The heap
+// (PriorityQueue with CompareDistance, 64 entries) is keyed by distance
+// alone, so equal distances come out in heap order; the rows are compared
+// after sorting equal distances by index.  The stop when both children are
+// missed and the heap is empty is the difference listed in the header.
+// Returns nnz, or -1 if a heap would exceed 64 entries.
+int64_t orc_query_ordered(void *h, const float *rays, int64_t nq,
+                          int64_t *offsets, int32_t *indices,
+                          float *distances)
+{
+  const Tree &t = *(const Tree *)h;
+  const int64_t N = t.n;
+  const float inf = std::numeric_limits<float>::infinity();
+  std::vector<std::vector<std::pair<int32_t, float>>> rows(nq);
+  std::atomic<bool> overflow{false};
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < nq; ++i)
+  {
+    const float *ray = rays + 6 * i;
+    auto &row = rows[i];
+    if (N == 0)
+      continue;
+    auto dist = [&](int64_t node) {
+      return node < N ? ray_distance(ray, &t.leaf_lo[3 * node], &t.leaf_hi[3 * node])
+                      : ray_distance(ray, &t.int_lo[3 * (node - N)], &t.int_hi[3 * (node - N)]);
+    };
+    if (N == 1)
+    {
+      const float d = dist(0);
+      if (d != inf)
+        row.push_back({(int32_t)t.perm[0], d});
+      continue;
+    }
+    using PID = std::pair<int64_t, float>;
+    auto further_first = [](const PID &a, const PID &b) { return a.second > b.second; };
+    std::vector<PID> heap;
+    int64_t node = N; // root
+    while (true)
+    {
+      if (node < N)
+      {
+        row.push_back({(int32_t)t.perm[node], dist(node)});
+        if (heap.empty())
+          break;
+        std::pop_heap(heap.begin(), heap.end(), further_first);
+        node = heap.back().first;
+        heap.pop_back();
+        continue;
+      }
+      const int64_t l = t.left_child[node - N];
+      const int64_t r = l < N ? t.leaf_rope[l] : t.rope[l - N];
+      const PID lp{l, dist(l)}, rp{r, dist(r)};
+      const PID &closer = lp.second < rp.second ? lp : rp;
+      const PID &further = lp.second < rp.second ? rp : lp;
+      if (!heap.empty() && heap.front().second < closer.second)
+      {
+        std::pop_heap(heap.begin(), heap.end(), further_first);
+        node = heap.back().first;
+        heap.pop_back();
+        if (closer.second < inf)
+        {
+          heap.push_back(closer);
+          std::push_heap(heap.begin(), heap.end(), further_first);
+        }
+      }
+      else if (closer.second < inf)
+        node = closer.first;
+      else
+        break; // both children missed, heap empty
+      if (further.second < inf)
+      {
+        heap.push_back(further);
+        std::push_heap(heap.begin(), heap.end(), further_first);
+      }
+      if (heap.size() > 64)
+      {
+        overflow = true;
+        break;
+      }
+    }
+  }
+  if (overflow)
+    return -1;
+  int64_t run = 0;
+  for (int64_t i = 0; i < nq; ++i)
+  {
+    offsets[i] = run;
+    for (size_t j = 0; j < rows[i].size(); ++j)
+    {
+      if (indices)
+        indices[run + j] = rows[i][j].first;
+      if (distances)
+        distances[run + j] = rows[i][j].second;
+    }
+    run += rows[i].size();
+  }
+  offsets[nq] = run;
+  return run;
 }
 
 // TreeTraversal.hpp:168-335: stack-based DFS, closest child first, bounded

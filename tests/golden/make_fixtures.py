@@ -177,6 +177,34 @@ def main():
              "indices": [1, 2, 3, 0, 2, 3, 0, 1, 3, 0, 1, 2]},
         ],
     }
+    # tstQueryTreeRay.cpp: ten unit boxes [i, i+1]^3 on the diagonal and rays
+    # given as origin + (unnormalised) direction; "ordered" rows are compared
+    # as sequences, the others as sets.  The single-leaf case is
+    # :212-243 (one box [0,1]^3), the empty tree :185-210.
+    diag = [[float(i)] * 3 + [float(i + 1)] * 3 for i in range(10)]
+    kat["rays"] = {
+        "source": "test/tstQueryTreeRay.cpp:29-243",
+        "cases": [
+            {"line": "29-58", "boxes": diag, "kind": "nearest", "k": 1,
+             "origins": [[0.0, 0.0, 0.0]], "directions": [[0.15, 0.1, 0.0]],
+             "offsets": [0, 1], "indices": [0]},
+            {"line": "60-87", "boxes": diag, "kind": "intersects",
+             "origins": [[0.0, 0.0, 0.0]], "directions": [[0.1, 0.1, 0.1]],
+             "offsets": [0, 10], "indices": list(range(10))},
+            {"line": "112-163", "boxes": diag, "kind": "ordered",
+             "origins": [[0.0, 0.0, 0.0], [10.0, 10.0, 10.0]],
+             "directions": [[0.1, 0.1, 0.1], [-0.1, -0.1, -0.1]],
+             "offsets": [0, 10, 20], "indices": list(range(10)) + list(range(9, -1, -1))},
+            {"line": "185-210", "boxes": [], "kind": "ordered",
+             "origins": [[0.0, 0.0, 0.0], [0.0, 0.0, 0.0]],
+             "directions": [[1.0, 0.0, 0.0], [1.0, 0.0, 0.0]],
+             "offsets": [0, 0, 0], "indices": []},
+            {"line": "212-243", "boxes": [[0.0, 0.0, 0.0, 1.0, 1.0, 1.0]], "kind": "ordered",
+             "origins": [[0.0, 0.0, 0.0], [4.0, 5.0, 6.0]],
+             "directions": [[1.0, 1.0, 1.0], [7.0, 8.0, 9.0]],
+             "offsets": [0, 1, 1], "indices": [0]},
+        ],
+    }
     with open(os.path.join(HERE, "reference_kat.json"), "w") as f:
         json.dump(kat, f, indent=1)
     print("wrote", os.path.join(HERE, "reference_kat.json"))

@@ -84,3 +84,46 @@ def diagonal_boxes(n=4096):
 def spheres_from_points(points, r):
     p = np.asarray(points, np.float32)
     return np.concatenate([p, np.full((p.shape[0], 1), np.float32(r), np.float32)], 1)
+
+
+def ray_inputs(n, seed, boxes=None):
+    """Origins and (unnormalised) directions of n test rays: origins in
+    [-0.2, 1.2)^3, Gaussian directions, and the awkward cases mixed in:
+    directions along an axis (two zero components, some of them -0), origins
+    on a face of one of `boxes`, and origins inside a box."""
+    g = np.random.default_rng(seed)
+    o = g.uniform(-0.2, 1.2, (n, 3)).astype(np.float32)
+    d = g.normal(size=(n, 3)).astype(np.float32)
+    ax = g.integers(0, 3, n)
+    sel = np.arange(n) % 5 == 1
+    for i in np.nonzero(sel)[0]:
+        keep = d[i, ax[i]]
+        d[i] = np.float32(0.0)
+        d[i, ax[i]] = keep
+        if i % 2:
+            d[i, (ax[i] + 1) % 3] = np.float32(-0.0)
+    if boxes is not None and len(boxes):
+        b = np.asarray(boxes, np.float32).reshape(-1, 6)
+        on_face = np.nonzero(np.arange(n) % 7 == 3)[0]
+        pick = g.integers(0, len(b), len(on_face))
+        for i, j in zip(on_face, pick):
+            o[i] = (b[j, :3] + b[j, 3:]) / np.float32(2)
+            o[i, ax[i]] = b[j, ax[i]]  # on the lower face along the axis
+        inside = np.nonzero(np.arange(n) % 7 == 5)[0]
+        pick = g.integers(0, len(b), len(inside))
+        o[inside] = (b[pick, :3] + b[pick, 3:]) / np.float32(2)
+    return o, d
+
+
+def canonical_ordered_rows(offsets, indices, distances):
+    """Rows of an ordered query with each run of equal distances sorted by
+    index; checks that every row is non-decreasing in distance."""
+    out = np.array(indices, copy=True)
+    offsets = np.asarray(offsets, np.int64)
+    for i in range(len(offsets) - 1):
+        a, b = offsets[i], offsets[i + 1]
+        d = np.asarray(distances[a:b])
+        assert np.all(d[1:] >= d[:-1]), f"row {i} not ordered by distance"
+        order = np.lexsort((out[a:b], d))
+        out[a:b] = out[a:b][order]
+    return out

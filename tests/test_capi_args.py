@@ -61,9 +61,9 @@ def test_queries_reject_bad_arguments():
                                                         None) == INVALID
     assert "radius" in _last_error()
     # unknown predicate kind
-    assert L.axb_bvh_query_intersects(dummy, 5, dummy, 4, _policy(), dummy, ctypes.byref(ind),
+    assert L.axb_bvh_query_intersects(dummy, 7, dummy, 4, _policy(), dummy, ctypes.byref(ind),
                                       ctypes.byref(nnz), None) == INVALID
-    assert L.axb_bruteforce_query_intersects(dummy, 5, dummy, 4, _policy(), dummy,
+    assert L.axb_bruteforce_query_intersects(dummy, 7, dummy, 4, _policy(), dummy,
                                              ctypes.byref(ind), ctypes.byref(nnz),
                                              None) == INVALID
     # null tree / outputs, query counts out of range
@@ -80,6 +80,24 @@ def test_queries_reject_bad_arguments():
                                           ctypes.byref(ind), ctypes.byref(dist),
                                           ctypes.byref(nnz), None) == INVALID
     assert "nq must be" in _last_error()
+
+
+def test_ray_arguments():
+    L = ax.lib()
+    ind = P()
+    nnz = I64()
+    dummy = P(16)
+    assert L.axb_make_rays(dummy, dummy, -1, dummy, None) == INVALID
+    assert L.axb_make_rays(None, dummy, 3, dummy, None) == INVALID
+    assert L.axb_make_rays(dummy, dummy, 0, dummy, None) == ax.AXB_OK
+    # kinds past the ray kinds, and the ordered kind on the brute force
+    assert L.axb_bvh_query_intersects(dummy, 5, dummy, 4, _policy(), dummy, ctypes.byref(ind),
+                                      ctypes.byref(nnz), None) == INVALID
+    assert L.axb_bruteforce_query_intersects(dummy, ax.PREDICATE_ORDERED_RAYS, dummy, 4,
+                                             _policy(), dummy, ctypes.byref(ind),
+                                             ctypes.byref(nnz), None) == INVALID
+    assert L.axb_bvh_query_nearest_rays(None, dummy, 4, 2, _policy(), dummy, ctypes.byref(ind),
+                                        ctypes.byref(nnz), None) == INVALID
 
 
 def test_energy_rejects_bad_arguments():
