@@ -118,6 +118,36 @@ ARBORX_HD float distance(Point const &a, Point const &b)
 static_assert(sizeof(Point) == 12 && sizeof(Box) == 24 && sizeof(Sphere) == 16,
               "facade geometry must match the C ABI layout");
 
+namespace Experimental
+{
+// geometry/ArborX_Ray.hpp:28-66: origin and unit direction.  The
+// constructor normalises the direction with the norm and the division in
+// double (misc/ArborX_Vector.hpp:41-55,77-87); a zero component is stored
+// as +0 (the library's slab test gives -0 no meaning of its own).  Same
+// bytes as a row of the C ABI's ray arrays.
+struct Ray
+{
+  Point _origin = {};
+  Point _direction = {};
+  Ray() = default;
+  ARBORX_HD Ray(Point const &origin, Point const &direction) : _origin(origin)
+  {
+    double r = 0;
+    for (int d = 0; d < 3; ++d)
+      r += static_cast<double>(direction[d]) * static_cast<double>(direction[d]);
+    const double mag = std::sqrt(r);
+    for (int d = 0; d < 3; ++d)
+    {
+      const float w = static_cast<float>(direction[d] / mag);
+      _direction[d] = w == 0.f ? 0.f : w;
+    }
+  }
+  ARBORX_HD Point const &origin() const { return _origin; }
+  ARBORX_HD Point const &direction() const { return _direction; }
+};
+static_assert(sizeof(Ray) == 24, "facade rays must match the C ABI layout");
+} // namespace Experimental
+
 // Owning device array (stands in for Kokkos::View<T*, HIP memory space>).
 template <class T>
 class DeviceVector
@@ -241,6 +271,25 @@ struct IntersectsBoxes
   Box const *boxes;
   size_t n;
 };
+// Intersects<Ray>, OrderedSpatial<Ray> (ordered_intersects) and
+// Nearest<Ray> batches over box trees (geometry/ArborX_Ray.hpp,
+// detail/ArborX_Predicates.hpp:104-155).
+struct IntersectsRays
+{
+  Ray const *rays;
+  size_t n;
+};
+struct OrderedIntersectsRays
+{
+  Ray const *rays;
+  size_t n;
+};
+struct RaysNearestK
+{
+  Ray const *rays;
+  size_t n;
+  int k;
+};
 
 // make_intersects(points, r): spheres of radius r (stored as float),
 // PredicateHelpers.hpp:99-105 (r > 0 asserted at :102).
@@ -253,6 +302,20 @@ inline PointsWithRadius make_intersects(DeviceVector<Point> const &points, float
 inline PointsNearestK make_nearest(DeviceVector<Point> const &points, int k)
 {
   return {points.data(), points.size(), k};
+}
+inline RaysNearestK make_nearest(DeviceVector<Ray> const &rays, int k)
+{
+  return {rays.data(), rays.size(), k};
+}
+// ordered_intersects / make_ordered_intersects over a batch of rays
+// (detail/ArborX_Predicates.hpp:148-155).
+inline OrderedIntersectsRays ordered_intersects(DeviceVector<Ray> const &rays)
+{
+  return {rays.data(), rays.size()};
+}
+inline OrderedIntersectsRays make_ordered_intersects(DeviceVector<Ray> const &rays)
+{
+  return {rays.data(), rays.size()};
 }
 } // namespace Experimental
 
@@ -269,6 +332,14 @@ inline Experimental::IntersectsBoxes intersects(DeviceVector<Box> const &b)
 inline Experimental::PointsNearestK nearest(DeviceVector<Point> const &p, int k = 1)
 {
   return {p.data(), p.size(), k};
+}
+inline Experimental::IntersectsRays intersects(DeviceVector<Experimental::Ray> const &r)
+{
+  return {r.data(), r.size()};
+}
+inline Experimental::RaysNearestK nearest(DeviceVector<Experimental::Ray> const &r, int k = 1)
+{
+  return {r.data(), r.size(), k};
 }
 
 template <class T>
@@ -395,6 +466,29 @@ public:
           c_policy(policy), offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
     finish(space, indices, ind, nnz, offsets, preds.n);
   }
+  // Rays over a tree of boxes: every box the ray meets (rows in traversal
+  // order), the same boxes by increasing (distance, index), or the first k
+  // of those (rows may be shorter than k).
+  void query(HipStream const &space, Experimental::IntersectsRays const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    rays_impl(space, AXB_PREDICATE_RAYS, preds.rays, preds.n, -1, indices, offsets, policy);
+  }
+  void query(HipStream const &space, Experimental::OrderedIntersectsRays const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    rays_impl(space, AXB_PREDICATE_ORDERED_RAYS, preds.rays, preds.n, -1, indices, offsets,
+              policy);
+  }
+  void query(HipStream const &space, Experimental::RaysNearestK const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    rays_impl(space, AXB_PREDICATE_ORDERED_RAYS, preds.rays, preds.n, preds.k, indices, offsets,
+              policy);
+  }
   void query(HipStream const &space, Experimental::PointsNearestK const &preds,
              DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
              Experimental::TraversalPolicy const &policy = {}) const
@@ -467,6 +561,24 @@ private:
       return;
     }
     indices.adopt(ind, (size_t)nnz, space.stream);
+  }
+  // kind: AXB_PREDICATE_RAYS or _ORDERED_RAYS; k >= 0: nearest(ray, k)
+  void rays_impl(HipStream const &space, int32_t kind, Experimental::Ray const *rays, size_t n,
+                 int k, DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+                 Experimental::TraversalPolicy const &policy) const
+  {
+    prepare(offsets, n);
+    int32_t *ind = nullptr;
+    int64_t nnz = 0;
+    if (_h && k >= 0)
+      Details::check(axb_bvh_query_nearest_rays(_h, (float const *)rays, (int64_t)n, k,
+                                                c_policy(policy), offsets.data(), &ind, &nnz,
+                                                (axb_stream_t)space.stream));
+    else if (_h)
+      Details::check(axb_bvh_query_intersects(_h, kind, (float const *)rays, (int64_t)n,
+                                              c_policy(policy), offsets.data(), &ind, &nnz,
+                                              (axb_stream_t)space.stream));
+    finish(space, indices, ind, nnz, offsets, n);
   }
   void nearest_impl(HipStream const &space, Experimental::PointsNearestK const &preds,
                     DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
@@ -584,6 +696,21 @@ public:
     if (_h)
       Details::check(axb_bruteforce_query_intersects(
           _h, AXB_PREDICATE_BOXES, (float const *)preds.boxes, (int64_t)preds.n,
+          c_policy(policy), offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
+    finish(space, indices, ind, nnz, offsets, preds.n);
+  }
+  // Intersects<Ray> over boxes (the reference's brute force has no ordered
+  // traversal).
+  void query(HipStream const &space, Experimental::IntersectsRays const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    prepare(offsets, preds.n);
+    int32_t *ind = nullptr;
+    int64_t nnz = 0;
+    if (_h)
+      Details::check(axb_bruteforce_query_intersects(
+          _h, AXB_PREDICATE_RAYS, (float const *)preds.rays, (int64_t)preds.n,
           c_policy(policy), offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
     finish(space, indices, ind, nnz, offsets, preds.n);
   }

@@ -64,6 +64,15 @@ struct Nearest
   int k;
   int index;
 };
+// Experimental::OrderedSpatial<Ray> (detail/ArborX_Predicates.hpp:104-126):
+// the callback sees the boxes the ray hits in increasing (distance, index)
+// order and may stop the walk with early_exit (first hit).
+template <class Geometry>
+struct OrderedSpatial
+{
+  Geometry geometry;
+  int index;
+};
 template <class P>
 ARBORX_HD auto const &getGeometry(P const &p)
 {
@@ -99,6 +108,10 @@ struct DeviceTree
   // bvh.query(PerThread, predicate, callback) (ArborX_LinearBVH.hpp:112-122)
   template <class Geometry, class Callback>
   __device__ void query(Experimental::PerThread, Intersects<Geometry> const &predicate,
+                        Callback const &callback) const;
+  template <class Callback>
+  __device__ void query(Experimental::PerThread,
+                        OrderedSpatial<Experimental::Ray> const &predicate,
                         Callback const &callback) const;
 };
 
@@ -157,9 +170,15 @@ __device__ __forceinline__ axb::SpatialPred spatial_of(Box const &b)
       {b._min_corner._coords[0], b._min_corner._coords[1], b._min_corner._coords[2]},
       {b._max_corner._coords[0], b._max_corner._coords[1], b._max_corner._coords[2]}});
 }
+__device__ __forceinline__ axb::SpatialPred spatial_of(Experimental::Ray const &r)
+{
+  return axb::ray_spatial_pred((float const *)&r);
+}
 template <class Geometry>
 constexpr int pred_kind()
 {
+  if constexpr (std::is_same_v<Geometry, Experimental::Ray>)
+    return axb::PRED_RAYS;
   return std::is_same_v<Geometry, Box> ? axb::PRED_BOXES : axb::PRED_SPHERES;
 }
 
@@ -175,7 +194,20 @@ __device__ void spatial_walk(DeviceTree const &t, Intersects<Geometry> const &pr
   CallbackSink<Intersects<Geometry>, Callback> sink{pred, cb};
   uint32_t vi = 0, vl = 0;
   constexpr int PK = pred_kind<Geometry>();
-  if (t.format == AXB_NODES_Q16)
+  if constexpr (PK == axb::PRED_RAYS)
+  {
+    // trees of boxes only (checked on the host)
+    if (t.geometry != AXB_GEOMETRY_BOXES)
+      return;
+    if (t.format == AXB_NODES_Q16)
+      axb::rope_walk<axb::NODES_Q16, axb::GEOM_BOXES, PK>(t, sp, 0u, t.total, ~0u, sink, vi,
+                                                          vl);
+    else
+      axb::rope_walk<axb::NODES_FP32, axb::GEOM_BOXES, PK>(t, sp, 0u, t.total, ~0u, sink, vi,
+                                                           vl);
+    return;
+  }
+  else if (t.format == AXB_NODES_Q16)
   {
     if (t.geometry == AXB_GEOMETRY_POINTS)
       axb::rope_walk<axb::NODES_Q16, axb::GEOM_POINTS, PK>(t, sp, 0u, t.total, ~0u, sink, vi,
@@ -193,6 +225,25 @@ __device__ void spatial_walk(DeviceTree const &t, Intersects<Geometry> const &pr
       axb::rope_walk<axb::NODES_FP32, axb::GEOM_BOXES, PK>(t, sp, 0u, t.total, ~0u, sink, vi,
                                                            vl);
   }
+}
+
+// One ray, best-first over a tree of boxes (TreeTraversal.hpp:338-489): the
+// callback gets the boxes the ray hits by increasing (distance, index) and
+// may stop the walk.  A walk whose heap of 64 pending entries overflows
+// stops and raises KERR_STACK in t.err (when set).
+template <class Callback>
+__device__ void spatial_walk(DeviceTree const &t,
+                             OrderedSpatial<Experimental::Ray> const &pred, Callback const &cb)
+{
+  if (t.n <= 0 || t.geometry != AXB_GEOMETRY_BOXES)
+    return;
+  const axb::RayPred ray = axb::ray_pred((float const *)&pred.geometry);
+  CallbackSink<OrderedSpatial<Experimental::Ray>, Callback> sink{pred, cb};
+  uint32_t vi = 0, vl = 0;
+  if (t.format == AXB_NODES_Q16)
+    axb::ordered_walk<axb::NODES_Q16>(t, ray, sink, vi, vl);
+  else
+    axb::ordered_walk<axb::NODES_FP32>(t, ray, sink, vi, vl);
 }
 
 // Device view of a brute-force index (axb_bruteforce_view).
@@ -228,13 +279,16 @@ __device__ void spatial_walk(BruteForceView const &t, Intersects<Geometry> const
   constexpr int PK = pred_kind<Geometry>();
   if (t.geometry == AXB_GEOMETRY_POINTS)
   {
-    for (int64_t j = 0; j < t.n; ++j)
-    {
-      const float *v = t.values + 3 * j;
-      const uint4 s0 = make_uint4(axb::f_bits(v[0]), axb::f_bits(v[1]), axb::f_bits(v[2]), 0u);
-      if (axb::leaf_hit<axb::GEOM_POINTS, PK>(sp, s0, s0) && !sink.hit((int32_t)j))
-        return;
-    }
+    // (rays: boxes only, checked on the host)
+    if constexpr (PK != axb::PRED_RAYS)
+      for (int64_t j = 0; j < t.n; ++j)
+      {
+        const float *v = t.values + 3 * j;
+        const uint4 s0 =
+            make_uint4(axb::f_bits(v[0]), axb::f_bits(v[1]), axb::f_bits(v[2]), 0u);
+        if (axb::leaf_hit<axb::GEOM_POINTS, PK>(sp, s0, s0) && !sink.hit((int32_t)j))
+          return;
+      }
   }
   else
   {
@@ -268,6 +322,21 @@ __device__ __forceinline__ Nearest<Point> predicate_at(Experimental::PointsNeare
 {
   return {b.points[i], b.k, i};
 }
+__device__ __forceinline__ Intersects<Experimental::Ray>
+predicate_at(Experimental::IntersectsRays b, int i)
+{
+  return {b.rays[i], i};
+}
+__device__ __forceinline__ OrderedSpatial<Experimental::Ray>
+predicate_at(Experimental::OrderedIntersectsRays b, int i)
+{
+  return {b.rays[i], i};
+}
+__device__ __forceinline__ Nearest<Experimental::Ray> predicate_at(Experimental::RaysNearestK b,
+                                                                   int i)
+{
+  return {b.rays[i], b.k, i};
+}
 
 inline int32_t c_kind(Experimental::PointsWithRadius const &) { return 2; }
 inline int32_t c_kind(Experimental::IntersectsSpheres const &) { return AXB_PREDICATE_SPHERES; }
@@ -284,12 +353,40 @@ inline float const *c_data(Experimental::IntersectsBoxes const &b)
 {
   return (float const *)b.boxes;
 }
+inline int32_t c_kind(Experimental::IntersectsRays const &) { return AXB_PREDICATE_RAYS; }
+inline int32_t c_kind(Experimental::OrderedIntersectsRays const &)
+{
+  return AXB_PREDICATE_ORDERED_RAYS;
+}
+inline float const *c_data(Experimental::IntersectsRays const &b)
+{
+  return (float const *)b.rays;
+}
+inline float const *c_data(Experimental::OrderedIntersectsRays const &b)
+{
+  return (float const *)b.rays;
+}
 
 template <class Batch>
 struct is_spatial_batch
     : std::bool_constant<std::is_same_v<Batch, Experimental::PointsWithRadius> ||
                          std::is_same_v<Batch, Experimental::IntersectsSpheres> ||
-                         std::is_same_v<Batch, Experimental::IntersectsBoxes>>
+                         std::is_same_v<Batch, Experimental::IntersectsBoxes> ||
+                         std::is_same_v<Batch, Experimental::IntersectsRays> ||
+                         std::is_same_v<Batch, Experimental::OrderedIntersectsRays>>
+{
+};
+template <class Batch>
+struct is_ray_batch
+    : std::bool_constant<std::is_same_v<Batch, Experimental::IntersectsRays> ||
+                         std::is_same_v<Batch, Experimental::OrderedIntersectsRays> ||
+                         std::is_same_v<Batch, Experimental::RaysNearestK>>
+{
+};
+template <class Batch>
+struct is_nearest_batch
+    : std::bool_constant<std::is_same_v<Batch, Experimental::PointsNearestK> ||
+                         std::is_same_v<Batch, Experimental::RaysNearestK>>
 {
 };
 
@@ -356,9 +453,9 @@ __global__ __launch_bounds__(256) void k_spatial_insert(View t, Batch batch, int
 }
 
 // ---- nearest: the rows of the library's kNN, callback per value in order
-template <class Callback>
-__global__ __launch_bounds__(256) void k_nearest_callback(Experimental::PointsNearestK batch,
-                                                          int64_t nq, const int64_t *offsets,
+template <class Batch, class Callback>
+__global__ __launch_bounds__(256) void k_nearest_callback(Batch batch, int64_t nq,
+                                                          const int64_t *offsets,
                                                           const int32_t *indices, Callback cb)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -368,9 +465,9 @@ __global__ __launch_bounds__(256) void k_nearest_callback(Experimental::PointsNe
   for (int64_t j = offsets[i]; j < offsets[i + 1]; ++j)
     cb(pred, indices[j]);
 }
-template <class Callback, class T, bool FILL>
-__global__ __launch_bounds__(256) void k_nearest_insert(Experimental::PointsNearestK batch,
-                                                        int64_t nq, const int64_t *knn_off,
+template <class Batch, class Callback, class T, bool FILL>
+__global__ __launch_bounds__(256) void k_nearest_insert(Batch batch, int64_t nq,
+                                                        const int64_t *knn_off,
                                                         const int32_t *knn_ind, Callback cb,
                                                         int64_t *offsets, T *out)
 {
@@ -420,6 +517,50 @@ inline DeviceTree index_view(BoundingVolumeHierarchy const &i)
   return device_tree(i.c_handle());
 }
 inline BruteForceView index_view(BruteForce const &i) { return bruteforce_view(i.c_handle()); }
+
+// Ray predicates need an index of boxes (the reference defines the ray
+// tests against boxes only).
+template <class Batch, class View>
+void check_ray_geometry(View const &t)
+{
+  if constexpr (is_ray_batch<Batch>::value)
+    if (t.n > 0 && t.geometry != AXB_GEOMETRY_BOXES)
+      throw std::invalid_argument("query: ray predicates need an index of boxes");
+}
+
+// (a template, so that every translation unit may instantiate it)
+template <class I>
+__global__ void k_zero(I *p, int64_t n)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = 0;
+}
+
+// Error word of an ordered walk (heap overflow), read after the launch.
+struct WalkErrors
+{
+  DeviceVector<uint32_t> word;
+  void attach(HipStream const &space, DeviceTree &t)
+  {
+    word.resize(1);
+    k_zero<uint32_t><<<1, 64, 0, space.stream>>>(word.data(), 1);
+    launch_check("query(callback): error word");
+    t.err = word.data();
+  }
+  void attach(HipStream const &, BruteForceView &) {}
+  void check_after(HipStream const &space)
+  {
+    if (word.size() == 0)
+      return;
+    uint32_t h = 0;
+    hip_check(hipMemcpyAsync(&h, word.data(), sizeof(h), hipMemcpyDeviceToHost, space.stream),
+              "hipMemcpyAsync");
+    space.fence();
+    if (h & axb::KERR_STACK)
+      throw SearchException("ordered query: a ray's walk exceeded its heap of 64 entries");
+  }
+};
 template <class Batch>
 DeviceVector<uint32_t> index_order(BoundingVolumeHierarchy const &i, HipStream const &space,
                                    Batch const &b, Experimental::TraversalPolicy const &policy)
@@ -442,33 +583,28 @@ void callback_query(Index const &index, HipStream const &space, Batch const &b,
   if constexpr (is_spatial_batch<Batch>::value)
   {
     auto t = index_view(index);
+    check_ray_geometry<Batch>(t);
+    WalkErrors errs;
+    if constexpr (std::is_same_v<Batch, Experimental::OrderedIntersectsRays>)
+      errs.attach(space, t);
     DeviceVector<uint32_t> order = index_order(index, space, b, policy);
     k_spatial_callback<<<blocks_of(b.n), 256, 0, space.stream>>>(t, b, (int64_t)b.n,
                                                                   order.data(), cb);
     launch_check("query(callback)");
     space.fence();
+    errs.check_after(space);
   }
   else
   {
-    static_assert(std::is_same_v<Batch, Experimental::PointsNearestK>,
-                  "unsupported predicates");
+    static_assert(is_nearest_batch<Batch>::value, "unsupported predicates");
     DeviceVector<int> ind;
     DeviceVector<int64_t> off;
     index.query(space, b, ind, off, policy);
-    k_nearest_callback<<<blocks_of(b.n), 256, 0, space.stream>>>(b, (int64_t)b.n, off.data(),
-                                                                  ind.data(), cb);
+    k_nearest_callback<Batch, Callback><<<blocks_of(b.n), 256, 0, space.stream>>>(
+        b, (int64_t)b.n, off.data(), ind.data(), cb);
     launch_check("query(nearest callback)");
     space.fence();
   }
-}
-
-// (a template, so that every translation unit may instantiate it)
-template <class I>
-__global__ void k_zero(I *p, int64_t n)
-{
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n)
-    p[i] = 0;
 }
 
 template <class Index, class Batch, class Callback, class T>
@@ -492,6 +628,10 @@ void callback_query_crs(Index const &index, HipStream const &space, Batch const 
   if constexpr (is_spatial_batch<Batch>::value)
   {
     auto t = index_view(index);
+    check_ray_geometry<Batch>(t);
+    WalkErrors errs;
+    if constexpr (std::is_same_v<Batch, Experimental::OrderedIntersectsRays>)
+      errs.attach(space, t);
     DeviceVector<uint32_t> order = index_order(index, space, b, policy);
     k_spatial_insert<decltype(t), Batch, Callback, T, false>
         <<<blocks_of(nq), 256, 0, space.stream>>>(t, b, nq, order.data(), cb, offsets.data(),
@@ -513,15 +653,15 @@ void callback_query_crs(Index const &index, HipStream const &space, Batch const 
       launch_check("query(callback, values, offsets): fill");
     }
     space.fence();
+    errs.check_after(space);
   }
   else
   {
-    static_assert(std::is_same_v<Batch, Experimental::PointsNearestK>,
-                  "unsupported predicates");
+    static_assert(is_nearest_batch<Batch>::value, "unsupported predicates");
     DeviceVector<int> ind;
     DeviceVector<int64_t> off;
     index.query(space, b, ind, off, policy);
-    k_nearest_insert<Callback, T, false><<<blocks_of(nq), 256, 0, space.stream>>>(
+    k_nearest_insert<Batch, Callback, T, false><<<blocks_of(nq), 256, 0, space.stream>>>(
         b, nq, off.data(), ind.data(), cb, offsets.data(), (T *)nullptr);
     launch_check("query(nearest callback, values, offsets): count");
     check(axb_exclusive_scan_i64(offsets.data(), offsets.data(), nq + 1,
@@ -534,7 +674,7 @@ void callback_query_crs(Index const &index, HipStream const &space, Batch const 
     values.resize((size_t)nnz);
     if (nnz > 0)
     {
-      k_nearest_insert<Callback, T, true><<<blocks_of(nq), 256, 0, space.stream>>>(
+      k_nearest_insert<Batch, Callback, T, true><<<blocks_of(nq), 256, 0, space.stream>>>(
           b, nq, off.data(), ind.data(), cb, offsets.data(), values.data());
       launch_check("query(nearest callback, values, offsets): fill");
     }
@@ -548,6 +688,13 @@ __device__ void DeviceTree::query(Experimental::PerThread, Intersects<Geometry> 
 {
   spatial_walk(*this, predicate, callback);
 }
+template <class Callback>
+__device__ void DeviceTree::query(Experimental::PerThread,
+                                  OrderedSpatial<Experimental::Ray> const &predicate,
+                                  Callback const &callback) const
+{
+  spatial_walk(*this, predicate, callback);
+}
 
 } // namespace Details
 
@@ -556,6 +703,13 @@ __device__ void DeviceTree::query(Experimental::PerThread, Intersects<Geometry> 
 template <class Geometry, class Callback>
 __device__ void query(Experimental::PerThread, Details::DeviceTree const &tree,
                       Intersects<Geometry> const &predicate, Callback const &callback)
+{
+  Details::spatial_walk(tree, predicate, callback);
+}
+template <class Callback>
+__device__ void query(Experimental::PerThread, Details::DeviceTree const &tree,
+                      OrderedSpatial<Experimental::Ray> const &predicate,
+                      Callback const &callback)
 {
   Details::spatial_walk(tree, predicate, callback);
 }

@@ -256,6 +256,147 @@ static void random_points(HipStream const &space, NodeFormat fmt)
   CHECK(off2.to_host() == ho && vals.to_host() == hi, "default-callback CRS equals query");
 }
 
+// First hit of each ray: the ordered callback stops at once.
+struct FirstHit
+{
+  int *first;
+  __device__ CallbackTreeTraversalControl operator()(
+      OrderedSpatial<Experimental::Ray> const &p, int v) const
+  {
+    first[getData(p)] = v;
+    return CallbackTreeTraversalControl::early_exit;
+  }
+};
+
+// query(PerThread, ordered ray, callback) from a user kernel: the first two
+// hits of each ray.
+__global__ void per_thread_rays(Details::DeviceTree tree, Experimental::Ray const *rays, int nq,
+                                int *firsts)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq)
+    return;
+  int c = 0;
+  firsts[2 * i] = firsts[2 * i + 1] = -1;
+  query(Experimental::PerThread{}, tree, OrderedSpatial<Experimental::Ray>{rays[i], i},
+        [&](OrderedSpatial<Experimental::Ray> const &, int v) {
+          firsts[2 * i + c] = v;
+          return ++c < 2 ? CallbackTreeTraversalControl::normal_continuation
+                         : CallbackTreeTraversalControl::early_exit;
+        });
+}
+
+// Ray callbacks over random boxes against the library's CRS rows: the
+// ordered callback's first hit and first two hits (PerThread), the CRS
+// built through an inserting ordered callback (same order), the void
+// Intersects<Ray> callback counts, and nearest(ray, k) per-value callbacks.
+static void random_rays(HipStream const &space, NodeFormat fmt)
+{
+  using Experimental::Ray;
+  int const n = 20000, nq = 4000;
+  std::mt19937 gen(11);
+  std::uniform_real_distribution<float> U(0.f, 1.f), O(-0.2f, 1.2f);
+  std::normal_distribution<float> Nd(0.f, 1.f);
+  std::vector<Box> hb(n);
+  for (auto &b : hb)
+  {
+    const float x = U(gen), y = U(gen), z = U(gen), e = 0.002f + 0.03f * U(gen);
+    b = Box({{x, y, z}}, {{x + e, y + e, z + e}});
+  }
+  std::vector<Ray> hr(nq);
+  for (int q = 0; q < nq; ++q)
+  {
+    Point dir{{Nd(gen), Nd(gen), Nd(gen)}};
+    if (q % 5 == 1)
+      dir[(q / 5) % 3] = 0.f;
+    hr[q] = Ray({{O(gen), O(gen), O(gen)}}, dir);
+  }
+  DeviceVector<Box> boxes(hb);
+  DeviceVector<Ray> rays(hr);
+  BoundingVolumeHierarchy tree(space, boxes, fmt);
+  DeviceVector<int> I;
+  DeviceVector<int64_t> Off;
+  tree.query(space, Experimental::ordered_intersects(rays), I, Off);
+  auto ho = Off.to_host();
+  auto hi = I.to_host();
+  CHECK(ho.back() > nq, "rays hit boxes");
+
+  DeviceVector<int> first(std::vector<int>(nq, -1));
+  tree.query(space, Experimental::ordered_intersects(rays), FirstHit{first.data()});
+  auto hf = first.to_host();
+  DeviceVector<int> two(2 * nq);
+  per_thread_rays<<<(nq + 255) / 256, 256, 0, space.stream>>>(tree.device_view(), rays.data(),
+                                                              nq, two.data());
+  Details::hip_check(hipStreamSynchronize(space.stream), "per_thread_rays");
+  auto h2 = two.to_host();
+  bool ok1 = true, ok2 = true;
+  for (int q = 0; q < nq; ++q)
+  {
+    const int64_t m = ho[q + 1] - ho[q];
+    ok1 = ok1 && hf[q] == (m > 0 ? hi[ho[q]] : -1);
+    ok2 = ok2 && h2[2 * q] == (m > 0 ? hi[ho[q]] : -1) && h2[2 * q + 1] == (m > 1 ? hi[ho[q] + 1] : -1);
+  }
+  CHECK(ok1, "ordered callback with early exit: the first hit");
+  CHECK(ok2, "query(PerThread) ordered: the first two hits");
+
+  DeviceVector<int> vals;
+  DeviceVector<int64_t> off2;
+  tree.query(
+      space, Experimental::ordered_intersects(rays),
+      [] __device__(OrderedSpatial<Ray> const &, int v, auto const &out) { out(v); }, vals, off2);
+  CHECK(off2.to_host() == ho && vals.to_host() == hi, "ordered inserting callback equals CRS");
+
+  DeviceVector<int> counts(std::vector<int>(nq, 0));
+  tree.query(space, intersects(rays), CountHits{counts.data()});
+  auto hc = counts.to_host();
+  bool ok = true;
+  for (int q = 0; q < nq; ++q)
+    ok = ok && hc[q] == ho[q + 1] - ho[q];
+  CHECK(ok, "Intersects<Ray> callback counts equal the ordered row lengths");
+  if (fmt == NodeFormat::FP32)
+  {
+    BruteForce bf(space, boxes);
+    DeviceVector<int> bcounts(std::vector<int>(nq, 0));
+    bf.query(space, intersects(rays), CountHits{bcounts.data()});
+    CHECK(bcounts.to_host() == hc, "brute force: Intersects<Ray> callback counts");
+  }
+
+  DeviceVector<int> NI;
+  DeviceVector<int64_t> NO;
+  tree.query(space, nearest(rays, 3), NI, NO);
+  DeviceVector<int> nvals;
+  DeviceVector<int64_t> noff;
+  tree.query(
+      space, nearest(rays, 3),
+      [] __device__(Nearest<Ray> const &, int v, auto const &out) { out(v); }, nvals, noff);
+  CHECK(noff.to_host() == NO.to_host() && nvals.to_host() == NI.to_host(),
+        "nearest(ray) inserting callback equals its CRS");
+  auto nho = NO.to_host();
+  auto nhi = NI.to_host();
+  ok = true;
+  for (int q = 0; q < nq && ok; ++q)
+  {
+    const int64_t m = std::min<int64_t>(3, ho[q + 1] - ho[q]);
+    ok = nho[q + 1] - nho[q] == m && std::equal(nhi.begin() + nho[q], nhi.begin() + nho[q + 1],
+                                                hi.begin() + ho[q]);
+  }
+  CHECK(ok, "nearest(ray, 3) rows are the first 3 ordered hits");
+
+  // rays need boxes
+  DeviceVector<Point> pts(std::vector<Point>{{{0.f, 0.f, 0.f}}, {{1.f, 1.f, 1.f}}});
+  BoundingVolumeHierarchy ptree(space, pts, fmt);
+  bool threw = false;
+  try
+  {
+    ptree.query(space, intersects(rays), CountHits{counts.data()});
+  }
+  catch (std::invalid_argument const &)
+  {
+    threw = true;
+  }
+  CHECK(threw, "ray callbacks over points throw");
+}
+
 // The brute-force index's callbacks against its own CRS query: counts per
 // predicate, and the CRS built through an inserting callback equals it,
 // row order included (values in index order on both sides).
@@ -331,6 +472,7 @@ int main()
     reference_cases(space, MakeBVH{fmt});
     early_exit(space, MakeBVH{fmt});
     random_points(space, fmt);
+    random_rays(space, fmt);
   }
   reference_cases(space, MakeBruteForce{});
   early_exit(space, MakeBruteForce{});

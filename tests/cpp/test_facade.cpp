@@ -341,6 +341,62 @@ static void neighbor_lists(HipStream const &space)
   }
 }
 
+// test/tstQueryTreeRay.cpp: ten unit boxes on the diagonal.  Ordered rows
+// are compared as sequences.
+template <class Make>
+static void rays(HipStream const &space, Make make)
+{
+  using Experimental::Ray;
+  std::vector<Box> diag;
+  for (int i = 0; i < 10; ++i)
+    diag.push_back(Box({{(float)i, (float)i, (float)i}},
+                       {{(float)i + 1, (float)i + 1, (float)i + 1}}));
+  DeviceVector<Box> v(diag);
+  auto index = make(space, v);
+  DeviceVector<int> ind;
+  DeviceVector<int64_t> off;
+  // :60-87 intersects(ray)
+  DeviceVector<Ray> r1(std::vector<Ray>{Ray({{0, 0, 0}}, {{.1f, .1f, .1f}})});
+  index.query(space, intersects(r1), ind, off);
+  CHECK(rows_of(ind, off) == rows({{0, 1, 2, 3, 4, 5, 6, 7, 8, 9}}), "rays: intersects");
+  // nearest and ordered ray queries: trees only, as in the reference
+  if constexpr (std::is_same_v<typename Make::index_type, BoundingVolumeHierarchy>)
+  {
+    // :29-58 nearest(ray, 1)
+    DeviceVector<Ray> r0(std::vector<Ray>{Ray({{0, 0, 0}}, {{.15f, .1f, 0.f}})});
+    index.query(space, nearest(r0, 1), ind, off);
+    CHECK(rows_of(ind, off) == rows({{0}}), "rays: nearest k = 1");
+    // :112-163 ordered intersections, forwards and backwards
+    DeviceVector<Ray> r2(std::vector<Ray>{Ray({{0, 0, 0}}, {{.1f, .1f, .1f}}),
+                                          Ray({{10, 10, 10}}, {{-.1f, -.1f, -.1f}})});
+    index.query(space, Experimental::make_ordered_intersects(r2), ind, off);
+    std::vector<int> want;
+    for (int i = 0; i < 10; ++i)
+      want.push_back(i);
+    for (int i = 9; i >= 0; --i)
+      want.push_back(i);
+    CHECK(ind.to_host() == want && (off.to_host() == std::vector<int64_t>{0, 10, 20}),
+          "rays: ordered intersections in order");
+    index.query(space, nearest(r2, 3), ind, off);
+    CHECK((ind.to_host() == std::vector<int>{0, 1, 2, 9, 8, 7}), "rays: nearest k = 3 in order");
+    // :212-243 one leaf; :185-210 empty tree
+    DeviceVector<Box> one(std::vector<Box>{Box({{0, 0, 0}}, {{1, 1, 1}})});
+    auto leaf = make(space, one);
+    DeviceVector<Ray> r3(std::vector<Ray>{Ray({{0, 0, 0}}, {{1, 1, 1}}),
+                                          Ray({{4, 5, 6}}, {{7, 8, 9}})});
+    leaf.query(space, Experimental::ordered_intersects(r3), ind, off);
+    CHECK(rows_of(ind, off) == rows({{0}, {}}), "rays: one leaf");
+    typename Make::index_type none;
+    none.query(space, Experimental::ordered_intersects(r3), ind, off);
+    CHECK(rows_of(ind, off) == rows({{}, {}}), "rays: empty tree");
+    // the constructor normalises in double; zero components are +0
+    Ray a({{0, 0, 0}}, {{3.f, -0.f, 4.f}});
+    CHECK(a.direction()[0] == 0.6f && a.direction()[2] == 0.8f &&
+              !std::signbit(a.direction()[1]),
+          "rays: normalised direction");
+  }
+}
+
 int main()
 {
   HipStream space;
@@ -350,7 +406,9 @@ int main()
     callbacks_nearest(space, MakeBVH{fmt});
     manufactured(space, MakeBVH{fmt});
     brute_force(space, MakeBVH{fmt});
+    rays(space, MakeBVH{fmt});
   }
+  rays(space, MakeBruteForce{});
   policy(space, MakeBVH{NodeFormat::FP32});
   degenerate(space, MakeBruteForce{});
   callbacks_nearest(space, MakeBruteForce{});
