@@ -692,7 +692,13 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
     if (st != AXB_OK)
       return st;
   }
+  // whole ordered rows: the rope walk collects the slots of the hit leaves
+  // and each row is then sorted by (distance, index); the best-first walk
+  // is kept for rows cut at `limit` (it stops early)
+  const bool sorted_rows = ordered && limit == INT32_MAX;
   auto first = [&](int mode, int32_t *out, int32_t B) {
+    if (sorted_rows)
+      return launch_ray_slots(tv, t->node_format, preds, nq, order, offsets, out, B, mode, s);
     if (ordered)
       return launch_ordered(tv, t->node_format, preds, nq, limit, order, offsets, out, B, mode,
                             s);
@@ -701,6 +707,9 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
   };
   // the overflow list holds rows; the fill pass takes them as its order
   auto fill = [&](const uint32_t *list, int64_t n_list, int32_t *out) {
+    if (sorted_rows)
+      return launch_ray_slots(tv, t->node_format, preds, n_list, list ? list : order, offsets,
+                              out, 0, MODE_FILL, s);
     if (ordered)
       return launch_ordered(tv, t->node_format, preds, n_list, limit, list ? list : order,
                             offsets, out, 0, MODE_FILL, s);
@@ -710,6 +719,18 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
   axb_status st = crs_query(t->d_err, nq, policy, order, false, first, fill, offsets, indices,
                             nnz_host, s, "query");
   dfree(order_blk, s);
+  if (st == AXB_OK && sorted_rows && *nnz_host > 0)
+  {
+    hipError_t e = launch_ray_sort_rows(tv, preds, nq, offsets, *indices, s);
+    if (e == hipSuccess)
+      e = sync_stream(s);
+    if (e != hipSuccess)
+    {
+      out_free(*indices, s);
+      *indices = nullptr;
+      return hip_fail(e, "query: ordered rows");
+    }
+  }
   return st;
 }
 

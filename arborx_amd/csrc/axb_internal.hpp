@@ -248,6 +248,14 @@ hipError_t launch_spatial(const TreeView &tv, int node_format, int geometry, int
 hipError_t launch_ordered(const TreeView &tv, int node_format, const float *rays, int64_t nq,
                           int32_t limit, const uint32_t *order, int64_t *offsets, int32_t *out,
                           int32_t buffer, int mode, hipStream_t s);
+// Intersects<Ray> rows holding the slots of the hit leaf records (box
+// trees), and their conversion to rows of values in (distance, index)
+// order: the rows of an ordered query without the best-first walk.
+hipError_t launch_ray_slots(const TreeView &tv, int node_format, const float *rays, int64_t nq,
+                            const uint32_t *order, int64_t *offsets, int32_t *out,
+                            int32_t buffer, int mode, hipStream_t s);
+hipError_t launch_ray_sort_rows(const TreeView &tv, const float *rays, int64_t nq,
+                                const int64_t *offsets, int32_t *rows, hipStream_t s);
 // Ray construction: rays[i] = {origins[i], directions[i] / |directions[i]|}
 // with the norm in double; a zero direction raises KERR_LAYOUT in err.
 hipError_t launch_make_rays(const float *origins, const float *directions, int64_t n,

@@ -36,7 +36,7 @@ namespace
 
 // One lane per predicate (in `order` when given): rope traversal of the
 // whole tree; row q of the CRS (counts, scratch rows or the final rows).
-template <int F, int G, int PK, int MODE>
+template <int F, int G, int PK, int MODE, bool SLOTS = false>
 __global__ __launch_bounds__(256) void k_spatial(
     TreeView tv, const float *__restrict__ preds, int64_t nq, float radius,
     const uint32_t *__restrict__ order, int64_t *__restrict__ offsets,
@@ -64,8 +64,23 @@ __global__ __launch_bounds__(256) void k_spatial(
   rw.base = MODE == MODE_FILL ? offsets[q] : (MODE == MODE_BUFFER ? q * (int64_t)buffer : 0);
   rw.limit = MODE == MODE_FILL ? offsets[q + 1] - rw.base : (int64_t)buffer;
   uint32_t v_int = 0, v_leaf = 0;
-  rope_walk<F, G, PK>(tv, pr, 0u, tv.total, ~0u, rw, v_int, v_leaf);
+  rope_walk<F, G, PK, SLOTS>(tv, pr, 0u, tv.total, ~0u, rw, v_int, v_leaf);
   row_epilogue(tv, rw, offsets, q, v_int, v_leaf);
+}
+
+// Rows of leaf slots hit by each ray -> rows of values in (distance, index)
+// order, one lane per row.
+__global__ __launch_bounds__(256) void k_ray_sort_rows(TreeView tv, const float *__restrict__ rays,
+                                                       int64_t nq,
+                                                       const int64_t *__restrict__ offsets,
+                                                       int32_t *__restrict__ rows)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq)
+    return;
+  const int64_t b = offsets[q], m = offsets[q + 1] - b;
+  if (m > 0)
+    sort_ray_row(tv, ray_pred(rays + 6 * q), rows + b, m);
 }
 
 // Row writer that stops the walk after `limit` hits.
@@ -1243,6 +1258,44 @@ hipError_t launch_knn(const TreeView &tv, int node_format, int geometry, const f
   if (node_format == NODES_Q16 && geometry == GEOM_POINTS)
     return launch_knn_fg<NODES_Q16, GEOM_POINTS>(tv, pts, nq, k, order, oi, od, s);
   return launch_knn_fg<NODES_Q16, GEOM_BOXES>(tv, pts, nq, k, order, oi, od, s);
+}
+
+template <int F>
+static hipError_t launch_ray_slots_f(const TreeView &tv, const float *rays, int64_t nq,
+                                     const uint32_t *order, int64_t *offsets, int32_t *out,
+                                     int32_t buffer, int mode, hipStream_t s)
+{
+  constexpr int G = GEOM_BOXES, PK = PRED_RAYS;
+  if (mode == MODE_COUNT)
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_COUNT, true>), dim3(grid_of(nq)), dim3(256), 0, s, tv,
+               rays, nq, 0.f, order, offsets, out, buffer);
+  else if (mode == MODE_FILL)
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_FILL, true>), dim3(grid_of(nq)), dim3(256), 0, s, tv,
+               rays, nq, 0.f, order, offsets, out, buffer);
+  else
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_BUFFER, true>), dim3(grid_of(nq)), dim3(256), 0, s, tv,
+               rays, nq, 0.f, order, offsets, out, buffer);
+  return hipGetLastError();
+}
+
+hipError_t launch_ray_slots(const TreeView &tv, int node_format, const float *rays, int64_t nq,
+                            const uint32_t *order, int64_t *offsets, int32_t *out,
+                            int32_t buffer, int mode, hipStream_t s)
+{
+  if (nq <= 0)
+    return hipSuccess;
+  if (node_format == NODES_FP32)
+    return launch_ray_slots_f<NODES_FP32>(tv, rays, nq, order, offsets, out, buffer, mode, s);
+  return launch_ray_slots_f<NODES_Q16>(tv, rays, nq, order, offsets, out, buffer, mode, s);
+}
+
+hipError_t launch_ray_sort_rows(const TreeView &tv, const float *rays, int64_t nq,
+                                const int64_t *offsets, int32_t *rows, hipStream_t s)
+{
+  if (nq <= 0)
+    return hipSuccess;
+  AXB_LAUNCH(k_ray_sort_rows, dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq, offsets, rows);
+  return hipGetLastError();
 }
 
 template <int F>

@@ -607,7 +607,9 @@ AXB_HD void rope_walk_points(const TV &tv, const SpatialPred &pr, uint32_t p, ui
   }
 }
 
-template <int F, int G, int PK, class TV, class Sink>
+// SLOTS: the sink receives the slot of each hit leaf record instead of its
+// value (read back by sort_ray_row).
+template <int F, int G, int PK, bool SLOTS = false, class TV, class Sink>
 AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
                      uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
 {
@@ -646,7 +648,8 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
         if (run)
         {
           ++v_leaf;
-          if (p != skip && leaf_hit<G, PK>(pr, a[j], b[j]) && !sink.hit(~(int32_t)a[j].w))
+          if (p != skip && leaf_hit<G, PK>(pr, a[j], b[j]) &&
+              !sink.hit(SLOTS ? (int32_t)p : ~(int32_t)a[j].w))
             stop = true;
           p += SL;
         }
@@ -1154,6 +1157,90 @@ AXB_HD void bitonic_merge(T (&a)[N])
         cas_keys(a[i], a[j]);
     }
 }
+
+// A row of box-leaf slots hit by a ray (rope_walk with SLOTS) becomes the
+// row of their values in increasing (distance, index) order -- the rows of
+// an ordered query without a best-first walk.  Up to 64 entries: keys
+// (distance bits, index) sorted in registers by a bitonic network; longer
+// rows: heap sort in place, keys recomputed from the records.
+template <int N, class TV>
+AXB_HD void sort_ray_row_n(const TV &tv, const RayPred &ray, int32_t *row, int m)
+{
+  uint64_t a[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+  {
+    a[j] = ~0ull;
+    if (j < m)
+    {
+      const uint32_t p = (uint32_t)row[j];
+      const uint4 s0 = tv.slots[p], s1 = tv.slots[p + 1];
+      const Box6 b = {{bits_f(s0.x), bits_f(s0.y), bits_f(s0.z)},
+                      {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
+      a[j] = ordered_key(ray_distance(ray, b), ~s0.w);
+    }
+  }
+  bitonic_sort<N>(a);
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    if (j < m)
+      row[j] = (int32_t)(uint32_t)a[j];
+}
+template <class TV>
+AXB_HD uint64_t ray_leaf_key(const TV &tv, const RayPred &ray, int32_t slot)
+{
+  const uint4 s0 = tv.slots[(uint32_t)slot], s1 = tv.slots[(uint32_t)slot + 1];
+  const Box6 b = {{bits_f(s0.x), bits_f(s0.y), bits_f(s0.z)},
+                  {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
+  return ordered_key(ray_distance(ray, b), ~s0.w);
+}
+template <class TV>
+AXB_HD void sort_ray_row(const TV &tv, const RayPred &ray, int32_t *row, int64_t m)
+{
+  if (m <= 8)
+    return sort_ray_row_n<8>(tv, ray, row, (int)m);
+  if (m <= 16)
+    return sort_ray_row_n<16>(tv, ray, row, (int)m);
+  if (m <= 32)
+    return sort_ray_row_n<32>(tv, ray, row, (int)m);
+  if (m <= 64)
+    return sort_ray_row_n<64>(tv, ray, row, (int)m);
+  // heap sort of the slots by key (max-heap, then the largest to the back)
+  auto sift = [&](int64_t j, int64_t end) {
+    const int32_t v = row[j];
+    const uint64_t kv = ray_leaf_key(tv, ray, v);
+    while (true)
+    {
+      int64_t c = 2 * j + 1;
+      if (c >= end)
+        break;
+      uint64_t kc = ray_leaf_key(tv, ray, row[c]);
+      if (c + 1 < end)
+      {
+        const uint64_t kr = ray_leaf_key(tv, ray, row[c + 1]);
+        if (kr > kc)
+          ++c, kc = kr;
+      }
+      if (kv >= kc)
+        break;
+      row[j] = row[c];
+      j = c;
+    }
+    row[j] = v;
+  };
+  for (int64_t j = m / 2 - 1; j >= 0; --j)
+    sift(j, m);
+  for (int64_t end = m - 1; end > 0; --end)
+  {
+    const int32_t t = row[0];
+    row[0] = row[end];
+    row[end] = t;
+    sift(0, end);
+  }
+  for (int64_t j = 0; j < m; ++j)
+    row[j] = ~(int32_t)tv.slots[(uint32_t)row[j]].w;
+}
+
 
 // Keys row[b, b + N) with entries from c on replaced by ~0.  The whole range
 // lies inside the row's capacity (the caller's contract), so every load is

@@ -265,7 +265,7 @@ static void run_rays(int64_t n, unsigned seed, int node_format)
   std::vector<int32_t> ref(nnz > 0 ? nnz : 1), sref(nnz > 0 ? nnz : 1);
   orc_bruteforce_rays_ordered(v.data(), n, rays.data(), nq, -1, off.data(), ref.data());
   orc_bruteforce_spatial(v.data(), n, 1, 3, rays.data(), nq, soff.data(), sref.data());
-  bool ok_spatial = true, ok_ordered = true, ok_cut = true;
+  bool ok_spatial = true, ok_ordered = true, ok_cut = true, ok_sorted = true;
   for (int64_t q = 0; q < nq; ++q)
   {
     const float *r6 = rays.data() + 6 * q;
@@ -282,6 +282,18 @@ static void run_rays(int64_t n, unsigned seed, int node_format)
     std::sort(row.begin(), row.end());
     ok_spatial = ok_spatial && row.size() == (size_t)(soff[q + 1] - soff[q]) &&
                  std::equal(row.begin(), row.end(), sref.begin() + soff[q]);
+    // slots of the hits, sorted into the ordered row
+    std::vector<int32_t> srow;
+    RowSink ss{&srow};
+    if (node_format == 0)
+      axb::rope_walk<axb::NODES_FP32, axb::GEOM_BOXES, axb::PRED_RAYS, true>(
+          tv, pr, 0u, (uint32_t)total, ~0u, ss, vi, vl);
+    else
+      axb::rope_walk<axb::NODES_Q16, axb::GEOM_BOXES, axb::PRED_RAYS, true>(
+          tv, pr, 0u, (uint32_t)total, ~0u, ss, vi, vl);
+    axb::sort_ray_row(tv, pr.ray, srow.data(), (int64_t)srow.size());
+    ok_sorted = ok_sorted && srow.size() == (size_t)(off[q + 1] - off[q]) &&
+                std::equal(srow.begin(), srow.end(), ref.begin() + off[q]);
     const size_t k = 1 + (size_t)(q % 4);
     LimitSink sc{&cut, k};
     bool fine;
@@ -307,6 +319,9 @@ static void run_rays(int64_t n, unsigned seed, int node_format)
   std::snprintf(what, sizeof what, "ordered rows cut at k: n %lld format %d", (long long)n,
                 node_format);
   CHECK(ok_cut, what);
+  std::snprintf(what, sizeof what, "ordered rows by slot sort: n %lld format %d", (long long)n,
+                node_format);
+  CHECK(ok_sorted, what);
   std::snprintf(what, sizeof what, "ray cases hit boxes: n %lld nnz %lld", (long long)n,
                 (long long)nnz);
   CHECK(n < 1000 || nnz > nq, what);
@@ -316,7 +331,7 @@ static void run_rays(int64_t n, unsigned seed, int node_format)
 int main()
 {
   for (int fmt : {0, 1})
-    for (int64_t n : {1, 2, 3, 5, 9, 40, 700, 6000})
+    for (int64_t n : {1, 2, 3, 5, 9, 40, 700, 6000, 60000})
       run_rays(n, 23u + (unsigned)n, fmt);
   for (int fmt : {0, 1})
   {
