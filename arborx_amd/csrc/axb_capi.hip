@@ -248,7 +248,10 @@ static axb_status hip_fail(hipError_t e, const char *where)
 }
 
 // Reads and clears a kernel error word (the stream must be synchronised).
-static axb_status check_kernel_error(uint32_t *d_err, hipStream_t s, const char *where)
+// `tolerated` (optional): a traversal-stack overflow alone is reported there
+// instead of as an error, for callers that redo the work another way.
+static axb_status check_kernel_error(uint32_t *d_err, hipStream_t s, const char *where,
+                                     uint32_t *tolerated = nullptr)
 {
   uint32_t h = 0;
   hipError_t e = hipMemcpyAsync(&h, d_err, sizeof(h), hipMemcpyDeviceToHost, s);
@@ -256,6 +259,12 @@ static axb_status check_kernel_error(uint32_t *d_err, hipStream_t s, const char 
     e = sync_stream(s);
   if (e != hipSuccess)
     return hip_fail(e, where);
+  if (h && tolerated && h == KERR_STACK)
+  {
+    (void)fill_u32(d_err, 0u, 1, s);
+    *tolerated = h;
+    return AXB_OK;
+  }
   if (h)
   {
     (void)fill_u32(d_err, 0u, 1, s);
@@ -561,7 +570,7 @@ template <class First, class Fill>
 static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy policy,
                             const uint32_t *ovf_order, bool ovf_emit_index, First first,
                             Fill fill, int64_t *offsets, int32_t **indices, int64_t *nnz_host,
-                            hipStream_t s, const char *what)
+                            hipStream_t s, const char *what, uint32_t *tolerated = nullptr)
 {
   const bool hard = policy.buffer_size < 0;
   int64_t B = policy.buffer_size < 0 ? -(int64_t)policy.buffer_size : policy.buffer_size;
@@ -643,7 +652,7 @@ static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy po
   }
   if (st != AXB_OK)
     return st;
-  st = check_kernel_error(d_err, s, what);
+  st = check_kernel_error(d_err, s, what, tolerated);
   if (st != AXB_OK)
   {
     out_free(*indices, s);
@@ -661,7 +670,7 @@ extern "C" {
 static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *preds, int64_t nq,
                                 float radius, axb_traversal_policy policy, int64_t *offsets,
                                 int32_t **indices, int64_t *nnz_host, axb_stream_t stream_,
-                                int32_t limit = INT32_MAX)
+                                int32_t limit = INT32_MAX, uint32_t *tolerated = nullptr)
 {
   if (!t || !offsets || !indices || !nnz_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: NULL argument");
@@ -717,7 +726,7 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
                           list ? list : order, offsets, out, 0, MODE_FILL, s);
   };
   axb_status st = crs_query(t->d_err, nq, policy, order, false, first, fill, offsets, indices,
-                            nnz_host, s, "query");
+                            nnz_host, s, "query", tolerated);
   dfree(order_blk, s);
   if (st == AXB_OK && sorted_rows && *nnz_host > 0)
   {
@@ -752,8 +761,66 @@ axb_status axb_bvh_query_nearest_rays(const axb_bvh *t, const float *rays, int64
                                       axb_stream_t stream)
 {
   // TreeTraversal.hpp:143-178: k < 1 returns nothing
-  return query_spatial(t, AXB_PREDICATE_ORDERED_RAYS, rays, nq, 0.f, policy, offsets, indices,
-                       nnz_host, stream, k > 0 ? k : 0);
+  uint32_t tol = 0;
+  axb_status st = query_spatial(t, AXB_PREDICATE_ORDERED_RAYS, rays, nq, 0.f, policy, offsets,
+                                indices, nnz_host, stream, k > 0 ? k : 0, &tol);
+  if (st != AXB_OK || !(tol & KERR_STACK))
+    return st;
+  // A ray's best-first walk outgrew its heap: the rows are then taken as
+  // the first k of the whole ordered rows (rope walk + per-row sort).
+  hipStream_t s = (hipStream_t)stream;
+  out_free(*indices, s);
+  *indices = nullptr;
+  *nnz_host = 0;
+  int64_t *full_off = (int64_t *)dalloc((size_t)(nq + 1) * sizeof(int64_t), s);
+  void *scan_ws = dalloc(scan_workspace_bytes(nq + 1), s);
+  if (!full_off || !scan_ws)
+  {
+    dfree(full_off, s);
+    dfree(scan_ws, s);
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bvh_query_nearest_rays: out of device memory");
+  }
+  int32_t *full_ind = nullptr;
+  int64_t full_nnz = 0;
+  axb_traversal_policy whole = policy;
+  whole.buffer_size = 0; // whole rows may be longer than k
+  st = query_spatial(t, AXB_PREDICATE_ORDERED_RAYS, rays, nq, 0.f, whole, full_off, &full_ind,
+                     &full_nnz, stream);
+  hipError_t e = hipSuccess;
+  int64_t nnz = 0;
+  if (st == AXB_OK)
+  {
+    e = launch_row_prefix_counts(full_off, nq, k, offsets, s);
+    if (e == hipSuccess)
+      e = exclusive_scan_i64(offsets, offsets, nq + 1, scan_ws, t->d_err, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&nnz, offsets + nq, sizeof(nnz), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+      e = sync_stream(s);
+    if (e == hipSuccess)
+    {
+      *indices = (int32_t *)out_alloc((size_t)nnz * sizeof(int32_t), s);
+      if (!*indices)
+        e = hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess)
+      e = launch_row_prefix_copy(full_off, full_ind, nq, offsets, *indices, s);
+    if (e == hipSuccess)
+      e = sync_stream(s);
+  }
+  out_free(full_ind, s);
+  dfree(full_off, s);
+  dfree(scan_ws, s);
+  if (st != AXB_OK)
+    return st;
+  if (e != hipSuccess)
+  {
+    out_free(*indices, s);
+    *indices = nullptr;
+    return hip_fail(e, "axb_bvh_query_nearest_rays");
+  }
+  *nnz_host = nnz;
+  return check_kernel_error(t->d_err, s, "axb_bvh_query_nearest_rays");
 }
 
 axb_status axb_make_rays(const float *origins, const float *directions, int64_t n,

@@ -256,6 +256,12 @@ hipError_t launch_ray_slots(const TreeView &tv, int node_format, const float *ra
                             int32_t buffer, int mode, hipStream_t s);
 hipError_t launch_ray_sort_rows(const TreeView &tv, const float *rays, int64_t nq,
                                 const int64_t *offsets, int32_t *rows, hipStream_t s);
+// Rows cut to their first k entries: counts[q] = min(k, row length) and
+// counts[nq] = 0 (scanned into offsets by the caller), then the copy.
+hipError_t launch_row_prefix_counts(const int64_t *offsets, int64_t nq, int32_t k,
+                                    int64_t *counts, hipStream_t s);
+hipError_t launch_row_prefix_copy(const int64_t *offsets, const int32_t *rows, int64_t nq,
+                                  const int64_t *out_offsets, int32_t *out, hipStream_t s);
 // Ray construction: rays[i] = {origins[i], directions[i] / |directions[i]|}
 // with the norm in double; a zero direction raises KERR_LAYOUT in err.
 hipError_t launch_make_rays(const float *origins, const float *directions, int64_t n,

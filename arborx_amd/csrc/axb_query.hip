@@ -83,6 +83,30 @@ __global__ __launch_bounds__(256) void k_ray_sort_rows(TreeView tv, const float 
     sort_ray_row(tv, ray_pred(rays + 6 * q), rows + b, m);
 }
 
+__global__ __launch_bounds__(256) void k_row_prefix_counts(const int64_t *__restrict__ offsets,
+                                                           int64_t nq, int32_t k,
+                                                           int64_t *__restrict__ counts)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > nq)
+    return;
+  const int64_t m = q < nq ? offsets[q + 1] - offsets[q] : 0;
+  counts[q] = m < (int64_t)k ? m : (int64_t)k;
+}
+__global__ __launch_bounds__(256) void k_row_prefix_copy(const int64_t *__restrict__ offsets,
+                                                         const int32_t *__restrict__ rows,
+                                                         int64_t nq,
+                                                         const int64_t *__restrict__ out_offsets,
+                                                         int32_t *__restrict__ out)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq)
+    return;
+  const int64_t a = offsets[q], b = out_offsets[q], m = out_offsets[q + 1] - b;
+  for (int64_t j = 0; j < m; ++j)
+    out[b + j] = rows[a + j];
+}
+
 // Row writer that stops the walk after `limit` hits.
 template <int MODE>
 struct LimitedRow
@@ -1287,6 +1311,24 @@ hipError_t launch_ray_slots(const TreeView &tv, int node_format, const float *ra
   if (node_format == NODES_FP32)
     return launch_ray_slots_f<NODES_FP32>(tv, rays, nq, order, offsets, out, buffer, mode, s);
   return launch_ray_slots_f<NODES_Q16>(tv, rays, nq, order, offsets, out, buffer, mode, s);
+}
+
+hipError_t launch_row_prefix_counts(const int64_t *offsets, int64_t nq, int32_t k,
+                                    int64_t *counts, hipStream_t s)
+{
+  AXB_LAUNCH(k_row_prefix_counts, dim3(grid_of(nq + 1)), dim3(256), 0, s, offsets, nq, k,
+             counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_prefix_copy(const int64_t *offsets, const int32_t *rows, int64_t nq,
+                                  const int64_t *out_offsets, int32_t *out, hipStream_t s)
+{
+  if (nq <= 0)
+    return hipSuccess;
+  AXB_LAUNCH(k_row_prefix_copy, dim3(grid_of(nq)), dim3(256), 0, s, offsets, rows, nq,
+             out_offsets, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_ray_sort_rows(const TreeView &tv, const float *rays, int64_t nq,

@@ -208,7 +208,7 @@ struct LimitSink
 // Rays against n boxes: the rope walk with ray predicates against the
 // oracle's brute force (sets, tree order), and the best-first walk against
 // the oracle's (distance, index)-sorted rows, whole and cut short.
-static void run_rays(int64_t n, unsigned seed, int node_format)
+static void run_rays(int64_t n, unsigned seed, int node_format, float size = 0.1f)
 {
   std::mt19937 gen(seed);
   std::uniform_real_distribution<float> U(0.f, 1.f), O(-0.2f, 1.2f);
@@ -217,7 +217,7 @@ static void run_rays(int64_t n, unsigned seed, int node_format)
   for (int64_t i = 0; i < n; ++i)
   {
     const float x = U(gen), y = U(gen), z = U(gen);
-    const float e = 0.004f + 0.1f * U(gen) * U(gen);
+    const float e = 0.004f + size * U(gen) * U(gen);
     v[6 * i] = x, v[6 * i + 1] = y, v[6 * i + 2] = z;
     v[6 * i + 3] = x + e, v[6 * i + 4] = y + e * U(gen), v[6 * i + 5] = z + e;
   }
@@ -266,6 +266,7 @@ static void run_rays(int64_t n, unsigned seed, int node_format)
   orc_bruteforce_rays_ordered(v.data(), n, rays.data(), nq, -1, off.data(), ref.data());
   orc_bruteforce_spatial(v.data(), n, 1, 3, rays.data(), nq, soff.data(), sref.data());
   bool ok_spatial = true, ok_ordered = true, ok_cut = true, ok_sorted = true;
+  int overflows = 0;
   for (int64_t q = 0; q < nq; ++q)
   {
     const float *r6 = rays.data() + 6 * q;
@@ -304,7 +305,15 @@ static void run_rays(int64_t n, unsigned seed, int node_format)
       fine = axb::ordered_walk<axb::NODES_Q16>(tv, pr.ray, so, vi, vl) &&
              axb::ordered_walk<axb::NODES_Q16>(tv, pr.ray, sc, vi, vl);
     const size_t m = (size_t)(off[q + 1] - off[q]);
-    ok_ordered = ok_ordered && fine && ord.size() == m &&
+    if (!fine)
+    {
+      // a heap of 64 pending entries was not enough: reported, never wrong
+      ++overflows;
+      ok_ordered = ok_ordered && (err & axb::KERR_STACK) != 0;
+      err &= ~axb::KERR_STACK;
+      continue;
+    }
+    ok_ordered = ok_ordered && ord.size() == m &&
                  std::equal(ord.begin(), ord.end(), ref.begin() + off[q]);
     ok_cut = ok_cut && cut.size() == std::min(k, m) &&
              std::equal(cut.begin(), cut.end(), ref.begin() + off[q]);
@@ -313,9 +322,11 @@ static void run_rays(int64_t n, unsigned seed, int node_format)
   std::snprintf(what, sizeof what, "ray rows (rope walk): n %lld format %d", (long long)n,
                 node_format);
   CHECK(ok_spatial && err == 0, what);
-  std::snprintf(what, sizeof what, "ordered rows (best-first): n %lld format %d", (long long)n,
-                node_format);
+  std::snprintf(what, sizeof what, "ordered rows (best-first): n %lld format %d, %d heap overflows",
+                (long long)n, node_format, overflows);
   CHECK(ok_ordered && err == 0, what);
+  if (overflows)
+    std::printf("note: %s\n", what);
   std::snprintf(what, sizeof what, "ordered rows cut at k: n %lld format %d", (long long)n,
                 node_format);
   CHECK(ok_cut, what);
@@ -331,8 +342,11 @@ static void run_rays(int64_t n, unsigned seed, int node_format)
 int main()
 {
   for (int fmt : {0, 1})
+  {
     for (int64_t n : {1, 2, 3, 5, 9, 40, 700, 6000, 60000})
       run_rays(n, 23u + (unsigned)n, fmt);
+    run_rays(30000, 5u, fmt, 0.4f); // rows of many hundreds
+  }
   for (int fmt : {0, 1})
   {
     for (int64_t n : {1, 2, 3, 4, 5, 6, 9, 17, 100, 1000, 20000})

@@ -88,6 +88,47 @@ def test_ray_rows_match_oracle(fmt):
     assert i.numel() == 0 and host(off).tolist() == [0] * (len(rays_h) + 1)
 
 
+@pytest.mark.parametrize("fmt", ["fp32", "q16"])
+def test_long_ray_rows(fmt):
+    """Large overlapping boxes: rows of hundreds of hits (ordered rows past
+    the 64 sorted in registers take the in-place heap sort; nearest(ray, k)
+    walks long best-first frontiers)."""
+    boxes = synthetic.lognormal_boxes(30_000, 80, mu=math.log(0.08))
+    o, d = ray_inputs(1_000, 81, boxes)
+    rays_h = oracle.make_rays(o, d)
+    rays = ax.Rays.from_normalized(dev(rays_h))
+    bvh = ax.BoundingVolumeHierarchy(dev(boxes), geometry="boxes", node_format=fmt)
+    bo, bi = oracle.bruteforce_rays_ordered(boxes, rays_h)
+    assert np.percentile(np.diff(bo), 50) > 64
+    i, off = bvh.query(ax.ordered_intersects(rays))
+    np.testing.assert_array_equal(host(off), bo)
+    np.testing.assert_array_equal(host(i), bi)
+    ko, ki = oracle.bruteforce_rays_ordered(boxes, rays_h, 100)
+    i, off = bvh.query(ax.nearest(rays, 100))
+    np.testing.assert_array_equal(host(off), ko)
+    np.testing.assert_array_equal(host(i), ki)
+
+
+@pytest.mark.parametrize("fmt", ["fp32", "q16"])
+def test_nearest_rays_past_the_heap(fmt):
+    """Boxes up to 0.4 wide: a best-first walk of many rays needs more than
+    its 64 pending entries (the CPU test of the walk counts them); the query
+    then takes the first k of the whole ordered rows, with the same rows."""
+    g = np.random.default_rng(82)
+    lo = g.uniform(size=(30_000, 3)).astype(np.float32)
+    e = (0.004 + 0.4 * g.uniform(size=(30_000, 1)) * g.uniform(size=(30_000, 1)))
+    boxes = np.concatenate([lo, lo + e.astype(np.float32)], axis=1)
+    o, d = ray_inputs(600, 83, boxes)
+    rays_h = oracle.make_rays(o, d)
+    rays = ax.Rays.from_normalized(dev(rays_h))
+    bvh = ax.BoundingVolumeHierarchy(dev(boxes), geometry="boxes", node_format=fmt)
+    for k in (1, 50, 400):
+        ko, ki = oracle.bruteforce_rays_ordered(boxes, rays_h, k)
+        i, off = bvh.query(ax.nearest(rays, k))
+        np.testing.assert_array_equal(host(off), ko)
+        np.testing.assert_array_equal(host(i), ki)
+
+
 @pytest.mark.parametrize("buffer_size", [0, 3, 64, -64])
 @pytest.mark.parametrize("sort", [True, False])
 def test_ordered_policies(buffer_size, sort):
