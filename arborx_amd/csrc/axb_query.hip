@@ -1022,8 +1022,7 @@ __global__ __launch_bounds__(256) void k_pred_morton32(
 // Buffered first pass -> CRS: row q's results sit at buf + q * buffer when
 // they fit (CrsGraphWrapperImpl.hpp:296-318, copy_valid_values); rows that
 // did not fit are written by a second traversal of those queries.
-// Scratch rows that fit -> their CRS rows: 16 lanes per row, 16 bytes per
-// lane per step (a 64-entry row is one step).
+// Scratch rows that fit -> their CRS rows: 16 lanes per row.
 __global__ void k_compact(const int32_t *__restrict__ buf, int64_t nq, int32_t buffer,
                           const int64_t *__restrict__ offsets, int32_t *__restrict__ out)
 {
@@ -1036,14 +1035,11 @@ __global__ void k_compact(const int32_t *__restrict__ buf, int64_t nq, int32_t b
   if (cnt > buffer)
     return;
   const int32_t *src = buf + q * (int64_t)buffer;
-  for (int64_t j = 4 * l; j < cnt; j += 64)
-  {
-    if (j + 4 <= cnt)
-      *(i32x4_u *)(out + o + j) = *(const i32x4_u *)(src + j);
-    else
-      for (int64_t u = j; u < cnt; ++u)
-        out[o + u] = src[u];
-  }
+  // one int per lane: the 16 lanes of a row read and write 64 contiguous
+  // bytes per step (16-byte accesses at the rows' unaligned starts measured
+  // 2% slower over the whole radius query)
+  for (int64_t j = l; j < cnt; j += 16)
+    out[o + j] = src[j];
 }
 
 // Rows whose result count exceeds the buffer, listed in processing
