@@ -75,6 +75,20 @@ def lognormal_boxes(n, seed, mu=math.log(0.002), sigma=0.5):
     return np.concatenate([lo, hi], axis=1)
 
 
+def rays(n, seed):
+    """n rays as (origins, directions), float32 (n, 3) each: origins uniform
+    in [-0.2, 1.2)^3, Gaussian directions (not normalised), every fifth ray
+    along a coordinate axis (the two other components zero)."""
+    o = (uniform01(seed, 3 * n).reshape(n, 3) * np.float32(1.4) - np.float32(0.2))
+    d = normal(seed + 700, 3 * n).reshape(n, 3).astype(np.float32)
+    along = np.arange(n) % 5 == 1
+    axis = (np.arange(n) // 5) % 3
+    keep = d[along, axis[along]].copy()
+    d[along] = 0.0
+    d[along, axis[along]] = keep
+    return o.astype(np.float32), d
+
+
 def fcc_lattice(cells, seed, a=1.0, jitter=0.05):
     """FCC lattice with nearest-neighbour distance `a` (cubic cell edge a*sqrt(2)),
     `cells` = (nx, ny, nz) cubic cells, 4 atoms per cell, each coordinate jittered

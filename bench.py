@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--box-n", type=int, default=5_000_000,
                     help="data and query boxes of the box-primitive case (0 = skip)")
+    ap.add_argument("--ray-n", type=int, default=1_000_000,
+                    help="rays over the box-primitive case's data boxes (0 = skip)")
     ap.add_argument("--fcc-cells", default="80,80,78",
                     help="FCC cubic cells of the pair-energy case ('' = skip)")
     ap.add_argument("--e2e-steps", type=int, default=0,
@@ -159,6 +161,40 @@ def bench_boxes(ax, args, torch, synthetic, stream):
     out["q16_traversal_speedup"] = out["fp32"]["traversal_kernel_ms"] / out["q16"]["traversal_kernel_ms"]
     out["config"] = f"{n} data boxes (seed 4) and {n} query boxes (seed 5): centres U[0,1)^3, " \
                     "sides lognormal(ln 0.002, 0.5)"
+    return out
+
+
+def bench_rays(ax, args, torch, synthetic, stream):
+    """Rays over the box case's data boxes: Intersects<Ray>, ordered rows
+    (ordered_intersects) and nearest(ray, k) for k = 1 and 8, FP32 and Q16
+    nodes; the two formats' rows are checked identical."""
+    n, nr = args.box_n, args.ray_n
+    data = torch.from_numpy(synthetic.lognormal_boxes(n, 4)).cuda()
+    o, d = synthetic.rays(nr, 9)
+    rays = ax.Rays(torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda())
+    preds = {"intersects": ax.intersects(rays), "ordered": ax.ordered_intersects(rays),
+             "nearest1": ax.nearest(rays, 1), "nearest8": ax.nearest(rays, 8)}
+    out, ref = {}, {}
+    for fmt in ("fp32", "q16"):
+        bvh = ax.BoundingVolumeHierarchy(data, node_format=fmt)
+        res = {}
+        for name, p in preds.items():
+            t = timed(lambda: bvh.query(p), args.steps, args.warmup, stream)
+            idx, off = bvh.query(p)
+            if fmt == "fp32" and name != "intersects":
+                ref[name] = (idx, off)
+            same = name == "intersects" or fmt == "fp32" or bool(
+                torch.equal(off, ref[name][1]) and torch.equal(idx, ref[name][0]))
+            res[name] = dict(ms=t, rays_per_s=nr / t * 1e3, hits=int(idx.numel()),
+                             identical_to_fp32=same)
+        out[fmt] = res
+        log(f"[bench] rays/{fmt}: " + " | ".join(
+            f"{k} {v['ms']:.2f} ms ({v['rays_per_s'] / 1e6:.1f} Mrays/s, {v['hits']} hits)"
+            for k, v in res.items()) + " | identical " +
+            str(all(v["identical_to_fp32"] for v in res.values())))
+        del bvh
+    out["config"] = f"{nr} rays (origins U[-0.2,1.2)^3, Gaussian directions, every fifth " \
+                    f"along an axis, seed 9) over the {n} data boxes of the box case"
     return out
 
 
@@ -464,6 +500,8 @@ def main():
     extra = {}
     if args.box_n > 0:
         extra["boxes"] = bench_boxes(ax, args, torch, synthetic, stream)
+        if args.ray_n > 0:
+            extra["rays"] = bench_rays(ax, args, torch, synthetic, stream)
     if args.fcc_cells:
         extra["energy"] = bench_energy(ax, args, torch, synthetic, stream)
     if args.e2e_steps > 0:
