@@ -69,7 +69,9 @@ __global__ __launch_bounds__(256) void k_spatial(
 }
 
 // Rows of leaf slots hit by each ray -> rows of values in (distance, index)
-// order, one lane per row.
+// order, one lane per row.  SHORT: the rows of at most 16 (few registers,
+// many wavefronts in flight); otherwise the longer ones.
+template <bool SHORT>
 __global__ __launch_bounds__(256) void k_ray_sort_rows(TreeView tv, const float *__restrict__ rays,
                                                        int64_t nq,
                                                        const int64_t *__restrict__ offsets,
@@ -79,8 +81,18 @@ __global__ __launch_bounds__(256) void k_ray_sort_rows(TreeView tv, const float 
   if (q >= nq)
     return;
   const int64_t b = offsets[q], m = offsets[q + 1] - b;
-  if (m > 0)
-    sort_ray_row(tv, ray_pred(rays + 6 * q), rows + b, m);
+  if (m == 0 || (m <= 16) != SHORT)
+    return;
+  const RayPred ray = ray_pred(rays + 6 * q);
+  if (SHORT)
+  {
+    if (m <= 8)
+      sort_ray_row_n<8>(tv, ray, rows + b, (int)m);
+    else
+      sort_ray_row_n<16>(tv, ray, rows + b, (int)m);
+  }
+  else
+    sort_ray_row(tv, ray, rows + b, m);
 }
 
 __global__ __launch_bounds__(256) void k_row_prefix_counts(const int64_t *__restrict__ offsets,
@@ -1332,7 +1344,10 @@ hipError_t launch_ray_sort_rows(const TreeView &tv, const float *rays, int64_t n
 {
   if (nq <= 0)
     return hipSuccess;
-  AXB_LAUNCH(k_ray_sort_rows, dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq, offsets, rows);
+  AXB_LAUNCH(k_ray_sort_rows<true>, dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq, offsets,
+             rows);
+  AXB_LAUNCH(k_ray_sort_rows<false>, dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq, offsets,
+             rows);
   return hipGetLastError();
 }
 
