@@ -865,7 +865,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 &&
   const int m = k < tv.n ? k : tv.n;
   uint32_t v_int = 0, v_leaf = 0;
   float R = knn_seed<F, G, K>(tv, qx, qy, qz, k, W, v_int, v_leaf);
-  const uint32_t row_at = (uint32_t)(g * cap); // batch * cap < 2^28
+  const uint32_t row_at = (uint32_t)(g * cap); // batch * cap + cap <= 2^32 - 1
   int c = 0;
   bool done = false;
   int rounds = 0;
@@ -1225,7 +1225,13 @@ static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq,
   // radius factor after an overflowing round (cap + 1 leaves within R):
   // aim at 2m leaves, m = min(k, n)
   const float shrink = std::cbrt((float)(2 * std::min(k, n)) / (float)(cap + 1));
-  const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t(1) << 31) / (cap * 8)));
+  // queries per launch: candidate rows of up to 8 GiB, row offsets within
+  // 32 bits (10M points, k = 16: 4 launches 13.3 ms uniform / 20.9 ms
+  // Gaussian mixture, 2 launches 12.7 / 17.5, 1 launch 12.4 / 14.8 -- every
+  // launch ends with a tail of the slowest wavefronts)
+  const int64_t batch = std::max<int64_t>(
+      1, std::min<int64_t>(nq, std::min<int64_t>((int64_t(1) << 33) / (cap * 8),
+                                                 (int64_t(UINT32_MAX) - cap) / cap)));
   uint2 *cand = (uint2 *)ws_get(s, WS_KNN_CAND, (size_t)batch * cap * sizeof(uint2));
   // unresolved queries of every batch, for one fallback launch at the end:
   // each of them is one lane walking alone, so launches of a few lanes per
