@@ -1229,10 +1229,14 @@ static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq,
   // 32 bits (10M points, k = 16: 4 launches 13.3 ms uniform / 20.9 ms
   // Gaussian mixture, 2 launches 12.7 / 17.5, 1 launch 12.4 / 14.8 -- every
   // launch ends with a tail of the slowest wavefronts)
-  const int64_t batch = std::max<int64_t>(
+  int64_t batch = std::max<int64_t>(
       1, std::min<int64_t>(nq, std::min<int64_t>((int64_t(1) << 33) / (cap * 8),
                                                  (int64_t(UINT32_MAX) - cap) / cap)));
-  uint2 *cand = (uint2 *)ws_get(s, WS_KNN_CAND, (size_t)batch * cap * sizeof(uint2));
+  // smaller launches when the device cannot spare that much
+  uint2 *cand = nullptr;
+  while (!(cand = (uint2 *)ws_get(s, WS_KNN_CAND, (size_t)batch * cap * sizeof(uint2))) &&
+         batch > 65536)
+    batch = (batch + 1) / 2;
   // unresolved queries of every batch, for one fallback launch at the end:
   // each of them is one lane walking alone, so launches of a few lanes per
   // batch would each cost the longest such walk
