@@ -57,6 +57,7 @@ EXPORTED_SYMBOLS = (
     "axb_bruteforce_create", "axb_bruteforce_destroy", "axb_bruteforce_size",
     "axb_bruteforce_bounds", "axb_bruteforce_query_intersects",
     "axb_bruteforce_query_intersects_radius", "axb_bruteforce_query_nearest",
+    "axb_bruteforce_query_nearest_rays",
     "axb_bruteforce_view",
     "axb_bvh_view", "axb_bvh_predicate_order", "axb_exclusive_scan_i64",
     "axb_bvh_num_slots", "axb_bvh_slots", "axb_bvh_export_topology",
@@ -148,6 +149,9 @@ def lib():
             "axb_bruteforce_query_nearest": (_I32, [_P, _P, _I64, _I32, _Policy, _P,
                                                     ctypes.POINTER(_P), ctypes.POINTER(_P),
                                                     ctypes.POINTER(_I64), _P]),
+            "axb_bruteforce_query_nearest_rays": (_I32, [_P, _P, _I64, _I32, _Policy, _P,
+                                                         ctypes.POINTER(_P),
+                                                         ctypes.POINTER(_I64), _P]),
             "axb_pair_energy": (_I32, [_P, _I64, _P, _P, ctypes.c_float, _P, _P, _P]),
             "axb_pair_energy_fd": (_I32, [_P, _I64, _P, _P, ctypes.c_float, _P, _I64,
                                           ctypes.c_double, _P, _P]),
@@ -483,14 +487,12 @@ def _crs_query(prefix, handle, predicates, policy, return_distances, stream):
             ctypes.byref(nnz), s))
         return _take_output(ind.value, nnz.value, torch.int32), offsets
     if isinstance(predicates, NearestRays):
-        if prefix != "axb_bvh":
-            raise TypeError("nearest(Rays) needs a BoundingVolumeHierarchy")
         r = predicates.rays.data
         nq = r.shape[0]
         offsets = torch.empty(nq + 1, dtype=torch.int64, device="cuda")
-        _check(L.axb_bvh_query_nearest_rays(handle, _ptr(r), nq, predicates.k, policy._c(),
-                                            _ptr(offsets), ctypes.byref(ind), ctypes.byref(nnz),
-                                            s))
+        _check(getattr(L, prefix + "_query_nearest_rays")(
+            handle, _ptr(r), nq, predicates.k, policy._c(), _ptr(offsets), ctypes.byref(ind),
+            ctypes.byref(nnz), s))
         return _take_output(ind.value, nnz.value, torch.int32), offsets
     if isinstance(predicates, (Intersects, OrderedIntersects)):
         g = predicates.geometry

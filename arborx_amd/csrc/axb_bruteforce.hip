@@ -137,6 +137,123 @@ __global__ __launch_bounds__(256) void k_bf_heap(const float *__restrict__ value
   L.finish(m);
 }
 
+// Nearest<Ray> over boxes (BruteForceImpl.hpp:155-229 with distance(Ray,
+// Box), ArborX_Ray.hpp:429-444): row q = the first m = min(k, hits) boxes
+// of ray q by (distance, index); boxes the ray misses are at +inf and never
+// qualify.  A first sweep counts the hits; the second keeps a max-heap of m
+// value indices in the row itself (keys recomputed from the values), then
+// sorts it.  MODE_BUFFER keeps the row in the scratch row when m fits.
+struct RayKeys
+{
+  const float *__restrict__ values;
+  RayPred ray;
+  AXB_D uint64_t at(int32_t j) const
+  {
+    const float *v = values + 6 * (int64_t)j;
+    const Box6 b = {{v[0], v[1], v[2]}, {v[3], v[4], v[5]}};
+    return ordered_key(ray_distance(ray, b), (uint32_t)j);
+  }
+};
+
+AXB_D void heap_sift_down(const RayKeys &src, int32_t *h, int j, int size, int32_t v,
+                          uint64_t kv)
+{
+  while (true)
+  {
+    int c = 2 * j + 1;
+    if (c >= size)
+      break;
+    uint64_t kc = src.at(h[c]);
+    if (c + 1 < size)
+    {
+      const uint64_t kr = src.at(h[c + 1]);
+      if (kr > kc)
+        ++c, kc = kr;
+    }
+    if (kv >= kc)
+      break;
+    h[j] = h[c];
+    j = c;
+  }
+  h[j] = v;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bf_ray_nearest(const float *__restrict__ values, int n,
+                                                        const float *__restrict__ rays,
+                                                        int64_t nq, int k,
+                                                        const uint32_t *__restrict__ list,
+                                                        int64_t *__restrict__ offsets,
+                                                        int32_t *__restrict__ out, int32_t buffer,
+                                                        uint32_t *__restrict__ err)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq)
+    return;
+  const int64_t q = list ? (int64_t)list[t] : t;
+  const RayKeys src{values, ray_pred(rays + 6 * q)};
+  int hits = 0;
+  for (int j = 0; j < n; ++j)
+  {
+    const float *v = values + 6 * (int64_t)j;
+    hits += ray_distance(src.ray, Box6{{v[0], v[1], v[2]}, {v[3], v[4], v[5]}}) < INFINITY;
+  }
+  const int m = hits < k ? hits : k;
+  int32_t *row;
+  if (MODE == MODE_COUNT || (MODE == MODE_BUFFER && m > buffer))
+  {
+    offsets[q] = m;
+    return;
+  }
+  if (MODE == MODE_BUFFER)
+    row = out + q * (int64_t)buffer;
+  else
+  {
+    row = out + offsets[q];
+    if (offsets[q + 1] - offsets[q] != m)
+    {
+      raise_error(err, KERR_LAYOUT);
+      return;
+    }
+  }
+  // max-heap of the m smallest keys met so far
+  int size = 0;
+  uint64_t top = 0;
+  for (int j = 0; j < n && m > 0; ++j)
+  {
+    const uint64_t kj = src.at(j);
+    if ((uint32_t)(kj >> 32) >= 0x7f800000u) // +inf: a miss
+      continue;
+    if (size < m)
+    {
+      int i = size++;
+      while (i > 0)
+      {
+        const int up = (i - 1) >> 1;
+        if (src.at(row[up]) >= kj)
+          break;
+        row[i] = row[up];
+        i = up;
+      }
+      row[i] = j;
+      top = src.at(row[0]);
+    }
+    else if (kj < top)
+    {
+      heap_sift_down(src, row, 0, size, j, kj);
+      top = src.at(row[0]);
+    }
+  }
+  for (int end = size - 1; end > 0; --end)
+  {
+    const int32_t v = row[end];
+    row[end] = row[0];
+    heap_sift_down(src, row, 0, end, v, src.at(v));
+  }
+  if (MODE == MODE_BUFFER)
+    offsets[q] = m;
+}
+
 inline unsigned blocks_of(int64_t n) { return (unsigned)div_up(n, 256); }
 
 template <int G, int PK>
@@ -212,6 +329,25 @@ hipError_t launch_bf_spatial(const float *values, int64_t n, int geometry, int p
                                      buffer, mode, err, s);
   return bf_spatial_g<GEOM_BOXES>(values, n, pred_kind, preds, nq, radius, list, offsets, out,
                                   buffer, mode, err, s);
+}
+
+hipError_t launch_bf_ray_nearest(const float *values, int64_t n, const float *rays, int64_t nq,
+                                 int k, const uint32_t *list, int64_t *offsets, int32_t *out,
+                                 int32_t buffer, int mode, uint32_t *err, hipStream_t s)
+{
+  if (nq <= 0)
+    return hipSuccess;
+  const dim3 grid(blocks_of(nq)), block(256);
+  if (mode == MODE_COUNT)
+    AXB_LAUNCH(k_bf_ray_nearest<MODE_COUNT>, grid, block, 0, s, values, (int)n, rays, nq, k,
+               list, offsets, out, buffer, err);
+  else if (mode == MODE_FILL)
+    AXB_LAUNCH(k_bf_ray_nearest<MODE_FILL>, grid, block, 0, s, values, (int)n, rays, nq, k,
+               list, offsets, out, buffer, err);
+  else
+    AXB_LAUNCH(k_bf_ray_nearest<MODE_BUFFER>, grid, block, 0, s, values, (int)n, rays, nq, k,
+               list, offsets, out, buffer, err);
+  return hipGetLastError();
 }
 
 hipError_t launch_bf_nearest(const float *values, int64_t n, int geometry, const float *pts,

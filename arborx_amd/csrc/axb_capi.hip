@@ -1435,6 +1435,44 @@ axb_status axb_bruteforce_query_intersects_radius(const axb_bruteforce *bf, cons
                           nnz_host, stream);
 }
 
+axb_status axb_bruteforce_query_nearest_rays(const axb_bruteforce *bf, const float *rays,
+                                             int64_t nq, int32_t k, axb_traversal_policy policy,
+                                             int64_t *offsets, int32_t **indices,
+                                             int64_t *nnz_host, axb_stream_t stream_)
+{
+  if (!bf || !offsets || !indices || !nnz_host)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_query_nearest_rays: NULL argument");
+  if (nq < 0 || nq >= (int64_t(1) << 31))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT,
+             "axb_bruteforce_query_nearest_rays: nq must be in [0, 2^31)");
+  if (nq > 0 && !rays)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_query_nearest_rays: rays is NULL");
+  if (bf->geometry != GEOM_BOXES && bf->n > 0)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT,
+             "axb_bruteforce_query_nearest_rays: ray predicates need boxes");
+  hipStream_t s = (hipStream_t)stream_;
+  *indices = nullptr;
+  *nnz_host = 0;
+  const int kk = k > 0 ? k : 0; // TreeTraversal.hpp:143-178: k < 1 returns nothing
+  if (nq == 0 || bf->n == 0 || kk == 0)
+  {
+    AXB_HIP_TRY(fill_u64((uint64_t *)offsets, 0ull, nq + 1, s));
+    *indices = (int32_t *)out_alloc(0, s);
+    AXB_HIP_TRY(sync_stream(s));
+    return AXB_OK;
+  }
+  auto first = [&](int mode, int32_t *out, int32_t B) {
+    return launch_bf_ray_nearest(bf->values, bf->n, rays, nq, kk, nullptr, offsets, out, B,
+                                 mode, bf->d_err, s);
+  };
+  auto fill = [&](const uint32_t *list, int64_t n_list, int32_t *out) {
+    return launch_bf_ray_nearest(bf->values, bf->n, rays, n_list, kk, list, offsets, out, 0,
+                                 MODE_FILL, bf->d_err, s);
+  };
+  return crs_query(bf->d_err, nq, policy, nullptr, false, first, fill, offsets, indices,
+                   nnz_host, s, "bruteforce nearest rays");
+}
+
 axb_status axb_bruteforce_query_nearest(const axb_bruteforce *bf, const float *points,
                                        int64_t nq, int32_t k, axb_traversal_policy policy,
                                        int64_t *offsets, int32_t **indices, float **distances,

@@ -296,6 +296,11 @@ hipError_t launch_h2f_fill(const int64_t *offsets, const int32_t *indices, int64
 // predicate list ? list[t] : t, nq of them); nearest rows of min(k, n)
 // (distance, index) pairs, for k <= 64 by the register selection, above by
 // a heap in the output row.
+// Nearest<Ray> over a brute-force index of boxes: rows of min(k, hits)
+// values by (distance, index), through the CRS row writers' modes.
+hipError_t launch_bf_ray_nearest(const float *values, int64_t n, const float *rays, int64_t nq,
+                                 int k, const uint32_t *list, int64_t *offsets, int32_t *out,
+                                 int32_t buffer, int mode, uint32_t *err, hipStream_t s);
 hipError_t launch_bf_spatial(const float *values, int64_t n, int geometry, int pred_kind,
                              const float *preds, int64_t nq, float radius, const uint32_t *list,
                              int64_t *offsets, int32_t *out, int32_t buffer, int mode,

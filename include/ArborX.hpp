@@ -699,6 +699,20 @@ public:
           c_policy(policy), offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
     finish(space, indices, ind, nnz, offsets, preds.n);
   }
+  // nearest(ray, k) over boxes: rows of at most k values by (distance, index).
+  void query(HipStream const &space, Experimental::RaysNearestK const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    prepare(offsets, preds.n);
+    int32_t *ind = nullptr;
+    int64_t nnz = 0;
+    if (_h)
+      Details::check(axb_bruteforce_query_nearest_rays(
+          _h, (float const *)preds.rays, (int64_t)preds.n, preds.k, c_policy(policy),
+          offsets.data(), &ind, &nnz, (axb_stream_t)space.stream));
+    finish(space, indices, ind, nnz, offsets, preds.n);
+  }
   // Intersects<Ray> over boxes (the reference's brute force has no ordered
   // traversal).
   void query(HipStream const &space, Experimental::IntersectsRays const &preds,

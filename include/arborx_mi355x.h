@@ -252,7 +252,8 @@ axb_status axb_bruteforce_bounds(const axb_bruteforce *bf, float out_host[6]);
  * (ArborX_BruteForce.hpp:66-91,151-178, BruteForceImpl.hpp:53-153): as for
  * axb_bvh_query_intersects / _radius; rows in increasing value index.
  * AXB_PREDICATE_RAYS over boxes is accepted; AXB_PREDICATE_ORDERED_RAYS is
- * not (the reference's brute force has no ordered traversal). */
+ * not (the reference's brute force has no ordered traversal; nearest rays:
+ * axb_bruteforce_query_nearest_rays). */
 axb_status axb_bruteforce_query_intersects(const axb_bruteforce *bf,
                                            int32_t predicate_kind,
                                            const float *predicates, int64_t nq,
@@ -272,6 +273,13 @@ axb_status axb_bruteforce_query_nearest(const axb_bruteforce *bf,
                                         int64_t *offsets, int32_t **indices,
                                         float **distances, int64_t *nnz_host,
                                         axb_stream_t stream);
+
+/* Nearest<Ray> over a brute-force index of boxes (BruteForceImpl.hpp:155-229
+ * with distance(Ray, Box)): as axb_bvh_query_nearest_rays. */
+axb_status axb_bruteforce_query_nearest_rays(const axb_bruteforce *bf, const float *rays,
+                                             int64_t nq, int32_t k, axb_traversal_policy policy,
+                                             int64_t *offsets, int32_t **indices,
+                                             int64_t *nnz_host, axb_stream_t stream);
 
 /* Read-only view of a brute-force index for device code (the callback
  * forms of query over it, include/ArborX_Callbacks.hpp): the index's device

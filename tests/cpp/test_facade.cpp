@@ -359,13 +359,16 @@ static void rays(HipStream const &space, Make make)
   DeviceVector<Ray> r1(std::vector<Ray>{Ray({{0, 0, 0}}, {{.1f, .1f, .1f}})});
   index.query(space, intersects(r1), ind, off);
   CHECK(rows_of(ind, off) == rows({{0, 1, 2, 3, 4, 5, 6, 7, 8, 9}}), "rays: intersects");
-  // nearest and ordered ray queries: trees only, as in the reference
+  // :29-58 nearest(ray, 1)
+  DeviceVector<Ray> r0(std::vector<Ray>{Ray({{0, 0, 0}}, {{.15f, .1f, 0.f}})});
+  index.query(space, nearest(r0, 1), ind, off);
+  CHECK(rows_of(ind, off) == rows({{0}}), "rays: nearest k = 1");
+  DeviceVector<Ray> rb(std::vector<Ray>{Ray({{10, 10, 10}}, {{-.1f, -.1f, -.1f}})});
+  index.query(space, nearest(rb, 3), ind, off);
+  CHECK((ind.to_host() == std::vector<int>{9, 8, 7}), "rays: nearest k = 3 in order");
+  // ordered ray queries: trees only, as in the reference
   if constexpr (std::is_same_v<typename Make::index_type, BoundingVolumeHierarchy>)
   {
-    // :29-58 nearest(ray, 1)
-    DeviceVector<Ray> r0(std::vector<Ray>{Ray({{0, 0, 0}}, {{.15f, .1f, 0.f}})});
-    index.query(space, nearest(r0, 1), ind, off);
-    CHECK(rows_of(ind, off) == rows({{0}}), "rays: nearest k = 1");
     // :112-163 ordered intersections, forwards and backwards
     DeviceVector<Ray> r2(std::vector<Ray>{Ray({{0, 0, 0}}, {{.1f, .1f, .1f}}),
                                           Ray({{10, 10, 10}}, {{-.1f, -.1f, -.1f}})});

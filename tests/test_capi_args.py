@@ -98,6 +98,12 @@ def test_ray_arguments():
                                              ctypes.byref(nnz), None) == INVALID
     assert L.axb_bvh_query_nearest_rays(None, dummy, 4, 2, _policy(), dummy, ctypes.byref(ind),
                                         ctypes.byref(nnz), None) == INVALID
+    assert L.axb_bruteforce_query_nearest_rays(None, dummy, 4, 2, _policy(), dummy,
+                                               ctypes.byref(ind), ctypes.byref(nnz),
+                                               None) == INVALID
+    assert L.axb_bruteforce_query_nearest_rays(dummy, dummy, -1, 2, _policy(), dummy,
+                                               ctypes.byref(ind), ctypes.byref(nnz),
+                                               None) == INVALID
 
 
 def test_energy_rejects_bad_arguments():

@@ -195,8 +195,22 @@ def test_bruteforce_rays():
     np.testing.assert_array_equal(host(i), bi)
     with pytest.raises(RuntimeError):
         bf.query(ax.ordered_intersects(rays))
-    with pytest.raises(TypeError):
-        bf.query(ax.nearest(rays, 2))
+    # nearest(ray, k): the first k boxes by (distance, index), rows may be short
+    for k, b in ((1, 0), (5, 0), (12, 0), (5, 3), (12, -16)):
+        ko, ki = oracle.bruteforce_rays_ordered(boxes, rays_h, k)
+        i, off = bf.query(ax.nearest(rays, k), ax.TraversalPolicy().setBufferSize(b))
+        np.testing.assert_array_equal(host(off), ko)
+        np.testing.assert_array_equal(host(i), ki)
+    assert np.max(np.diff(ko)) == 12 and np.min(np.diff(ko)) < 12
+    # long rows (a heap of 100 over boxes up to 0.1 wide) against the tree
+    big = synthetic.lognormal_boxes(4_000, 84, mu=math.log(0.1))
+    bfb = ax.BruteForce(dev(big), geometry="boxes")
+    ko, ki = oracle.bruteforce_rays_ordered(big, rays_h[:500], 100)
+    sub = ax.Rays.from_normalized(dev(rays_h[:500]))
+    i, off = bfb.query(ax.nearest(sub, 100))
+    np.testing.assert_array_equal(host(off), ko)
+    np.testing.assert_array_equal(host(i), ki)
+    assert np.max(np.diff(ko)) == 100
 
 
 def test_rays_need_boxes():
@@ -206,5 +220,6 @@ def test_rays_need_boxes():
     for p in (ax.intersects(rays), ax.ordered_intersects(rays), ax.nearest(rays, 2)):
         with pytest.raises(RuntimeError, match="boxes"):
             bvh.query(p)
-    with pytest.raises(RuntimeError):
-        ax.BruteForce(dev(pts)).query(ax.intersects(rays))
+    for p in (ax.intersects(rays), ax.nearest(rays, 2)):
+        with pytest.raises(RuntimeError):
+            ax.BruteForce(dev(pts)).query(p)
