@@ -223,3 +223,30 @@ def test_rays_need_boxes():
     for p in (ax.intersects(rays), ax.nearest(rays, 2)):
         with pytest.raises(RuntimeError):
             ax.BruteForce(dev(pts)).query(p)
+
+
+def test_ray_edge_cases():
+    """No rays, k = 0, empty indexes, a single box (TreeTraversal.hpp:48-59,
+    143-178: degenerate trees and k < 1)."""
+    none = ax.Rays(dev(np.zeros((0, 3), np.float32)), dev(np.zeros((0, 3), np.float32)))
+    assert none.data.shape == (0, 6)
+    one = dev(np.array([[0, 0, 0, 1, 1, 1]], np.float32))
+    rays = ax.Rays(dev(np.array([[0.5, 0.5, -1.0], [2.0, 2.0, 2.0]], np.float32)),
+                   dev(np.array([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0]], np.float32)))
+    for index in (ax.BoundingVolumeHierarchy(one, geometry="boxes", node_format="q16"),
+                  ax.BruteForce(one, geometry="boxes")):
+        i, off = index.query(ax.intersects(none))
+        assert host(off).tolist() == [0] and i.numel() == 0
+        i, off = index.query(ax.nearest(rays, 0))
+        assert host(off).tolist() == [0, 0, 0] and i.numel() == 0
+        i, off = index.query(ax.nearest(rays, 3))
+        assert host(off).tolist() == [0, 1, 1] and host(i).tolist() == [0]
+        i, off = index.query(ax.intersects(rays))
+        assert host(off).tolist() == [0, 1, 1] and host(i).tolist() == [0]
+    bvh = ax.BoundingVolumeHierarchy(one, geometry="boxes")
+    i, off = bvh.query(ax.ordered_intersects(rays))
+    assert host(off).tolist() == [0, 1, 1] and host(i).tolist() == [0]
+    for empty in (ax.BoundingVolumeHierarchy(), ax.BruteForce()):
+        for p in (ax.intersects(rays), ax.nearest(rays, 2)):
+            i, off = empty.query(p)
+            assert host(off).tolist() == [0, 0, 0] and i.numel() == 0
