@@ -69,9 +69,10 @@ __global__ __launch_bounds__(256) void k_spatial(
 }
 
 // Rows of leaf slots hit by each ray -> rows of values in (distance, index)
-// order, one lane per row.  SHORT: the rows of at most 16 (few registers,
-// many wavefronts in flight); otherwise the longer ones.
-template <bool SHORT>
+// order, one lane per row, one launch per row-length class so that the
+// short rows keep few registers and many wavefronts in flight: CLASS 0
+// rows of up to 16, 1 up to 32, 2 the longer ones.
+template <int CLASS>
 __global__ __launch_bounds__(256) void k_ray_sort_rows(TreeView tv, const float *__restrict__ rays,
                                                        int64_t nq,
                                                        const int64_t *__restrict__ offsets,
@@ -81,16 +82,19 @@ __global__ __launch_bounds__(256) void k_ray_sort_rows(TreeView tv, const float 
   if (q >= nq)
     return;
   const int64_t b = offsets[q], m = offsets[q + 1] - b;
-  if (m == 0 || (m <= 16) != SHORT)
+  const int cls = m <= 16 ? 0 : (m <= 32 ? 1 : 2);
+  if (m == 0 || cls != CLASS)
     return;
   const RayPred ray = ray_pred(rays + 6 * q);
-  if (SHORT)
+  if (CLASS == 0)
   {
     if (m <= 8)
       sort_ray_row_n<8>(tv, ray, rows + b, (int)m);
     else
       sort_ray_row_n<16>(tv, ray, rows + b, (int)m);
   }
+  else if (CLASS == 1)
+    sort_ray_row_n<32>(tv, ray, rows + b, (int)m);
   else
     sort_ray_row(tv, ray, rows + b, m);
 }
@@ -1354,9 +1358,11 @@ hipError_t launch_ray_sort_rows(const TreeView &tv, const float *rays, int64_t n
 {
   if (nq <= 0)
     return hipSuccess;
-  AXB_LAUNCH(k_ray_sort_rows<true>, dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq, offsets,
+  AXB_LAUNCH(k_ray_sort_rows<0>, dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq, offsets,
              rows);
-  AXB_LAUNCH(k_ray_sort_rows<false>, dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq, offsets,
+  AXB_LAUNCH(k_ray_sort_rows<1>, dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq, offsets,
+             rows);
+  AXB_LAUNCH(k_ray_sort_rows<2>, dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq, offsets,
              rows);
   return hipGetLastError();
 }
