@@ -315,8 +315,9 @@ def bench_e2e(ax, args, torch, synthetic, stream, fmt):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
-              "design is described in DESIGN.md and not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the index "
+              "is built and queried on one device; sharding over several GPUs is a design, "
+              "not code, here", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
