@@ -64,6 +64,7 @@ EXPORTED_SYMBOLS = (
     "axb_sqrt_threshold", "axb_morton64_host", "axb_q16_encode_host", "axb_profile_enable",
     "axb_profile_clear",
     "axb_profile_num_records", "axb_profile_record", "axb_profile_visits",
+    "axb_debug_set_limits",
 )
 
 
@@ -167,6 +168,7 @@ def lib():
             "axb_profile_record": (_I32, [_I32, ctypes.c_char_p, _I32,
                                           ctypes.POINTER(ctypes.c_double)]),
             "axb_profile_visits": (_I32, [ctypes.POINTER(_I64)]),
+            "axb_debug_set_limits": (_I32, [_I64, _I64, _I32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -666,6 +668,26 @@ class profile:
                                fallback_queries=v[4], fallback_internal=v[5],
                                fallback_leaf=v[6], rounds=v[8], multi_round_queries=v[9])
         _check(L.axb_profile_enable(0))
+        return False
+
+
+class debug_limits:
+    """Lowers limits the library derives from the device, so that small
+    inputs take the paths of large ones (tests only; process-wide):
+
+    with arborx_amd.debug_limits(knn_batch_max=1000, knn_rounds=1):
+        ...   # kNN in launches of 1000 queries, most to the exact fallback
+    """
+
+    def __init__(self, knn_batch_max=0, scratch_rows_bytes=0, knn_rounds=0):
+        self.args = (int(knn_batch_max), int(scratch_rows_bytes), int(knn_rounds))
+
+    def __enter__(self):
+        _check(lib().axb_debug_set_limits(*self.args))
+        return self
+
+    def __exit__(self, *exc):
+        _check(lib().axb_debug_set_limits(0, 0, 0))
         return False
 
 

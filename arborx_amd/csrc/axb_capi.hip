@@ -158,44 +158,89 @@ struct WsBlock
 {
   void *p = nullptr;
   size_t bytes = 0;
+  bool busy = false; // leased to a call
 };
 std::mutex g_ws_mutex;
 std::map<WsKey, std::array<WsBlock, WS_NUM>> g_ws;
+
+std::atomic<int64_t> g_lim_knn_batch{0};
+std::atomic<int64_t> g_lim_rows_bytes{0};
+std::atomic<int32_t> g_lim_knn_rounds{0};
 } // namespace
 
-void *ws_get(hipStream_t s, int slot, size_t bytes)
+Limits debug_limits()
+{
+  return Limits{g_lim_knn_batch.load(), g_lim_rows_bytes.load(), g_lim_knn_rounds.load()};
+}
+
+void WsLease::acquire(hipStream_t s_, int slot_, size_t bytes)
 {
   HostScope h("host:ws_get");
-  int dev = 0;
+  release();
+  s = s_;
+  slot = slot_;
   if (hipGetDevice(&dev) != hipSuccess)
   {
     (void)hipGetLastError();
-    return nullptr;
+    return;
   }
-  std::lock_guard<std::mutex> lock(g_ws_mutex);
-  WsBlock &b = g_ws[WsKey{dev, s}][slot];
-  if (b.bytes >= bytes && b.p)
-    return b.p;
-  if (b.p)
-    dfree(b.p, s);
-  b.p = dalloc(bytes, s);
-  b.bytes = b.p ? bytes : 0;
-  return b.p;
+  {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    WsBlock &b = g_ws[WsKey{dev, s}][slot];
+    if (!b.busy)
+    {
+      // the block's earlier users ran on this same stream, so a free
+      // enqueued here follows their kernels
+      if (!(b.p && b.bytes >= bytes))
+      {
+        if (b.p)
+          dfree(b.p, s);
+        b.p = dalloc(bytes, s);
+        b.bytes = b.p ? bytes : 0;
+      }
+      if (b.p)
+      {
+        b.busy = true;
+        cached = true;
+        p = b.p;
+      }
+      return;
+    }
+  }
+  p = dalloc(bytes, s);
+  cached = false;
 }
 
-// Releases every cached workspace (after the device is idle).
+void WsLease::release()
+{
+  if (!p)
+    return;
+  if (cached)
+  {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    auto it = g_ws.find(WsKey{dev, s});
+    if (it != g_ws.end())
+      it->second[slot].busy = false;
+  }
+  else
+    dfree(p, s);
+  p = nullptr;
+  cached = false;
+}
+
+// Releases every cached workspace not leased to a call (after the device
+// is idle).
 static void ws_release_all()
 {
   std::lock_guard<std::mutex> lock(g_ws_mutex);
   for (auto &kv : g_ws)
     for (WsBlock &b : kv.second)
-      if (b.p)
+      if (b.p && !b.busy)
       {
         (void)hipSetDevice(kv.first.dev);
         (void)hipFree(b.p);
         b = WsBlock{};
       }
-  g_ws.clear();
 }
 
 // Result arrays the caller owns.
@@ -283,14 +328,35 @@ static axb_status check_kernel_error(uint32_t *d_err, hipStream_t s, const char 
   return AXB_OK;
 }
 
-static TreeView view_of(const axb_bvh *t)
+// The kernel error word of one call: calls running at the same time on
+// other streams never see (or clear) each other's flags.
+struct CallErr
+{
+  uint32_t *p = nullptr;
+  hipStream_t s;
+  explicit CallErr(hipStream_t s_) : s(s_)
+  {
+    p = (uint32_t *)dalloc(sizeof(uint32_t), s);
+    if (p && fill_u32(p, 0u, 1, s) != hipSuccess)
+    {
+      dfree(p, s);
+      p = nullptr;
+    }
+  }
+  CallErr(const CallErr &) = delete;
+  CallErr &operator=(const CallErr &) = delete;
+  ~CallErr() { dfree(p, s); }
+};
+
+// Device view of a tree for one call's kernels (err: that call's word).
+static TreeView view_of(const axb_bvh *t, uint32_t *err)
 {
   TreeView tv;
   tv.slots = t->b.slots;
   tv.total = (uint32_t)t->b.total_slots;
   tv.n = (int32_t)t->n;
   tv.C = t->b.C;
-  tv.err = t->d_err;
+  tv.err = err;
   tv.visits = visit_counters();
   for (int d = 0; d < 3; ++d)
   {
@@ -353,6 +419,17 @@ axb_status axb_free(void *device_ptr, axb_stream_t stream)
 }
 
 float axb_sqrt_threshold(float r) { return sqrt_threshold(r); }
+
+axb_status axb_debug_set_limits(int64_t knn_batch_max, int64_t scratch_rows_bytes,
+                                int32_t knn_rounds)
+{
+  if (knn_batch_max < 0 || scratch_rows_bytes < 0 || knn_rounds < -1 || knn_rounds > 64)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_debug_set_limits: bad limit");
+  g_lim_knn_batch.store(knn_batch_max);
+  g_lim_rows_bytes.store(scratch_rows_bytes);
+  g_lim_knn_rounds.store(knn_rounds);
+  return AXB_OK;
+}
 
 uint64_t axb_morton64_host(const float p[3], const float scene[6])
 {
@@ -498,6 +575,9 @@ static axb_status morton_order(const float *scene, uint32_t *d_err, int pred_kin
   *order_out = nullptr;
   if (nq < 2)
     return AXB_OK;
+  // the sort's look-back words carry 30-bit counts
+  if (nq >= MAX_SORTED_PREDICATES)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: sort_predicates needs fewer than 2^30 predicates");
   const size_t ws_bytes = sort_workspace_bytes(nq, 4, 4);
   // one block: keys0 keys1 vals0 vals1 hist workspace
   const size_t a = (size_t)nq * 4;
@@ -524,11 +604,11 @@ static axb_status morton_order(const float *scene, uint32_t *d_err, int pred_kin
   return AXB_OK;
 }
 
-static axb_status predicate_order(const axb_bvh *t, int pred_kind, const float *preds,
-                                  int64_t nq, hipStream_t s, uint32_t **order_out,
-                                  void **to_free)
+static axb_status predicate_order(const axb_bvh *t, uint32_t *err, int pred_kind,
+                                  const float *preds, int64_t nq, hipStream_t s,
+                                  uint32_t **order_out, void **to_free)
 {
-  return morton_order(t->b.scene, t->d_err, pred_kind, preds, nq, s, order_out, to_free);
+  return morton_order(t->b.scene, err, pred_kind, preds, nq, s, order_out, to_free);
 }
 
 } // extern "C"
@@ -542,6 +622,9 @@ static int64_t scratch_rows_budget()
   int dev = 0;
   (void)hipGetDevice(&dev);
   std::atomic<int64_t> &b = budget[dev & 63];
+  const int64_t lim = debug_limits().scratch_rows_bytes;
+  if (lim > 0)
+    return lim;
   int64_t v = b.load(std::memory_order_relaxed);
   if (v == 0)
   {
@@ -581,6 +664,7 @@ static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy po
     B = fit >= auto_rows ? auto_rows : (fit >= 16 ? fit : 0);
   }
   void *scan_ws = dalloc(scan_workspace_bytes(nq + 1), s);
+  WsLease rows;
   int32_t *buf = nullptr;
   uint32_t *ovf = nullptr, *ovf_n = nullptr;
   axb_status st = AXB_OK;
@@ -589,7 +673,8 @@ static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy po
   uint32_t n_ovf = 0;
   if (e == hipSuccess && B > 0)
   {
-    buf = (int32_t *)ws_get(s, WS_ROWS, (size_t)nq * B * sizeof(int32_t));
+    rows.acquire(s, WS_ROWS, (size_t)nq * B * sizeof(int32_t));
+    buf = (int32_t *)rows.p;
     ovf = (uint32_t *)dalloc((size_t)nq * sizeof(uint32_t), s);
     ovf_n = (uint32_t *)dalloc(sizeof(uint32_t), s);
     e = (buf && ovf && ovf_n) ? hipSuccess : hipErrorOutOfMemory;
@@ -682,6 +767,8 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: nq must be in [0, 2^31)");
   if (nq > 0 && !preds)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: predicates is NULL");
+  if (policy.sort_predicates && nq >= MAX_SORTED_PREDICATES)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: sort_predicates needs fewer than 2^30 predicates");
   hipStream_t s = (hipStream_t)stream_;
   *indices = nullptr;
   *nnz_host = 0;
@@ -692,13 +779,16 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
     AXB_HIP_TRY(sync_stream(s));
     return AXB_OK;
   }
-  const TreeView tv = view_of(t);
+  CallErr err(s);
+  if (!err.p)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "query: out of device memory");
+  const TreeView tv = view_of(t, err.p);
   uint32_t *order = nullptr;
   void *order_blk = nullptr;
   if (policy.sort_predicates)
   {
-    axb_status st = predicate_order(t, ordered ? PRED_RAYS : pred_kind, preds, nq, s, &order,
-                                    &order_blk);
+    axb_status st = predicate_order(t, err.p, ordered ? PRED_RAYS : pred_kind, preds, nq, s,
+                                    &order, &order_blk);
     if (st != AXB_OK)
       return st;
   }
@@ -731,7 +821,7 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
   // launch of few, long walks, 5 ms of latency; rows of 96 avoid it
   // (intersects 21.2 -> 16.1 ms).  Radius rows: 64 (7.64 ms, 96: 7.75 ms).
   const bool rays = pred_kind == PRED_RAYS || ordered;
-  axb_status st = crs_query(t->d_err, nq, policy, order, false, first, fill, offsets, indices,
+  axb_status st = crs_query(err.p, nq, policy, order, false, first, fill, offsets, indices,
                             nnz_host, s, "query", tolerated, rays ? 96 : 64);
   dfree(order_blk, s);
   if (st == AXB_OK && sorted_rows && *nnz_host > 0)
@@ -786,6 +876,13 @@ axb_status axb_bvh_query_nearest_rays(const axb_bvh *t, const float *rays, int64
     dfree(scan_ws, s);
     AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bvh_query_nearest_rays: out of device memory");
   }
+  CallErr err(s);
+  if (!err.p)
+  {
+    dfree(full_off, s);
+    dfree(scan_ws, s);
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bvh_query_nearest_rays: out of device memory");
+  }
   int32_t *full_ind = nullptr;
   int64_t full_nnz = 0;
   axb_traversal_policy whole = policy;
@@ -798,7 +895,7 @@ axb_status axb_bvh_query_nearest_rays(const axb_bvh *t, const float *rays, int64
   {
     e = launch_row_prefix_counts(full_off, nq, k, offsets, s);
     if (e == hipSuccess)
-      e = exclusive_scan_i64(offsets, offsets, nq + 1, scan_ws, t->d_err, s);
+      e = exclusive_scan_i64(offsets, offsets, nq + 1, scan_ws, err.p, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(&nnz, offsets + nq, sizeof(nnz), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess)
@@ -825,8 +922,15 @@ axb_status axb_bvh_query_nearest_rays(const axb_bvh *t, const float *rays, int64
     *indices = nullptr;
     return hip_fail(e, "axb_bvh_query_nearest_rays");
   }
+  st = check_kernel_error(err.p, s, "axb_bvh_query_nearest_rays");
+  if (st != AXB_OK)
+  {
+    out_free(*indices, s);
+    *indices = nullptr;
+    return st;
+  }
   *nnz_host = nnz;
-  return check_kernel_error(t->d_err, s, "axb_bvh_query_nearest_rays");
+  return AXB_OK;
 }
 
 axb_status axb_make_rays(const float *origins, const float *directions, int64_t n,
@@ -872,7 +976,7 @@ axb_status axb_bvh_view(const axb_bvh *t, axb_tree_view *out)
 {
   if (!t || !out)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_view: NULL argument");
-  const TreeView tv = view_of(t);
+  const TreeView tv = view_of(t, nullptr);
   out->slots = t->n > 0 ? (const void *)tv.slots : nullptr;
   out->end_slot = t->n > 0 ? (int64_t)tv.total : 0;
   out->n = t->n;
@@ -900,8 +1004,9 @@ axb_status axb_bvh_predicate_order(const axb_bvh *t, int32_t predicate_kind,
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: unknown predicate kind");
   if (predicate_kind == AXB_PREDICATE_ORDERED_RAYS)
     predicate_kind = PRED_RAYS;
-  if (nq < 0 || nq >= (int64_t(1) << 31))
-    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: nq must be in [0, 2^31)");
+  // the sort's look-back words carry 30-bit counts
+  if (nq < 0 || nq >= MAX_SORTED_PREDICATES)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: nq must be in [0, 2^30)");
   hipStream_t s = (hipStream_t)stream_;
   if (nq == 0)
     return AXB_OK;
@@ -910,9 +1015,12 @@ axb_status axb_bvh_predicate_order(const axb_bvh *t, int32_t predicate_kind,
     AXB_HIP_TRY(launch_iota_u32(order, nq, s));
     return AXB_OK;
   }
+  CallErr err(s);
+  if (!err.p)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bvh_predicate_order: out of device memory");
   uint32_t *ord = nullptr;
   void *blk = nullptr;
-  axb_status st = predicate_order(t, predicate_kind, preds, nq, s, &ord, &blk);
+  axb_status st = predicate_order(t, err.p, predicate_kind, preds, nq, s, &ord, &blk);
   if (st != AXB_OK)
     return st;
   hipError_t e = hipMemcpyAsync(order, ord, (size_t)nq * sizeof(uint32_t),
@@ -920,7 +1028,7 @@ axb_status axb_bvh_predicate_order(const axb_bvh *t, int32_t predicate_kind,
   dfree(blk, s);
   if (e != hipSuccess)
     return hip_fail(e, "axb_bvh_predicate_order");
-  return check_kernel_error(t->d_err, s, "axb_bvh_predicate_order");
+  return check_kernel_error(err.p, s, "axb_bvh_predicate_order");
 }
 
 axb_status axb_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n,
@@ -969,7 +1077,10 @@ axb_status axb_bvh_neighbor_list(const axb_bvh *t, float radius, int32_t full,
     AXB_HIP_TRY(sync_stream(s));
     return AXB_OK;
   }
-  const TreeView tv = view_of(t);
+  CallErr err(s);
+  if (!err.p)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bvh_neighbor_list: out of device memory");
+  const TreeView tv = view_of(t, err.p);
   uint32_t *rowof = (uint32_t *)dalloc((size_t)n * sizeof(uint32_t), s);
   if (!rowof)
     AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bvh_neighbor_list: out of device memory");
@@ -984,7 +1095,7 @@ axb_status axb_bvh_neighbor_list(const axb_bvh *t, float radius, int32_t full,
     return launch_nlist(tv, t->node_format, f, radius, n_list, list, offsets, out, 0, nullptr,
                         MODE_FILL, s);
   };
-  axb_status st = crs_query(t->d_err, n, policy, rowof, true, first, fill, offsets, indices,
+  axb_status st = crs_query(err.p, n, policy, rowof, true, first, fill, offsets, indices,
                             nnz_host, s, "axb_bvh_neighbor_list");
   dfree(rowof, s);
   return st;
@@ -1077,6 +1188,9 @@ axb_status axb_bvh_query_nearest(const axb_bvh *t, const float *points, int64_t 
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_nearest: nq must be in [0, 2^31)");
   if (nq > 0 && !points)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_nearest: points is NULL");
+  if (policy.sort_predicates && nq >= MAX_SORTED_PREDICATES)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT,
+             "axb_bvh_query_nearest: sort_predicates needs fewer than 2^30 predicates");
   hipStream_t s = (hipStream_t)stream_;
   const int64_t per = (k < 1 || t->n == 0) ? 0 : std::min<int64_t>(k, t->n);
   const int64_t nnz = nq * per;
@@ -1085,6 +1199,9 @@ axb_status axb_bvh_query_nearest(const axb_bvh *t, const float *points, int64_t 
   *indices = nullptr;
   if (distances)
     *distances = nullptr;
+  CallErr err(s);
+  if (!err.p)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bvh_query_nearest: out of device memory");
   AXB_HIP_TRY(launch_knn_offsets(offsets, nq, per, s));
   *indices = (int32_t *)out_alloc((size_t)nnz * sizeof(int32_t), s);
   float *dist = distances ? (float *)out_alloc((size_t)nnz * sizeof(float), s) : nullptr;
@@ -1109,7 +1226,7 @@ axb_status axb_bvh_query_nearest(const axb_bvh *t, const float *points, int64_t 
     }
     if (e == hipSuccess && policy.sort_predicates)
     {
-      axb_status st = predicate_order(t, PRED_RADIUS, points, nq, s, &order, &order_blk);
+      axb_status st = predicate_order(t, err.p, PRED_RADIUS, points, nq, s, &order, &order_blk);
       if (st != AXB_OK)
       {
         out_free(*indices, s);
@@ -1120,14 +1237,14 @@ axb_status axb_bvh_query_nearest(const axb_bvh *t, const float *points, int64_t 
       }
     }
     if (e == hipSuccess)
-      e = launch_knn(view_of(t), t->node_format, t->geometry, points, nq, k, order, *indices,
+      e = launch_knn(view_of(t, err.p), t->node_format, t->geometry, points, nq, k, order, *indices,
                      dist ? dist : tmp_dist, s);
   }
   if (e == hipSuccess)
     e = sync_stream(s);
   dfree(order_blk, s);
   dfree(tmp_dist, s);
-  axb_status st = e == hipSuccess ? check_kernel_error(t->d_err, s, "axb_bvh_query_nearest")
+  axb_status st = e == hipSuccess ? check_kernel_error(err.p, s, "axb_bvh_query_nearest")
                                   : hip_fail(e, "axb_bvh_query_nearest");
   if (st != AXB_OK)
   {
@@ -1249,7 +1366,7 @@ axb_status axb_pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
   float *scene = nullptr;
   uint32_t *d_err = nullptr;
   hipError_t e = hipSuccess;
-  if (12 * n > (int64_t(64) << 20))
+  if (12 * n > (int64_t(64) << 20) && n < MAX_SORTED_PREDICATES)
   {
     scene = (float *)dalloc(6 * sizeof(float), s);
     d_err = (uint32_t *)dalloc(sizeof(uint32_t), s);
@@ -1396,17 +1513,20 @@ static axb_status bf_query_spatial(const axb_bruteforce *bf, int pred_kind, cons
     AXB_HIP_TRY(sync_stream(s));
     return AXB_OK;
   }
+  CallErr err(s);
+  if (!err.p)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "bruteforce query: out of device memory");
   // predicates in the caller's order: the values are read in the same order
   // by every lane, so there is no locality to gain from sorting them
   auto first = [&](int mode, int32_t *out, int32_t B) {
     return launch_bf_spatial(bf->values, bf->n, bf->geometry, pred_kind, preds, nq, radius,
-                             nullptr, offsets, out, B, mode, bf->d_err, s);
+                             nullptr, offsets, out, B, mode, err.p, s);
   };
   auto fill = [&](const uint32_t *list, int64_t n_list, int32_t *out) {
     return launch_bf_spatial(bf->values, bf->n, bf->geometry, pred_kind, preds, n_list, radius,
-                             list, offsets, out, 0, MODE_FILL, bf->d_err, s);
+                             list, offsets, out, 0, MODE_FILL, err.p, s);
   };
-  return crs_query(bf->d_err, nq, policy, nullptr, false, first, fill, offsets, indices,
+  return crs_query(err.p, nq, policy, nullptr, false, first, fill, offsets, indices,
                    nnz_host, s, "bruteforce query");
 }
 
@@ -1467,15 +1587,18 @@ axb_status axb_bruteforce_query_nearest_rays(const axb_bruteforce *bf, const flo
     AXB_HIP_TRY(sync_stream(s));
     return AXB_OK;
   }
+  CallErr err(s);
+  if (!err.p)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "bruteforce nearest rays: out of device memory");
   auto first = [&](int mode, int32_t *out, int32_t B) {
     return launch_bf_ray_nearest(bf->values, bf->n, rays, nq, kk, nullptr, offsets, out, B,
-                                 mode, bf->d_err, s);
+                                 mode, err.p, s);
   };
   auto fill = [&](const uint32_t *list, int64_t n_list, int32_t *out) {
     return launch_bf_ray_nearest(bf->values, bf->n, rays, n_list, kk, list, offsets, out, 0,
-                                 MODE_FILL, bf->d_err, s);
+                                 MODE_FILL, err.p, s);
   };
-  return crs_query(bf->d_err, nq, policy, nullptr, false, first, fill, offsets, indices,
+  return crs_query(err.p, nq, policy, nullptr, false, first, fill, offsets, indices,
                    nnz_host, s, "bruteforce nearest rays");
 }
 
@@ -1499,6 +1622,9 @@ axb_status axb_bruteforce_query_nearest(const axb_bruteforce *bf, const float *p
   *indices = nullptr;
   if (distances)
     *distances = nullptr;
+  CallErr err(s);
+  if (!err.p)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bruteforce_query_nearest: out of device memory");
   AXB_HIP_TRY(launch_knn_offsets(offsets, nq, per, s));
   *indices = (int32_t *)out_alloc((size_t)nnz * sizeof(int32_t), s);
   float *dist = distances ? (float *)out_alloc((size_t)nnz * sizeof(float), s) : nullptr;
@@ -1526,7 +1652,7 @@ axb_status axb_bruteforce_query_nearest(const axb_bruteforce *bf, const float *p
   if (e == hipSuccess)
     e = sync_stream(s);
   dfree(tmp_dist, s);
-  axb_status st = e == hipSuccess ? check_kernel_error(bf->d_err, s, "axb_bruteforce_query_nearest")
+  axb_status st = e == hipSuccess ? check_kernel_error(err.p, s, "axb_bruteforce_query_nearest")
                                   : hip_fail(e, "axb_bruteforce_query_nearest");
   if (st != AXB_OK)
   {

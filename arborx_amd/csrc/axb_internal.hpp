@@ -53,16 +53,42 @@ struct HostScope
   bool on_;
   std::chrono::steady_clock::time_point t0_;
 };
-// Cached scratch buffer of at least `bytes` for calls on stream s (grow-only,
-// kept between calls; released by axb_trim_memory).  nullptr when out of
-// memory.
+// Scratch buffer of at least `bytes` for one call on stream s.  Each
+// (device, stream, slot) keeps one grow-only block between calls (released
+// by axb_trim_memory); a call takes it for its duration, and a call that
+// finds it taken (another host thread on the same stream) gets a block of
+// its own, freed when the lease ends.  p is nullptr when out of memory.
 enum WsSlot
 {
   WS_ROWS = 0,     // scratch rows of the radius CRS
   WS_KNN_CAND = 1, // kNN candidate rows
   WS_NUM = 2
 };
-void *ws_get(hipStream_t s, int slot, size_t bytes);
+struct WsLease
+{
+  void *p = nullptr;
+  hipStream_t s = nullptr;
+  int dev = 0, slot = 0;
+  bool cached = false;
+  WsLease() = default;
+  WsLease(const WsLease &) = delete;
+  WsLease &operator=(const WsLease &) = delete;
+  // takes a block of at least `bytes` (a lease held already is released first)
+  void acquire(hipStream_t s, int slot, size_t bytes);
+  void release();
+  ~WsLease() { release(); }
+};
+
+// Limits that are normally derived from the device, lowered by tests
+// (axb_debug_set_limits) to run the multi-launch and fallback paths on
+// small inputs.  0 = default.
+struct Limits
+{
+  int64_t knn_batch_max;      // kNN queries per collect launch
+  int64_t scratch_rows_bytes; // scratch rows of a buffered CRS query
+  int32_t knn_rounds;         // collect rounds before the exact fallback
+};
+Limits debug_limits();
 // hipStreamSynchronize, timed as "host:stream_sync" while profiling.
 hipError_t sync_stream(hipStream_t s);
 #define AXB_LAUNCH(kernel, grid, block, shmem, stream, ...)                    \
@@ -76,6 +102,10 @@ hipError_t sync_stream(hipStream_t s);
 unsigned long long *visit_counters();
 
 // ------------------------------------------------------------ sort / scan
+// The onesweep passes carry 30-bit counts in their look-back words, so a
+// sort takes fewer than 2^30 keys (trees: n < 2^30; predicate batches that
+// are sorted by Morton code: nq < 2^30).
+constexpr int64_t MAX_SORTED_PREDICATES = int64_t(1) << 30;
 constexpr int SORT_BLOCK = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
@@ -269,6 +299,17 @@ hipError_t launch_make_rays(const float *origins, const float *directions, int64
 hipError_t launch_knn(const TreeView &tv, int node_format, int geometry, const float *pts,
                       int64_t nq, int k, const uint32_t *order, int32_t *oi, float *od,
                       hipStream_t s);
+// Buffers of a kNN call for k <= 64: candidate rows for `batch` queries
+// (batch * knn_row_capacity(k, n) entries, batch * capacity + capacity
+// below 2^32), the list of unresolved queries (nq entries) and its counter.
+struct KnnScratch
+{
+  uint2 *cand;
+  int64_t batch;
+  uint32_t *ovf;
+  uint32_t *ovf_n;
+};
+int knn_row_capacity(int k, int n);
 hipError_t launch_knn_offsets(int64_t *offsets, int64_t nq, int64_t per, hipStream_t s);
 hipError_t launch_pred_morton32(int pred_kind, const float *preds, int64_t nq,
                                 const float *scene, uint32_t *keys, uint32_t *hist,

@@ -857,8 +857,8 @@ template <int F, int G, int K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 && G == GEOM_POINTS) ? 8 : 1))) void k_knn_collect(
     TreeView tv, const float *__restrict__ pts, int64_t t0, int64_t tn, int k, int W,
     const uint32_t *__restrict__ order, uint2 *__restrict__ cand, int cap, float shrink,
-    uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_n, int32_t *__restrict__ oi,
-    float *__restrict__ od)
+    int max_rounds, uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_n,
+    int32_t *__restrict__ oi, float *__restrict__ od)
 {
   const int64_t g = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= tn)
@@ -875,7 +875,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 &&
   int rounds = 0;
   float R_hi = R;   // radius known to hold >= k points
   float R_lo = 0.f; // radius known to hold < k points
-  for (int round = 0; round < KNN_ROUNDS; ++round)
+  for (int round = 0; round < max_rounds; ++round)
   {
     ++rounds;
     c = knn_collect_walk<F, G>(tv, qx, qy, qz, R, R_hi, R_lo, shrink, cand, row_at, cap, v_int,
@@ -1213,22 +1213,65 @@ hipError_t launch_spatial(const TreeView &tv, int node_format, int geometry, int
                                        offsets, out, buffer, mode, s);
 }
 
+// Shape of the collect launches for k <= 64 (the candidate-row capacity and
+// the radius factor after an overflowing round).
+struct KnnShape
+{
+  int W, cap;
+  float shrink;
+};
+static KnnShape knn_shape(int k, int n)
+{
+  KnnShape sh;
+  // seed window: 6k leaves (measured at 10M points, k = 16: 3k 13.9 ms,
+  // 4k 13.2, 6k 13.1, 8k 13.15 -- a tighter seed radius against a longer
+  // seed scan)
+  sh.W = std::min(n, std::max(6 * k, 16));
+  // row capacity: about 6k, a multiple of 8 (10M points, k = 16: 4k 13.1 ms
+  // uniform / 20.0 ms Gaussian mixture, 6k 12.85 / 20.9, 8k 13.1 / 23.2 --
+  // a full row costs an in-place shrink, a long row a longer selection)
+  sh.cap = 6 * k > 64 ? (6 * k + 7) / 8 * 8 : 64;
+  // radius factor after an overflowing round (cap + 1 leaves within R):
+  // aim at 2m leaves, m = min(k, n)
+  sh.shrink = std::cbrt((float)(2 * std::min(k, n)) / (float)(sh.cap + 1));
+  return sh;
+}
+
+int knn_row_capacity(int k, int n) { return knn_shape(k, n).cap; }
+
+template <int F, int G, int K>
+static hipError_t knn_run(const TreeView &tv, const float *pts, int64_t nq, int k,
+                          const uint32_t *order, int32_t *oi, float *od, const KnnScratch &ws,
+                          hipStream_t s)
+{
+  const KnnShape sh = knn_shape(k, tv.n);
+  const int lim = debug_limits().knn_rounds;
+  const int rounds = lim > 0 ? lim : (lim < 0 ? 0 : KNN_ROUNDS);
+  hipError_t e = fill_u32(ws.ovf_n, 0u, 1, s);
+  for (int64_t t0 = 0; e == hipSuccess && t0 < nq; t0 += ws.batch)
+  {
+    const int64_t tn = std::min(ws.batch, nq - t0);
+    AXB_LAUNCH((k_knn_collect<F, G, K>), dim3(grid_of(tn)), dim3(256), 0, s, tv, pts, t0, tn, k,
+               sh.W, order, ws.cand, sh.cap, sh.shrink, rounds, ws.ovf, ws.ovf_n, oi, od);
+    e = hipGetLastError();
+  }
+  // unresolved queries of every batch, in one launch at the end: each of
+  // them is one lane walking alone, so launches of a few lanes per batch
+  // would each cost the longest such walk
+  if (e == hipSuccess)
+  {
+    AXB_LAUNCH((k_knn_fallback<F, G, K>), dim3((unsigned)std::min<int64_t>(grid_of(nq), 512)),
+               dim3(256), 0, s, tv, pts, k, order, ws.ovf, ws.ovf_n, oi, od);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
 template <int F, int G, int K>
 static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq, int k,
                                const uint32_t *order, int32_t *oi, float *od, hipStream_t s)
 {
-  const int n = tv.n;
-  // seed window: 6k leaves (measured at 10M points, k = 16: 3k 13.9 ms,
-  // 4k 13.2, 6k 13.1, 8k 13.15 -- a tighter seed radius against a longer
-  // seed scan)
-  const int W = std::min(n, std::max(6 * k, 16));
-  // row capacity: about 6k, a multiple of 8 (10M points, k = 16: 4k 13.1 ms
-  // uniform / 20.0 ms Gaussian mixture, 6k 12.85 / 20.9, 8k 13.1 / 23.2 --
-  // a full row costs an in-place shrink, a long row a longer selection)
-  const int cap = 6 * k > 64 ? (6 * k + 7) / 8 * 8 : 64;
-  // radius factor after an overflowing round (cap + 1 leaves within R):
-  // aim at 2m leaves, m = min(k, n)
-  const float shrink = std::cbrt((float)(2 * std::min(k, n)) / (float)(cap + 1));
+  const int cap = knn_shape(k, tv.n).cap;
   // queries per launch: candidate rows of up to 8 GiB, row offsets within
   // 32 bits (10M points, k = 16: 4 launches 13.3 ms uniform / 20.9 ms
   // Gaussian mixture, 2 launches 12.7 / 17.5, 1 launch 12.4 / 14.8 -- every
@@ -1236,34 +1279,25 @@ static hipError_t knn_pipeline(const TreeView &tv, const float *pts, int64_t nq,
   int64_t batch = std::max<int64_t>(
       1, std::min<int64_t>(nq, std::min<int64_t>((int64_t(1) << 33) / (cap * 8),
                                                  (int64_t(UINT32_MAX) - cap) / cap)));
+  const int64_t lim = debug_limits().knn_batch_max;
+  if (lim > 0)
+    batch = std::min(batch, lim);
   // smaller launches when the device cannot spare that much
-  uint2 *cand = nullptr;
-  while (!(cand = (uint2 *)ws_get(s, WS_KNN_CAND, (size_t)batch * cap * sizeof(uint2))) &&
-         batch > 65536)
+  WsLease rows;
+  while (true)
+  {
+    rows.acquire(s, WS_KNN_CAND, (size_t)batch * cap * sizeof(uint2));
+    if (rows.p || batch <= 65536)
+      break;
     batch = (batch + 1) / 2;
-  // unresolved queries of every batch, for one fallback launch at the end:
-  // each of them is one lane walking alone, so launches of a few lanes per
-  // batch would each cost the longest such walk
-  uint32_t *ovf = (uint32_t *)dalloc((size_t)nq * sizeof(uint32_t), s);
-  uint32_t *ovf_n = (uint32_t *)dalloc(sizeof(uint32_t), s);
-  hipError_t e = (cand && ovf && ovf_n) ? hipSuccess : hipErrorOutOfMemory;
-  if (e == hipSuccess)
-    e = fill_u32(ovf_n, 0u, 1, s);
-  for (int64_t t0 = 0; e == hipSuccess && t0 < nq; t0 += batch)
-  {
-    const int64_t tn = std::min(batch, nq - t0);
-    AXB_LAUNCH((k_knn_collect<F, G, K>), dim3(grid_of(tn)), dim3(256), 0, s, tv, pts, t0, tn, k,
-               W, order, cand, cap, shrink, ovf, ovf_n, oi, od);
-    e = hipGetLastError();
   }
+  KnnScratch ws{(uint2 *)rows.p, batch, (uint32_t *)dalloc((size_t)nq * sizeof(uint32_t), s),
+                (uint32_t *)dalloc(sizeof(uint32_t), s)};
+  hipError_t e = (ws.cand && ws.ovf && ws.ovf_n) ? hipSuccess : hipErrorOutOfMemory;
   if (e == hipSuccess)
-  {
-    AXB_LAUNCH((k_knn_fallback<F, G, K>), dim3((unsigned)std::min<int64_t>(grid_of(nq), 512)),
-               dim3(256), 0, s, tv, pts, k, order, ovf, ovf_n, oi, od);
-    e = hipGetLastError();
-  }
-  dfree(ovf, s);
-  dfree(ovf_n, s);
+    e = knn_run<F, G, K>(tv, pts, nq, k, order, oi, od, ws, s);
+  dfree(ws.ovf, s);
+  dfree(ws.ovf_n, s);
   return e;
 }
 

@@ -341,7 +341,8 @@ axb_status axb_bvh_view(const axb_bvh *bvh, axb_tree_view *out_host);
  * predicate_kind: AXB_PREDICATE_SPHERES / _BOXES, 2 for points (3
  * floats each), or AXB_PREDICATE_RAYS / _ORDERED_RAYS (a ray's centroid is
  * its origin, ArborX_Ray.hpp:76-80).  order: device uint32, nq entries
- * (caller-allocated). */
+ * (caller-allocated), nq < 2^30 (the sort's counts are 30-bit; queries
+ * with sort_predicates set take the same limit). */
 axb_status axb_bvh_predicate_order(const axb_bvh *bvh, int32_t predicate_kind,
                                    const float *predicates, int64_t nq,
                                    uint32_t *order, axb_stream_t stream);
@@ -390,6 +391,17 @@ axb_status axb_profile_record(int32_t i, char *name, int32_t name_len,
  * four counters for the nearest queries that took the exact fallback walk,
  * then {collect rounds run, queries needing more than one round, 0, 0}. */
 axb_status axb_profile_visits(int64_t out_host[12]);
+
+/* Test hook (no reference counterpart): lowers limits the library normally
+ * derives from the device, so that small inputs run the paths large ones
+ * take -- kNN queries per collect launch (several launches and one
+ * fallback launch after them), bytes of the scratch rows of a buffered
+ * CRS query (below the rows' need: plain count + fill), and collect rounds
+ * before a kNN query goes to the exact fallback walk (-1: none, every query
+ * takes the fallback).  0 restores a default.  Process-wide; set it between
+ * calls. */
+axb_status axb_debug_set_limits(int64_t knn_batch_max, int64_t scratch_rows_bytes,
+                                int32_t knn_rounds);
 
 /* Host helper used by the query kernels, exported for unit tests: the largest
  * float s with RN(sqrt(s)) <= r, so that `d2 <= t` decides exactly the

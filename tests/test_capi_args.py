@@ -124,3 +124,36 @@ def test_size_and_bounds_of_null_handles():
     assert L.axb_bruteforce_bounds(None, ctypes.cast(out, P)) == INVALID
     L.axb_bvh_destroy(None)  # no-ops
     L.axb_bruteforce_destroy(None)
+
+
+def test_sorted_predicate_batches_below_2_30():
+    """The predicate sort's look-back words hold 30-bit counts: a batch of
+    2^30 or more predicates cannot be sorted by Morton code, and asking for
+    it is an argument error (checked before any device work)."""
+    L = ax.lib()
+    ind = P()
+    dist = P()
+    nnz = I64()
+    dummy = P(16)
+    big = 1 << 30
+    sort = ax.TraversalPolicy(sort_predicates=True)._c()
+    assert L.axb_bvh_query_intersects(dummy, ax.PREDICATE_SPHERES, dummy, big, sort, dummy,
+                                      ctypes.byref(ind), ctypes.byref(nnz), None) == INVALID
+    assert "2^30" in _last_error()
+    assert L.axb_bvh_query_intersects_radius(dummy, dummy, big, 0.5, sort, dummy,
+                                             ctypes.byref(ind), ctypes.byref(nnz),
+                                             None) == INVALID
+    assert L.axb_bvh_query_nearest(dummy, dummy, big, 4, sort, dummy, ctypes.byref(ind),
+                                   ctypes.byref(dist), ctypes.byref(nnz), None) == INVALID
+    assert "2^30" in _last_error()
+    assert L.axb_bvh_predicate_order(dummy, ctypes.c_int32(ax.PREDICATE_SPHERES), dummy,
+                                     I64(big), dummy, None) == INVALID
+
+
+def test_debug_limits_arguments():
+    L = ax.lib()
+    assert L.axb_debug_set_limits(-1, 0, 0) == INVALID
+    assert L.axb_debug_set_limits(0, -1, 0) == INVALID
+    assert L.axb_debug_set_limits(0, 0, 65) == INVALID
+    assert L.axb_debug_set_limits(0, 0, -2) == INVALID
+    assert L.axb_debug_set_limits(0, 0, 0) == ax.AXB_OK

@@ -384,6 +384,41 @@ def test_pair_energy_fd_subregion():
     assert np.linalg.norm(g - fd) / np.linalg.norm(fd) <= 1e-5
 
 
+def test_pair_energy_closed_forms():
+    """The GPU energy against closed forms, not only against the oracle: a
+    dimer (E = phi(r) - phi(rc), dE/dx_0 = -phi'(r) along the axis) and the
+    interior atoms of a perfect FCC crystal (E per atom = 1/2 sum over the
+    six shells n_s (phi(r_s) - phi(rc)), zero gradient).  Float pair terms,
+    double sums: relative tolerance 1e-6 on E."""
+    from tests.test_oracle import fcc_integer_block, lj
+    rc = 2.5
+    r = 1.5
+    x = np.array([[0.25, 0.5, 0.75], [0.25 + r, 0.5, 0.75]], np.float32)
+    bvh = ax.BoundingVolumeHierarchy(dev(x))
+    idx, off = bvh.query(ax.make_intersects(dev(x), rc))
+    e, g = ax.pair_energy(dev(x), off, idx, rc)
+    phi, dphi = lj(r)
+    phic, _ = lj(rc)
+    assert float(host(e)) == pytest.approx(phi - phic, rel=1e-6)
+    np.testing.assert_allclose(host(g)[0], [-dphi, 0, 0], rtol=1e-5, atol=1e-6)
+    x = fcc_integer_block(7)
+    rcf = float(np.float32(2.5 * math.sqrt(2.0)))
+    bvh = ax.BoundingVolumeHierarchy(dev(x))
+    idx, off = bvh.query(ax.make_intersects(dev(x), rcf))
+    off_h, idx_h = host(off), host(idx)
+    interior = np.nonzero(np.abs(x - 7.0).max(1) <= 2.0)[0]
+    cnt = np.zeros(x.shape[0], np.int64)
+    cnt[interior] = np.diff(off_h)[interior]
+    o2 = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+    i2 = np.concatenate([idx_h[off_h[i]:off_h[i + 1]] for i in interior]).astype(np.int32)
+    e, g = ax.pair_energy(dev(x), dev(o2), dev(i2), rcf)
+    phic, _ = lj(rcf)
+    shells = {2: 12, 4: 6, 6: 24, 8: 12, 10: 24, 12: 8}
+    per_atom = 0.5 * sum(n * (lj(math.sqrt(s))[0] - phic) for s, n in shells.items())
+    assert float(host(e)) == pytest.approx(len(interior) * per_atom, rel=1e-6)
+    assert np.abs(host(g)[interior]).max() <= 1e-5
+
+
 @pytest.mark.parametrize("recompute", [False, True])
 def test_pair_energy_autograd(recompute):
     rc = 2.5

@@ -1,0 +1,182 @@
+"""GPU parity at the headline size: 10M points uniform in [0,1)^3 (seed 2)
+and 10M points from the 1000-centre Gaussian mixture (sigma 0.01, clamped to
+the cube, seed 3), both node formats, against the oracle on the host's
+OpenMP cores:
+
+* Morton codes, permutation and the whole reference-form topology
+  bit-exact, and the pre-order node arrays bit-exact;
+* radius rows at about 32 neighbours identical over all 10M queries (same
+  rows, same order within each row);
+* kNN k = 16 rows and distances bit-exact over all 10M queries.
+
+The size-dependent paths of the library run here under a check: one kNN
+launch with gigabytes of candidate rows, the Gaussian mixture's clusters
+and its points clamped onto the cube's faces and edges, several kNN launches with a
+ragged last one and the exact fallback walk (forced through
+axb_debug_set_limits), and the radius query's plain count + fill path and
+its overflow refill (scratch-row budget lowered the same way)."""
+import numpy as np
+import pytest
+import torch
+
+import arborx_amd as ax
+import oracle
+from arborx_amd import synthetic
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
+
+N = 10_000_000
+K = 16
+# Radii for about 32 neighbours per point (self included): the uniform
+# set's closed form (3 * 32 / (4 pi N))^(1/3), and for the mixture the
+# radius a bisection on the oracle's mean count over every 100th point
+# settles at (31.99999 neighbours).
+RADIUS = {"uniform": float(np.float32(synthetic.radius_for_neighbours(N, 32))),
+          "gaussian": float(np.float32(0.00284968))}
+
+
+def dataset(kind):
+    if kind == "uniform":
+        return synthetic.uniform_points(N, 2)
+    return synthetic.gaussian_mixture_points(N, 3, n_centres=1000, sigma=0.01)
+
+
+class Case:
+    """Points, oracle tree and oracle results of one dataset (computed once)."""
+
+    def __init__(self, kind):
+        self.kind = kind
+        self.pts = dataset(kind)
+        self.tree = oracle.Tree(self.pts)
+        self._radius = None
+        self._knn = None
+        self.dev = torch.from_numpy(self.pts).cuda()
+
+    def radius_rows(self):
+        if self._radius is None:
+            r = np.float32(RADIUS[self.kind])
+            sph = np.concatenate([self.pts, np.full((N, 1), r, np.float32)], 1)
+            self._radius = self.tree.query_spatial(0, sph)
+        return self._radius
+
+    def knn_rows(self):
+        if self._knn is None:
+            self._knn = self.tree.query_knn(self.pts, K)
+        return self._knn
+
+
+_cases = {}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _release_cases():
+    yield
+    _cases.clear()
+    ax.trim_memory()
+
+
+def case(kind):
+    if kind not in _cases:
+        _cases[kind] = Case(kind)
+    return _cases[kind]
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("kind", ["uniform", "gaussian"])
+def test_topology_and_nodes_at_10m(kind):
+    c = case(kind)
+    bvh = ax.BoundingVolumeHierarchy(c.dev, node_format="q16", keep_topology=True)
+    np.testing.assert_array_equal(np.array(bvh.bounds(), np.float32), c.tree.bounds())
+    e = bvh.export_topology()
+    r = c.tree.export()
+    np.testing.assert_array_equal(host(e["keys"]).view(np.uint64), r["keys"])
+    np.testing.assert_array_equal(host(e["perm"]).view(np.uint32), r["perm"])
+    np.testing.assert_array_equal(host(e["leaf_rope"]), r["leaf_rope"])
+    np.testing.assert_array_equal(host(e["left_child"]), r["left_child"])
+    np.testing.assert_array_equal(host(e["rope"]), r["rope"])
+    np.testing.assert_array_equal(host(e["boxes"]).view(np.uint32), r["boxes"].view(np.uint32))
+    if kind == "gaussian":
+        # clamping puts about 269k points on the cube's faces and 1.5k on
+        # its edges (coordinates exactly 0 or 1), where many codes share
+        # their high bits
+        on = ((c.pts == 0) | (c.pts == 1)).sum(1)
+        assert (on >= 1).sum() > 200_000 and (on >= 2).sum() > 1000
+    del e, bvh
+    for fmt, code in (("q16", 1), ("fp32", 0)):
+        bvh = ax.BoundingVolumeHierarchy(c.dev, node_format=fmt)
+        ref = c.tree.linearize(code, flat_max=ax.FLAT_SUBTREE_MAX)
+        got = host(bvh.slots()).view(np.uint32)
+        assert got.shape == ref.shape, fmt
+        np.testing.assert_array_equal(got, ref)
+        del bvh
+
+
+@pytest.mark.parametrize("fmt", ["q16", "fp32"])
+@pytest.mark.parametrize("kind", ["uniform", "gaussian"])
+def test_radius_rows_at_10m(kind, fmt):
+    c = case(kind)
+    o_ref, i_ref = c.radius_rows()
+    mean = o_ref[-1] / N
+    assert 30.0 < mean < 34.0, mean
+    bvh = ax.BoundingVolumeHierarchy(c.dev, node_format=fmt)
+    idx, off = bvh.query(ax.make_intersects(c.dev, RADIUS[kind]))
+    np.testing.assert_array_equal(host(off), o_ref)
+    np.testing.assert_array_equal(host(idx), i_ref)
+
+
+@pytest.mark.parametrize("fmt", ["q16", "fp32"])
+@pytest.mark.parametrize("kind", ["uniform", "gaussian"])
+def test_knn_rows_at_10m(kind, fmt):
+    c = case(kind)
+    _, ki_ref, kd_ref = c.knn_rows()
+    bvh = ax.BoundingVolumeHierarchy(c.dev, node_format=fmt)
+    ki, ko, kd = bvh.query(ax.make_nearest(c.dev, K), return_distances=True)
+    np.testing.assert_array_equal(host(ki), ki_ref)
+    np.testing.assert_array_equal(host(kd).view(np.uint32), kd_ref.view(np.uint32))
+
+
+def test_knn_several_launches_and_exact_fallback_at_10m():
+    """kNN in launches of 1,000,003 queries (ten launches, the last one
+    ragged), then with no collect rounds at all on a 1M-query slice, so that
+    every query of the slice takes the exact fallback walk."""
+    c = case("gaussian")
+    _, ki_ref, kd_ref = c.knn_rows()
+    bvh = ax.BoundingVolumeHierarchy(c.dev, node_format="q16")
+    with ax.debug_limits(knn_batch_max=1_000_003):
+        with ax.profile(timing=True) as prof:
+            ki, _, kd = bvh.query(ax.make_nearest(c.dev, K), return_distances=True)
+    launches = sum(len(v) for k, v in prof.kernels.items() if "k_knn_collect" in k)
+    assert launches == 10, launches
+    np.testing.assert_array_equal(host(ki), ki_ref)
+    np.testing.assert_array_equal(host(kd).view(np.uint32), kd_ref.view(np.uint32))
+    m = 1_000_000
+    q = c.dev[3 * N // 5: 3 * N // 5 + m].contiguous()
+    with ax.debug_limits(knn_rounds=-1):
+        with ax.profile(timing=False, visits=True) as pv:
+            ki, _, kd = bvh.query(ax.make_nearest(q, K), return_distances=True)
+    assert pv.visits["fallback_queries"] == m
+    a = 3 * N // 5 * K
+    np.testing.assert_array_equal(host(ki), ki_ref[a:a + m * K])
+    np.testing.assert_array_equal(host(kd).view(np.uint32), kd_ref[a:a + m * K].view(np.uint32))
+
+
+@pytest.mark.parametrize("rows_bytes", [4 * N * 20, 1])
+def test_radius_count_fill_and_refill_at_10m(rows_bytes):
+    """Scratch rows of 20 results (most rows overflow and are walked again
+    in place), and no scratch rows at all (count pass, scan, fill pass)."""
+    c = case("uniform")
+    o_ref, i_ref = c.radius_rows()
+    bvh = ax.BoundingVolumeHierarchy(c.dev, node_format="q16")
+    with ax.debug_limits(scratch_rows_bytes=rows_bytes):
+        with ax.profile(timing=True) as prof:
+            idx, off = bvh.query(ax.make_intersects(c.dev, RADIUS["uniform"]))
+    names = " ".join(prof.kernels)
+    if rows_bytes == 1:
+        assert "MODE_COUNT" in names and "MODE_BUFFER" not in names
+    else:
+        assert "MODE_BUFFER" in names and "MODE_FILL" in names
+    np.testing.assert_array_equal(host(off), o_ref)
+    np.testing.assert_array_equal(host(idx), i_ref)

@@ -412,6 +412,98 @@ def test_pair_energy_gradient_fd_f64():
             assert abs(fd - g[i, d]) <= 1e-6 * max(1.0, abs(g[i, d])), (i, d, fd, g[i, d])
 
 
+def lj(r):
+    """Unshifted Lennard-Jones 4(r^-12 - r^-6) and its derivative, closed form."""
+    return 4.0 * (r ** -12 - r ** -6), 4.0 * (-12.0 * r ** -13 + 6.0 * r ** -7)
+
+
+def full_rows(x, rc):
+    t = oracle.Tree(x)
+    return t.query_spatial(0, spheres_from_points(x, np.float32(rc)))
+
+
+@pytest.mark.parametrize("r", [1.0, 1.5, 0.875, 2.25])
+def test_pair_energy_dimer_closed_form(r):
+    """Two atoms at distance r < rc: every pair appears in both rows of the
+    full list, so E = 1/2 * 2 * (phi(r) - phi(rc)) = phi(r) - phi(rc), and
+    dE/dx_0 = phi'(r) (x_0 - x_1) / r.  A dropped 1/2 doubles E, a dropped
+    shift moves it by phi(rc); both fail here."""
+    rc = 2.5
+    x = np.array([[0.25, 0.5, 0.75], [0.25 + r, 0.5, 0.75]], np.float32)
+    off, idx = full_rows(x, rc)
+    assert list(off) == [0, 2, 4]
+    e, g = oracle.pair_energy_f64(x, off, idx, rc)
+    phi, dphi = lj(r)
+    phic, _ = lj(rc)
+    assert e == pytest.approx(phi - phic, rel=1e-14, abs=1e-15)
+    np.testing.assert_allclose(g[0], [-dphi, 0.0, 0.0], rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(g[1], [dphi, 0.0, 0.0], rtol=1e-13, atol=1e-15)
+
+
+def test_pair_energy_dimer_at_minimum():
+    """At r = 2^(1/6) phi(r) = -1 and phi'(r) = 0: E = -1 - phi(rc).  The
+    float position is off by one rounding, which moves E by O(eps^2) only."""
+    rc = 2.5
+    r = np.float32(2.0 ** (1.0 / 6.0))
+    x = np.array([[0.0, 0.0, 0.0], [0.0, r, 0.0]], np.float32)
+    off, idx = full_rows(x, rc)
+    e, g = oracle.pair_energy_f64(x, off, idx, rc)
+    phic, _ = lj(rc)
+    assert e == pytest.approx(-1.0 - phic, abs=1e-12)
+    assert np.abs(g).max() <= 1e-5
+    # a pair outside the cutoff contributes nothing
+    x2 = np.array([[0.0, 0.0, 0.0], [0.0, 2.5001, 0.0]], np.float32)
+    off2, idx2 = full_rows(x2, rc)
+    e2, g2 = oracle.pair_energy_f64(x2, off2, idx2, rc)
+    assert e2 == 0.0 and not g2.any()
+
+
+def fcc_integer_block(m):
+    """Unjittered FCC block on integer coordinates: cubic cell edge 2, basis
+    (0,0,0) (1,1,0) (1,0,1) (0,1,1), m cells per side.  Nearest-neighbour
+    distance sqrt(2); squared pair distances are exact integers in float and
+    double."""
+    basis = np.array([[0, 0, 0], [1, 1, 0], [1, 0, 1], [0, 1, 1]], np.float64)
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1)
+    p = (2.0 * g.reshape(-1, 1, 3) + basis[None]).reshape(-1, 3)
+    return p.astype(np.float32)
+
+
+def test_pair_energy_fcc_interior_closed_form():
+    """Interior atoms of a perfect FCC crystal with cutoff 2.5 nearest-
+    neighbour distances: shells at squared distance 2, 4, 6, 8, 10, 12 (in
+    units where the nearest-neighbour distance is sqrt(2)) hold 12, 6, 24,
+    12, 24, 8 atoms; the next shell (14) lies outside 2.5 * sqrt(2).  With
+    only atom i's row present, E = 1/2 sum_s n_s (phi(r_s) - phi(rc)), and
+    the gradient of i vanishes by symmetry."""
+    x = fcc_integer_block(7)
+    rc = 2.5 * math.sqrt(2.0)
+    rcf = float(np.float32(rc))
+    off, idx = full_rows(x, rcf)
+    centre = np.array([7.0, 7.0, 7.0])
+    interior = np.nonzero(np.abs(x - centre).max(1) <= 2.0)[0]
+    assert len(interior) >= 20
+    shells = {2: 12, 4: 6, 6: 24, 8: 12, 10: 24, 12: 8}
+    phic, _ = lj(rcf)
+    per_atom = 0.5 * sum(n * (lj(math.sqrt(s))[0] - phic) for s, n in shells.items())
+    for i in interior[:20]:
+        row = idx[off[i]:off[i + 1]]
+        assert len(row) == 87  # 86 neighbours + the atom itself
+        o1 = np.zeros_like(off)
+        o1[i + 1:] = len(row)
+        e, g = oracle.pair_energy_f64(x, o1, row, rcf)
+        assert e == pytest.approx(per_atom, rel=1e-13)
+        assert np.abs(g[i]).max() <= 1e-12
+    # all interior rows together: E = |interior| * per_atom
+    keep = np.zeros(x.shape[0], bool)
+    keep[interior] = True
+    cnt = np.where(keep, np.diff(off), 0)
+    o2 = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+    i2 = np.concatenate([idx[off[i]:off[i + 1]] for i in interior]).astype(np.int32)
+    e_all, _ = oracle.pair_energy_f64(x, o2, i2, rcf)
+    assert e_all == pytest.approx(len(interior) * per_atom, rel=1e-12)
+
+
 # ------------------------------------------------------------ neighbour lists
 def _kat_radius(r):
     return np.float32(np.inf) if r == "inf" else np.float32(r)
