@@ -17,8 +17,10 @@ The roofline object is for the dominant kernel, with its duration taken from
 HIP events the library records around each launch on its own stream and its
 algorithmic bytes from the node records the traversal reads (counted by the
 kernel itself in an untimed pass).  cpu_baseline times the CPU oracle
-(C++/OpenMP restatement of ArborX's Serial/OpenMP path) on a 1M-point
-sample of the same workload.
+(C++/OpenMP restatement of ArborX's Serial/OpenMP path) on bounded samples
+of the same workload, on every OpenMP thread and on one thread (median of 5
+steps after a warm-up).  The 100M-point moving-points loop (3 steps per node
+format by default) follows the 10M cases.
 """
 import argparse
 import json
@@ -44,7 +46,10 @@ def parse():
     ap.add_argument("--neighbours", type=int, default=32)
     ap.add_argument("--datasets", default="uniform,gaussian")
     ap.add_argument("--formats", default="q16,fp32")
-    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000,
+                    help="points of the CPU baseline on all OpenMP threads")
+    ap.add_argument("--cpu-serial-sample", type=int, default=200_000,
+                    help="points of the CPU baseline on one thread")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--box-n", type=int, default=5_000_000,
@@ -53,7 +58,7 @@ def parse():
                     help="rays over the box-primitive case's data boxes (0 = skip)")
     ap.add_argument("--fcc-cells", default="80,80,78",
                     help="FCC cubic cells of the pair-energy case ('' = skip)")
-    ap.add_argument("--e2e-steps", type=int, default=0,
+    ap.add_argument("--e2e-steps", type=int, default=3,
                     help="steps of the moving-points loop (0 = skip)")
     ap.add_argument("--e2e-n", type=int, default=100_000_000)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -67,17 +72,38 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def calibrate_radius(ax, p, r, target, sample=100, iters=4):
-    """Radius with about `target` neighbours per point (mean over every
-    `sample`-th point), by the cube-root rule on measured counts (untimed)."""
+def calibrate_radius(ax, p, r, target, sample=100, iters=24):
+    """Radius with `target` neighbours per point on average (over every
+    `sample`-th point, untimed): bracket by doubling / halving from r, then
+    bisect on the measured mean until the bracket is one float apart or the
+    mean is within 0.05% of the target."""
+    import numpy as np
     bvh = ax.BoundingVolumeHierarchy(p)
     q = p[::sample].contiguous()
+
+    def mean(rr):
+        _, off = bvh.query(ax.make_intersects(q, float(np.float32(rr))))
+        return float(off[-1].item()) / q.shape[0]
+
+    lo, hi = r, r
+    while mean(lo) > target:
+        lo *= 0.5
+    while mean(hi) < target:
+        hi *= 2.0
     for _ in range(iters):
-        _, off = bvh.query(ax.make_intersects(q, r))
-        mean = float(off[-1].item()) / q.shape[0]
-        r = r * (target / max(mean, 1e-3)) ** (1.0 / 3.0)
-    import numpy as np
-    return float(np.float32(r))
+        mid = float(np.float32(0.5 * (lo + hi)))
+        if not (lo < mid < hi):
+            break
+        m = mean(mid)
+        if abs(m - target) <= 5e-4 * target:
+            lo = hi = mid
+            break
+        if m < target:
+            lo = mid
+        else:
+            hi = mid
+    best = min((lo, hi), key=lambda rr: abs(mean(rr) - target))
+    return float(np.float32(best))
 
 
 PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_uniform10M_q16.json")
@@ -312,6 +338,90 @@ def bench_e2e(ax, args, torch, synthetic, stream, fmt):
     return out
 
 
+def sig(v, digits=6):
+    """Floats of a JSON tree to `digits` significant digits (small values
+    such as relative errors keep their magnitude)."""
+    if isinstance(v, float):
+        return float(f"{v:.{digits}g}") if math.isfinite(v) else v
+    if isinstance(v, dict):
+        return {k: sig(x, digits) for k, x in v.items()}
+    if isinstance(v, (list, tuple)):
+        return [sig(x, digits) for x in v]
+    return v
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_step_times(oracle, np, synthetic, n, neighbours, k, reps):
+    """Per-phase seconds of `reps` repetitions (after one warm-up) of one
+    step -- build, two-pass radius CRS, kNN -- of the oracle over the first
+    n points of the uniform set, with the radius for `neighbours` at that
+    N.  Wall clock around each phase (the oracle returns when its OpenMP
+    loops have joined)."""
+    pts = synthetic.uniform_points(n, 2)
+    r = np.float32(synthetic.radius_for_neighbours(n, neighbours))
+    sph = np.concatenate([pts, np.full((n, 1), r, np.float32)], 1)
+    out = []
+    for rep in range(reps + 1):
+        t0 = time.perf_counter()
+        t = oracle.Tree(pts)
+        t1 = time.perf_counter()
+        t.query_spatial(0, sph)
+        t2 = time.perf_counter()
+        t.query_knn(pts, k)
+        t3 = time.perf_counter()
+        del t
+        if rep > 0:
+            out.append((t1 - t0, t2 - t1, t3 - t2))
+    return out
+
+
+def cpu_baseline(args, K, synthetic):
+    """The oracle (C++/OpenMP restatement of ArborX's Serial/OpenMP path) on
+    the host: median of 5 steps after a warm-up, on every OpenMP thread and
+    on one thread ("Serial"), each on a bounded sample of the uniform
+    workload at the same neighbour count and k."""
+    import numpy as np
+
+    import oracle
+
+    threads = oracle.num_threads()
+    res = {}
+    for label, nt, n in (("all_threads", threads, min(args.cpu_sample, args.n)),
+                         ("serial", 1, min(args.cpu_serial_sample, args.n))):
+        oracle.set_num_threads(nt)
+        reps = cpu_step_times(oracle, np, synthetic, n, args.neighbours, K, 5)
+        step = sorted(sum(r) for r in reps)
+        med = step[len(step) // 2]
+        res[label] = dict(threads=nt, n_points=n, median_step_s=med,
+                          queries_per_s=n / med,
+                          median_build_s=sorted(r[0] for r in reps)[2],
+                          median_radius_s=sorted(r[1] for r in reps)[2],
+                          median_knn_s=sorted(r[2] for r in reps)[2],
+                          steps_s=[sum(r) for r in reps])
+    oracle.set_num_threads(threads)
+    a, s1 = res["all_threads"], res["serial"]
+    return dict(value=round(a["queries_per_s"], 1), unit="queries/s", cores=threads,
+                kind="port", cpu_model=cpu_model(),
+                serial_value=round(s1["queries_per_s"], 1),
+                sample=(f"CPU restatement of ArborX Serial/OpenMP (the oracle): one step = "
+                        f"build + two-pass radius CRS ({args.neighbours} nbrs) + kNN k={K} over "
+                        f"the first N points of the uniform set, queries = points; median of 5 "
+                        f"after 1 warm-up; value: N={a['n_points']} on {threads} OpenMP threads "
+                        f"({a['median_step_s']:.2f} s/step); serial_value: "
+                        f"N={s1['n_points']} on 1 thread ({s1['median_step_s']:.2f} s/step)"),
+                runs=res)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -473,29 +583,10 @@ def main():
                         dominant_kernel=dom[0], visits_radius=vr, visits_knn=vk)
         log(f"[bench] roofline: {roofline}")
 
-    # ---- CPU baseline: oracle on host cores, bounded sample
+    # ---- CPU baseline: oracle on host cores, bounded samples
     cpu = None
     if not args.no_cpu_baseline:
-        import oracle
-
-        n_cpu = min(args.cpu_sample, N)
-        ptsc = synthetic.uniform_points(n_cpu, 2)
-        rc = float(np.float32(synthetic.radius_for_neighbours(n_cpu, args.neighbours)))
-        threads = oracle.num_threads()
-        t0 = time.perf_counter()
-        t = oracle.Tree(ptsc)
-        t1 = time.perf_counter()
-        sph = np.concatenate([ptsc, np.full((n_cpu, 1), np.float32(rc), np.float32)], 1)
-        o, _ = t.query_spatial(0, sph)
-        t2 = time.perf_counter()
-        t.query_knn(ptsc, K)
-        t3 = time.perf_counter()
-        cpu = dict(value=round(n_cpu / (t3 - t0), 1), unit="queries/s", cores=threads,
-                   kind="port",
-                   sample=f"first {n_cpu} points of the uniform set, r for {args.neighbours} "
-                          f"neighbours at that N, k={K}; build {t1 - t0:.2f}s + radius "
-                          f"{t2 - t1:.2f}s + kNN {t3 - t2:.2f}s on {threads} OpenMP threads "
-                          "(CPU restatement of ArborX Serial/OpenMP)")
+        cpu = cpu_baseline(args, K, synthetic)
         log(f"[bench] cpu baseline: {cpu}")
 
     extra = {}
@@ -528,17 +619,14 @@ def main():
                                "Gaussian mixture): BVH build + radius (~32 nbrs) + kNN k=16, "
                                "queries = points",
                    "n_points": N, "radius": radius, "k": K, "headline": head_key},
-        "cases": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv)
-                      for kk, vv in v.items()} for k, v in results.items()},
+        "cases": sig(results),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     for key, v in extra.items():
-        line[key] = {kk: (round(vv, 4) if isinstance(vv, float) else
-                          ({a: (round(b, 4) if isinstance(b, float) else b) for a, b in vv.items()}
-                           if isinstance(vv, dict) else vv)) for kk, vv in v.items()}
+        line[key] = sig(v)
     if kernels:
-        line["kernels_one_step_ms"] = {k: round(v["total_ms"], 4) for k, v in kernels.items()}
+        line["kernels_one_step_ms"] = sig({k: v["total_ms"] for k, v in kernels.items()})
     s = json.dumps(line)
     if args.json_out:
         with open(args.json_out, "w") as f:
