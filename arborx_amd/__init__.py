@@ -45,6 +45,9 @@ PREDICATE_SPHERES = 0
 PREDICATE_BOXES = 1
 PREDICATE_RAYS = 3
 PREDICATE_ORDERED_RAYS = 4
+PREDICATE_ORDERED_SPHERES = 5
+PREDICATE_ORDERED_BOXES = 6
+PREDICATE_ORDERED_POINTS = 7
 
 # Symbols of include/arborx_mi355x.h (checked by the CPU test suite).
 EXPORTED_SYMBOLS = (
@@ -237,6 +240,14 @@ class Boxes:
         self.data = _dev_f32(lohi, 6)
 
 
+class Points:
+    """nq points, tensor (nq, 3) (ArborX::Point), the geometry of
+    ordered_intersects(point)."""
+
+    def __init__(self, xyz):
+        self.data = _dev_f32(xyz, 3)
+
+
 class Rays:
     """nq rays, Experimental::Ray(origin, direction) over a batch
     (src/geometry/ArborX_Ray.hpp:28-66): `data` is a (nq, 6) tensor
@@ -273,14 +284,16 @@ class Intersects:
 
 
 class OrderedIntersects:
-    """Experimental::OrderedSpatial<Ray> over a batch
-    (detail/ArborX_Predicates.hpp:104-155): rows list the boxes each ray
-    hits in increasing (distance, index) order."""
+    """Experimental::OrderedSpatial<Geometry> over a batch
+    (detail/ArborX_Predicates.hpp:104-155): for Rays, rows list the boxes
+    each ray hits in increasing (distance, index) order; for Spheres, Boxes
+    or Points every value is at a finite distance, so each row lists all the
+    index's values by (distance(geometry, value), index)."""
 
-    def __init__(self, rays):
-        if not isinstance(rays, Rays):
-            raise TypeError("ordered_intersects() takes Rays")
-        self.geometry = rays
+    def __init__(self, geometry):
+        if not isinstance(geometry, (Rays, Spheres, Boxes, Points)):
+            raise TypeError("ordered_intersects() takes Rays, Spheres, Boxes or Points")
+        self.geometry = geometry
 
 
 class _PointsWithRadius:
@@ -318,14 +331,16 @@ def nearest(points, k=1):
     return Nearest(points, k)
 
 
-def ordered_intersects(rays):
-    """Experimental::ordered_intersects (detail/ArborX_Predicates.hpp:148-155)."""
-    return OrderedIntersects(rays)
+def ordered_intersects(geometry):
+    """Experimental::ordered_intersects (detail/ArborX_Predicates.hpp:148-155)
+    over a batch of Rays, Spheres, Boxes or Points."""
+    return OrderedIntersects(geometry)
 
 
-def make_ordered_intersects(rays):
-    """Experimental::make_ordered_intersects(rays) over a batch of Rays."""
-    return OrderedIntersects(rays)
+def make_ordered_intersects(geometry):
+    """Experimental::make_ordered_intersects over a batch of Rays, Spheres,
+    Boxes or Points."""
+    return OrderedIntersects(geometry)
 
 
 def make_intersects(points, r):
@@ -499,7 +514,8 @@ def _crs_query(prefix, handle, predicates, policy, return_distances, stream):
     if isinstance(predicates, (Intersects, OrderedIntersects)):
         g = predicates.geometry
         if isinstance(predicates, OrderedIntersects):
-            kind = PREDICATE_ORDERED_RAYS
+            kind = {Rays: PREDICATE_ORDERED_RAYS, Spheres: PREDICATE_ORDERED_SPHERES,
+                    Boxes: PREDICATE_ORDERED_BOXES, Points: PREDICATE_ORDERED_POINTS}[type(g)]
         elif isinstance(g, Rays):
             kind = PREDICATE_RAYS
         else:

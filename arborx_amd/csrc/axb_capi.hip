@@ -749,6 +749,68 @@ static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy po
   return AXB_OK;
 }
 
+// The geometry kind of the device's ordered walk for an ordered predicate
+// kind of the C ABI (-1: not one of them).
+static int ordered_geom_kind(int pred_kind)
+{
+  switch (pred_kind)
+  {
+  case AXB_PREDICATE_ORDERED_SPHERES:
+    return PRED_SPHERES;
+  case AXB_PREDICATE_ORDERED_BOXES:
+    return PRED_BOXES;
+  case AXB_PREDICATE_ORDERED_POINTS:
+    return PRED_POINTS;
+  default:
+    return -1;
+  }
+}
+
+// CRS of OrderedSpatial<Sphere | Box | Point>: every row lists all n values
+// by (distance, index), offsets[q] = q n.  The traversal policy's buffer
+// size only matters when negative (an error if n exceeds it, as the
+// reference's overflow check); sorting predicates changes nothing.
+static axb_status query_ordered_rows(const axb_bvh *t, int geom_kind, const float *preds,
+                                     int64_t nq, axb_traversal_policy policy, int64_t *offsets,
+                                     int32_t **indices, int64_t *nnz_host, hipStream_t s)
+{
+  const int64_t n = t->n;
+  if (policy.buffer_size < 0 && n > -(int64_t)policy.buffer_size)
+    AXB_FAIL(AXB_ERR_BUFFER_OVERFLOW, "query: a row has more results than the hard buffer "
+                                      "size (TraversalPolicy buffer_size < 0)");
+  CallErr err(s);
+  if (!err.p)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "query: out of device memory");
+  const TreeView tv = view_of(t, err.p);
+  const int64_t nnz = nq * n;
+  hipError_t e = launch_knn_offsets(offsets, nq, n, s);
+  if (e == hipSuccess)
+  {
+    *indices = (int32_t *)out_alloc((size_t)nnz * sizeof(int32_t), s);
+    if (!*indices)
+      e = hipErrorOutOfMemory;
+  }
+  if (e == hipSuccess)
+    e = launch_ordered_rows(tv, t->node_format, t->geometry, geom_kind, preds, nq, *indices, s);
+  if (e == hipSuccess)
+    e = sync_stream(s);
+  if (e != hipSuccess)
+  {
+    out_free(*indices, s);
+    *indices = nullptr;
+    return hip_fail(e, "query: ordered rows");
+  }
+  const axb_status st = check_kernel_error(err.p, s, "query: ordered rows");
+  if (st != AXB_OK)
+  {
+    out_free(*indices, s);
+    *indices = nullptr;
+    return st;
+  }
+  *nnz_host = nnz;
+  return AXB_OK;
+}
+
 extern "C" {
 
 // Spatial CRS queries; pred_kind AXB_PREDICATE_ORDERED_RAYS takes the
@@ -779,6 +841,9 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
     AXB_HIP_TRY(sync_stream(s));
     return AXB_OK;
   }
+  const int geom_kind = ordered_geom_kind(pred_kind);
+  if (geom_kind >= 0)
+    return query_ordered_rows(t, geom_kind, preds, nq, policy, offsets, indices, nnz_host, s);
   CallErr err(s);
   if (!err.p)
     AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "query: out of device memory");
@@ -845,7 +910,8 @@ axb_status axb_bvh_query_intersects(const axb_bvh *t, int32_t predicate_kind,
                                     int32_t **indices, int64_t *nnz_host, axb_stream_t stream)
 {
   if (predicate_kind != AXB_PREDICATE_SPHERES && predicate_kind != AXB_PREDICATE_BOXES &&
-      predicate_kind != AXB_PREDICATE_RAYS && predicate_kind != AXB_PREDICATE_ORDERED_RAYS)
+      predicate_kind != AXB_PREDICATE_RAYS && predicate_kind != AXB_PREDICATE_ORDERED_RAYS &&
+      ordered_geom_kind(predicate_kind) < 0)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_intersects: unknown predicate kind");
   return query_spatial(t, predicate_kind, predicates, nq, 0.f, policy, offsets, indices,
                        nnz_host, stream);
@@ -1000,10 +1066,18 @@ axb_status axb_bvh_predicate_order(const axb_bvh *t, int32_t predicate_kind,
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: NULL argument");
   if (predicate_kind != AXB_PREDICATE_SPHERES && predicate_kind != AXB_PREDICATE_BOXES &&
       predicate_kind != PRED_RADIUS && predicate_kind != AXB_PREDICATE_RAYS &&
-      predicate_kind != AXB_PREDICATE_ORDERED_RAYS)
+      predicate_kind != AXB_PREDICATE_ORDERED_RAYS && ordered_geom_kind(predicate_kind) < 0)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: unknown predicate kind");
+  // the Morton code of a predicate's centroid: rays by their origin, an
+  // ordered sphere / box / point like the unordered one
   if (predicate_kind == AXB_PREDICATE_ORDERED_RAYS)
     predicate_kind = PRED_RAYS;
+  else if (predicate_kind == AXB_PREDICATE_ORDERED_SPHERES)
+    predicate_kind = PRED_SPHERES;
+  else if (predicate_kind == AXB_PREDICATE_ORDERED_BOXES)
+    predicate_kind = PRED_BOXES;
+  else if (predicate_kind == AXB_PREDICATE_ORDERED_POINTS)
+    predicate_kind = PRED_RADIUS;
   // the sort's look-back words carry 30-bit counts
   if (nq < 0 || nq >= MAX_SORTED_PREDICATES)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_predicate_order: nq must be in [0, 2^30)");

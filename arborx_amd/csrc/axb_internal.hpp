@@ -284,6 +284,11 @@ hipError_t launch_ordered(const TreeView &tv, int node_format, const float *rays
 hipError_t launch_ray_slots(const TreeView &tv, int node_format, const float *rays, int64_t nq,
                             const uint32_t *order, int64_t *offsets, int32_t *out,
                             int32_t buffer, int mode, hipStream_t s);
+// Whole rows of OrderedSpatial<Sphere | Box | Point> (pred_kind PRED_SPHERES,
+// PRED_BOXES, PRED_POINTS): row q = rows[q * n, (q + 1) * n), every leaf's
+// value in (distance, index) order.
+hipError_t launch_ordered_rows(const TreeView &tv, int node_format, int geometry, int pred_kind,
+                               const float *preds, int64_t nq, int32_t *rows, hipStream_t s);
 hipError_t launch_ray_sort_rows(const TreeView &tv, const float *rays, int64_t nq,
                                 const int64_t *offsets, int32_t *rows, hipStream_t s);
 // Rows cut to their first k entries: counts[q] = min(k, row length) and

@@ -99,6 +99,38 @@ __global__ __launch_bounds__(256) void k_ray_sort_rows(TreeView tv, const float 
     sort_ray_row(tv, ray, rows + b, m);
 }
 
+// Whole rows of an ordered query with a sphere, box or point
+// (OrderedSpatial<G>, every leaf at a finite distance): row q holds the
+// slots of all n leaves (leaf j at S_leaf * j + S_int * C[j]) ...
+template <int F, int G>
+__global__ __launch_bounds__(256) void k_leaf_slot_rows(TreeView tv, int64_t nq,
+                                                        int32_t *__restrict__ rows)
+{
+  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  const int64_t n = tv.n;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq * n)
+    return;
+  const int64_t j = i % n;
+  rows[i] = (int32_t)(SL * (uint32_t)j + SI * (uint32_t)tv.C[j]);
+}
+// ... then sorted by (distance(G, leaf), index) and turned into values, one
+// lane per row.
+template <int G, int PK>
+__global__ __launch_bounds__(256) void k_ordered_sort_rows(TreeView tv,
+                                                           const float *__restrict__ preds,
+                                                           int64_t nq,
+                                                           int32_t *__restrict__ rows)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq)
+    return;
+  constexpr int stride = PK == PRED_SPHERES ? 4 : (PK == PRED_BOXES ? 6 : 3);
+  const OrderedGeom g = ordered_geom(PK, preds + stride * q);
+  const int64_t n = tv.n;
+  sort_ray_row<G>(tv, g, rows + q * n, n);
+}
+
 __global__ __launch_bounds__(256) void k_row_prefix_counts(const int64_t *__restrict__ offsets,
                                                            int64_t nq, int32_t k,
                                                            int64_t *__restrict__ counts)
@@ -1399,6 +1431,38 @@ hipError_t launch_ray_sort_rows(const TreeView &tv, const float *rays, int64_t n
   AXB_LAUNCH(k_ray_sort_rows<2>, dim3(grid_of(nq)), dim3(256), 0, s, tv, rays, nq, offsets,
              rows);
   return hipGetLastError();
+}
+
+template <int F, int G>
+static hipError_t launch_ordered_rows_fg(const TreeView &tv, int pred_kind, const float *preds,
+                                         int64_t nq, int32_t *rows, hipStream_t s)
+{
+  AXB_LAUNCH((k_leaf_slot_rows<F, G>), dim3(grid_of(nq * (int64_t)tv.n)), dim3(256), 0, s, tv,
+             nq, rows);
+  if (pred_kind == PRED_SPHERES)
+    AXB_LAUNCH((k_ordered_sort_rows<G, PRED_SPHERES>), dim3(grid_of(nq)), dim3(256), 0, s, tv,
+               preds, nq, rows);
+  else if (pred_kind == PRED_BOXES)
+    AXB_LAUNCH((k_ordered_sort_rows<G, PRED_BOXES>), dim3(grid_of(nq)), dim3(256), 0, s, tv,
+               preds, nq, rows);
+  else
+    AXB_LAUNCH((k_ordered_sort_rows<G, PRED_POINTS>), dim3(grid_of(nq)), dim3(256), 0, s, tv,
+               preds, nq, rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_ordered_rows(const TreeView &tv, int node_format, int geometry, int pred_kind,
+                               const float *preds, int64_t nq, int32_t *rows, hipStream_t s)
+{
+  if (nq <= 0 || tv.n <= 0)
+    return hipSuccess;
+  if (node_format == NODES_Q16)
+    return geometry == GEOM_POINTS
+               ? launch_ordered_rows_fg<NODES_Q16, GEOM_POINTS>(tv, pred_kind, preds, nq, rows, s)
+               : launch_ordered_rows_fg<NODES_Q16, GEOM_BOXES>(tv, pred_kind, preds, nq, rows, s);
+  return geometry == GEOM_POINTS
+             ? launch_ordered_rows_fg<NODES_FP32, GEOM_POINTS>(tv, pred_kind, preds, nq, rows, s)
+             : launch_ordered_rows_fg<NODES_FP32, GEOM_BOXES>(tv, pred_kind, preds, nq, rows, s);
 }
 
 template <int F>

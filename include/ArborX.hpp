@@ -290,6 +290,25 @@ struct RaysNearestK
   size_t n;
   int k;
 };
+// OrderedSpatial<Sphere | Box | Point> batches (ordered_intersects of a
+// sphere, a box or a point, any tree): values by increasing
+// (distance(geometry, value), index) -- every value, since all distances
+// are finite.
+struct OrderedIntersectsSpheres
+{
+  Sphere const *spheres;
+  size_t n;
+};
+struct OrderedIntersectsBoxes
+{
+  Box const *boxes;
+  size_t n;
+};
+struct OrderedIntersectsPoints
+{
+  Point const *points;
+  size_t n;
+};
 
 // make_intersects(points, r): spheres of radius r (stored as float),
 // PredicateHelpers.hpp:99-105 (r > 0 asserted at :102).
@@ -316,6 +335,24 @@ inline OrderedIntersectsRays ordered_intersects(DeviceVector<Ray> const &rays)
 inline OrderedIntersectsRays make_ordered_intersects(DeviceVector<Ray> const &rays)
 {
   return {rays.data(), rays.size()};
+}
+inline OrderedIntersectsSpheres ordered_intersects(DeviceVector<Sphere> const &s)
+{
+  return {s.data(), s.size()};
+}
+inline OrderedIntersectsBoxes ordered_intersects(DeviceVector<Box> const &b)
+{
+  return {b.data(), b.size()};
+}
+inline OrderedIntersectsPoints ordered_intersects(DeviceVector<Point> const &p)
+{
+  return {p.data(), p.size()};
+}
+template <class T>
+inline auto make_ordered_intersects(DeviceVector<T> const &g)
+    -> decltype(ordered_intersects(g))
+{
+  return ordered_intersects(g);
 }
 } // namespace Experimental
 
@@ -489,6 +526,29 @@ public:
     rays_impl(space, AXB_PREDICATE_ORDERED_RAYS, preds.rays, preds.n, preds.k, indices, offsets,
               policy);
   }
+  // ordered_intersects of spheres, boxes or points: every value of the
+  // tree per row, by (distance, index).
+  void query(HipStream const &space, Experimental::OrderedIntersectsSpheres const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    rays_impl(space, AXB_PREDICATE_ORDERED_SPHERES, preds.spheres, preds.n, -1, indices,
+              offsets, policy);
+  }
+  void query(HipStream const &space, Experimental::OrderedIntersectsBoxes const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    rays_impl(space, AXB_PREDICATE_ORDERED_BOXES, preds.boxes, preds.n, -1, indices, offsets,
+              policy);
+  }
+  void query(HipStream const &space, Experimental::OrderedIntersectsPoints const &preds,
+             DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    rays_impl(space, AXB_PREDICATE_ORDERED_POINTS, preds.points, preds.n, -1, indices, offsets,
+              policy);
+  }
   void query(HipStream const &space, Experimental::PointsNearestK const &preds,
              DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
              Experimental::TraversalPolicy const &policy = {}) const
@@ -562,9 +622,11 @@ private:
     }
     indices.adopt(ind, (size_t)nnz, space.stream);
   }
-  // kind: AXB_PREDICATE_RAYS or _ORDERED_RAYS; k >= 0: nearest(ray, k)
-  void rays_impl(HipStream const &space, int32_t kind, Experimental::Ray const *rays, size_t n,
-                 int k, DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
+  // kind: AXB_PREDICATE_RAYS or one of the ordered kinds; k >= 0:
+  // nearest(ray, k)
+  template <class G>
+  void rays_impl(HipStream const &space, int32_t kind, G const *rays, size_t n, int k,
+                 DeviceVector<int> &indices, DeviceVector<int64_t> &offsets,
                  Experimental::TraversalPolicy const &policy) const
   {
     prepare(offsets, n);

@@ -64,9 +64,10 @@ struct Nearest
   int k;
   int index;
 };
-// Experimental::OrderedSpatial<Ray> (detail/ArborX_Predicates.hpp:104-126):
-// the callback sees the boxes the ray hits in increasing (distance, index)
-// order and may stop the walk with early_exit (first hit).
+// Experimental::OrderedSpatial<G> (detail/ArborX_Predicates.hpp:104-126):
+// the callback sees values in increasing (distance(G, value), index) order
+// -- for a ray the boxes it hits, for a sphere, box or point every value --
+// and may stop the walk with early_exit (first hit).
 template <class Geometry>
 struct OrderedSpatial
 {
@@ -109,9 +110,8 @@ struct DeviceTree
   template <class Geometry, class Callback>
   __device__ void query(Experimental::PerThread, Intersects<Geometry> const &predicate,
                         Callback const &callback) const;
-  template <class Callback>
-  __device__ void query(Experimental::PerThread,
-                        OrderedSpatial<Experimental::Ray> const &predicate,
+  template <class Geometry, class Callback>
+  __device__ void query(Experimental::PerThread, OrderedSpatial<Geometry> const &predicate,
                         Callback const &callback) const;
 };
 
@@ -227,23 +227,59 @@ __device__ void spatial_walk(DeviceTree const &t, Intersects<Geometry> const &pr
   }
 }
 
-// One ray, best-first over a tree of boxes (TreeTraversal.hpp:338-489): the
-// callback gets the boxes the ray hits by increasing (distance, index) and
-// may stop the walk.  A walk whose heap of 64 pending entries overflows
-// stops and raises KERR_STACK in t.err (when set).
-template <class Callback>
-__device__ void spatial_walk(DeviceTree const &t,
-                             OrderedSpatial<Experimental::Ray> const &pred, Callback const &cb)
+// The device's form of an ordered predicate's geometry: a ray, or the
+// geometry of OrderedSpatial<Sphere | Box | Point>.
+__device__ __forceinline__ axb::RayPred ordered_of(Experimental::Ray const &r)
 {
-  if (t.n <= 0 || t.geometry != AXB_GEOMETRY_BOXES)
+  return axb::ray_pred((float const *)&r);
+}
+__device__ __forceinline__ axb::OrderedGeom ordered_of(Sphere const &s)
+{
+  const float q[4] = {s._centroid._coords[0], s._centroid._coords[1], s._centroid._coords[2],
+                      s._radius};
+  return axb::ordered_geom(axb::PRED_SPHERES, q);
+}
+__device__ __forceinline__ axb::OrderedGeom ordered_of(Box const &b)
+{
+  const float q[6] = {b._min_corner._coords[0], b._min_corner._coords[1],
+                      b._min_corner._coords[2], b._max_corner._coords[0],
+                      b._max_corner._coords[1], b._max_corner._coords[2]};
+  return axb::ordered_geom(axb::PRED_BOXES, q);
+}
+__device__ __forceinline__ axb::OrderedGeom ordered_of(Point const &p)
+{
+  return axb::ordered_geom(axb::PRED_POINTS, p._coords);
+}
+
+// One ordered predicate, best-first (TreeTraversal.hpp:338-489): the
+// callback gets the values by increasing (distance, index) -- a ray: the
+// boxes it hits (trees of boxes only); a sphere, box or point: every value
+// -- and may stop the walk.  A walk whose heap of 64 pending entries
+// overflows stops and raises KERR_STACK in t.err (when set).
+template <class Geometry, class Callback>
+__device__ void spatial_walk(DeviceTree const &t, OrderedSpatial<Geometry> const &pred,
+                             Callback const &cb)
+{
+  constexpr bool ray = std::is_same_v<Geometry, Experimental::Ray>;
+  if (t.n <= 0 || (ray && t.geometry != AXB_GEOMETRY_BOXES))
     return;
-  const axb::RayPred ray = axb::ray_pred((float const *)&pred.geometry);
-  CallbackSink<OrderedSpatial<Experimental::Ray>, Callback> sink{pred, cb};
+  const auto g = ordered_of(pred.geometry);
+  CallbackSink<OrderedSpatial<Geometry>, Callback> sink{pred, cb};
   uint32_t vi = 0, vl = 0;
-  if (t.format == AXB_NODES_Q16)
-    axb::ordered_walk<axb::NODES_Q16>(t, ray, sink, vi, vl);
-  else
-    axb::ordered_walk<axb::NODES_FP32>(t, ray, sink, vi, vl);
+  if (ray || t.geometry == AXB_GEOMETRY_BOXES)
+  {
+    if (t.format == AXB_NODES_Q16)
+      axb::ordered_walk<axb::NODES_Q16, 64, axb::GEOM_BOXES>(t, g, sink, vi, vl);
+    else
+      axb::ordered_walk<axb::NODES_FP32, 64, axb::GEOM_BOXES>(t, g, sink, vi, vl);
+  }
+  else if constexpr (!ray)
+  {
+    if (t.format == AXB_NODES_Q16)
+      axb::ordered_walk<axb::NODES_Q16, 64, axb::GEOM_POINTS>(t, g, sink, vi, vl);
+    else
+      axb::ordered_walk<axb::NODES_FP32, 64, axb::GEOM_POINTS>(t, g, sink, vi, vl);
+  }
 }
 
 // Device view of a brute-force index (axb_bruteforce_view).
@@ -337,6 +373,21 @@ __device__ __forceinline__ Nearest<Experimental::Ray> predicate_at(Experimental:
 {
   return {b.rays[i], b.k, i};
 }
+__device__ __forceinline__ OrderedSpatial<Sphere>
+predicate_at(Experimental::OrderedIntersectsSpheres b, int i)
+{
+  return {b.spheres[i], i};
+}
+__device__ __forceinline__ OrderedSpatial<Box> predicate_at(Experimental::OrderedIntersectsBoxes b,
+                                                            int i)
+{
+  return {b.boxes[i], i};
+}
+__device__ __forceinline__ OrderedSpatial<Point>
+predicate_at(Experimental::OrderedIntersectsPoints b, int i)
+{
+  return {b.points[i], i};
+}
 
 inline int32_t c_kind(Experimental::PointsWithRadius const &) { return 2; }
 inline int32_t c_kind(Experimental::IntersectsSpheres const &) { return AXB_PREDICATE_SPHERES; }
@@ -366,14 +417,46 @@ inline float const *c_data(Experimental::OrderedIntersectsRays const &b)
 {
   return (float const *)b.rays;
 }
+inline int32_t c_kind(Experimental::OrderedIntersectsSpheres const &)
+{
+  return AXB_PREDICATE_ORDERED_SPHERES;
+}
+inline int32_t c_kind(Experimental::OrderedIntersectsBoxes const &)
+{
+  return AXB_PREDICATE_ORDERED_BOXES;
+}
+inline int32_t c_kind(Experimental::OrderedIntersectsPoints const &)
+{
+  return AXB_PREDICATE_ORDERED_POINTS;
+}
+inline float const *c_data(Experimental::OrderedIntersectsSpheres const &b)
+{
+  return (float const *)b.spheres;
+}
+inline float const *c_data(Experimental::OrderedIntersectsBoxes const &b)
+{
+  return (float const *)b.boxes;
+}
+inline float const *c_data(Experimental::OrderedIntersectsPoints const &b)
+{
+  return (float const *)b.points;
+}
 
+template <class Batch>
+struct is_ordered_batch
+    : std::bool_constant<std::is_same_v<Batch, Experimental::OrderedIntersectsRays> ||
+                         std::is_same_v<Batch, Experimental::OrderedIntersectsSpheres> ||
+                         std::is_same_v<Batch, Experimental::OrderedIntersectsBoxes> ||
+                         std::is_same_v<Batch, Experimental::OrderedIntersectsPoints>>
+{
+};
 template <class Batch>
 struct is_spatial_batch
     : std::bool_constant<std::is_same_v<Batch, Experimental::PointsWithRadius> ||
                          std::is_same_v<Batch, Experimental::IntersectsSpheres> ||
                          std::is_same_v<Batch, Experimental::IntersectsBoxes> ||
                          std::is_same_v<Batch, Experimental::IntersectsRays> ||
-                         std::is_same_v<Batch, Experimental::OrderedIntersectsRays>>
+                         is_ordered_batch<Batch>::value>
 {
 };
 template <class Batch>
@@ -558,7 +641,7 @@ struct WalkErrors
               "hipMemcpyAsync");
     space.fence();
     if (h & axb::KERR_STACK)
-      throw SearchException("ordered query: a ray's walk exceeded its heap of 64 entries");
+      throw SearchException("ordered query: a walk exceeded its heap of 64 entries");
   }
 };
 template <class Batch>
@@ -585,7 +668,7 @@ void callback_query(Index const &index, HipStream const &space, Batch const &b,
     auto t = index_view(index);
     check_ray_geometry<Batch>(t);
     WalkErrors errs;
-    if constexpr (std::is_same_v<Batch, Experimental::OrderedIntersectsRays>)
+    if constexpr (is_ordered_batch<Batch>::value)
       errs.attach(space, t);
     DeviceVector<uint32_t> order = index_order(index, space, b, policy);
     k_spatial_callback<<<blocks_of(b.n), 256, 0, space.stream>>>(t, b, (int64_t)b.n,
@@ -630,7 +713,7 @@ void callback_query_crs(Index const &index, HipStream const &space, Batch const 
     auto t = index_view(index);
     check_ray_geometry<Batch>(t);
     WalkErrors errs;
-    if constexpr (std::is_same_v<Batch, Experimental::OrderedIntersectsRays>)
+    if constexpr (is_ordered_batch<Batch>::value)
       errs.attach(space, t);
     DeviceVector<uint32_t> order = index_order(index, space, b, policy);
     k_spatial_insert<decltype(t), Batch, Callback, T, false>
@@ -688,9 +771,9 @@ __device__ void DeviceTree::query(Experimental::PerThread, Intersects<Geometry> 
 {
   spatial_walk(*this, predicate, callback);
 }
-template <class Callback>
+template <class Geometry, class Callback>
 __device__ void DeviceTree::query(Experimental::PerThread,
-                                  OrderedSpatial<Experimental::Ray> const &predicate,
+                                  OrderedSpatial<Geometry> const &predicate,
                                   Callback const &callback) const
 {
   spatial_walk(*this, predicate, callback);
@@ -706,10 +789,9 @@ __device__ void query(Experimental::PerThread, Details::DeviceTree const &tree,
 {
   Details::spatial_walk(tree, predicate, callback);
 }
-template <class Callback>
+template <class Geometry, class Callback>
 __device__ void query(Experimental::PerThread, Details::DeviceTree const &tree,
-                      OrderedSpatial<Experimental::Ray> const &predicate,
-                      Callback const &callback)
+                      OrderedSpatial<Geometry> const &predicate, Callback const &callback)
 {
   Details::spatial_walk(tree, predicate, callback);
 }

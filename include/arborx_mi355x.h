@@ -83,7 +83,18 @@ enum
    * same boxes, each row in increasing (distance, index) order, distance =
    * the ray parameter where it enters the box (ArborX_Ray.hpp:429-444).  The
    * reference leaves equal distances in the order of its traversal heap. */
-  AXB_PREDICATE_ORDERED_RAYS = 4
+  AXB_PREDICATE_ORDERED_RAYS = 4,
+  /* OrderedSpatial<Sphere | Box | Point> (ordered_intersects of a sphere
+   * {x,y,z,r}, nq x 4; a box {lo, hi}, nq x 6; a point, nq x 3), any tree:
+   * the ordered traversal reports every value at a finite distance
+   * (TreeTraversal.hpp:338-489 pushes every child below +inf), so each row
+   * lists all n values in increasing (distance, index) order, distance =
+   * distance(geometry, value) (geometry/algorithms/ArborX_Distance.hpp:
+   * 52-208).  Rows are n long: meant for small trees (the callback form with
+   * early exit is the common use). */
+  AXB_PREDICATE_ORDERED_SPHERES = 5,
+  AXB_PREDICATE_ORDERED_BOXES = 6,
+  AXB_PREDICATE_ORDERED_POINTS = 7
 };
 
 /* ArborX::Experimental::TraversalPolicy

@@ -686,7 +686,55 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
   }
 }
 
-// ------------------------------------------------------- ordered (rays)
+// ------------------------------------------------------- ordered walks
+// Geometry of an ordered (best-first) query other than a ray:
+// OrderedSpatial<Sphere | Box | Point> (detail/ArborX_Predicates.hpp:104-126)
+// orders the values by distance(geometry, value) (Distance.hpp:52-208).
+// kind: PRED_SPHERES {centre, r}, PRED_BOXES {lo, hi}, PRED_POINTS {p}.
+constexpr int PRED_POINTS = 5;
+struct OrderedGeom
+{
+  int kind;
+  float q[6];
+};
+AXB_HD OrderedGeom ordered_geom(int kind, const float *p)
+{
+  OrderedGeom g;
+  g.kind = kind;
+  const int m = kind == PRED_SPHERES ? 4 : (kind == PRED_BOXES ? 6 : 3);
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+    g.q[j] = j < m ? p[j] : 0.f;
+  return g;
+}
+// distance(geometry, box): point-box (closest point, Distance.hpp:72-80);
+// box-box (gaps of the separated axes, Distance.hpp:159-195: an axis where
+// the boxes overlap adds +0, which leaves the sum unchanged);
+// sphere-box (max(distance(centre, box) - r, 0), Distance.hpp:196-208).
+// A point value is the box [p, p]: distance(point q, [p, p]) is
+// distance(q, p), distance(box, [p, p]) is distance(p, box) and
+// distance(sphere, [p, p]) is distance(p, sphere) bit for bit (the squared
+// terms only change sign inside), the reference's reverse dispatch
+// (ReverseDispatch.hpp:32-46) for the last two.
+AXB_HD float ordered_distance(const OrderedGeom &g, const Box6 &b)
+{
+  if (g.kind == PRED_BOXES)
+  {
+    float s = 0.f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      const float t = kmax(kmax(g.q[d] - b.hi[d], b.lo[d] - g.q[3 + d]), 0.f);
+      s = s + t * t;
+    }
+    return sqrt_rn(s);
+  }
+  const float dc = sqrt_rn(dist2_pbox(g.q[0], g.q[1], g.q[2], b));
+  return g.kind == PRED_SPHERES ? kmax(dc - g.q[3], 0.f) : dc;
+}
+AXB_HD float ordered_distance(const RayPred &r, const Box6 &b) { return ray_distance(r, b); }
+
+// ------------------------------------------------------- best-first heap
 // Binary min-heap of (key, payload) pairs in private arrays, for the
 // best-first walk below.
 template <int CAP>
@@ -748,10 +796,13 @@ AXB_HD uint64_t ordered_key(float d, uint32_t tie)
   return ((uint64_t)f_bits(d) << 32) | tie;
 }
 
-// Best-first walk of a box tree for one ray, the reference's ordered
-// spatial traversal (TreeTraversal.hpp:338-489): the leaves the ray hits
-// are reported in increasing (distance, index) order, distance =
-// ray_distance; sink.hit(value) returns false to stop (early exit,
+// Best-first walk for one ordered predicate, the reference's ordered
+// spatial traversal (TreeTraversal.hpp:338-489): a ray (RayPred, trees of
+// boxes: the leaves it hits) or the geometry of OrderedSpatial<Sphere | Box
+// | Point> (OrderedGeom, any tree: every leaf, since their distances are
+// finite); leaves are reported in increasing (distance, index) order,
+// distance = ordered_distance (G: the trees' leaf geometry);
+// sink.hit(value) returns false to stop (early exit,
 // Callbacks.hpp:96-118).  The reference reports equal distances in the
 // order of its heap; here they come by index.  Node distances bound those
 // of the leaves below, so taking entries smallest key first reports leaves
@@ -766,13 +817,16 @@ AXB_HD uint64_t ordered_key(float d, uint32_t tie)
 // decoding against the scene, and is decoded down that spine when taken
 // from the heap.  Returns false when the heap is full (KERR_STACK; the walk
 // stops).  TV: a tree view with `slots`, `total`, `scene`, `ss`, `err`.
-template <int F, int CAP = 64, class TV, class Sink>
-AXB_HD bool ordered_walk(const TV &tv, const RayPred &ray, Sink &sink, uint32_t &v_int,
+template <int F, int CAP = 64, int G = GEOM_BOXES, class TV, class P, class Sink>
+AXB_HD bool ordered_walk(const TV &tv, const P &ray, Sink &sink, uint32_t &v_int,
                          uint32_t &v_leaf)
 {
-  constexpr uint32_t SL = 2, SI = internal_slots<F>();
+  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
+  // a point leaf is the box [p, p]
   auto leaf_box = [](uint4 a, uint4 b) {
-    return Box6{{bits_f(a.x), bits_f(a.y), bits_f(a.z)}, {bits_f(b.x), bits_f(b.y), bits_f(b.z)}};
+    return G == GEOM_POINTS
+               ? Box6{{bits_f(a.x), bits_f(a.y), bits_f(a.z)}, {bits_f(a.x), bits_f(a.y), bits_f(a.z)}}
+               : Box6{{bits_f(a.x), bits_f(a.y), bits_f(a.z)}, {bits_f(b.x), bits_f(b.y), bits_f(b.z)}};
   };
   OrderedHeap<CAP> h;
   const uint4 r0 = tv.slots[0], r1 = tv.slots[1];
@@ -786,7 +840,7 @@ AXB_HD bool ordered_walk(const TV &tv, const RayPred &ray, Sink &sink, uint32_t 
       if ((int32_t)a0.w >= 0)
         break;
       ++v_leaf;
-      const float d = ray_distance(ray, leaf_box(a0, tv.slots[q + 1]));
+      const float d = ordered_distance(ray, leaf_box(a0, SL == 2 ? tv.slots[q + 1] : a0));
       if (d < INFINITY)
         h.push(ordered_key(d, ORDERED_LEAF | ~(uint32_t)a0.w), 0);
     }
@@ -804,7 +858,7 @@ AXB_HD bool ordered_walk(const TV &tv, const RayPred &ray, Sink &sink, uint32_t 
   Box6 cb;
   node_box<F>(tv, r0, r1, false, tv.scene, tv.ss, cb);
   ++v_int;
-  if (!(ray_distance(ray, cb) < INFINITY))
+  if (!(ordered_distance(ray, cb) < INFINITY))
     return true;
   bool have_box = true;
   while (true)
@@ -858,8 +912,8 @@ AXB_HD bool ordered_walk(const TV &tv, const RayPred &ray, Sink &sink, uint32_t 
     };
     auto leaf_child = [&](uint32_t q, uint4 a0) {
       ++v_leaf;
-      const Box6 lb = leaf_box(a0, tv.slots[q + 1]);
-      const float d = ray_distance(ray, lb);
+      const Box6 lb = leaf_box(a0, SL == 2 ? tv.slots[q + 1] : a0);
+      const float d = ordered_distance(ray, lb);
       if (d < INFINITY)
         offer(ordered_key(d, ORDERED_LEAF | ~(uint32_t)a0.w), 0, false, lb);
     };
@@ -867,7 +921,7 @@ AXB_HD bool ordered_walk(const TV &tv, const RayPred &ray, Sink &sink, uint32_t 
       ++v_int;
       Box6 nb;
       node_box<F>(tv, a0, F == NODES_FP32 ? tv.slots[q + 1] : a0, left, frame, fs, nb);
-      const float d = ray_distance(ray, nb);
+      const float d = ordered_distance(ray, nb);
       if (d < INFINITY)
         offer(ordered_key(d, 0u), (uint64_t)q | ((uint64_t)(left ? anchor : q) << 32), true, nb);
     };
@@ -1158,28 +1212,27 @@ AXB_HD void bitonic_merge(T (&a)[N])
     }
 }
 
-// A row of box-leaf slots hit by a ray (rope_walk with SLOTS) becomes the
-// row of their values in increasing (distance, index) order -- the rows of
-// an ordered query without a best-first walk.  Up to 64 entries: keys
-// (distance bits, index) sorted in registers by a bitonic network; longer
-// rows: heap sort in place, keys recomputed from the records.
-template <int N, class TV>
-AXB_HD void sort_ray_row_n(const TV &tv, const RayPred &ray, int32_t *row, int m)
+// A row of leaf slots (rope_walk with SLOTS, or every leaf) becomes the
+// row of their values in increasing (distance, index) order for an
+// ordered predicate -- the rows of an ordered query without a best-first
+// walk.  Up to 64 entries: keys (distance bits, index) sorted in registers
+// by a bitonic network; longer rows: heap sort in place, keys recomputed
+// from the records.  G: the leaf geometry (a point leaf is [p, p]).
+template <int G, class TV, class P>
+AXB_HD uint64_t ordered_leaf_key(const TV &tv, const P &pred, uint32_t p)
+{
+  const uint4 s0 = tv.slots[p], s1 = G == GEOM_BOXES ? tv.slots[p + 1] : s0;
+  const Box6 b = {{bits_f(s0.x), bits_f(s0.y), bits_f(s0.z)},
+                  {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
+  return ordered_key(ordered_distance(pred, b), ~s0.w);
+}
+template <int N, int G = GEOM_BOXES, class TV, class P>
+AXB_HD void sort_ray_row_n(const TV &tv, const P &ray, int32_t *row, int m)
 {
   uint64_t a[N];
 #pragma unroll
   for (int j = 0; j < N; ++j)
-  {
-    a[j] = ~0ull;
-    if (j < m)
-    {
-      const uint32_t p = (uint32_t)row[j];
-      const uint4 s0 = tv.slots[p], s1 = tv.slots[p + 1];
-      const Box6 b = {{bits_f(s0.x), bits_f(s0.y), bits_f(s0.z)},
-                      {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
-      a[j] = ordered_key(ray_distance(ray, b), ~s0.w);
-    }
-  }
+    a[j] = j < m ? ordered_leaf_key<G>(tv, ray, (uint32_t)row[j]) : ~0ull;
   bitonic_sort<N>(a);
 #pragma unroll
   for (int j = 0; j < N; ++j)
@@ -1189,35 +1242,33 @@ AXB_HD void sort_ray_row_n(const TV &tv, const RayPred &ray, int32_t *row, int m
 template <class TV>
 AXB_HD uint64_t ray_leaf_key(const TV &tv, const RayPred &ray, int32_t slot)
 {
-  const uint4 s0 = tv.slots[(uint32_t)slot], s1 = tv.slots[(uint32_t)slot + 1];
-  const Box6 b = {{bits_f(s0.x), bits_f(s0.y), bits_f(s0.z)},
-                  {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
-  return ordered_key(ray_distance(ray, b), ~s0.w);
+  return ordered_leaf_key<GEOM_BOXES>(tv, ray, (uint32_t)slot);
 }
-template <class TV>
-AXB_HD void sort_ray_row(const TV &tv, const RayPred &ray, int32_t *row, int64_t m)
+template <int G = GEOM_BOXES, class TV, class P>
+AXB_HD void sort_ray_row(const TV &tv, const P &ray, int32_t *row, int64_t m)
 {
   if (m <= 8)
-    return sort_ray_row_n<8>(tv, ray, row, (int)m);
+    return sort_ray_row_n<8, G>(tv, ray, row, (int)m);
   if (m <= 16)
-    return sort_ray_row_n<16>(tv, ray, row, (int)m);
+    return sort_ray_row_n<16, G>(tv, ray, row, (int)m);
   if (m <= 32)
-    return sort_ray_row_n<32>(tv, ray, row, (int)m);
+    return sort_ray_row_n<32, G>(tv, ray, row, (int)m);
   if (m <= 64)
-    return sort_ray_row_n<64>(tv, ray, row, (int)m);
+    return sort_ray_row_n<64, G>(tv, ray, row, (int)m);
   // heap sort of the slots by key (max-heap, then the largest to the back)
+  auto key = [&](int32_t slot) { return ordered_leaf_key<G>(tv, ray, (uint32_t)slot); };
   auto sift = [&](int64_t j, int64_t end) {
     const int32_t v = row[j];
-    const uint64_t kv = ray_leaf_key(tv, ray, v);
+    const uint64_t kv = key(v);
     while (true)
     {
       int64_t c = 2 * j + 1;
       if (c >= end)
         break;
-      uint64_t kc = ray_leaf_key(tv, ray, row[c]);
+      uint64_t kc = key(row[c]);
       if (c + 1 < end)
       {
-        const uint64_t kr = ray_leaf_key(tv, ray, row[c + 1]);
+        const uint64_t kr = key(row[c + 1]);
         if (kr > kc)
           ++c, kc = kr;
       }

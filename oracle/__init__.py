@@ -38,6 +38,15 @@ def _ptr(a):
     return a.ctypes.data_as(P) if a is not None else None
 
 
+# Predicate kinds of the ordered traversal (those of the C ABI) and their
+# floats per predicate.
+ORDERED_RAYS = 4
+ORDERED_SPHERES = 5
+ORDERED_BOXES = 6
+ORDERED_POINTS = 7
+ORDERED_STRIDE = {ORDERED_RAYS: 6, ORDERED_SPHERES: 4, ORDERED_BOXES: 6, ORDERED_POINTS: 3}
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -69,6 +78,9 @@ def lib():
             "orc_make_rays": (I64, [P, P, I64, P]),
             "orc_bruteforce_rays_ordered": (I64, [P, I64, P, I64, I64, P, P]),
             "orc_query_ordered": (I64, [P, P, I64, P, P, P]),
+            "orc_query_ordered_kind": (I64, [P, I32, P, I64, P, P, P]),
+            "orc_bruteforce_ordered": (I64, [P, I64, I32, I32, P, I64, I64, P, P]),
+            "orc_ordered_distance": (ctypes.c_float, [I32, P, I32, P]),
             "orc_pair_energy_f64": (ctypes.c_double, [P, I64, P, P, ctypes.c_double, P]),
             "orc_neighbor_list": (I64, [P, I64, ctypes.c_float, I32, P, P]),
             "orc_expand_half_to_full": (I64, [P, P, I64, P, P]),
@@ -188,18 +200,22 @@ class Tree:
         slots = lib().orc_linearize(self._h, node_format, int(flat_max), _ptr(words))
         return words[:slots]
 
-    def query_ordered(self, rays):
-        """The reference's ordered (best-first) ray traversal -> (offsets,
-        indices, distances), rows in traversal order."""
-        r = _f32(rays, 6)
+    def query_ordered(self, rays, kind=ORDERED_RAYS):
+        """The reference's ordered (best-first) traversal -> (offsets,
+        indices, distances), rows in traversal order.  kind: ORDERED_RAYS
+        (nq, 6) {origin, direction}, ORDERED_SPHERES (nq, 4), ORDERED_BOXES
+        (nq, 6) or ORDERED_POINTS (nq, 3)."""
+        r = _f32(rays, ORDERED_STRIDE[kind])
         nq = r.shape[0]
         offsets = np.zeros(nq + 1, np.int64)
-        nnz = lib().orc_query_ordered(self._h, _ptr(r), nq, _ptr(offsets), None, None)
+        nnz = lib().orc_query_ordered_kind(self._h, kind, _ptr(r), nq, _ptr(offsets), None,
+                                           None)
         if nnz < 0:
             raise RuntimeError("ordered traversal: heap of 64 entries exceeded")
         idx = np.zeros(max(nnz, 1), np.int32)
         dist = np.zeros(max(nnz, 1), np.float32)
-        lib().orc_query_ordered(self._h, _ptr(r), nq, _ptr(offsets), _ptr(idx), _ptr(dist))
+        lib().orc_query_ordered_kind(self._h, kind, _ptr(r), nq, _ptr(offsets), _ptr(idx),
+                                     _ptr(dist))
         return offsets, idx[:nnz], dist[:nnz]
 
     def query_spatial(self, kind, preds, with_visits=False):
@@ -268,6 +284,29 @@ def make_rays(origins, directions):
     if bad:
         raise ValueError("make_rays: zero direction")
     return out
+
+
+def bruteforce_ordered(values, geometry, kind, preds, limit=-1):
+    """Rows of the values at a finite distance from each ordered predicate
+    (kind ORDERED_RAYS / _SPHERES / _BOXES / _POINTS), in (distance, index)
+    order, at most `limit` per row (-1: all) -> (offsets, indices)."""
+    v = _f32(values, 3 if geometry == 0 else 6)
+    q = _f32(preds, ORDERED_STRIDE[kind])
+    nq = q.shape[0]
+    offsets = np.zeros(nq + 1, np.int64)
+    nnz = lib().orc_bruteforce_ordered(_ptr(v), v.shape[0], geometry, kind, _ptr(q), nq, limit,
+                                       _ptr(offsets), None)
+    idx = np.zeros(max(nnz, 1), np.int32)
+    lib().orc_bruteforce_ordered(_ptr(v), v.shape[0], geometry, kind, _ptr(q), nq, limit,
+                                 _ptr(offsets), _ptr(idx))
+    return offsets, idx[:nnz]
+
+
+def ordered_distance(kind, pred, geometry, value):
+    """distance(predicate geometry, value) of the ordered traversal."""
+    q = np.ascontiguousarray(pred, dtype=np.float32)
+    v = np.ascontiguousarray(value, dtype=np.float32)
+    return float(lib().orc_ordered_distance(kind, _ptr(q), geometry, _ptr(v)))
 
 
 def bruteforce_rays_ordered(boxes, rays, limit=-1):

@@ -110,6 +110,42 @@ static inline bool intersects_bb(const float *alo, const float *ahi,
   return true;
 }
 
+// Distance.hpp:159-195 (distance box-box): per axis the gap between the
+// boxes when they are apart (a_min > b_max: a_min - b_max; b_min > a_max:
+// b_min - a_max), squared and summed from 0 over the separated axes, then
+// Kokkos::sqrt.
+static inline float dist_bb(const float *alo, const float *ahi, const float *blo,
+                            const float *bhi)
+{
+  float s = 0;
+  for (int d = 0; d < 3; ++d)
+  {
+    if (alo[d] > bhi[d])
+    {
+      const float t = alo[d] - bhi[d];
+      s += t * t;
+    }
+    else if (blo[d] > ahi[d])
+    {
+      const float t = blo[d] - ahi[d];
+      s += t * t;
+    }
+  }
+  return std::sqrt(s);
+}
+// Kokkos::max(a, b) = (a < b) ? b : a.
+static inline float kokkos_max(float a, float b) { return (a < b) ? b : a; }
+// Distance.hpp:196-208 (distance sphere-box): max(distance(centre, box) - r, 0).
+static inline float dist_sbox(const float *c, float r, const float *lo, const float *hi)
+{
+  return kokkos_max(dist_pbox(c, lo, hi) - r, 0.f);
+}
+// Distance.hpp:82-94 (distance point-sphere): max(distance(p, centre) - r, 0).
+static inline float dist_psphere(const float *p, const float *c, float r)
+{
+  return kokkos_max(dist_pp(p, c) - r, 0.f);
+}
+
 // ---------------------------------------------------------------- rays
 // A ray is 6 floats {origin, unit direction} (ArborX_Ray.hpp:28-66).
 // Slab intersection, ArborX_Ray.hpp:107-153.
@@ -1012,17 +1048,50 @@ int64_t orc_make_rays(const float *origins, const float *dirs, int64_t n,
   return bad;
 }
 
-// Ordered rows by brute force: every box the ray hits, sorted by
-// (distance(ray, box), index), at most `limit` per row (limit < 0: all).
-// This is the row of an OrderedSpatial<Ray> query (limit < 0) and of a
+// OrderedSpatial<G>::distance(other) = distance(G, other)
+// (Predicates.hpp:104-126) for the predicate kinds of the C ABI: 4 rays
+// {origin, direction} (trees of boxes), 5 spheres {centre, r}, 6 boxes
+// {lo, hi}, 7 points.  `other` is a leaf's value (a point when `point`,
+// lo = the point) or a box; pairs the reference defines one way round only
+// go through its reverse dispatch (ReverseDispatch.hpp:32-46):
+// distance(sphere, point) = distance(point, sphere), distance(box, point)
+// = distance(point, box).
+static inline int ordered_stride(int kind)
+{
+  return kind == 5 ? 4 : kind == 7 ? 3 : 6;
+}
+static inline float ordered_dist(int kind, const float *q, bool point, const float *lo,
+                                 const float *hi)
+{
+  switch (kind)
+  {
+  case 4:
+    return ray_distance(q, lo, hi);
+  case 5:
+    return point ? dist_psphere(lo, q, q[3]) : dist_sbox(q, q[3], lo, hi);
+  case 6:
+    return point ? dist_pbox(lo, q, q + 3) : dist_bb(q, q + 3, lo, hi);
+  default:
+    return point ? dist_pp(q, lo) : dist_pbox(q, lo, hi);
+  }
+}
+float orc_ordered_distance(int kind, const float *pred, int geometry, const float *value)
+{
+  return ordered_dist(kind, pred, geometry == 0, value, geometry == 0 ? value : value + 3);
+}
+
+// Ordered rows by brute force: every value at a finite distance (the
+// traversal below never pushes +inf; rays: the boxes the ray hits, other
+// geometries: every value), sorted by (distance, index), at most `limit`
+// per row (limit < 0: all).  For rays this is also the row of a
 // Nearest<Ray> query with k = limit: the kNN traversal only keeps leaves at
 // a distance below its radius, which starts at +inf
 // (TreeTraversal.hpp:255-275), so boxes the ray misses never qualify.
-int64_t orc_bruteforce_rays_ordered(const float *boxes, int64_t n,
-                                    const float *rays, int64_t nq,
-                                    int64_t limit, int64_t *offsets,
-                                    int32_t *indices)
+int64_t orc_bruteforce_ordered(const float *values, int64_t n, int geometry, int kind,
+                               const float *preds, int64_t nq, int64_t limit,
+                               int64_t *offsets, int32_t *indices)
 {
+  const int vs = geometry == 0 ? 3 : 6, ps = ordered_stride(kind);
   std::vector<std::vector<int32_t>> rows(nq);
 #pragma omp parallel for schedule(dynamic, 16)
   for (int64_t i = 0; i < nq; ++i)
@@ -1030,7 +1099,9 @@ int64_t orc_bruteforce_rays_ordered(const float *boxes, int64_t n,
     std::vector<Cand> hits;
     for (int64_t j = 0; j < n; ++j)
     {
-      const float d = ray_distance(rays + 6 * i, boxes + 6 * j, boxes + 6 * j + 3);
+      const float *v = values + vs * j;
+      const float d = ordered_dist(kind, preds + ps * i, geometry == 0, v,
+                                   geometry == 0 ? v : v + 3);
       if (d < std::numeric_limits<float>::infinity())
         hits.push_back({d, (int32_t)j});
     }
@@ -1050,6 +1121,12 @@ int64_t orc_bruteforce_rays_ordered(const float *boxes, int64_t n,
   offsets[nq] = run;
   return run;
 }
+int64_t orc_bruteforce_rays_ordered(const float *boxes, int64_t n, const float *rays,
+                                    int64_t nq, int64_t limit, int64_t *offsets,
+                                    int32_t *indices)
+{
+  return orc_bruteforce_ordered(boxes, n, 1, 4, rays, nq, limit, offsets, indices);
+}
 
 // TreeTraversal.hpp:383-489: the ordered spatial traversal over the
 // reference-form tree.  OneLeafTree: the leaf if its distance is finite.
@@ -1063,11 +1140,12 @@ int64_t orc_bruteforce_rays_ordered(const float *boxes, int64_t n,
 // after sorting equal distances by index.  The stop when both children are
 // missed and the heap is empty is the difference listed in the header.
 // Returns nnz, or -1 if a heap would exceed 64 entries.
-int64_t orc_query_ordered(void *h, const float *rays, int64_t nq,
-                          int64_t *offsets, int32_t *indices,
-                          float *distances)
+int64_t orc_query_ordered_kind(void *h, int kind, const float *preds, int64_t nq,
+                               int64_t *offsets, int32_t *indices, float *distances)
 {
   const Tree &t = *(const Tree *)h;
+  const int ps = ordered_stride(kind);
+  const bool point_leaves = t.geometry == 0;
   const int64_t N = t.n;
   const float inf = std::numeric_limits<float>::infinity();
   std::vector<std::vector<std::pair<int32_t, float>>> rows(nq);
@@ -1075,13 +1153,15 @@ int64_t orc_query_ordered(void *h, const float *rays, int64_t nq,
 #pragma omp parallel for schedule(dynamic, 16)
   for (int64_t i = 0; i < nq; ++i)
   {
-    const float *ray = rays + 6 * i;
+    const float *q = preds + ps * i;
     auto &row = rows[i];
     if (N == 0)
       continue;
     auto dist = [&](int64_t node) {
-      return node < N ? ray_distance(ray, &t.leaf_lo[3 * node], &t.leaf_hi[3 * node])
-                      : ray_distance(ray, &t.int_lo[3 * (node - N)], &t.int_hi[3 * (node - N)]);
+      return node < N ? ordered_dist(kind, q, point_leaves, &t.leaf_lo[3 * node],
+                                     &t.leaf_hi[3 * node])
+                      : ordered_dist(kind, q, false, &t.int_lo[3 * (node - N)],
+                                     &t.int_hi[3 * (node - N)]);
     };
     if (N == 1)
     {
@@ -1155,6 +1235,12 @@ int64_t orc_query_ordered(void *h, const float *rays, int64_t nq,
   }
   offsets[nq] = run;
   return run;
+}
+
+int64_t orc_query_ordered(void *h, const float *rays, int64_t nq, int64_t *offsets,
+                          int32_t *indices, float *distances)
+{
+  return orc_query_ordered_kind(h, 4, rays, nq, offsets, indices, distances);
 }
 
 // TreeTraversal.hpp:168-335: stack-based DFS, closest child first, bounded

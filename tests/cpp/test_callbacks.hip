@@ -440,6 +440,129 @@ static void bruteforce_random(HipStream const &space)
 
 // The XCD block remap of the traversal kernels must be a bijection of
 // [0, nblocks) (host-side check of the device helper).
+// tstQueryTreeCallbackQueryPerThread.cpp:36-110: ten points (i, i, i);
+// thread i queries the box [i - .5, i + .5]^3 with query(PerThread, ...):
+// intersects(box) -- every match is point i -- and ordered_intersects(box)
+// with a callback that records whether its first value is i and returns
+// early_exit.
+__global__ void per_thread_reference(Details::DeviceTree tree, int n, int *ok)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float c = (float)i;
+  const Box box({{c - .5f, c - .5f, c - .5f}}, {{c + .5f, c + .5f, c + .5f}});
+  int s1 = 1, s2 = 0, calls = 0;
+  query(Experimental::PerThread{}, tree, Intersects<Box>{box, i},
+        [&](Intersects<Box> const &, int v) { s1 = s1 && v == i; });
+  query(Experimental::PerThread{}, tree, OrderedSpatial<Box>{box, i},
+        [&](OrderedSpatial<Box> const &, int v) {
+          s2 = v == i;
+          ++calls;
+          return CallbackTreeTraversalControl::early_exit;
+        });
+  ok[i] = s1 && s2 && calls == 1;
+}
+static void per_thread_reference_case(HipStream const &space, NodeFormat fmt)
+{
+  const int n = 10;
+  std::vector<Point> hp(n);
+  for (int i = 0; i < n; ++i)
+    hp[i] = Point{{(float)i, (float)i, (float)i}};
+  DeviceVector<Point> points(hp);
+  BoundingVolumeHierarchy tree(space, points, fmt);
+  DeviceVector<int> ok(std::vector<int>(n, 0));
+  per_thread_reference<<<1, 64, 0, space.stream>>>(tree.device_view(), n, ok.data());
+  Details::hip_check(hipStreamSynchronize(space.stream), "per_thread_reference");
+  CHECK(ok.to_host() == std::vector<int>(n, 1), "PerThread intersects / ordered_intersects(box)");
+}
+
+// Ordered spheres, boxes and points over random points and boxes: the
+// callback form (every value, in order; early exit after the first three)
+// against the CRS rows of the same predicates.
+template <class G>
+struct OrderedPrefix
+{
+  int *out; // 3 per predicate
+  __device__ CallbackTreeTraversalControl operator()(OrderedSpatial<G> const &p, int v) const
+  {
+    int *o = out + 3 * getData(p);
+    int j = 0;
+    while (j < 3 && o[j] >= 0)
+      ++j;
+    if (j < 3)
+      o[j] = v;
+    return j + 1 < 3 ? CallbackTreeTraversalControl::normal_continuation
+                     : CallbackTreeTraversalControl::early_exit;
+  }
+};
+template <class G, class Batch, class Tree>
+static void ordered_prefix_check(HipStream const &space, Tree const &tree, Batch const &b,
+                                 int nq, int n, const char *what)
+{
+  DeviceVector<int> I;
+  DeviceVector<int64_t> Off;
+  tree.query(space, b, I, Off);
+  auto ho = Off.to_host();
+  auto hi = I.to_host();
+  bool ok = ho.size() == (size_t)nq + 1 && ho.back() == (int64_t)nq * n;
+  DeviceVector<int> pre(std::vector<int>(3 * nq, -1));
+  tree.query(space, b, OrderedPrefix<G>{pre.data()});
+  auto hpre = pre.to_host();
+  for (int q = 0; ok && q < nq; ++q)
+    for (int j = 0; j < 3; ++j)
+      ok = ok && hpre[3 * q + j] == (j < n ? hi[ho[q] + j] : -1);
+  // every value once per row
+  for (int q = 0; ok && q < nq; ++q)
+  {
+    std::vector<int> r(hi.begin() + ho[q], hi.begin() + ho[q + 1]);
+    std::sort(r.begin(), r.end());
+    for (int j = 0; j < n; ++j)
+      ok = ok && r[j] == j;
+  }
+  CHECK(ok, what);
+}
+static void random_ordered(HipStream const &space, NodeFormat fmt)
+{
+  const int n = 50, nq = 300; // the ordered walk's heap holds the few pending entries
+  std::mt19937 gen(21);
+  std::uniform_real_distribution<float> U(0.f, 1.f), O(-0.2f, 1.2f);
+  std::vector<Point> hp(n);
+  std::vector<Box> hb(n);
+  for (int i = 0; i < n; ++i)
+  {
+    hp[i] = Point{{U(gen), U(gen), U(gen)}};
+    const float e = 0.05f * U(gen);
+    hb[i] = Box(hp[i], Point{{hp[i][0] + e, hp[i][1] + e, hp[i][2] + e}});
+  }
+  std::vector<Sphere> hs(nq);
+  std::vector<Box> hq(nq);
+  std::vector<Point> hc(nq);
+  for (int q = 0; q < nq; ++q)
+  {
+    hc[q] = Point{{O(gen), O(gen), O(gen)}};
+    hs[q] = Sphere{hc[q], 0.2f * U(gen)};
+    const float e = 0.2f * U(gen);
+    hq[q] = Box(hc[q], Point{{hc[q][0] + e, hc[q][1] + e, hc[q][2] + e}});
+  }
+  DeviceVector<Point> points(hp), centres(hc);
+  DeviceVector<Box> boxes(hb), qboxes(hq);
+  DeviceVector<Sphere> spheres(hs);
+  BoundingVolumeHierarchy tp(space, points, fmt), tb(space, boxes, fmt);
+  ordered_prefix_check<Sphere>(space, tp, Experimental::ordered_intersects(spheres), nq, n,
+                               "ordered spheres over points: callback prefix = CRS");
+  ordered_prefix_check<Box>(space, tp, Experimental::ordered_intersects(qboxes), nq, n,
+                            "ordered boxes over points: callback prefix = CRS");
+  ordered_prefix_check<Point>(space, tp, Experimental::ordered_intersects(centres), nq, n,
+                              "ordered points over points: callback prefix = CRS");
+  ordered_prefix_check<Sphere>(space, tb, Experimental::ordered_intersects(spheres), nq, n,
+                               "ordered spheres over boxes: callback prefix = CRS");
+  ordered_prefix_check<Box>(space, tb, Experimental::ordered_intersects(qboxes), nq, n,
+                            "ordered boxes over boxes: callback prefix = CRS");
+  ordered_prefix_check<Point>(space, tb, Experimental::ordered_intersects(centres), nq, n,
+                              "ordered points over boxes: callback prefix = CRS");
+}
+
 static void xcd_remap()
 {
   bool ok = true;
@@ -473,6 +596,8 @@ int main()
     early_exit(space, MakeBVH{fmt});
     random_points(space, fmt);
     random_rays(space, fmt);
+    per_thread_reference_case(space, fmt);
+    random_ordered(space, fmt);
   }
   reference_cases(space, MakeBruteForce{});
   early_exit(space, MakeBruteForce{});

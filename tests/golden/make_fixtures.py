@@ -205,6 +205,17 @@ def main():
              "offsets": [0, 1, 1], "indices": [0]},
         ],
     }
+    # tstQueryTreeCallbackQueryPerThread.cpp:81-110: ten points (i, i, i),
+    # attach_indices values; query i is ordered_intersects of the box
+    # [i - .5, i + .5]^3 with a callback that records whether the first
+    # value it sees is i, then returns early_exit.  (:36-79: the same with
+    # intersects(box), every match being i.)
+    pts10 = [[float(i)] * 3 for i in range(10)]
+    boxes10 = [[i - 0.5] * 3 + [i + 0.5] * 3 for i in range(10)]
+    kat["per_thread"] = {
+        "source": "test/tstQueryTreeCallbackQueryPerThread.cpp:36-110",
+        "points": pts10, "query_boxes": boxes10, "first_value": list(range(10)),
+    }
     with open(os.path.join(HERE, "reference_kat.json"), "w") as f:
         json.dump(kat, f, indent=1)
     print("wrote", os.path.join(HERE, "reference_kat.json"))

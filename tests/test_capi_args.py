@@ -60,12 +60,14 @@ def test_queries_reject_bad_arguments():
                                                         ctypes.byref(ind), ctypes.byref(nnz),
                                                         None) == INVALID
     assert "radius" in _last_error()
-    # unknown predicate kind
-    assert L.axb_bvh_query_intersects(dummy, 7, dummy, 4, _policy(), dummy, ctypes.byref(ind),
-                                      ctypes.byref(nnz), None) == INVALID
-    assert L.axb_bruteforce_query_intersects(dummy, 7, dummy, 4, _policy(), dummy,
-                                             ctypes.byref(ind), ctypes.byref(nnz),
-                                             None) == INVALID
+    # unknown predicate kinds; the brute-force index has no ordered traversal
+    for kind in (-1, 2, 8, 42):
+        assert L.axb_bvh_query_intersects(dummy, kind, dummy, 4, _policy(), dummy,
+                                          ctypes.byref(ind), ctypes.byref(nnz), None) == INVALID
+    for kind in (-1, 2, 4, 5, 6, 7, 8):
+        assert L.axb_bruteforce_query_intersects(dummy, kind, dummy, 4, _policy(), dummy,
+                                                 ctypes.byref(ind), ctypes.byref(nnz),
+                                                 None) == INVALID
     # null tree / outputs, query counts out of range
     assert L.axb_bvh_query_intersects(None, ax.PREDICATE_SPHERES, dummy, 4, _policy(), dummy,
                                       ctypes.byref(ind), ctypes.byref(nnz), None) == INVALID
@@ -90,8 +92,8 @@ def test_ray_arguments():
     assert L.axb_make_rays(dummy, dummy, -1, dummy, None) == INVALID
     assert L.axb_make_rays(None, dummy, 3, dummy, None) == INVALID
     assert L.axb_make_rays(dummy, dummy, 0, dummy, None) == ax.AXB_OK
-    # kinds past the ray kinds, and the ordered kind on the brute force
-    assert L.axb_bvh_query_intersects(dummy, 5, dummy, 4, _policy(), dummy, ctypes.byref(ind),
+    # kinds past the ordered kinds, and the ordered kind on the brute force
+    assert L.axb_bvh_query_intersects(dummy, 8, dummy, 4, _policy(), dummy, ctypes.byref(ind),
                                       ctypes.byref(nnz), None) == INVALID
     assert L.axb_bruteforce_query_intersects(dummy, ax.PREDICATE_ORDERED_RAYS, dummy, 4,
                                              _policy(), dummy, ctypes.byref(ind),

@@ -29,10 +29,12 @@ N = 10_000_000
 K = 16
 # Radii for about 32 neighbours per point (self included): the uniform
 # set's closed form (3 * 32 / (4 pi N))^(1/3), and for the mixture the
-# radius a bisection on the oracle's mean count over every 100th point
-# settles at (31.99999 neighbours).
+# radius a bisection on the oracle's mean count over every 10th point
+# settles at (32.00001 there, 32.11 over all points).  (Point i belongs to
+# centre i % 1000, so a sample of every 100th point would see 10 of the
+# 1000 clusters only.)
 RADIUS = {"uniform": float(np.float32(synthetic.radius_for_neighbours(N, 32))),
-          "gaussian": float(np.float32(0.00284968))}
+          "gaussian": float(np.float32(0.0030986322))}
 
 
 def dataset(kind):
