@@ -56,7 +56,8 @@ EXPORTED_SYMBOLS = (
     "axb_bvh_query_intersects", "axb_bvh_query_intersects_radius",
     "axb_bvh_query_nearest", "axb_bvh_query_nearest_rays", "axb_make_rays",
     "axb_bvh_neighbor_list", "axb_find_neighbor_list",
-    "axb_expand_half_to_full", "axb_pair_energy", "axb_pair_energy_fd",
+    "axb_expand_half_to_full", "axb_pair_energy", "axb_pair_energy_noalloc",
+    "axb_pair_energy_fd",
     "axb_bruteforce_create", "axb_bruteforce_destroy", "axb_bruteforce_size",
     "axb_bruteforce_bounds", "axb_bruteforce_query_intersects",
     "axb_bruteforce_query_intersects_radius", "axb_bruteforce_query_nearest",
@@ -157,6 +158,7 @@ def lib():
                                                          ctypes.POINTER(_P),
                                                          ctypes.POINTER(_I64), _P]),
             "axb_pair_energy": (_I32, [_P, _I64, _P, _P, ctypes.c_float, _P, _P, _P]),
+            "axb_pair_energy_noalloc": (_I32, [_P, _I64, _P, _P, ctypes.c_float, _P, _P, _P]),
             "axb_pair_energy_fd": (_I32, [_P, _I64, _P, _P, ctypes.c_float, _P, _I64,
                                           ctypes.c_double, _P, _P]),
             "axb_bvh_num_slots": (_I64, [_P]),

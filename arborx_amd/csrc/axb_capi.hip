@@ -1464,6 +1464,26 @@ axb_status axb_pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
   return st;
 }
 
+axb_status axb_pair_energy_noalloc(const float *xyz, int64_t n, const int64_t *offsets,
+                                   const int32_t *indices, float cutoff, double *energy,
+                                   float *grad, axb_stream_t stream)
+{
+  static_assert(AXB_PAIR_ENERGY_NOALLOC_MAX_N == ENERGY_SCRATCH_BLOCKS * 256,
+                "scratch size and the documented limit differ");
+  if (n < 0 || (n > 0 && (!xyz || !offsets)))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_pair_energy_noalloc: bad arguments");
+  if (n > AXB_PAIR_ENERGY_NOALLOC_MAX_N)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT,
+             "axb_pair_energy_noalloc: n exceeds AXB_PAIR_ENERGY_NOALLOC_MAX_N (2^27 points)");
+  if (!(cutoff > 0.f))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_pair_energy_noalloc: cutoff must be > 0");
+  const hipError_t e = pair_energy_noalloc(xyz, n, offsets, indices, cutoff, energy, grad,
+                                           (hipStream_t)stream);
+  if (e != hipSuccess)
+    return hip_fail(e, "axb_pair_energy_noalloc");
+  return AXB_OK;
+}
+
 axb_status axb_pair_energy_fd(const float *xyz, int64_t n, const int64_t *offsets,
                               const int32_t *indices, float cutoff, const int32_t *atoms,
                               int64_t m, double h, double *fd, axb_stream_t stream)

@@ -11,6 +11,8 @@
 // the row once (the fused kernel costs one gather of x_j per pair).
 #include "axb_internal.hpp"
 
+#include <mutex>
+
 namespace axb
 {
 namespace
@@ -134,29 +136,106 @@ __global__ __launch_bounds__(256) void k_energy_fd(const float *__restrict__ xyz
   fd[t] = (e2[0] - e2[1]) / (2.0 * h);
 }
 
+// Scratch of the allocation-free entry point: the per-block partial sums
+// (one double per 256 rows).  Every GPU the library runs on holds its own
+// copy until the process ends.  Host code never passes `energy_scratch`
+// itself (in host code the name is a stand-in, not a device address); it
+// uses the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double energy_scratch[ENERGY_SCRATCH_BLOCKS];
+
+hipError_t launch_energy(const float *xyz, int64_t n, const int64_t *offsets,
+                         const int32_t *indices, float phic, const uint32_t *order,
+                         double *partials, int64_t nb, double *energy, float *grad,
+                         hipStream_t s, bool profiled)
+{
+  if (profiled)
+  {
+    if (grad)
+      AXB_LAUNCH(k_energy<true>, dim3((unsigned)nb), dim3(256), 0, s, xyz, n, offsets,
+                 indices, phic, order, partials, grad);
+    else
+      AXB_LAUNCH(k_energy<false>, dim3((unsigned)nb), dim3(256), 0, s, xyz, n, offsets,
+                 indices, phic, order, partials, grad);
+    if (energy)
+      AXB_LAUNCH(k_reduce_f64, dim3(1), dim3(256), 0, s, partials, nb, energy);
+  }
+  else
+  {
+    if (grad)
+      hipLaunchKernelGGL(k_energy<true>, dim3((unsigned)nb), dim3(256), 0, s, xyz, n, offsets,
+                         indices, phic, order, partials, grad);
+    else
+      hipLaunchKernelGGL(k_energy<false>, dim3((unsigned)nb), dim3(256), 0, s, xyz, n,
+                         offsets, indices, phic, order, partials, grad);
+    if (energy)
+      hipLaunchKernelGGL(k_reduce_f64, dim3(1), dim3(256), 0, s, partials, nb, energy);
+  }
+  return hipGetLastError();
+}
+
+float shifted_phi_cutoff(float cutoff)
+{
+  const double irc2 = 1.0 / ((double)cutoff * (double)cutoff);
+  const double irc6 = irc2 * irc2 * irc2;
+  return (float)(4.0 * irc6 * (irc6 - 1.0));
+}
+
 } // namespace
 
 hipError_t pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
                        const int32_t *indices, float cutoff, const uint32_t *order,
                        double *energy, float *grad, hipStream_t s)
 {
-  const double irc2 = 1.0 / ((double)cutoff * (double)cutoff);
-  const double irc6 = irc2 * irc2 * irc2;
-  const float phic = (float)(4.0 * irc6 * (irc6 - 1.0));
+  const float phic = shifted_phi_cutoff(cutoff);
   const int64_t nb = div_up(n > 0 ? n : 1, 256);
   double *partials = (double *)dalloc(nb * sizeof(double), s);
   if (!partials)
     return hipErrorOutOfMemory;
-  if (grad)
-    AXB_LAUNCH(k_energy<true>, dim3((unsigned)nb), dim3(256), 0, s, xyz, n, offsets,
-               indices, phic, order, partials, grad);
-  else
-    AXB_LAUNCH(k_energy<false>, dim3((unsigned)nb), dim3(256), 0, s, xyz, n, offsets,
-               indices, phic, order, partials, grad);
-  if (energy)
-    AXB_LAUNCH(k_reduce_f64, dim3(1), dim3(256), 0, s, partials, nb, energy);
+  const hipError_t e =
+      launch_energy(xyz, n, offsets, indices, phic, order, partials, nb, energy, grad, s, true);
   dfree(partials, s);
-  return hipGetLastError();
+  return e;
+}
+
+// Device address of this GPU's energy_scratch, looked up once per device
+// (with that device current) and kept.
+static hipError_t energy_scratch_ptr(double **out)
+{
+  static std::mutex m;
+  static double *ptrs[MAX_DEVICES] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return e;
+  if (dev < 0 || dev >= MAX_DEVICES)
+    return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(m);
+  if (!ptrs[dev])
+  {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(energy_scratch));
+    if (e != hipSuccess)
+      return e;
+    ptrs[dev] = (double *)p;
+  }
+  *out = ptrs[dev];
+  return hipSuccess;
+}
+
+hipError_t pair_energy_noalloc(const float *xyz, int64_t n, const int64_t *offsets,
+                               const int32_t *indices, float cutoff, double *energy,
+                               float *grad, hipStream_t s)
+{
+  const int64_t nb = div_up(n > 0 ? n : 1, 256);
+  if (nb > ENERGY_SCRATCH_BLOCKS)
+    return hipErrorInvalidValue;
+  double *partials = nullptr;
+  const hipError_t e = energy_scratch_ptr(&partials);
+  if (e != hipSuccess)
+    return e;
+  // every block writes its own partial, so nothing needs clearing first
+  return launch_energy(xyz, n, offsets, indices, shifted_phi_cutoff(cutoff), nullptr, partials,
+                       nb, energy, grad, s, false);
 }
 
 hipError_t pair_energy_fd(const float *xyz, const int64_t *offsets, const int32_t *indices,

@@ -314,6 +314,22 @@ axb_status axb_pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
                            const int32_t *indices, float cutoff,
                            double *energy, float *grad, axb_stream_t stream);
 
+/* The same energy and gradient without any allocation (no reference
+ * counterpart).  For callers that cannot allocate device memory: code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc.  Its
+ * scratch (one double of partial sums per 256 rows) is a static device array
+ * of AXB_PAIR_ENERGY_NOALLOC_MAX_N / 256 doubles (4 MiB) that every GPU the
+ * library runs on keeps until the process ends; n above
+ * AXB_PAIR_ENERGY_NOALLOC_MAX_N is rejected with AXB_ERR_INVALID_ARGUMENT.
+ * Rows are taken in the caller's order (axb_pair_energy may reorder large
+ * inputs, which changes E only by the summation order of its partials; the
+ * gradient is bit-identical).  Calls that overlap on one GPU share its
+ * scratch and race: serialise them (one stream, or events). */
+#define AXB_PAIR_ENERGY_NOALLOC_MAX_N (INT64_C(1) << 27)
+axb_status axb_pair_energy_noalloc(const float *xyz, int64_t n, const int64_t *offsets,
+                                   const int32_t *indices, float cutoff,
+                                   double *energy, float *grad, axb_stream_t stream);
+
 /* Central finite differences of the same energy in double precision for the
  * atoms listed in `atoms` (device int32, m entries): fd[3*a + d] =
  * (E(x + h e_ad) - E(x - h e_ad)) / 2h with the neighbour set held fixed
