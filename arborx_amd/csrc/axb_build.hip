@@ -433,7 +433,7 @@ __global__ __launch_bounds__(256) void k_lin_q16(const INode *__restrict__ inode
       q16_encode_axis(f.lo[d], f.hi[d], r.lo[d], r.hi[d], q[d], q[3 + d]);
       const float s = q16_scale(f.lo[d], f.hi[d]);
       dec.lo[d] = q16_lo(f.lo[d], s, q[d]);
-      dec.hi[d] = q16_hi(f.hi[d], s, q[3 + d]);
+      dec.hi[d] = q16_lo(f.lo[d], s, q[3 + d]);
     }
     const uint32_t pos = SI * (uint32_t)(C[r.first] - r.h) + SL * (uint32_t)r.first;
     const uint32_t len = (uint32_t)(r.last - r.first + 1);

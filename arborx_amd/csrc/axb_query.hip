@@ -728,10 +728,42 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
 // {index, d2 bits}: read as one little-endian 64-bit word that is the
 // (d2, index) key, d2 bits high (d2 >= 0, so bit order is value order).  Stops as soon as more than `cap` leaves qualify;
 // returns the count (cap + 1 on overflow).
+// Candidates of a point walk gather in LDS, KNN_STAGE per lane (entry j of
+// lane l at st[j * 256 + l]), and leave for the row as whole 64-byte
+// chunks: each chunk of the row is written once, complete (a row's lanes
+// write far apart, so single 8-byte stores left most of each 128-byte line
+// to be written back partially, several times).
+constexpr int KNN_STAGE = 8;
+struct CandStage
+{
+  uint2 *st;
+  int lane;
+  AXB_D void put(int c, uint2 e) { st[(c & (KNN_STAGE - 1)) * 256 + lane] = e; }
+  // the chunk holding entries [c0, c0 + 8) of the row, c0 a multiple of 8
+  AXB_D void flush(uint2 *__restrict__ cand, uint32_t row_at, int c0) const
+  {
+    uint4 *dst = reinterpret_cast<uint4 *>(cand + row_at + c0);
+#pragma unroll
+    for (int j = 0; j < KNN_STAGE / 2; ++j)
+    {
+      const uint2 a = st[(2 * j) * 256 + lane], b = st[(2 * j + 1) * 256 + lane];
+      dst[j] = make_uint4(a.x, a.y, b.x, b.y);
+    }
+  }
+  // reloads the partial chunk of a row of c entries from the row
+  AXB_D void reload(const uint2 *__restrict__ cand, uint32_t row_at, int c)
+  {
+    const int c0 = c & ~(KNN_STAGE - 1);
+    for (int j = 0; j < (c & (KNN_STAGE - 1)); ++j)
+      st[j * 256 + lane] = cand[row_at + c0 + j];
+  }
+};
+
 template <int F, int G>
 AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, float &R,
                            float &R_hi, float R_lo, float shrink, uint2 *__restrict__ cand,
-                           uint32_t row_at, int cap, uint32_t &v_int, uint32_t &v_leaf)
+                           uint32_t row_at, int cap, CandStage &stage, uint32_t &v_int,
+                           uint32_t &v_leaf)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
   float thr = sqrt_threshold(R);
@@ -745,6 +777,8 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
     // leaf passed over so far lay outside the old radius, so the row still
     // holds every leaf seen within the new one, and the walk goes on (R_hi:
     // the last radius known to hold at least the row's capacity).
+    // Candidates go through the lane's LDS stage (cap is a multiple of 8, so
+    // a full row has been written out whole).
     int c = 0;
     uint32_t p = 0;
     const uint32_t end = tv.total;
@@ -786,9 +820,15 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
           }
           if (c == cap)
             return cap + 1; // still full: left to the next round
+          stage.reload(cand, row_at, c);
         }
         if (d2 <= thr)
-          cand[row_at + c++] = make_uint2((uint32_t)~w, f_bits(d2));
+        {
+          stage.put(c, make_uint2((uint32_t)~w, f_bits(d2)));
+          ++c;
+          if ((c & (KNN_STAGE - 1)) == 0)
+            stage.flush(cand, row_at, c - KNN_STAGE);
+        }
       }
       if (!leaf && !pass && w == 0)
       {
@@ -799,6 +839,8 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
       desc = !leaf && pass;
       p += leaf ? 1u : (pass ? SI : (uint32_t)w);
     }
+    if (c & (KNN_STAGE - 1)) // the last, partial chunk (the rest of it unused)
+      stage.flush(cand, row_at, c & ~(KNN_STAGE - 1));
     return c;
   }
   int c = 0;
@@ -892,9 +934,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 &&
     int max_rounds, uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_n,
     int32_t *__restrict__ oi, float *__restrict__ od)
 {
+  __shared__ uint2 s_stage[G == GEOM_POINTS ? KNN_STAGE * 256 : 1];
   const int64_t g = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= tn)
     return;
+  CandStage stage{s_stage, (int)threadIdx.x};
   const int64_t t = t0 + g;
   const int64_t q = order ? (int64_t)order[t] : t;
   const float qx = pts[3 * q], qy = pts[3 * q + 1], qz = pts[3 * q + 2];
@@ -910,8 +954,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 &&
   for (int round = 0; round < max_rounds; ++round)
   {
     ++rounds;
-    c = knn_collect_walk<F, G>(tv, qx, qy, qz, R, R_hi, R_lo, shrink, cand, row_at, cap, v_int,
-                               v_leaf);
+    c = knn_collect_walk<F, G>(tv, qx, qy, qz, R, R_hi, R_lo, shrink, cand, row_at, cap, stage,
+                               v_int, v_leaf);
     if (c >= m && c <= cap)
     {
       done = true;

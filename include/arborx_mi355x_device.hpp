@@ -222,25 +222,30 @@ AXB_HD float sqrt_threshold(float r)
 }
 
 // ---------------------------------------------------------------- Q16
-// Node box quantised on the 65535-step grid of a frame box F:
-//   lo(q) = fma(q, s, F.lo),  hi(q) = fma(-(65535 - q), s, F.hi),
-//   s = (F.hi - F.lo) * (1/65535)
-// (one correctly rounded fma each, identical on the host and the device).
-// qlo is the largest q with lo(q) <= box.lo and qhi the smallest with
-// hi(q) >= box.hi, so the decoded box always contains the exact one.
-constexpr float Q16_INV = 1.0f / 65535.0f;
+// Node box quantised on the 65535-step grid of a frame box F, one formula
+// for both faces:
+//   v(q) = fma(q, s, F.lo),   s = fma(F.hi - F.lo, C, 2^-149),
+//   C = 0x1.00010ap-16
+// (correctly rounded fmas, identical on the host and the device).  C
+// exceeds (1 + 2^-21) / 65535, so s > (F.hi - F.lo) / 65535 however the
+// difference and the product round, and v(65535) >= F.hi; the added
+// smallest denormal keeps s > 0 where the product would underflow.  qlo is the
+// largest q with v(q) <= box.lo and qhi the smallest with v(q) >= box.hi
+// (both exist for a box inside its frame), so the decoded box always
+// contains the exact one.  Decoding needs the frame's lower corner and
+// step only.
+constexpr float Q16_INV_UP = 0x1.00010ap-16f;
 
-AXB_HD float q16_scale(float flo, float fhi) { return (fhi - flo) * Q16_INV; }
+AXB_HD float q16_scale(float flo, float fhi)
+{
+  return __builtin_fmaf(fhi - flo, Q16_INV_UP, 0x1p-149f);
+}
 AXB_HD float q16_lo(float flo, float s, uint32_t q)
 {
   return __builtin_fmaf((float)q, s, flo);
 }
-AXB_HD float q16_hi(float fhi, float s, uint32_t q)
-{
-  return __builtin_fmaf(-(float)(65535u - q), s, fhi);
-}
-// Largest q in [0, 65535] with lo(q) <= v (0 when there is none).  lo(q)
-// is non-decreasing in q; the search starts from the window around the
+// Largest q in [0, 65535] with v(q) <= v (0 when there is none).  v(q) is
+// non-decreasing in q; the search starts from the window around the
 // estimate (v - flo) / s and widens to the full range when the window does
 // not bracket the answer, so the result is the plain binary search's.
 AXB_HD uint32_t q16_lo_index(float flo, float s, float v)
@@ -262,20 +267,20 @@ AXB_HD uint32_t q16_lo_index(float flo, float s, float v)
   return (uint32_t)a;
 }
 
-// Smallest q in [0, 65535] with hi(q) >= v (65535 when there is none);
-// hi(q) is non-decreasing in q.  Same bracketing as q16_lo_index.
-AXB_HD uint32_t q16_hi_index(float fhi, float s, float v)
+// Smallest q in [0, 65535] with v(q) >= v (65535 when there is none).
+// Same bracketing as q16_lo_index.
+AXB_HD uint32_t q16_hi_index(float flo, float s, float v)
 {
-  const float est = (fhi - v) / s; // steps below fhi
-  const int q = est >= 0.f ? (est <= 65535.f ? 65535 - (int)est : 0) : 65535;
+  const float est = (v - flo) / s;
+  const int q = est >= 0.f ? (est <= 65535.f ? (int)est : 65535) : 0; // NaN -> 0
   int a = q - 2 < 0 ? 0 : q - 2, b = q + 2 > 65535 ? 65535 : q + 2;
-  if (!((b == 65535 || q16_hi(fhi, s, (uint32_t)b) >= v) &&
-        (a == 0 || q16_hi(fhi, s, (uint32_t)(a - 1)) < v)))
+  if (!((b == 65535 || q16_lo(flo, s, (uint32_t)b) >= v) &&
+        (a == 0 || q16_lo(flo, s, (uint32_t)(a - 1)) < v)))
     a = 0, b = 65535;
   while (a < b)
   {
     const int m = (a + b) >> 1;
-    if (q16_hi(fhi, s, (uint32_t)m) >= v)
+    if (q16_lo(flo, s, (uint32_t)m) >= v)
       b = m;
     else
       a = m + 1;
@@ -293,19 +298,20 @@ AXB_HD void q16_encode_axis(float flo, float fhi, float vlo, float vhi,
     return;
   }
   qlo = q16_lo_index(flo, s, vlo);
-  qhi = q16_hi_index(fhi, s, vhi);
+  qhi = q16_hi_index(flo, s, vhi);
 }
 
-// Decodes a Q16 record (words x, y, z) against frame (flo, fhi, scale).
-AXB_HD void q16_decode(uint32_t w0, uint32_t w1, uint32_t w2, const Box6 &f,
+// Decodes a Q16 record (words x, y, z) against a frame (lower corner flo,
+// step s).
+AXB_HD void q16_decode(uint32_t w0, uint32_t w1, uint32_t w2, const float flo[3],
                        const float s[3], Box6 &out)
 {
-  out.lo[0] = q16_lo(f.lo[0], s[0], w0 & 0xffffu);
-  out.lo[1] = q16_lo(f.lo[1], s[1], w0 >> 16);
-  out.lo[2] = q16_lo(f.lo[2], s[2], w1 & 0xffffu);
-  out.hi[0] = q16_hi(f.hi[0], s[0], w1 >> 16);
-  out.hi[1] = q16_hi(f.hi[1], s[1], w2 & 0xffffu);
-  out.hi[2] = q16_hi(f.hi[2], s[2], w2 >> 16);
+  out.lo[0] = q16_lo(flo[0], s[0], w0 & 0xffffu);
+  out.lo[1] = q16_lo(flo[1], s[1], w0 >> 16);
+  out.lo[2] = q16_lo(flo[2], s[2], w1 & 0xffffu);
+  out.hi[0] = q16_lo(flo[0], s[0], w1 >> 16);
+  out.hi[1] = q16_lo(flo[1], s[1], w2 & 0xffffu);
+  out.hi[2] = q16_lo(flo[2], s[2], w2 >> 16);
 }
 
 // ---------------------------------------------------------------- XCDs
@@ -352,16 +358,14 @@ AXB_HD void node_box(const TV &tv, uint4 s0, uint4 s1, bool desc, const Box6 &pa
   {
     // frame chosen by selects rather than a branch: lanes of a wavefront
     // disagree on `desc` all the time
-    Box6 f;
-    float fs[3];
+    float flo[3], fs[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
-      f.lo[d] = desc ? par.lo[d] : tv.scene.lo[d];
-      f.hi[d] = desc ? par.hi[d] : tv.scene.hi[d];
+      flo[d] = desc ? par.lo[d] : tv.scene.lo[d];
       fs[d] = desc ? ps[d] : tv.ss[d];
     }
-    q16_decode(s0.x, s0.y, s0.z, f, fs, b);
+    q16_decode(s0.x, s0.y, s0.z, flo, fs, b);
   }
 }
 

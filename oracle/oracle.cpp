@@ -664,46 +664,44 @@ void orc_export(void *h, uint64_t *keys, uint32_t *perm, int32_t *leaf_rope,
 // (HappyTreeFriends.hpp:66-85: right child = rope of left child).
 //
 // Q16 encoding (no reference counterpart): a node's box is quantised on a
-// 65535-step grid of a frame box F with s = (F.hi - F.lo) * (1/65535):
-//   lo(q) = fma(q, s, F.lo),   hi(q) = fma(-(65535-q), s, F.hi)
-// (correctly rounded fused multiply-add, as on the GPU)
-// qmin = largest q with lo(q) <= box.lo, qmax = smallest q with
-// hi(q) >= box.hi (outward rounding => decoded box contains the exact one).
-// Frame = decoded box of the parent for a left child (reached by descent),
-// the scene box for the root and every right child (reached by a rope).
-static const float Q16_INV = 1.0f / 65535.0f;
+// 65535-step grid of a frame box F, one formula for both faces,
+//   v(q) = fma(q, s, F.lo),  s = fma(F.hi - F.lo, C, 2^-149),
+//   C = 0x1.00010ap-16
+// (correctly rounded fused multiply-adds, as on the GPU; C > (1 + 2^-21) /
+// 65535, so v(65535) >= F.hi; the smallest denormal keeps s > 0); qmin = largest q with v(q) <= box.lo, qmax =
+// smallest q with v(q) >= box.hi (outward rounding => decoded box contains
+// the exact one).  Frame = decoded box of the parent for a left child
+// (reached by descent), the scene box for the root and every right child
+// (reached by a rope).
+static const float Q16_INV_UP = 0x1.00010ap-16f;
 
-static inline float q16_lo(float flo, float s, int q) { return std::fma((float)q, s, flo); }
-static inline float q16_hi(float fhi, float s, int q)
-{
-  return std::fma(-(float)(65535 - q), s, fhi);
-}
+static inline float q16_v(float flo, float s, int q) { return std::fma((float)q, s, flo); }
 static inline void q16_encode_axis(float flo, float fhi, float vlo, float vhi,
                                    uint16_t &qlo, uint16_t &qhi)
 {
-  float s = (fhi - flo) * Q16_INV;
+  float s = std::fma(fhi - flo, Q16_INV_UP, 0x1p-149f);
   if (!(s > 0.f))
   {
     qlo = 0, qhi = 65535;
     return;
   }
-  // largest q with lo(q) <= vlo (lo(0) == flo <= vlo); lo is monotone in q
+  // largest q with v(q) <= vlo (v(0) == flo <= vlo); v is monotone in q
   int a = 0, b = 65535;
   while (a < b)
   {
     int m = (a + b + 1) >> 1;
-    if (q16_lo(flo, s, m) <= vlo)
+    if (q16_v(flo, s, m) <= vlo)
       a = m;
     else
       b = m - 1;
   }
   qlo = (uint16_t)a;
-  // smallest q with hi(q) >= vhi (hi(65535) == fhi >= vhi)
+  // smallest q with v(q) >= vhi (v(65535) >= fhi >= vhi)
   a = 0, b = 65535;
   while (a < b)
   {
     int m = (a + b) >> 1;
-    if (q16_hi(fhi, s, m) >= vhi)
+    if (q16_v(flo, s, m) >= vhi)
       b = m;
     else
       a = m + 1;
@@ -719,9 +717,9 @@ void orc_q16_encode_box(const float *frame6, const float *box6, uint16_t *q6,
     uint16_t a, b;
     q16_encode_axis(frame6[d], frame6[3 + d], box6[d], box6[3 + d], a, b);
     q6[d] = a, q6[3 + d] = b;
-    float s = (frame6[3 + d] - frame6[d]) * Q16_INV;
-    decoded6[d] = (s > 0.f) ? q16_lo(frame6[d], s, a) : frame6[d];
-    decoded6[3 + d] = (s > 0.f) ? q16_hi(frame6[3 + d], s, b) : frame6[3 + d];
+    float s = std::fma(frame6[3 + d] - frame6[d], Q16_INV_UP, 0x1p-149f);
+    decoded6[d] = (s > 0.f) ? q16_v(frame6[d], s, a) : frame6[d];
+    decoded6[3 + d] = (s > 0.f) ? q16_v(frame6[d], s, b) : frame6[3 + d];
   }
 }
 

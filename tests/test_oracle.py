@@ -262,10 +262,31 @@ def test_flattened_layout(flat_max):
         assert row == list(i_ref[o_ref[q]:o_ref[q + 1]])
 
 
+def _fma32(a, b, c):
+    """Correctly rounded float32 fma (exact rational, then the nearest
+    float32, ties to even)."""
+    from fractions import Fraction
+    exact = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    f = np.float32(float(exact))
+    cands = [f, np.nextafter(f, np.float32(np.inf)), np.nextafter(f, np.float32(-np.inf))]
+    best = min(cands, key=lambda v: (abs(Fraction(float(v)) - exact),
+                                     int(np.float32(v).view(np.uint32)) & 1))
+    return np.float32(best)
+
+
+Q16_C = np.float32(float.fromhex("0x1.00010ap-16"))
+
+
 def test_q16_encoder_is_tight_and_conservative():
+    """v(q) = fma(q, s, F.lo) with s = fma(F.hi - F.lo, C, 2^-149): the
+    decoded box contains the box, one step inwards would cut it, and the
+    grid reaches the frame's top (v(65535) >= F.hi)."""
     rng = np.random.default_rng(7)
-    for _ in range(2000):
+    tiny = np.float32(2.0 ** -149)
+    for t in range(600):
         f = np.sort(rng.uniform(-5, 5, (2, 3)).astype(np.float32), axis=0)
+        if t % 50 == 0:
+            f[1, 0] = np.nextafter(f[0, 0], np.float32(np.inf))  # one ulp wide
         frame = np.concatenate([f[0], f[1]])
         b = np.sort(rng.uniform(0, 1, (2, 3)), axis=0)
         box = np.concatenate([f[0] + (f[1] - f[0]) * b[0], f[0] + (f[1] - f[0]) * b[1]]).astype(np.float32)
@@ -273,13 +294,15 @@ def test_q16_encoder_is_tight_and_conservative():
         q, dec = oracle.q16_encode_box(frame, box)
         assert np.all(dec[:3] <= box[:3]) and np.all(dec[3:] >= box[3:])
         for d in range(3):
-            s = np.float32((frame[3 + d] - frame[d]) * np.float32(1.0 / 65535.0))
-            fma = lambda a, b, c: np.float32(float(a) * float(b) + float(c))  # noqa: E731
-            # (the float64 product of two floats is exact; one rounding to float)
-            if s > 0 and q[d] < 65535:  # one step up would cut the box
-                assert fma(np.float32(int(q[d]) + 1), s, frame[d]) > box[d]
-            if s > 0 and q[3 + d] > 0:
-                assert fma(-np.float32(65535 - (int(q[3 + d]) - 1)), s, frame[3 + d]) < box[3 + d]
+            s = _fma32(np.float32(frame[3 + d] - frame[d]), Q16_C, tiny)
+            assert s > 0
+            assert _fma32(np.float32(65535), s, frame[d]) >= frame[3 + d]
+            assert dec[d] == _fma32(np.float32(int(q[d])), s, frame[d])
+            assert dec[3 + d] == _fma32(np.float32(int(q[3 + d])), s, frame[d])
+            if q[d] < 65535:  # one step up would cut the box
+                assert _fma32(np.float32(int(q[d]) + 1), s, frame[d]) > box[d]
+            if q[3 + d] > 0:  # one step down would cut it
+                assert _fma32(np.float32(int(q[3 + d]) - 1), s, frame[d]) < box[3 + d]
 
 
 # ------------------------------------------------------------ queries vs brute force
