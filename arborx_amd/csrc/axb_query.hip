@@ -36,7 +36,9 @@ namespace
 
 // One lane per predicate (in `order` when given): rope traversal of the
 // whole tree; row q of the CRS (counts, scratch rows or the final rows).
-template <int F, int G, int PK, int MODE, bool SLOTS = false>
+// COUNT: the launch adds its visits to the visit counters (profiling
+// passes); without it the walk carries no counters.
+template <int F, int G, int PK, int MODE, bool SLOTS = false, bool COUNT = true>
 __global__ __launch_bounds__(256) void k_spatial(
     TreeView tv, const float *__restrict__ preds, int64_t nq, float radius,
     const uint32_t *__restrict__ order, int64_t *__restrict__ offsets,
@@ -65,7 +67,7 @@ __global__ __launch_bounds__(256) void k_spatial(
   rw.limit = MODE == MODE_FILL ? offsets[q + 1] - rw.base : (int64_t)buffer;
   uint32_t v_int = 0, v_leaf = 0;
   rope_walk<F, G, PK, SLOTS>(tv, pr, 0u, tv.total, ~0u, rw, v_int, v_leaf);
-  row_epilogue(tv, rw, offsets, q, v_int, v_leaf);
+  row_epilogue(tv, rw, offsets, q, COUNT ? v_int : 0u, COUNT ? v_leaf : 0u);
 }
 
 // Rows of leaf slots hit by each ray -> rows of values in (distance, index)
@@ -782,62 +784,80 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
     int c = 0;
     uint32_t p = 0;
     const uint32_t end = tv.total;
-    Box6 par = tv.scene;
-    float ps[3] = {tv.ss[0], tv.ss[1], tv.ss[2]};
-    bool desc = false;
-    while (p < end)
+    WalkFrame fr;
+    walk_frame_init(tv, fr);
+    while (true)
     {
-      const uint4 s0 = tv.slots[p];
-      const uint4 s1 = F == NODES_FP32 ? tv.slots[p + 1] : s0;
-      const int32_t w = (int32_t)s0.w;
-      const bool leaf = w < 0;
-      Box6 bx;
-      node_box<F>(tv, s0, s1, desc, par, ps, bx);
-      const float cx = leaf ? bits_f(s0.x) : clamp_box(qx, bx.lo[0], bx.hi[0]);
-      const float cy = leaf ? bits_f(s0.y) : clamp_box(qy, bx.lo[1], bx.hi[1]);
-      const float cz = leaf ? bits_f(s0.z) : clamp_box(qz, bx.lo[2], bx.hi[2]);
-      const float d2 = dist2_pp(qx, qy, qz, cx, cy, cz);
-      const bool pass = d2 <= thr;
-      v_leaf += leaf;
-      v_int += !leaf;
-      if (leaf && pass)
+      // the walk proper, until its end or a leaf within R finds the row
+      // full; the rare shrink below stays out of this loop, so that the
+      // loop carries neither R nor its bounds
+      bool full = false;
+      uint2 held = make_uint2(0u, 0u); // the leaf that found the row full
+      while (p < end)
       {
-        if (c == cap)
+        const uint4 s0 = tv.slots[p];
+        const uint4 s1 = F == NODES_FP32 ? tv.slots[p + 1] : s0;
+        const int32_t w = (int32_t)s0.w;
+        const bool leaf = w < 0;
+        Box6 bx;
+        walk_node_box<F>(s0, s1, fr, bx);
+        const float cx = leaf ? bits_f(s0.x) : clamp_box(qx, bx.lo[0], bx.hi[0]);
+        const float cy = leaf ? bits_f(s0.y) : clamp_box(qy, bx.lo[1], bx.hi[1]);
+        const float cz = leaf ? bits_f(s0.z) : clamp_box(qz, bx.lo[2], bx.hi[2]);
+        const float d2 = dist2_pp(qx, qy, qz, cx, cy, cz);
+        const bool pass = d2 <= thr;
+        v_leaf += leaf;
+        v_int += !leaf;
+        if (!leaf && !pass && w == 0)
         {
-          for (int t = 0; t < 3 && c == cap && R * shrink > R_lo; ++t)
-          {
-            R_hi = R;
-            R = R * shrink;
-            thr = sqrt_threshold(R);
-            int kept = 0;
-            for (int j = 0; j < c; ++j)
-            {
-              const uint2 e = cand[row_at + j];
-              if (bits_f(e.y) <= thr)
-                cand[row_at + kept++] = e;
-            }
-            c = kept;
-          }
-          if (c == cap)
-            return cap + 1; // still full: left to the next round
-          stage.reload(cand, row_at, c);
+          raise_error(tv.err, KERR_LAYOUT);
+          return c;
         }
-        if (d2 <= thr)
+        walk_frame_next<F>(tv, !leaf && pass, bx, fr);
+        p += leaf ? 1u : (pass ? SI : (uint32_t)w);
+        // A candidate goes to the stage; every lane writes its next free
+        // stage entry each step (without a candidate that entry stays
+        // free), which keeps the step free of a branch the wavefront would
+        // take at almost every step (some lane has a candidate).
+        const bool hit = leaf && pass;
+        const uint2 e = make_uint2((uint32_t)~w, f_bits(d2));
+        if (hit && c == cap)
         {
-          stage.put(c, make_uint2((uint32_t)~w, f_bits(d2)));
-          ++c;
-          if ((c & (KNN_STAGE - 1)) == 0)
-            stage.flush(cand, row_at, c - KNN_STAGE);
+          full = true;
+          held = e;
+          break;
         }
+        stage.put(c, e);
+        c += hit;
+        if (hit && (c & (KNN_STAGE - 1)) == 0)
+          stage.flush(cand, row_at, c - KNN_STAGE);
       }
-      if (!leaf && !pass && w == 0)
+      if (!full)
+        break;
+      for (int t = 0; t < 3 && c == cap && R * shrink > R_lo; ++t)
       {
-        raise_error(tv.err, KERR_LAYOUT);
-        return c;
+        R_hi = R;
+        R = R * shrink;
+        thr = sqrt_threshold(R);
+        int kept = 0;
+        for (int j = 0; j < c; ++j)
+        {
+          const uint2 e = cand[row_at + j];
+          if (bits_f(e.y) <= thr)
+            cand[row_at + kept++] = e;
+        }
+        c = kept;
       }
-      set_frame<F>(bx, par, ps); // read only when the next record descends
-      desc = !leaf && pass;
-      p += leaf ? 1u : (pass ? SI : (uint32_t)w);
+      if (c == cap)
+        return cap + 1; // still full: left to the next round
+      stage.reload(cand, row_at, c);
+      if (bits_f(held.y) <= thr)
+      {
+        stage.put(c, held);
+        ++c;
+        if ((c & (KNN_STAGE - 1)) == 0)
+          stage.flush(cand, row_at, c - KNN_STAGE);
+      }
     }
     if (c & (KNN_STAGE - 1)) // the last, partial chunk (the rest of it unused)
       stage.flush(cand, row_at, c & ~(KNN_STAGE - 1));
@@ -927,7 +947,7 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
 // walk then fits the 64 registers of full occupancy without spilling.
 constexpr int KNN_ROUNDS = 6;
 
-template <int F, int G, int K>
+template <int F, int G, int K, bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 && G == GEOM_POINTS) ? 8 : 1))) void k_knn_collect(
     TreeView tv, const float *__restrict__ pts, int64_t t0, int64_t tn, int k, int W,
     const uint32_t *__restrict__ order, uint2 *__restrict__ cand, int cap, float shrink,
@@ -980,7 +1000,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 &&
     select_row<K>(cand, row_at, c, m, oi, od, q * m);
   else
     ovf[atomicAdd(ovf_n, 1u)] = (uint32_t)t;
-  if (tv.visits)
+  if (COUNT && tv.visits)
   {
     atomicAdd(tv.visits + 0, 1ull);
     atomicAdd(tv.visits + 1, (unsigned long long)v_int);
@@ -1227,21 +1247,31 @@ __global__ void k_h2f_fill(const int64_t *__restrict__ offsets,
 inline unsigned grid_of(int64_t n) { return (unsigned)div_up(n, 256); }
 
 
+template <int F, int G, int PK, bool COUNT>
+static hipError_t launch_spatial_fgc(const TreeView &tv, const float *preds, int64_t nq,
+                                     float radius, const uint32_t *order, int64_t *offsets,
+                                     int32_t *out, int32_t buffer, int mode, hipStream_t s)
+{
+  if (mode == MODE_COUNT)
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_COUNT, false, COUNT>), dim3(grid_of(nq)), dim3(256), 0,
+               s, tv, preds, nq, radius, order, offsets, out, buffer);
+  else if (mode == MODE_FILL)
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_FILL, false, COUNT>), dim3(grid_of(nq)), dim3(256), 0,
+               s, tv, preds, nq, radius, order, offsets, out, buffer);
+  else
+    AXB_LAUNCH((k_spatial<F, G, PK, MODE_BUFFER, false, COUNT>), dim3(grid_of(nq)), dim3(256),
+               0, s, tv, preds, nq, radius, order, offsets, out, buffer);
+  return hipGetLastError();
+}
 template <int F, int G, int PK>
 static hipError_t launch_spatial_fg(const TreeView &tv, const float *preds, int64_t nq,
                                     float radius, const uint32_t *order, int64_t *offsets,
                                     int32_t *out, int32_t buffer, int mode, hipStream_t s)
 {
-  if (mode == MODE_COUNT)
-    AXB_LAUNCH((k_spatial<F, G, PK, MODE_COUNT>), dim3(grid_of(nq)), dim3(256), 0, s, tv, preds,
-               nq, radius, order, offsets, out, buffer);
-  else if (mode == MODE_FILL)
-    AXB_LAUNCH((k_spatial<F, G, PK, MODE_FILL>), dim3(grid_of(nq)), dim3(256), 0, s, tv, preds,
-               nq, radius, order, offsets, out, buffer);
-  else
-    AXB_LAUNCH((k_spatial<F, G, PK, MODE_BUFFER>), dim3(grid_of(nq)), dim3(256), 0, s, tv, preds,
-               nq, radius, order, offsets, out, buffer);
-  return hipGetLastError();
+  return tv.visits ? launch_spatial_fgc<F, G, PK, true>(tv, preds, nq, radius, order, offsets,
+                                                         out, buffer, mode, s)
+                   : launch_spatial_fgc<F, G, PK, false>(tv, preds, nq, radius, order, offsets,
+                                                          out, buffer, mode, s);
 }
 
 template <int PK>
@@ -1299,9 +1329,9 @@ struct KnnShape
 static KnnShape knn_shape(int k, int n)
 {
   KnnShape sh;
-  // seed window: 6k leaves (measured at 10M points, k = 16: 3k 13.9 ms,
-  // 4k 13.2, 6k 13.1, 8k 13.15 -- a tighter seed radius against a longer
-  // seed scan)
+  // seed window: 6k leaves (measured at 10M points, k = 16, Q16 nodes:
+  // 3k 11.2 ms uniform / 15.5 ms Gaussian mixture, 4k 10.9 / 14.5, 6k
+  // 11.0 / 13.8 -- a tighter seed radius against a longer seed scan)
   sh.W = std::min(n, std::max(6 * k, 16));
   // row capacity: about 6k, a multiple of 8 (10M points, k = 16: 4k 13.1 ms
   // uniform / 20.0 ms Gaussian mixture, 6k 12.85 / 20.9, 8k 13.1 / 23.2 --
@@ -1327,8 +1357,13 @@ static hipError_t knn_run(const TreeView &tv, const float *pts, int64_t nq, int 
   for (int64_t t0 = 0; e == hipSuccess && t0 < nq; t0 += ws.batch)
   {
     const int64_t tn = std::min(ws.batch, nq - t0);
-    AXB_LAUNCH((k_knn_collect<F, G, K>), dim3(grid_of(tn)), dim3(256), 0, s, tv, pts, t0, tn, k,
-               sh.W, order, ws.cand, sh.cap, sh.shrink, rounds, ws.ovf, ws.ovf_n, oi, od);
+    if (tv.visits)
+      AXB_LAUNCH((k_knn_collect<F, G, K, true>), dim3(grid_of(tn)), dim3(256), 0, s, tv, pts, t0,
+                 tn, k, sh.W, order, ws.cand, sh.cap, sh.shrink, rounds, ws.ovf, ws.ovf_n, oi, od);
+    else
+      AXB_LAUNCH((k_knn_collect<F, G, K, false>), dim3(grid_of(tn)), dim3(256), 0, s, tv, pts,
+                 t0, tn, k, sh.W, order, ws.cand, sh.cap, sh.shrink, rounds, ws.ovf, ws.ovf_n, oi,
+                 od);
     e = hipGetLastError();
   }
   // unresolved queries of every batch, in one launch at the end: each of

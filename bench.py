@@ -107,22 +107,23 @@ def calibrate_radius(ax, p, r, target, sample=97, iters=24):
     return float(np.float32(best))
 
 
-PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_uniform10M_q16.json")
+PMC = os.path.join(ROOT, "profiles", "pmc_kernels_uniform10M.json")
+# VALU issue capacity of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction
+# per SIMD every 4 cycles (16 lanes wide), 2.4 GHz peak engine clock
+VALU_PEAK_INSTS_S = 256 * 4 * 2.4e9 / 4
 
 
-def measured_traffic(kernel, case):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC runs
-    (FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy kernel), for
-    the workload they were taken on; None otherwise."""
-    if case != "uniform/q16" or not os.path.exists(PMC_TRAFFIC):
+def measured(kernel, case):
+    """The committed rocprofv3 PMC figures of `kernel` (its base name) in
+    `case` ("uniform/q16", "boxes/fp32", ...): HBM bytes per dispatch
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known size) and
+    SQ instruction counts; None when the workload was not profiled."""
+    if not os.path.exists(PMC):
         return None
-    with open(PMC_TRAFFIC) as f:
+    with open(PMC) as f:
         data = json.load(f)
     base = kernel.strip("()").split("<")[0]
-    for name, v in data["kernels"].items():
-        if name.split("<")[0] == base:
-            return v["hbm_bytes_per_dispatch"]
-    return None
+    return data.get("cases", {}).get(case, {}).get(base)
 
 
 def timed(fn, steps, warmup, stream):
@@ -179,11 +180,17 @@ def bench_boxes(ax, args, torch, synthetic, stream):
                         query_qps=n / tq * 1e3, hits_per_query=v["results"] / n,
                         internal_visits_per_query=v["internal"] / n,
                         leaf_visits_per_query=v["leaf"] / n,
-                        bytes_per_query=alg / n, traversal_GBps=alg / (trav * 1e-3) / 1e9,
+                        bytes_per_query=alg / n, traversal_alg_GBps=alg / (trav * 1e-3) / 1e9,
                         identical_to_fp32=identical)
+        pm = measured("k_spatial", f"boxes/{fmt}")
+        if pm:
+            # measured HBM bytes of the traversal (rocprofv3 PMC, committed)
+            out[fmt]["traversal_hbm_bytes_per_query"] = pm["hbm_bytes"] / n
+            out[fmt]["traversal_hbm_GBps"] = pm["hbm_bytes"] / (trav * 1e-3) / 1e9
         log(f"[bench] boxes/{fmt}: build {tb:.2f} ms | query {tq:.2f} ms "
             f"({n / tq / 1e3:.1f} Mq/s, {v['results'] / n:.3f} hits/q) | traversal "
-            f"{trav:.2f} ms, {alg / n:.0f} B/q, {out[fmt]['traversal_GBps']:.0f} GB/s | "
+            f"{trav:.2f} ms, {alg / n:.0f} B/q, {out[fmt]['traversal_alg_GBps']:.0f} GB/s "
+            "(algorithmic) | "
             f"identical {identical}")
     out["q16_traversal_speedup"] = out["fp32"]["traversal_kernel_ms"] / out["q16"]["traversal_kernel_ms"]
     out["config"] = f"{n} data boxes (seed 4) and {n} query boxes (seed 5): centres U[0,1)^3, " \
@@ -268,8 +275,8 @@ def bench_energy(ax, args, torch, synthetic, stream):
     out = dict(n_points=n, cells=list(cells), cutoff=rc, mean_neighbours=nnz / n,
                energy=float(e.item()), csr_ms=t_csr, forward_ms=t_f, fwd_bwd_ms=t_fb,
                fwd_bwd_over_forward=t_fb / t_f, forward_kernel_ms=k_f, fwd_bwd_kernel_ms=k_fb,
-               forward_GBps=alg_f / (k_f * 1e-3) / 1e9,
-               fwd_bwd_GBps=(alg_f + n * 12) / (k_fb * 1e-3) / 1e9,
+               forward_alg_GBps=alg_f / (k_f * 1e-3) / 1e9,
+               fwd_bwd_alg_GBps=(alg_f + n * 12) / (k_fb * 1e-3) / 1e9,
                autograd_step_ms=t_ag, autograd_recompute_csr_step_ms=t_agr,
                grad_rel_err_vs_fp64_fd=rel, fd_atoms=1000, fd_h=1e-6)
     log(f"[bench] energy: N={n}, {nnz / n:.1f} nbrs | CSR {t_csr:.2f} ms | fwd {t_f:.3f} ms "
@@ -574,14 +581,31 @@ def main():
             tot_ms = v["total_ms"]
         launches = kernels[name]["launches"]
         achieved = alg / (tot_ms * 1e-3) / 1e9
-        traffic = measured_traffic(name, head_key)
+        avg_ms = tot_ms / launches
+        pm = measured(name, head_key)
+        traffic = int(pm["hbm_bytes"]) if pm else None
         roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                         unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                         kernel=name, launches_per_call=launches,
-                        avg_launch_ms=round(tot_ms / launches, 4),
+                        avg_launch_ms=round(avg_ms, 4),
                         alg_bytes_per_launch=int(alg / launches),
                         alg_bytes_per_unit=round(alg / N, 1), unit_of_work=per,
                         dominant_kernel=dom[0], visits_radius=vr, visits_knn=vk)
+        if pm:
+            # What actually binds the kernel, from the committed rocprofv3
+            # counters of the same workload: measured HBM bytes per launch
+            # against the 8 TB/s peak, and VALU wave-instructions per launch
+            # against the device's issue rate.  Algorithmic bytes exceed the
+            # measured ones: node records hit in L2 and the Infinity Cache.
+            hbm = pm["hbm_bytes"] / (avg_ms * 1e-3) / 1e9
+            issue_ms = pm["SQ_INSTS_VALU"] / VALU_PEAK_INSTS_S * 1e3
+            roofline.update(
+                hbm_measured_GBps=round(hbm, 1), hbm_measured_frac=round(hbm / HBM_PEAK_GBS, 4),
+                valu=dict(insts_per_launch=int(pm["SQ_INSTS_VALU"]),
+                          peak_insts_per_s=VALU_PEAK_INSTS_S, issue_ms=round(issue_ms, 4),
+                          issue_frac=round(issue_ms / avg_ms, 4)),
+                binding=("VALU issue" if issue_ms / avg_ms >= 0.7 else "memory latency"),
+                counters_source=os.path.relpath(PMC, ROOT))
         log(f"[bench] roofline: {roofline}")
 
     # ---- CPU baseline: oracle on host cores, bounded samples

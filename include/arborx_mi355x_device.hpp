@@ -244,13 +244,26 @@ AXB_HD float q16_lo(float flo, float s, uint32_t q)
 {
   return __builtin_fmaf((float)q, s, flo);
 }
+// 1 / s: the hardware's approximate reciprocal on the device, a division on
+// the host.  It only seeds the searches below, whose result does not
+// depend on the seed.
+AXB_HD float approx_rcp(float s)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(s);
+#else
+  return 1.0f / s;
+#endif
+}
+
 // Largest q in [0, 65535] with v(q) <= v (0 when there is none).  v(q) is
 // non-decreasing in q; the search starts from the window around the
-// estimate (v - flo) / s and widens to the full range when the window does
-// not bracket the answer, so the result is the plain binary search's.
-AXB_HD uint32_t q16_lo_index(float flo, float s, float v)
+// estimate (v - flo) / s (rs: about 1 / s) and widens to the full range when
+// the window does not bracket the answer, so the result is the plain binary
+// search's.
+AXB_HD uint32_t q16_lo_index(float flo, float s, float rs, float v)
 {
-  const float est = (v - flo) / s;
+  const float est = (v - flo) * rs;
   const int q = est >= 0.f ? (est <= 65535.f ? (int)est : 65535) : 0; // NaN -> 0
   int a = q - 2 < 0 ? 0 : q - 2, b = q + 2 > 65535 ? 65535 : q + 2;
   if (!((a == 0 || q16_lo(flo, s, (uint32_t)a) <= v) &&
@@ -269,9 +282,9 @@ AXB_HD uint32_t q16_lo_index(float flo, float s, float v)
 
 // Smallest q in [0, 65535] with v(q) >= v (65535 when there is none).
 // Same bracketing as q16_lo_index.
-AXB_HD uint32_t q16_hi_index(float flo, float s, float v)
+AXB_HD uint32_t q16_hi_index(float flo, float s, float rs, float v)
 {
-  const float est = (v - flo) / s;
+  const float est = (v - flo) * rs;
   const int q = est >= 0.f ? (est <= 65535.f ? (int)est : 65535) : 0; // NaN -> 0
   int a = q - 2 < 0 ? 0 : q - 2, b = q + 2 > 65535 ? 65535 : q + 2;
   if (!((b == 65535 || q16_lo(flo, s, (uint32_t)b) >= v) &&
@@ -297,8 +310,9 @@ AXB_HD void q16_encode_axis(float flo, float fhi, float vlo, float vhi,
     qlo = 0, qhi = 65535;
     return;
   }
-  qlo = q16_lo_index(flo, s, vlo);
-  qhi = q16_hi_index(flo, s, vhi);
+  const float rs = approx_rcp(s);
+  qlo = q16_lo_index(flo, s, rs, vlo);
+  qhi = q16_hi_index(flo, s, rs, vhi);
 }
 
 // Decodes a Q16 record (words x, y, z) against a frame (lower corner flo,
@@ -498,22 +512,70 @@ AXB_HD SpatialPred ray_spatial_pred(const float *p6)
   return p;
 }
 
-// State a point walk carries from one record to the next: the Q16 frame
-// (box and scale of the record just left, read only when the walk descended
-// from it) and whether it descended.
+// State a point walk carries from one record to the next: the Q16 frame of
+// the next record -- lower corner and step of the box just decoded when the
+// walk descends into it, else the scene's -- chosen by selects at the end
+// of each step (lanes of a wavefront disagree on descending all the time).
+// The scene's frame is kept in per-lane registers (slo, sst): a select can
+// read one scalar register at most besides its condition, so with the
+// scene's values in scalar registers every step would first copy six of
+// them to vector registers.
 struct WalkFrame
 {
-  Box6 par;
-  float ps[3];
-  bool desc;
+  float lo[3], s[3];
+  float slo[3], sst[3];
 };
+
+// x in a vector register, opaque to the compiler (so that it stays there)
+AXB_HD float lane_copy(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  float y;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+  return y;
+#else
+  return x;
+#endif
+}
 
 template <class TV>
 AXB_HD void walk_frame_init(const TV &tv, WalkFrame &fr)
 {
-  fr.par = tv.scene;
-  fr.ps[0] = tv.ss[0], fr.ps[1] = tv.ss[1], fr.ps[2] = tv.ss[2];
-  fr.desc = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    fr.slo[d] = lane_copy(tv.scene.lo[d]);
+    fr.sst[d] = lane_copy(tv.ss[d]);
+    fr.lo[d] = fr.slo[d], fr.s[d] = fr.sst[d];
+  }
+}
+
+// Box of the internal record s0/s1 of a point walk (Q16: against fr).
+template <int F>
+AXB_HD void walk_node_box(uint4 s0, uint4 s1, const WalkFrame &fr, Box6 &b)
+{
+  if (F == NODES_FP32)
+  {
+    b.lo[0] = bits_f(s0.x), b.lo[1] = bits_f(s0.y), b.lo[2] = bits_f(s0.z);
+    b.hi[0] = bits_f(s1.x), b.hi[1] = bits_f(s1.y), b.hi[2] = bits_f(s1.z);
+  }
+  else
+    q16_decode(s0.x, s0.y, s0.z, fr.lo, fr.s, b);
+}
+
+// The frame of the next record: b when the walk descends into it.
+template <int F, class TV>
+AXB_HD void walk_frame_next(const TV &, bool desc, const Box6 &b, WalkFrame &fr)
+{
+  if (F == NODES_Q16)
+  {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      fr.lo[d] = desc ? b.lo[d] : fr.slo[d];
+      fr.s[d] = desc ? q16_scale(b.lo[d], b.hi[d]) : fr.sst[d];
+    }
+  }
 }
 
 // One record of the point walk at slot p, leaf and internal node through
@@ -531,7 +593,7 @@ AXB_HD bool point_step(const TV &tv, const SpatialPred &pr, uint32_t &p, uint32_
   const int32_t w = (int32_t)s0.w;
   const bool leaf = w < 0;
   Box6 bx; // meaningless for a leaf record, and unused then
-  node_box<F>(tv, s0, s1, fr.desc, fr.par, fr.ps, bx);
+  walk_node_box<F>(s0, s1, fr, bx);
   const float cx = leaf ? bits_f(s0.x) : clamp_box(pr.cx, bx.lo[0], bx.hi[0]);
   const float cy = leaf ? bits_f(s0.y) : clamp_box(pr.cy, bx.lo[1], bx.hi[1]);
   const float cz = leaf ? bits_f(s0.z) : clamp_box(pr.cz, bx.lo[2], bx.hi[2]);
@@ -545,10 +607,7 @@ AXB_HD bool point_step(const TV &tv, const SpatialPred &pr, uint32_t &p, uint32_
     raise_error(tv.err, KERR_LAYOUT);
     return false;
   }
-  // the frame of the next record is this box only when it descends; it is
-  // stored every step (selects, no branch) and read only then
-  set_frame<F>(bx, fr.par, fr.ps);
-  fr.desc = !leaf && pass;
+  walk_frame_next<F>(tv, !leaf && pass, bx, fr);
   p += leaf ? 1u : (pass ? SI : (uint32_t)w);
   return true;
 }
@@ -624,9 +683,8 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
   }
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
   constexpr int LG = leaf_group<G>();
-  Box6 par;
-  float ps[3];
-  bool desc = false;
+  WalkFrame fr;
+  walk_frame_init(tv, fr);
   while (p < end)
   {
     const uint4 s0 = tv.slots[p];
@@ -660,32 +718,23 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
       }
       if (stop)
         break;
-      desc = false;
+      walk_frame_next<F>(tv, false, Box6{}, fr);
     }
     else
     {
       ++v_int;
       Box6 bx;
-      node_box<F>(tv, s0, s1, desc, par, ps, bx);
+      walk_node_box<F>(s0, s1, fr, bx);
       const bool pass = PK == PRED_RAYS    ? ray_hits(pr.ray, bx)
                         : PK == PRED_BOXES ? intersects_bb(pr.qb, bx)
                                            : dist2_pnode(pr.cx, pr.cy, pr.cz, bx) <= pr.thr;
-      if (pass)
+      if (!pass && w == 0)
       {
-        set_frame<F>(bx, par, ps);
-        desc = true;
-        p += SI;
+        raise_error(tv.err, KERR_LAYOUT);
+        break;
       }
-      else
-      {
-        if (w == 0)
-        {
-          raise_error(tv.err, KERR_LAYOUT);
-          break;
-        }
-        desc = false;
-        p += (uint32_t)w;
-      }
+      walk_frame_next<F>(tv, pass, bx, fr);
+      p += pass ? SI : (uint32_t)w;
     }
   }
 }
