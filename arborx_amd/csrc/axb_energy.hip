@@ -26,6 +26,17 @@ AXB_D double wave_sum_f64(double v)
   return v;
 }
 
+// A group of ROW_LANES consecutive lanes per row: lane l of the group takes
+// the row's pairs l, l + ROW_LANES, ...; the group then adds its lanes'
+// sums by xor shuffles.  The group reads its row's indices as contiguous
+// 32-byte pieces and gathers the neighbours of one point at a time, so the
+// gathers of a wavefront stay in a small region (its 8 rows are
+// consecutive); one lane per row spread each step's loads over 64 rows.
+// A block takes ROWS_PER_BLOCK rows at a time, in a grid-stride loop over
+// a grid of at most ENERGY_SCRATCH_BLOCKS blocks (the partial sums).
+constexpr int ROW_LANES = 8;
+constexpr int ROWS_PER_BLOCK = 256 / ROW_LANES;
+
 template <bool WANT_G>
 __global__ __launch_bounds__(256) void k_energy(const float *__restrict__ xyz, int64_t n,
                                                 const int64_t *__restrict__ offsets,
@@ -35,43 +46,64 @@ __global__ __launch_bounds__(256) void k_energy(const float *__restrict__ xyz, i
                                                 float *__restrict__ grad)
 {
   // rows of neighbouring points share neighbours: rows are taken in Morton
-  // order of their points (`order`), one contiguous range of blocks per XCD
-  // (its L2), so the x_j gathered by a wavefront and by its XCD overlap
+  // order of their points (`order`, when given), one contiguous range of
+  // blocks per XCD (its L2), so the x_j gathered by a wavefront and by its
+  // XCD overlap
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t t = (int64_t)blk * blockDim.x + threadIdx.x;
+  const int sub = (int)(threadIdx.x % ROW_LANES);
   double e = 0.0;
-  if (t < n)
+  for (int64_t base = (int64_t)blk * ROWS_PER_BLOCK; base < n;
+       base += (int64_t)gridDim.x * ROWS_PER_BLOCK)
   {
-    const int64_t i = order ? (int64_t)order[t] : t;
-    const float xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
-    double gx = 0.0, gy = 0.0, gz = 0.0;
-    const int64_t b = offsets[i], end = offsets[i + 1];
-    for (int64_t p = b; p < end; ++p)
+    const int64_t t = base + threadIdx.x / ROW_LANES;
+    double er = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+    int64_t i = 0;
+    if (t < n)
     {
-      const int64_t j = indices[p];
-      if (j == i)
-        continue;
-      const float dx = xi - xyz[3 * j], dy = yi - xyz[3 * j + 1], dz = zi - xyz[3 * j + 2];
-      float r2 = dx * dx;
-      r2 = r2 + dy * dy;
-      r2 = r2 + dz * dz;
-      const float ir2 = 1.0f / r2;
-      const float ir6 = ir2 * ir2 * ir2;
-      e += (double)(4.0f * ir6 * (ir6 - 1.0f) - phic);
-      if (WANT_G)
+      i = order ? (int64_t)order[t] : t;
+      const float xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+      const int64_t end = offsets[i + 1];
+      for (int64_t p = offsets[i] + sub; p < end; p += ROW_LANES)
       {
-        const float f = 24.0f * ir2 * ir6 * (1.0f - 2.0f * ir6);
-        gx += (double)(f * dx);
-        gy += (double)(f * dy);
-        gz += (double)(f * dz);
+        const int64_t j = indices[p];
+        if (j == i)
+          continue;
+        const float dx = xi - xyz[3 * j], dy = yi - xyz[3 * j + 1], dz = zi - xyz[3 * j + 2];
+        float r2 = dx * dx;
+        r2 = r2 + dy * dy;
+        r2 = r2 + dz * dz;
+        const float ir2 = 1.0f / r2;
+        const float ir6 = ir2 * ir2 * ir2;
+        er += (double)(4.0f * ir6 * (ir6 - 1.0f) - phic);
+        if (WANT_G)
+        {
+          const float f = 24.0f * ir2 * ir6 * (1.0f - 2.0f * ir6);
+          gx += (double)(f * dx);
+          gy += (double)(f * dy);
+          gz += (double)(f * dz);
+        }
       }
     }
-    e *= 0.5;
-    if (WANT_G)
+#pragma unroll
+    for (int o = ROW_LANES / 2; o > 0; o >>= 1)
     {
-      grad[3 * i] = (float)gx;
-      grad[3 * i + 1] = (float)gy;
-      grad[3 * i + 2] = (float)gz;
+      er += __shfl_xor(er, o, 64);
+      if (WANT_G)
+      {
+        gx += __shfl_xor(gx, o, 64);
+        gy += __shfl_xor(gy, o, 64);
+        gz += __shfl_xor(gz, o, 64);
+      }
+    }
+    if (t < n && sub == 0)
+    {
+      e += 0.5 * er;
+      if (WANT_G)
+      {
+        grad[3 * i] = (float)gx;
+        grad[3 * i + 1] = (float)gy;
+        grad[3 * i + 2] = (float)gz;
+      }
     }
   }
   __shared__ double s[4];
@@ -82,6 +114,14 @@ __global__ __launch_bounds__(256) void k_energy(const float *__restrict__ xyz, i
   __syncthreads();
   if (threadIdx.x == 0)
     partials[blk] = ((s[0] + s[1]) + s[2]) + s[3];
+}
+
+// Blocks of k_energy for n rows (and partial sums): at most
+// ENERGY_SCRATCH_BLOCKS, the grid-stride loop takes the rest.
+int64_t energy_blocks(int64_t n)
+{
+  const int64_t need = div_up(n > 0 ? n : 1, ROWS_PER_BLOCK);
+  return need < ENERGY_SCRATCH_BLOCKS ? need : ENERGY_SCRATCH_BLOCKS;
 }
 
 __global__ __launch_bounds__(256) void k_reduce_f64(const double *__restrict__ partials,
@@ -187,7 +227,7 @@ hipError_t pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
                        double *energy, float *grad, hipStream_t s)
 {
   const float phic = shifted_phi_cutoff(cutoff);
-  const int64_t nb = div_up(n > 0 ? n : 1, 256);
+  const int64_t nb = energy_blocks(n);
   double *partials = (double *)dalloc(nb * sizeof(double), s);
   if (!partials)
     return hipErrorOutOfMemory;
@@ -226,9 +266,9 @@ hipError_t pair_energy_noalloc(const float *xyz, int64_t n, const int64_t *offse
                                const int32_t *indices, float cutoff, double *energy,
                                float *grad, hipStream_t s)
 {
-  const int64_t nb = div_up(n > 0 ? n : 1, 256);
-  if (nb > ENERGY_SCRATCH_BLOCKS)
+  if (n > ENERGY_SCRATCH_BLOCKS * 256)
     return hipErrorInvalidValue;
+  const int64_t nb = energy_blocks(n);
   double *partials = nullptr;
   const hipError_t e = energy_scratch_ptr(&partials);
   if (e != hipSuccess)

@@ -358,8 +358,9 @@ hipError_t launch_bf_nearest(const float *values, int64_t n, int geometry, const
 hipError_t pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
                        const int32_t *indices, float cutoff, const uint32_t *order,
                        double *energy, float *grad, hipStream_t s);
-// The same over the library's static scratch (no allocation): n up to
-// ENERGY_SCRATCH_BLOCKS * 256 rows, rows in the caller's order.
+// The same over the library's static scratch of ENERGY_SCRATCH_BLOCKS
+// partial sums (no allocation): n up to ENERGY_SCRATCH_BLOCKS * 256 rows,
+// rows in the caller's order.
 constexpr int64_t ENERGY_SCRATCH_BLOCKS = int64_t(1) << 19; // 4 MiB of partial sums
 constexpr int MAX_DEVICES = 64;
 hipError_t pair_energy_noalloc(const float *xyz, int64_t n, const int64_t *offsets,

@@ -317,9 +317,9 @@ axb_status axb_pair_energy(const float *xyz, int64_t n, const int64_t *offsets,
 /* The same energy and gradient without any allocation (no reference
  * counterpart).  For callers that cannot allocate device memory: code under
  * HIP graph capture, serving loops, harnesses that forbid hipMalloc.  Its
- * scratch (one double of partial sums per 256 rows) is a static device array
- * of AXB_PAIR_ENERGY_NOALLOC_MAX_N / 256 doubles (4 MiB) that every GPU the
- * library runs on keeps until the process ends; n above
+ * scratch (the partial sums of the energy kernel's blocks, at most 2^19) is
+ * a static device array of 2^19 doubles (4 MiB) that every GPU the library
+ * runs on keeps until the process ends; n above
  * AXB_PAIR_ENERGY_NOALLOC_MAX_N is rejected with AXB_ERR_INVALID_ARGUMENT.
  * Rows are taken in the caller's order (axb_pair_energy may reorder large
  * inputs, which changes E only by the summation order of its partials; the
