@@ -5,7 +5,7 @@
 //   Morton64 codes   SpaceFillingCurves.hpp:63-84    -> k_morton_hist (also
 //                    builds the radix-sort digit histograms in the same pass)
 //   sortObjects      SortUtils.hpp:26-43             -> axb_sort.hip
-//   hierarchy        TreeConstruction.hpp:71-322     -> k_climb
+//   hierarchy        TreeConstruction.hpp:71-322     -> k_climb_local, k_climb_global
 // and adds the pre-order linearisation the traversal kernels read
 // (k_lin_*): the same topology stored depth-first so that a node's left child
 // is the next record and its rope is `slot + skip`.
@@ -258,51 +258,235 @@ AXB_D void read_sibling(bool is_leaf, int j, const float4 *leafrec, const INode 
   }
 }
 
-// TreeConstruction.hpp:197-311: one thread per leaf climbs toward the root;
-// the first thread to reach a parent stops (CAS on ranges), the second
-// finishes it.  Writes INode records, sorted leaf records, the spine data for
-// the pre-order positions and (optionally) the reference-form ropes.
-template <int G, bool ROPES>
-__global__ __launch_bounds__(256) void k_climb(
-    const float *__restrict__ values, int n, const uint64_t *__restrict__ keys,
-    const uint32_t *__restrict__ perm, int32_t *ranges, INode *inodes,
-    float4 *leafrec, int32_t *__restrict__ leaf_rope, int32_t *__restrict__ c,
-    int32_t *__restrict__ top)
+// The climb of TreeConstruction.hpp:197-311 (one thread per leaf climbs
+// toward the root; the first thread to reach a parent stops, the second
+// finishes it) in two kernels.  k_climb_local: a block of CLIMB_LEAVES threads
+// takes as many consecutive leaves and climbs as long as the parent's split
+// lies between two of its own leaves, with the hand-off flags, the deltas of
+// its splits and the boxes of its nodes in LDS; a node whose parent split lies
+// between two blocks (at most one per block side) is handed to
+// k_climb_global, which continues with the flags in device memory.  Both
+// write the INode records, the sorted leaf records, the spine data for the
+// pre-order positions and (optionally) the reference-form ropes.
+constexpr int CLIMB_LEAVES = 256;
+
+template <int G>
+AXB_D void load_leaf(const float *__restrict__ values, uint32_t orig, float4 *leafrec,
+                     int i0, Box6 &bv)
 {
-  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i0 >= n)
-    return;
-  const int nint = n - 1;
-  const uint32_t orig = perm[i0];
-  Box6 bv;
   if (G == GEOM_POINTS)
   {
     const float *v = values + 3 * (int64_t)orig;
     float x = v[0], y = v[1], z = v[2];
     // Box{} expanded by the point (Expand.hpp:42-57)
-    bv.lo[0] = kmin(FLT_MAX, x), bv.lo[1] = kmin(FLT_MAX, y),
-    bv.lo[2] = kmin(FLT_MAX, z);
-    bv.hi[0] = kmax(-FLT_MAX, x), bv.hi[1] = kmax(-FLT_MAX, y),
-    bv.hi[2] = kmax(-FLT_MAX, z);
-    st_coh128(leafrec + i0, make_uint4(f_bits(x), f_bits(y), f_bits(z), ~orig));
+    bv.lo[0] = kmin(FLT_MAX, x), bv.lo[1] = kmin(FLT_MAX, y), bv.lo[2] = kmin(FLT_MAX, z);
+    bv.hi[0] = kmax(-FLT_MAX, x), bv.hi[1] = kmax(-FLT_MAX, y), bv.hi[2] = kmax(-FLT_MAX, z);
+    ((uint4 *)leafrec)[i0] = make_uint4(f_bits(x), f_bits(y), f_bits(z), ~orig);
   }
   else
   {
     const float *v = values + 6 * (int64_t)orig;
     for (int d = 0; d < 3; ++d)
       bv.lo[d] = kmin(FLT_MAX, v[d]), bv.hi[d] = kmax(-FLT_MAX, v[3 + d]);
-    st_coh128(leafrec + 2 * i0,
-              make_uint4(f_bits(bv.lo[0]), f_bits(bv.lo[1]), f_bits(bv.lo[2]), ~orig));
-    st_coh128(leafrec + 2 * i0 + 1,
-              make_uint4(f_bits(bv.hi[0]), f_bits(bv.hi[1]), f_bits(bv.hi[2]), 0u));
+    ((uint4 *)leafrec)[2 * i0] =
+        make_uint4(f_bits(bv.lo[0]), f_bits(bv.lo[1]), f_bits(bv.lo[2]), ~orig);
+    ((uint4 *)leafrec)[2 * i0 + 1] =
+        make_uint4(f_bits(bv.hi[0]), f_bits(bv.hi[1]), f_bits(bv.hi[2]), 0u);
   }
-  int range_left = i0, range_right = i0;
+}
+
+// Rope of a node (TreeConstruction.hpp:172-195) from the deltas of its last
+// split and the next one.
+AXB_D int32_t rope_of(int nint, int n, int range_right, int64_t delta_right,
+                      int64_t delta_next)
+{
+  if (range_right == nint)
+    return -1;
+  return delta_right < delta_next ? range_right + 1 : range_right + 1 + n;
+}
+
+template <int G, bool ROPES>
+__global__ __launch_bounds__(CLIMB_LEAVES) void k_climb_local(
+    const float *__restrict__ values, int n, const uint64_t *__restrict__ keys,
+    const uint32_t *__restrict__ perm, int32_t *__restrict__ ranges,
+    INode *__restrict__ inodes, float4 *__restrict__ leafrec,
+    int32_t *__restrict__ leaf_rope, int32_t *__restrict__ c, int32_t *__restrict__ top,
+    int4 *__restrict__ pending, int32_t *__restrict__ spine_count,
+    int32_t *__restrict__ spine_list)
+{
+  constexpr int B = CLIMB_LEAVES;
+  constexpr int LW = G == GEOM_POINTS ? 3 : 6; // words of a leaf box in LDS
+  __shared__ int64_t s_delta[B + 2];           // delta(base - 1 + j)
+  __shared__ int32_t s_range[B];               // flag of split base + j
+  __shared__ float s_node[7][B];               // lo, hi, spine height of node base + j
+  __shared__ float s_leaf[LW][B];              // box of leaf base + j
+  __shared__ int4 s_pend[2];                   // nodes at the left / right block split
+  __shared__ int32_t s_sp[B];                  // first leaves of stored left spines
+  __shared__ int32_t s_nsp;
+  const int t = threadIdx.x;
+  const int base = blockIdx.x * B;
+  const int m = min(B, n - base);
+  const int nint = n - 1;
+  for (int j = t; j < B + 2; j += B)
+    s_delta[j] = apetrei_delta(keys, nint, base - 1 + j);
+  s_range[t] = -1;
+  if (t < 2)
+    s_pend[t] = make_int4(-1, 0, 0, 0);
+  if (t == 0)
+    s_nsp = 0;
+  Box6 bv;
+  if (t < m)
+  {
+    load_leaf<G>(values, perm[base + t], leafrec, base + t, bv);
+    for (int d = 0; d < 3; ++d)
+    {
+      s_leaf[d][t] = bv.lo[d];
+      if (G == GEOM_BOXES)
+        s_leaf[3 + d][t] = bv.hi[d];
+    }
+  }
+  __syncthreads();
+  if (t < m)
+  {
+    const int i0 = base + t;
+    int range_left = i0, range_right = i0;
+    int64_t dl = s_delta[t], dr = s_delta[t + 1];
+    if (ROPES)
+      leaf_rope[i0] = rope_of(nint, n, i0, dr, s_delta[t + 2]);
+    int cur = i0;  // reference encoding of the current node
+    int cur_h = 0; // left-spine height of the current node (0 for a leaf)
+    do
+    {
+      const bool is_left = dr < dl;
+      const int ap = is_left ? range_right : range_left - 1;
+      if (ap < base || ap > base + m - 2)
+      {
+        // the parent's split lies between two blocks
+        s_pend[is_left ? 1 : 0] = make_int4(cur, cur_h, range_left, range_right);
+        break;
+      }
+      // this node's LDS record is written before the flag moves
+      __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0)
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      int32_t old = -1;
+      __hip_atomic_compare_exchange_strong(&s_range[ap - base], &old,
+                                           is_left ? range_left : range_right,
+                                           __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      if (old == -1)
+        break;
+      int left_child, h_left, right_node, h_right, sib;
+      bool sib_leaf;
+      if (is_left)
+      {
+        range_right = old;
+        left_child = cur;
+        h_left = cur_h;
+        sib = ap + 1;
+        sib_leaf = sib == range_right;
+        dr = s_delta[range_right - base + 1];
+        right_node = sib_leaf ? sib : sib + n;
+      }
+      else
+      {
+        range_left = old;
+        sib = ap;
+        sib_leaf = sib == range_left;
+        dl = s_delta[range_left - base];
+        left_child = sib_leaf ? sib : sib + n;
+        right_node = cur;
+        h_right = cur_h;
+      }
+      const int js = sib - base;
+      int sh = 0;
+      for (int d = 0; d < 3; ++d)
+      {
+        const float slo = sib_leaf ? s_leaf[d][js] : s_node[d][js];
+        const float shi =
+            sib_leaf ? s_leaf[G == GEOM_POINTS ? d : 3 + d][js] : s_node[3 + d][js];
+        bv.lo[d] = kmin(bv.lo[d], slo);
+        bv.hi[d] = kmax(bv.hi[d], shi);
+      }
+      if (!sib_leaf)
+        sh = __float_as_int(s_node[6][js]);
+      if (is_left)
+        h_right = sh;
+      else
+        h_left = sh;
+      const int kp = dr < dl ? range_right : range_left;
+      // spine height among stored nodes; 0 = not stored (at most FLAT_MAX
+      // leaves); stored nodes are the top part of every left spine
+      const int hp =
+          range_right - range_left + 1 > FLAT_MAX ? (left_child < n ? 0 : h_left) + 1 : 0;
+      const int32_t rope =
+          ROPES ? rope_of(nint, n, range_right, dr, s_delta[range_right - base + 2]) : -1;
+      uint4 *w = (uint4 *)(inodes + kp);
+      w[0] = make_uint4(f_bits(bv.lo[0]), f_bits(bv.lo[1]), f_bits(bv.lo[2]),
+                        (uint32_t)left_child);
+      w[1] = make_uint4(f_bits(bv.hi[0]), f_bits(bv.hi[1]), f_bits(bv.hi[2]), (uint32_t)rope);
+      w[2] = make_uint4((uint32_t)range_left, (uint32_t)range_right, (uint32_t)hp, 0u);
+      const int jk = kp - base;
+      for (int d = 0; d < 3; ++d)
+        s_node[d][jk] = bv.lo[d], s_node[3 + d][jk] = bv.hi[d];
+      s_node[6][jk] = __int_as_float(hp);
+      if (right_node >= n)
+      {
+        // the right child tops a left spine starting at leaf ap + 1
+        c[ap + 1] = h_right;
+        top[ap + 1] = right_node - n;
+        if (h_right > 0)
+          s_sp[atomicAdd(&s_nsp, 1)] = ap + 1;
+      }
+      cur = kp + n;
+      cur_h = hp;
+      if (kp == 0)
+      {
+        c[0] = hp;
+        top[0] = 0;
+        if (hp > 0)
+          s_sp[atomicAdd(&s_nsp, 1)] = 0;
+      }
+    } while (cur != n);
+  }
+  __syncthreads();
+  const int nsp = s_nsp;
+  if (t < nsp)
+    spine_list[base + t] = s_sp[t];
+  if (t == 0)
+    spine_count[blockIdx.x] = nsp;
+  // first arrivals at the block's own splits (the global climb completes
+  // their parents); the split after the block starts unclaimed
+  if (t < m - 1)
+    ranges[base + t] = s_range[t];
+  else if (t == m - 1 && base + t < nint)
+    ranges[base + t] = -1;
+  if (t < 2)
+    pending[2 * blockIdx.x + t] = s_pend[t];
+}
+
+// Continues the climb of the nodes k_climb_local handed over (one thread per
+// hand-off slot), with the flags in device memory.
+template <int G, bool ROPES>
+__global__ __launch_bounds__(256) void k_climb_global(
+    const int4 *__restrict__ pending, int npending, int n,
+    const uint64_t *__restrict__ keys, int32_t *ranges, INode *inodes, float4 *leafrec,
+    int32_t *__restrict__ c, int32_t *__restrict__ top, int32_t *__restrict__ spine_count,
+    int32_t *__restrict__ spine_list)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= npending)
+    return;
+  const int4 e = pending[j];
+  if (e.x < 0)
+    return;
+  const int nint = n - 1;
+  int cur = e.x, cur_h = e.y, range_left = e.z, range_right = e.w;
+  Box6 bv;
+  int unused_h;
+  read_sibling<G>(cur < n, cur < n ? cur : cur - n, leafrec, inodes, bv, unused_h);
   int64_t dl = apetrei_delta(keys, nint, range_left - 1);
   int64_t dr = apetrei_delta(keys, nint, range_right);
-  if (ROPES)
-    leaf_rope[i0] = apetrei_rope(keys, nint, n, range_right, dr);
-  int cur = i0;   // reference encoding of the current node
-  int cur_h = 0;  // left-spine height of the current node (0 for a leaf)
   do
   {
     const bool is_left = dr < dl;
@@ -344,8 +528,6 @@ __global__ __launch_bounds__(256) void k_climb(
       bv.hi[d] = kmax(bv.hi[d], sb.hi[d]);
     }
     const int kp = dr < dl ? range_right : range_left;
-    // spine height among stored nodes; 0 = not stored (at most FLAT_MAX
-    // leaves); stored nodes are the top part of every left spine
     const int hp =
         range_right - range_left + 1 > FLAT_MAX ? (left_child < n ? 0 : h_left) + 1 : 0;
     const int32_t rope = ROPES ? apetrei_rope(keys, nint, n, range_right, dr) : -1;
@@ -355,11 +537,18 @@ __global__ __launch_bounds__(256) void k_climb(
     st_coh128(w + 1, make_uint4(f_bits(bv.hi[0]), f_bits(bv.hi[1]), f_bits(bv.hi[2]),
                                 (uint32_t)rope));
     st_coh128(w + 2, make_uint4((uint32_t)range_left, (uint32_t)range_right, (uint32_t)hp, 0u));
+    // first leaves of stored left spines topped here (the right child's and,
+    // at the root, the root's), appended to the list of the local-climb
+    // block holding the leaf
     if (right_node >= n)
     {
-      // the right child tops a left spine starting at leaf ap + 1
       c[ap + 1] = h_right;
       top[ap + 1] = right_node - n;
+      if (h_right > 0)
+      {
+        const int cb = (ap + 1) / CLIMB_LEAVES;
+        spine_list[cb * CLIMB_LEAVES + atomicAdd(&spine_count[cb], 1)] = ap + 1;
+      }
     }
     cur = kp + n;
     cur_h = hp;
@@ -367,6 +556,8 @@ __global__ __launch_bounds__(256) void k_climb(
     {
       c[0] = hp;
       top[0] = 0;
+      if (hp > 0)
+        spine_list[atomicAdd(&spine_count[0], 1)] = 0;
     }
   } while (cur != n);
 }
@@ -398,13 +589,16 @@ __global__ __launch_bounds__(256) void k_lin_fp32(const INode *__restrict__ inod
   slots[pos + 1] = make_uint4(f_bits(r.hi[0]), f_bits(r.hi[1]), f_bits(r.hi[2]), 0u);
 }
 
-// Q16: one thread per left spine (rooted at the root or at a right child,
-// whose frame is the scene box); walking down the spine, each left child is
-// encoded relative to the decoded box of its parent.
+// Q16: one thread per stored left spine (rooted at the root or at a right
+// child, whose frame is the scene box), a wavefront per list of spines that
+// the climb made for CLIMB_LEAVES consecutive leaves; walking down the spine,
+// each left child is encoded relative to the decoded box of its parent.
 template <int G>
 __global__ __launch_bounds__(256) void k_lin_q16(const INode *__restrict__ inodes,
                                                  int n,
-                                                 const int32_t *__restrict__ c,
+                                                 const int32_t *__restrict__ spine_count,
+                                                 const int32_t *__restrict__ spine_list,
+                                                 int nlists,
                                                  const int32_t *__restrict__ top,
                                                  const int32_t *__restrict__ C,
                                                  const float *__restrict__ scene,
@@ -412,43 +606,59 @@ __global__ __launch_bounds__(256) void k_lin_q16(const INode *__restrict__ inode
                                                  uint32_t total, uint32_t *err)
 {
   constexpr uint32_t SL = leaf_slots<G>(), SI = 1;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n || c[j] == 0)
+  const int cb = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (cb >= nlists)
     return;
-  Box6 f;
-  for (int d = 0; d < 3; ++d)
-    f.lo[d] = scene[d], f.hi[d] = scene[3 + d];
-  int k = top[j];
-  while (true)
+  const int cnt = spine_count[cb];
+  if (cnt > CLIMB_LEAVES)
   {
-    const INode r = inodes[k];
-    if (r.h == 0) // the rest of the spine is not stored
-      break;
-    uint32_t q[6];
-    Box6 dec;
-    for (int d = 0; d < 3; ++d)
-    {
-      if (r.lo[d] < f.lo[d] || r.hi[d] > f.hi[d])
-        raise_error(err, KERR_FRAME);
-      q16_encode_axis(f.lo[d], f.hi[d], r.lo[d], r.hi[d], q[d], q[3 + d]);
-      const float s = q16_scale(f.lo[d], f.hi[d]);
-      dec.lo[d] = q16_lo(f.lo[d], s, q[d]);
-      dec.hi[d] = q16_lo(f.lo[d], s, q[3 + d]);
-    }
-    const uint32_t pos = SI * (uint32_t)(C[r.first] - r.h) + SL * (uint32_t)r.first;
-    const uint32_t len = (uint32_t)(r.last - r.first + 1);
-    const uint32_t skip = SL * len + SI * (uint32_t)(C[r.last] - C[r.first] + r.h);
-    if (pos + SI > total || pos + skip > total)
+    raise_error(err, KERR_LAYOUT);
+    return;
+  }
+  for (int i = threadIdx.x & 63; i < cnt; i += 64)
+  {
+    const int j = spine_list[cb * CLIMB_LEAVES + i];
+    if (j < 0 || j >= n)
     {
       raise_error(err, KERR_LAYOUT);
       return;
     }
-    slots[pos] = make_uint4(q[0] | (q[1] << 16), q[2] | (q[3] << 16),
-                            q[4] | (q[5] << 16), skip);
-    if (r.left_child < n)
-      break;
-    k = r.left_child - n;
-    f = dec;
+    Box6 f;
+    for (int d = 0; d < 3; ++d)
+      f.lo[d] = scene[d], f.hi[d] = scene[3 + d];
+    const uint32_t cj = (uint32_t)C[j]; // every node of the spine has first leaf j
+    int k = top[j];
+    while (true)
+    {
+      const INode r = inodes[k];
+      if (r.h == 0) // the rest of the spine is not stored
+        break;
+      uint32_t q[6];
+      Box6 dec;
+      for (int d = 0; d < 3; ++d)
+      {
+        if (r.lo[d] < f.lo[d] || r.hi[d] > f.hi[d])
+          raise_error(err, KERR_FRAME);
+        q16_encode_axis(f.lo[d], f.hi[d], r.lo[d], r.hi[d], q[d], q[3 + d]);
+        const float s = q16_scale(f.lo[d], f.hi[d]);
+        dec.lo[d] = q16_lo(f.lo[d], s, q[d]);
+        dec.hi[d] = q16_lo(f.lo[d], s, q[3 + d]);
+      }
+      const uint32_t pos = SI * (cj - (uint32_t)r.h) + SL * (uint32_t)j;
+      const uint32_t len = (uint32_t)(r.last - j + 1);
+      const uint32_t skip = SL * len + SI * ((uint32_t)C[r.last] - cj + (uint32_t)r.h);
+      if (r.first != j || pos + SI > total || pos + skip > total)
+      {
+        raise_error(err, KERR_LAYOUT);
+        return;
+      }
+      slots[pos] = make_uint4(q[0] | (q[1] << 16), q[2] | (q[3] << 16),
+                              q[4] | (q[5] << 16), skip);
+      if (r.left_child < n)
+        break;
+      k = r.left_child - n;
+      f = dec;
+    }
   }
 }
 
@@ -519,6 +729,16 @@ __global__ void k_export_topology(const INode *inodes, int nint,
   if (boxes)
     for (int d = 0; d < 3; ++d)
       boxes[6 * k + d] = r.lo[d], boxes[6 * k + 3 + d] = r.hi[d];
+}
+
+__global__ void k_build_status(const float *scene, const int32_t *C, int64_t n,
+                               uint32_t *status)
+{
+  const int t = threadIdx.x;
+  if (t == 0)
+    status[1] = n > 1 ? (uint32_t)C[n - 1] : 0u;
+  else if (t < 7)
+    status[1 + t] = f_bits(scene[t - 1]);
 }
 
 inline unsigned grid_for(int64_t n, int block = 256)
@@ -618,10 +838,15 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
   float4 *leafrec = (float4 *)dalloc((size_t)n * leafrec_f4 * sizeof(float4), stream);
   int32_t *c = (int32_t *)dalloc(n * sizeof(int32_t), stream);
   int32_t *top = (int32_t *)dalloc(n * sizeof(int32_t), stream);
+  const int64_t nlists = div_up(n, CLIMB_LEAVES);
+  int4 *pending = (int4 *)dalloc(2 * nlists * sizeof(int4), stream);
+  int32_t *spine_count = (int32_t *)dalloc(nlists * sizeof(int32_t), stream);
+  int32_t *spine_list = (int32_t *)dalloc(nlists * CLIMB_LEAVES * sizeof(int32_t), stream);
   out.C = (int32_t *)dalloc(n * sizeof(int32_t), stream);
   int32_t *leaf_rope = keep_topology ? (int32_t *)dalloc(n * sizeof(int32_t), stream) : nullptr;
   if (!keys0 || !keys1 || !vals0 || !vals1 || !hist || !partials || !ws || !ranges ||
-      !inodes || !leafrec || !c || !top || !out.C || (keep_topology && !leaf_rope))
+      !inodes || !leafrec || !c || !top || !pending || !spine_count ||
+      !spine_list || !out.C || (keep_topology && !leaf_rope))
     return hipErrorOutOfMemory;
 
   // ---- scene box
@@ -654,31 +879,37 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
     return e;
 
   // ---- hierarchy
-  e = fill_u32((uint32_t *)ranges, 0xffffffffu, nint, stream);
-  if (e == hipSuccess)
-    e = fill_u32((uint32_t *)c, 0u, n, stream);
+  e = fill_u32((uint32_t *)c, 0u, n, stream);
   if (e != hipSuccess)
     return e;
   {
-    const unsigned g = grid_for(n);
+    const unsigned g = (unsigned)div_up(n, CLIMB_LEAVES);
+    const int npend = 2 * (int)g;
+    const unsigned gp = grid_for(npend);
+#define AXB_CLIMB(GEO, ROPE)                                                                  \
+  do                                                                                          \
+  {                                                                                           \
+    AXB_LAUNCH((k_climb_local<GEO, ROPE>), dim3(g), dim3(CLIMB_LEAVES), 0, stream, values,    \
+               ni, keys0, vals0, ranges, inodes, leafrec, leaf_rope, c, top, pending,         \
+               spine_count, spine_list);                                                      \
+    AXB_LAUNCH((k_climb_global<GEO, ROPE>), dim3(gp), dim3(256), 0, stream, pending, npend,   \
+               ni, keys0, ranges, inodes, leafrec, c, top, spine_count, spine_list);          \
+  } while (0)
     if (geometry == GEOM_POINTS)
     {
       if (keep_topology)
-        AXB_LAUNCH((k_climb<GEOM_POINTS, true>), dim3(g), dim3(256), 0, stream,
-                           values, ni, keys0, vals0, ranges, inodes, leafrec, leaf_rope, c, top);
+        AXB_CLIMB(GEOM_POINTS, true);
       else
-        AXB_LAUNCH((k_climb<GEOM_POINTS, false>), dim3(g), dim3(256), 0, stream,
-                           values, ni, keys0, vals0, ranges, inodes, leafrec, leaf_rope, c, top);
+        AXB_CLIMB(GEOM_POINTS, false);
     }
     else
     {
       if (keep_topology)
-        AXB_LAUNCH((k_climb<GEOM_BOXES, true>), dim3(g), dim3(256), 0, stream, values,
-                           ni, keys0, vals0, ranges, inodes, leafrec, leaf_rope, c, top);
+        AXB_CLIMB(GEOM_BOXES, true);
       else
-        AXB_LAUNCH((k_climb<GEOM_BOXES, false>), dim3(g), dim3(256), 0, stream,
-                           values, ni, keys0, vals0, ranges, inodes, leafrec, leaf_rope, c, top);
+        AXB_CLIMB(GEOM_BOXES, false);
     }
+#undef AXB_CLIMB
   }
   e = inclusive_scan_i32(c, out.C, n, ws, err, stream);
   if (e != hipSuccess)
@@ -708,15 +939,17 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
     {
       if (geometry == GEOM_POINTS)
       {
-        AXB_LAUNCH(k_lin_q16<GEOM_POINTS>, dim3(gl), dim3(256), 0, stream, inodes, ni,
-                           c, top, out.C, out.scene, out.slots, (uint32_t)out.max_slots, err);
+        AXB_LAUNCH(k_lin_q16<GEOM_POINTS>, dim3(grid_for(nlists, 4)), dim3(256), 0, stream,
+                   inodes, ni, spine_count, spine_list, (int)nlists, top, out.C, out.scene,
+                   out.slots, (uint32_t)out.max_slots, err);
         AXB_LAUNCH((k_lin_leaves<GEOM_POINTS, 1>), dim3(gl), dim3(256), 0, stream,
                            leafrec, ni, out.C, out.slots, (uint32_t)out.max_slots, err);
       }
       else
       {
-        AXB_LAUNCH(k_lin_q16<GEOM_BOXES>, dim3(gl), dim3(256), 0, stream, inodes, ni,
-                           c, top, out.C, out.scene, out.slots, (uint32_t)out.max_slots, err);
+        AXB_LAUNCH(k_lin_q16<GEOM_BOXES>, dim3(grid_for(nlists, 4)), dim3(256), 0, stream,
+                   inodes, ni, spine_count, spine_list, (int)nlists, top, out.C, out.scene,
+                   out.slots, (uint32_t)out.max_slots, err);
         AXB_LAUNCH((k_lin_leaves<GEOM_BOXES, 1>), dim3(gl), dim3(256), 0, stream,
                            leafrec, ni, out.C, out.slots, (uint32_t)out.max_slots, err);
       }
@@ -747,6 +980,9 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
   dfree(leafrec, stream);
   dfree(c, stream);
   dfree(top, stream);
+  dfree(pending, stream);
+  dfree(spine_count, stream);
+  dfree(spine_list, stream);
   if (keep_topology)
   {
     out.keys = keys0;
@@ -761,6 +997,12 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
     dfree(inodes, stream);
   }
   return e;
+}
+
+hipError_t build_status(const BuildOut &b, int64_t n, uint32_t *status, hipStream_t stream)
+{
+  AXB_LAUNCH(k_build_status, dim3(1), dim3(64), 0, stream, b.scene, b.C, n, status);
+  return hipGetLastError();
 }
 
 hipError_t export_topology(const INode *inodes, int64_t nint, int32_t *left_child,

@@ -295,15 +295,10 @@ static axb_status hip_fail(hipError_t e, const char *where)
 // Reads and clears a kernel error word (the stream must be synchronised).
 // `tolerated` (optional): a traversal-stack overflow alone is reported there
 // instead of as an error, for callers that redo the work another way.
-static axb_status check_kernel_error(uint32_t *d_err, hipStream_t s, const char *where,
-                                     uint32_t *tolerated = nullptr)
+// kernel_error_word: the same for an error word already copied to the host.
+static axb_status kernel_error_word(uint32_t h, uint32_t *d_err, hipStream_t s,
+                                    const char *where, uint32_t *tolerated = nullptr)
 {
-  uint32_t h = 0;
-  hipError_t e = hipMemcpyAsync(&h, d_err, sizeof(h), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess)
-    e = sync_stream(s);
-  if (e != hipSuccess)
-    return hip_fail(e, where);
   if (h && tolerated && h == KERR_STACK)
   {
     (void)fill_u32(d_err, 0u, 1, s);
@@ -326,6 +321,18 @@ static axb_status check_kernel_error(uint32_t *d_err, hipStream_t s, const char 
     return AXB_ERR_INTERNAL;
   }
   return AXB_OK;
+}
+
+static axb_status check_kernel_error(uint32_t *d_err, hipStream_t s, const char *where,
+                                     uint32_t *tolerated = nullptr)
+{
+  uint32_t h = 0;
+  hipError_t e = hipMemcpyAsync(&h, d_err, sizeof(h), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = sync_stream(s);
+  if (e != hipSuccess)
+    return hip_fail(e, where);
+  return kernel_error_word(h, d_err, s, where, tolerated);
 }
 
 // The kernel error word of one call: calls running at the same time on
@@ -459,7 +466,8 @@ axb_status axb_bvh_create(const float *values, int64_t n, int32_t geometry, int3
   t->node_format = node_format;
   t->keep_topology = (flags & AXB_BUILD_KEEP_TOPOLOGY) != 0;
   t->stream = s;
-  t->d_err = (uint32_t *)dalloc(sizeof(uint32_t), s);
+  // error word, then the build's summary (stored internal nodes, scene box)
+  t->d_err = (uint32_t *)dalloc(BUILD_STATUS_WORDS * sizeof(uint32_t), s);
   if (!t->d_err)
   {
     delete t;
@@ -472,13 +480,16 @@ axb_status axb_bvh_create(const float *values, int64_t n, int32_t geometry, int3
     AXB_FAIL(AXB_ERR_HIP, "axb_bvh_create: kernel launch failed");
   }
   hipError_t e = build_tree(values, n, geometry, node_format, t->keep_topology, t->d_err, s, t->b);
-  int32_t c_last = 0; // stored internal nodes (C[n - 1])
+  // one copy to the host: error word, stored internal nodes (C[n - 1]), scene
+  uint32_t status[BUILD_STATUS_WORDS] = {};
   if (e == hipSuccess)
-    e = hipMemcpyAsync(t->bounds, t->b.scene, sizeof(t->bounds), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && n > 0)
-    e = hipMemcpyAsync(&c_last, t->b.C + (n - 1), sizeof(c_last), hipMemcpyDeviceToHost, s);
+    e = build_status(t->b, n, t->d_err, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(status, t->d_err, sizeof(status), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess)
     e = sync_stream(s);
+  const int32_t c_last = (int32_t)status[1];
+  std::memcpy(t->bounds, status + 2, sizeof(t->bounds));
   if (e == hipSuccess && n > 0)
   {
     const int64_t sl = geometry == AXB_GEOMETRY_POINTS ? 1 : 2;
@@ -495,7 +506,7 @@ axb_status axb_bvh_create(const float *values, int64_t n, int32_t geometry, int3
     axb_bvh_destroy(t);
     return st;
   }
-  axb_status st = check_kernel_error(t->d_err, s, "axb_bvh_create");
+  axb_status st = kernel_error_word(status[0], t->d_err, s, "axb_bvh_create");
   if (st != AXB_OK)
   {
     axb_bvh_destroy(t);

@@ -266,6 +266,10 @@ hipError_t scene_box(const float *values, int64_t n, int geometry, float *scene,
                      hipStream_t stream);
 hipError_t build_tree(const float *values, int64_t n, int geometry, int node_format,
                       bool keep_topology, uint32_t *err, hipStream_t stream, BuildOut &out);
+// Words of a build's status block: {error word, C[n - 1] (0 when n < 2),
+// scene box (6 floats)}; build_status fills words 1..7 after build_tree.
+constexpr int BUILD_STATUS_WORDS = 8;
+hipError_t build_status(const BuildOut &b, int64_t n, uint32_t *status, hipStream_t stream);
 hipError_t export_topology(const INode *inodes, int64_t nint, int32_t *left_child, int32_t *rope,
                            float *boxes, hipStream_t stream);
 hipError_t launch_spatial(const TreeView &tv, int node_format, int geometry, int pred_kind,

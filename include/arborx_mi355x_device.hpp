@@ -266,9 +266,17 @@ AXB_HD uint32_t q16_lo_index(float flo, float s, float rs, float v)
   const float est = (v - flo) * rs;
   const int q = est >= 0.f ? (est <= 65535.f ? (int)est : 65535) : 0; // NaN -> 0
   int a = q - 2 < 0 ? 0 : q - 2, b = q + 2 > 65535 ? 65535 : q + 2;
-  if (!((a == 0 || q16_lo(flo, s, (uint32_t)a) <= v) &&
-        (b == 65535 || q16_lo(flo, s, (uint32_t)(b + 1)) > v)))
-    a = 0, b = 65535;
+  if ((a == 0 || q16_lo(flo, s, (uint32_t)a) <= v) &&
+      (b == 65535 || q16_lo(flo, s, (uint32_t)(b + 1)) > v))
+  {
+    // the answer is in [a, b] and {j : v(j) <= v} is a prefix of it
+    int cnt = 0;
+#pragma unroll
+    for (int j = 1; j < 5; ++j)
+      cnt += (a + j <= b && q16_lo(flo, s, (uint32_t)(a + j)) <= v);
+    return (uint32_t)(a + cnt);
+  }
+  a = 0, b = 65535;
   while (a < b)
   {
     const int m = (a + b + 1) >> 1;
@@ -287,9 +295,17 @@ AXB_HD uint32_t q16_hi_index(float flo, float s, float rs, float v)
   const float est = (v - flo) * rs;
   const int q = est >= 0.f ? (est <= 65535.f ? (int)est : 65535) : 0; // NaN -> 0
   int a = q - 2 < 0 ? 0 : q - 2, b = q + 2 > 65535 ? 65535 : q + 2;
-  if (!((b == 65535 || q16_lo(flo, s, (uint32_t)b) >= v) &&
-        (a == 0 || q16_lo(flo, s, (uint32_t)(a - 1)) < v)))
-    a = 0, b = 65535;
+  if ((b == 65535 || q16_lo(flo, s, (uint32_t)b) >= v) &&
+      (a == 0 || q16_lo(flo, s, (uint32_t)(a - 1)) < v))
+  {
+    // the answer is in [a, b] and {j : v(j) >= v} is a suffix of it
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      cnt += (a + j < b && q16_lo(flo, s, (uint32_t)(a + j)) < v);
+    return (uint32_t)(a + cnt);
+  }
+  a = 0, b = 65535;
   while (a < b)
   {
     const int m = (a + b) >> 1;
