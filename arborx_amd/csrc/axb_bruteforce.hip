@@ -55,9 +55,7 @@ __global__ __launch_bounds__(256) void k_bf_spatial(const float *__restrict__ va
     pr = sphere_pred(p[0], p[1], p[2], PK == PRED_SPHERES ? p[3] : radius);
   }
   RowWriter<MODE> rw;
-  rw.out = out;
-  rw.base = MODE == MODE_FILL ? offsets[q] : (MODE == MODE_BUFFER ? q * (int64_t)buffer : 0);
-  rw.limit = MODE == MODE_FILL ? offsets[q + 1] - rw.base : (int64_t)buffer;
+  rw.init(out, offsets, q, buffer);
   for (int64_t j = 0; j < n; ++j)
   {
     uint4 s0, s1;

@@ -198,44 +198,49 @@ typedef int32_t i32x4_u __attribute__((ext_vector_type(4), aligned(4)));
 template <int MODE>
 struct RowWriter
 {
-  int32_t *out;
-  int64_t base, limit; // FILL: row start and length; BUFFER: limit = buffer
-  int64_t cnt = 0;
-  int32_t h0 = 0, h1 = 0, h2 = 0;
+  int32_t *row;   // the row's first entry (unused when counting)
+  uint32_t limit; // entries the row holds
+  uint32_t cnt = 0;
 
+  // Row q: FILL writes the final row at offsets[q], BUFFER a scratch row of
+  // `buffer` entries, COUNT nothing.
+  AXB_D void init(int32_t *out, const int64_t *offsets, int64_t q, int32_t buffer)
+  {
+    const int64_t base =
+        MODE == MODE_FILL ? offsets[q] : (MODE == MODE_BUFFER ? q * (int64_t)buffer : 0);
+    row = out + base;
+    limit = MODE == MODE_FILL ? (uint32_t)(offsets[q + 1] - base) : (uint32_t)buffer;
+  }
+  // Hits are written in groups of four (one 16-byte store): the last three
+  // wait in h0..h2, shifted along at every hit (the walk calls this in
+  // nearly every step of a wavefront, some lane having a hit, so a hit
+  // costs three moves and a compare).
+  int32_t h0 = 0, h1 = 0, h2 = 0;
   AXB_D bool hit(int32_t v)
   {
-    if (MODE != MODE_COUNT && cnt < limit)
+    if (MODE != MODE_COUNT)
     {
-      const int j = (int)(cnt & 3);
-      if (j == 3)
-        *(i32x4_u *)(out + base + cnt - 3) = (i32x4_u){h0, h1, h2, v};
-      else
-      {
-        h0 = j == 0 ? v : h0;
-        h1 = j == 1 ? v : h1;
-        h2 = j == 2 ? v : h2;
-      }
+      if ((cnt & 3u) == 3u && cnt < limit)
+        *(i32x4_u *)(row + cnt - 3) = (i32x4_u){h0, h1, h2, v};
+      h0 = h1, h1 = h2, h2 = v;
     }
     ++cnt;
     return true;
   }
-  // Writes the last partial group of the kept hits.
+  // Writes the last partial group of the hits (nothing for a row that
+  // overflowed: its count sends it to be walked again).
   AXB_D void finish()
   {
-    if (MODE == MODE_COUNT)
+    if (MODE == MODE_COUNT || cnt > limit)
       return;
-    const int64_t kept = cnt < limit ? cnt : limit;
-    const int rem = (int)(kept & 3);
-    if (rem > 0)
-    {
-      const int64_t at = base + kept - rem;
-      out[at] = h0;
-      if (rem > 1)
-        out[at + 1] = h1;
-      if (rem > 2)
-        out[at + 2] = h2;
-    }
+    const uint32_t rem = cnt & 3u;
+    int32_t *at = row + (cnt - rem);
+    if (rem == 3)
+      at[0] = h0, at[1] = h1, at[2] = h2;
+    else if (rem == 2)
+      at[0] = h1, at[1] = h2;
+    else if (rem == 1)
+      at[0] = h2;
   }
 };
 

@@ -62,9 +62,7 @@ __global__ __launch_bounds__(256) void k_spatial(
     pr = sphere_pred(p[0], p[1], p[2], PK == PRED_SPHERES ? p[3] : radius);
   }
   RowWriter<MODE> rw;
-  rw.out = out;
-  rw.base = MODE == MODE_FILL ? offsets[q] : (MODE == MODE_BUFFER ? q * (int64_t)buffer : 0);
-  rw.limit = MODE == MODE_FILL ? offsets[q + 1] - rw.base : (int64_t)buffer;
+  rw.init(out, offsets, q, buffer);
   uint32_t v_int = 0, v_leaf = 0;
   rope_walk<F, G, PK, SLOTS>(tv, pr, 0u, tv.total, ~0u, rw, v_int, v_leaf);
   row_epilogue(tv, rw, offsets, q, COUNT ? v_int : 0u, COUNT ? v_leaf : 0u);
@@ -186,9 +184,7 @@ __global__ __launch_bounds__(256) void k_ordered(TreeView tv, const float *__res
   const int64_t q = order ? (int64_t)order[t] : t;
   const RayPred ray = ray_pred(rays + 6 * q);
   RowWriter<MODE> rw;
-  rw.out = out;
-  rw.base = MODE == MODE_FILL ? offsets[q] : (MODE == MODE_BUFFER ? q * (int64_t)buffer : 0);
-  rw.limit = MODE == MODE_FILL ? offsets[q + 1] - rw.base : (int64_t)buffer;
+  rw.init(out, offsets, q, buffer);
   LimitedRow<MODE> sink{&rw, (int64_t)limit};
   uint32_t v_int = 0, v_leaf = 0;
   if (limit > 0)
@@ -255,9 +251,7 @@ __global__ __launch_bounds__(256) void k_nlist(TreeView tv, float radius, int64_
     rowof[L] = (uint32_t)q;
   const SpatialPred pr = sphere_pred(bits_f(s.x), bits_f(s.y), bits_f(s.z), radius);
   RowWriter<MODE> rw;
-  rw.out = out;
-  rw.base = MODE == MODE_FILL ? offsets[q] : (MODE == MODE_BUFFER ? q * (int64_t)buffer : 0);
-  rw.limit = MODE == MODE_FILL ? offsets[q + 1] - rw.base : (int64_t)buffer;
+  rw.init(out, offsets, q, buffer);
   uint32_t v_int = 0, v_leaf = 0;
   rope_walk<F, GEOM_POINTS, PRED_SPHERES>(tv, pr, 0u, FULL ? tv.total : self, self, rw, v_int,
                                           v_leaf);
@@ -741,6 +735,7 @@ struct CandStage
   uint2 *st;
   int lane;
   AXB_D void put(int c, uint2 e) { st[(c & (KNN_STAGE - 1)) * 256 + lane] = e; }
+  AXB_D uint2 get(int c) const { return st[(c & (KNN_STAGE - 1)) * 256 + lane]; }
   // the chunk holding entries [c0, c0 + 8) of the row, c0 a multiple of 8
   AXB_D void flush(uint2 *__restrict__ cand, uint32_t row_at, int c0) const
   {
@@ -784,16 +779,17 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
     int c = 0;
     uint32_t p = 0;
     const uint32_t end = tv.total;
+    uint32_t step = 1; // 0 after a zero rope: a corrupt node array
     WalkFrame fr;
     walk_frame_init(tv, fr);
     while (true)
     {
       // the walk proper, until its end or a leaf within R finds the row
-      // full; the rare shrink below stays out of this loop, so that the
-      // loop carries neither R nor its bounds
-      bool full = false;
-      uint2 held = make_uint2(0u, 0u); // the leaf that found the row full
-      while (p < end)
+      // full: c reaches cap + 1 and that leaf waits in stage entry cap (or a
+      // zero rope ends it).  The loop carries nothing but p, c, the step and
+      // the frame; the rare shrink below stays out of it, so that it carries
+      // neither R nor its bounds.
+      while (step != 0 && p < end && c <= cap)
       {
         const uint4 s0 = tv.slots[p];
         const uint4 s1 = F == NODES_FP32 ? tv.slots[p + 1] : s0;
@@ -808,32 +804,25 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
         const bool pass = d2 <= thr;
         v_leaf += leaf;
         v_int += !leaf;
-        if (!leaf && !pass && w == 0)
-        {
-          raise_error(tv.err, KERR_LAYOUT);
-          return c;
-        }
         walk_frame_next<F>(tv, !leaf && pass, bx, fr);
-        p += leaf ? 1u : (pass ? SI : (uint32_t)w);
+        step = leaf ? 1u : (pass ? SI : (uint32_t)w);
+        p += step;
         // A candidate goes to the stage; every lane writes its next free
         // stage entry each step (without a candidate that entry stays
         // free), which keeps the step free of a branch the wavefront would
         // take at almost every step (some lane has a candidate).
         const bool hit = leaf && pass;
-        const uint2 e = make_uint2((uint32_t)~w, f_bits(d2));
-        if (hit && c == cap)
-        {
-          full = true;
-          held = e;
-          break;
-        }
-        stage.put(c, e);
+        stage.put(c, make_uint2((uint32_t)~w, f_bits(d2)));
         c += hit;
         if (hit && (c & (KNN_STAGE - 1)) == 0)
           stage.flush(cand, row_at, c - KNN_STAGE);
       }
-      if (!full)
+      if (step == 0 || c <= cap)
         break;
+      // the row's cap entries are all written out (cap is a multiple of the
+      // stage's size); the leaf that found it full is in the stage
+      const uint2 held = stage.get(cap);
+      c = cap;
       for (int t = 0; t < 3 && c == cap && R * shrink > R_lo; ++t)
       {
         R_hi = R;
@@ -858,6 +847,11 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
         if ((c & (KNN_STAGE - 1)) == 0)
           stage.flush(cand, row_at, c - KNN_STAGE);
       }
+    }
+    if (step == 0)
+    {
+      raise_error(tv.err, KERR_LAYOUT);
+      return c;
     }
     if (c & (KNN_STAGE - 1)) // the last, partial chunk (the rest of it unused)
       stage.flush(cand, row_at, c & ~(KNN_STAGE - 1));
