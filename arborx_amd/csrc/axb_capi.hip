@@ -579,9 +579,14 @@ axb_status axb_bvh_export_topology(const axb_bvh *t, uint64_t *keys, uint32_t *p
 }
 
 // Morton-32 processing order of predicates (CrsGraphWrapperImpl.hpp:407-419).
+// exact: the permutation that sorts the centroids by their 30-bit Morton
+// codes (ties by position); otherwise by the top 24 bits of the codes (8 per
+// axis, cells of 1/256 of the scene per axis), one radix pass fewer -- the
+// processing order of a query batch only decides which predicates share a
+// wavefront, never a result.
 static axb_status morton_order(const float *scene, uint32_t *d_err, int pred_kind,
                                const float *preds, int64_t nq, hipStream_t s,
-                               uint32_t **order_out, void **to_free)
+                               uint32_t **order_out, void **to_free, bool exact = false)
 {
   *order_out = nullptr;
   if (nq < 2)
@@ -589,6 +594,7 @@ static axb_status morton_order(const float *scene, uint32_t *d_err, int pred_kin
   // the sort's look-back words carry 30-bit counts
   if (nq >= MAX_SORTED_PREDICATES)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: sort_predicates needs fewer than 2^30 predicates");
+  const int passes = exact ? 4 : 3;
   const size_t ws_bytes = sort_workspace_bytes(nq, 4, 4);
   // one block: keys0 keys1 vals0 vals1 hist workspace
   const size_t a = (size_t)nq * 4;
@@ -602,24 +608,25 @@ static axb_status morton_order(const float *scene, uint32_t *d_err, int pred_kin
   uint32_t *hist = (uint32_t *)(blk + off_h);
   hipError_t e = fill_u32(hist, 0u, 4 * 256, s);
   if (e == hipSuccess)
-    e = launch_pred_morton32(pred_kind, preds, nq, scene, k0, hist, s);
+    e = launch_pred_morton32(pred_kind, preds, nq, scene, exact ? 0 : 6, k0, hist, s);
   if (e == hipSuccess)
-    e = radix_sort_pairs_u32(k0, k1, v0, v1, nq, 4, hist, blk + off_ws, d_err, s);
+    e = radix_sort_pairs_u32(k0, k1, v0, v1, nq, passes, hist, blk + off_ws, d_err, s);
   if (e != hipSuccess)
   {
     dfree(blk, s);
     return hip_fail(e, "query: predicate sort");
   }
-  *order_out = v0;
+  // each pass moves the pairs to the other buffer
+  *order_out = passes % 2 == 0 ? v0 : v1;
   *to_free = blk;
   return AXB_OK;
 }
 
 static axb_status predicate_order(const axb_bvh *t, uint32_t *err, int pred_kind,
                                   const float *preds, int64_t nq, hipStream_t s,
-                                  uint32_t **order_out, void **to_free)
+                                  uint32_t **order_out, void **to_free, bool exact = false)
 {
-  return morton_order(t->b.scene, err, pred_kind, preds, nq, s, order_out, to_free);
+  return morton_order(t->b.scene, err, pred_kind, preds, nq, s, order_out, to_free, exact);
 }
 
 } // extern "C"
@@ -1105,7 +1112,7 @@ axb_status axb_bvh_predicate_order(const axb_bvh *t, int32_t predicate_kind,
     AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_bvh_predicate_order: out of device memory");
   uint32_t *ord = nullptr;
   void *blk = nullptr;
-  axb_status st = predicate_order(t, err.p, predicate_kind, preds, nq, s, &ord, &blk);
+  axb_status st = predicate_order(t, err.p, predicate_kind, preds, nq, s, &ord, &blk, true);
   if (st != AXB_OK)
     return st;
   hipError_t e = hipMemcpyAsync(order, ord, (size_t)nq * sizeof(uint32_t),

@@ -325,8 +325,10 @@ struct KnnScratch
 };
 int knn_row_capacity(int k, int n);
 hipError_t launch_knn_offsets(int64_t *offsets, int64_t nq, int64_t per, hipStream_t s);
+// keys = the predicates' 30-bit Morton codes >> shift, and the histograms of
+// their four 8-bit digits
 hipError_t launch_pred_morton32(int pred_kind, const float *preds, int64_t nq,
-                                const float *scene, uint32_t *keys, uint32_t *hist,
+                                const float *scene, int shift, uint32_t *keys, uint32_t *hist,
                                 hipStream_t s);
 hipError_t launch_compact(const int32_t *buf, int64_t nq, int32_t buffer,
                           const int64_t *offsets, int32_t *out, hipStream_t s);

@@ -1084,7 +1084,7 @@ __global__ void k_knn_offsets(int64_t *offsets, int64_t nq, int64_t per)
 // digit histograms of the 4 radix passes.
 template <int PK>
 __global__ __launch_bounds__(256) void k_pred_morton32(
-    const float *__restrict__ preds, int64_t nq, const float *__restrict__ scene,
+    const float *__restrict__ preds, int64_t nq, const float *__restrict__ scene, int shift,
     uint32_t *__restrict__ keys, uint32_t *__restrict__ hist)
 {
   __shared__ uint32_t s_h[4 * 256];
@@ -1112,8 +1112,10 @@ __global__ __launch_bounds__(256) void k_pred_morton32(
       const float *v = preds + (PK == PRED_SPHERES ? 4 : 3) * i;
       cx = v[0], cy = v[1], cz = v[2];
     }
-    uint32_t k = morton32(translate_and_scale(cx, a0, b0), translate_and_scale(cy, a1, b1),
-                          translate_and_scale(cz, a2, b2));
+    const uint32_t k = morton32(translate_and_scale(cx, a0, b0),
+                                translate_and_scale(cy, a1, b1),
+                                translate_and_scale(cz, a2, b2)) >>
+                       shift;
     keys[i] = k;
 #pragma unroll
     for (int p = 0; p < 4; ++p)
@@ -1583,22 +1585,22 @@ hipError_t launch_knn_offsets(int64_t *offsets, int64_t nq, int64_t per, hipStre
 }
 
 hipError_t launch_pred_morton32(int pred_kind, const float *preds, int64_t nq,
-                                const float *scene, uint32_t *keys, uint32_t *hist,
+                                const float *scene, int shift, uint32_t *keys, uint32_t *hist,
                                 hipStream_t s)
 {
   const unsigned g = (unsigned)std::min<int64_t>(2048, div_up(nq, 256));
   if (pred_kind == PRED_RAYS)
-    AXB_LAUNCH(k_pred_morton32<PRED_RAYS>, dim3(g), dim3(256), 0, s, preds, nq, scene, keys,
+    AXB_LAUNCH(k_pred_morton32<PRED_RAYS>, dim3(g), dim3(256), 0, s, preds, nq, scene, shift, keys,
                hist);
   else if (pred_kind == PRED_SPHERES)
     AXB_LAUNCH(k_pred_morton32<PRED_SPHERES>, dim3(g), dim3(256), 0, s, preds, nq,
-                       scene, keys, hist);
+                       scene, shift, keys, hist);
   else if (pred_kind == PRED_BOXES)
     AXB_LAUNCH(k_pred_morton32<PRED_BOXES>, dim3(g), dim3(256), 0, s, preds, nq, scene,
-                       keys, hist);
+               shift, keys, hist);
   else
     AXB_LAUNCH(k_pred_morton32<PRED_RADIUS>, dim3(g), dim3(256), 0, s, preds, nq,
-                       scene, keys, hist);
+                       scene, shift, keys, hist);
   return hipGetLastError();
 }
 
