@@ -627,12 +627,14 @@ __global__ __launch_bounds__(256) void k_lin_q16(const INode *__restrict__ inode
     for (int d = 0; d < 3; ++d)
       f.lo[d] = scene[d], f.hi[d] = scene[3 + d];
     const uint32_t cj = (uint32_t)C[j]; // every node of the spine has first leaf j
-    int k = top[j];
-    while (true)
+    INode r = inodes[top[j]];
+    while (r.h != 0) // a node with h = 0 and the rest of the spine are not stored
     {
-      const INode r = inodes[k];
-      if (r.h == 0) // the rest of the spine is not stored
-        break;
+      // the left child's record is requested before this node is encoded
+      const bool more = r.left_child >= n;
+      INode rn;
+      if (more)
+        rn = inodes[r.left_child - n];
       uint32_t q[6];
       Box6 dec;
       for (int d = 0; d < 3; ++d)
@@ -654,9 +656,9 @@ __global__ __launch_bounds__(256) void k_lin_q16(const INode *__restrict__ inode
       }
       slots[pos] = make_uint4(q[0] | (q[1] << 16), q[2] | (q[3] << 16),
                               q[4] | (q[5] << 16), skip);
-      if (r.left_child < n)
+      if (!more)
         break;
-      k = r.left_child - n;
+      r = rn;
       f = dec;
     }
   }
