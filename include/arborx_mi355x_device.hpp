@@ -594,40 +594,6 @@ AXB_HD void walk_frame_next(const TV &, bool desc, const Box6 &b, WalkFrame &fr)
   }
 }
 
-// One record of the point walk at slot p, leaf and internal node through
-// the same code: the closest point of a leaf is the point, that of a node
-// box the clamp of the centre.  Advances p; returns false when the walk
-// stops here (the sink asked to, or a zero rope, reported as a layout
-// error).  A leaf at slot `skip` is not reported.
-template <int F, class TV, class Sink>
-AXB_HD bool point_step(const TV &tv, const SpatialPred &pr, uint32_t &p, uint32_t skip,
-                       WalkFrame &fr, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
-{
-  constexpr uint32_t SI = internal_slots<F>();
-  const uint4 s0 = tv.slots[p];
-  const uint4 s1 = F == NODES_FP32 ? tv.slots[p + 1] : s0;
-  const int32_t w = (int32_t)s0.w;
-  const bool leaf = w < 0;
-  Box6 bx; // meaningless for a leaf record, and unused then
-  walk_node_box<F>(s0, s1, fr, bx);
-  const float cx = leaf ? bits_f(s0.x) : clamp_box(pr.cx, bx.lo[0], bx.hi[0]);
-  const float cy = leaf ? bits_f(s0.y) : clamp_box(pr.cy, bx.lo[1], bx.hi[1]);
-  const float cz = leaf ? bits_f(s0.z) : clamp_box(pr.cz, bx.lo[2], bx.hi[2]);
-  const bool pass = dist2_pp(pr.cx, pr.cy, pr.cz, cx, cy, cz) <= pr.thr;
-  v_leaf += leaf;
-  v_int += !leaf;
-  if (leaf && pass && p != skip && !sink.hit(~w))
-    return false;
-  if (!leaf && !pass && w == 0)
-  {
-    raise_error(tv.err, KERR_LAYOUT);
-    return false;
-  }
-  walk_frame_next<F>(tv, !leaf && pass, bx, fr);
-  p += leaf ? 1u : (pass ? SI : (uint32_t)w);
-  return true;
-}
-
 // Exact test of a leaf record (s0, and s1 for a box leaf) against a sphere
 // or box predicate: Intersects.hpp:51-114 with the sphere decided by the
 // squared-distance threshold.
@@ -679,11 +645,34 @@ template <int F, class TV, class Sink>
 AXB_HD void rope_walk_points(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
                              uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
 {
+  constexpr uint32_t SI = internal_slots<F>();
   WalkFrame fr;
   walk_frame_init(tv, fr);
-  while (p < end && point_step<F>(tv, pr, p, skip, fr, sink, v_int, v_leaf))
+  // The loop's only exits are its condition (a zero rope, reported below,
+  // ends it as the end of the array does) and a sink asking to stop.
+  uint32_t step = 1;
+  while (step != 0 && p < end)
   {
+    const uint4 s0 = tv.slots[p];
+    const uint4 s1 = F == NODES_FP32 ? tv.slots[p + 1] : s0;
+    const int32_t w = (int32_t)s0.w;
+    const bool leaf = w < 0;
+    Box6 bx; // meaningless for a leaf record, and unused then
+    walk_node_box<F>(s0, s1, fr, bx);
+    const float cx = leaf ? bits_f(s0.x) : clamp_box(pr.cx, bx.lo[0], bx.hi[0]);
+    const float cy = leaf ? bits_f(s0.y) : clamp_box(pr.cy, bx.lo[1], bx.hi[1]);
+    const float cz = leaf ? bits_f(s0.z) : clamp_box(pr.cz, bx.lo[2], bx.hi[2]);
+    const bool pass = dist2_pp(pr.cx, pr.cy, pr.cz, cx, cy, cz) <= pr.thr;
+    v_leaf += leaf;
+    v_int += !leaf;
+    if (leaf && pass && p != skip && !sink.hit(~w))
+      return;
+    walk_frame_next<F>(tv, !leaf && pass, bx, fr);
+    step = leaf ? 1u : (pass ? SI : (uint32_t)w);
+    p += step;
   }
+  if (step == 0)
+    raise_error(tv.err, KERR_LAYOUT);
 }
 
 // SLOTS: the sink receives the slot of each hit leaf record instead of its
