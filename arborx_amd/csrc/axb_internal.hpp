@@ -244,8 +244,63 @@ struct RowWriter
   }
 };
 
+// Row writer of the point walks (sphere predicates over point leaves): hits
+// gather in LDS, ROW_STAGE per lane (entry j of lane l at st[j * 256], st
+// = the block's stage + l), and leave for the row as whole 64-byte chunks,
+// written back to back: a row's 16-byte groups written at different steps
+// left its lines to be written back partially, several times.
+constexpr int ROW_STAGE = 16;
 template <int MODE>
-AXB_D void row_epilogue(const TreeView &tv, RowWriter<MODE> &rw, int64_t *offsets, int64_t q,
+struct StagedRow
+{
+  int32_t *row;
+  uint32_t limit;
+  uint32_t cnt = 0;
+  int32_t *st;
+
+  AXB_D void init(int32_t *out, const int64_t *offsets, int64_t q, int32_t buffer,
+                  int32_t *stage)
+  {
+    const int64_t base =
+        MODE == MODE_FILL ? offsets[q] : (MODE == MODE_BUFFER ? q * (int64_t)buffer : 0);
+    row = out + base;
+    limit = MODE == MODE_FILL ? (uint32_t)(offsets[q + 1] - base) : (uint32_t)buffer;
+    st = stage;
+  }
+  AXB_D bool hit(int32_t v)
+  {
+    if (MODE != MODE_COUNT)
+    {
+      const uint32_t j = cnt & (ROW_STAGE - 1);
+      st[j * 256] = v;
+      // the chunk [cnt - 15, cnt] is complete and inside the row
+      if (j == ROW_STAGE - 1 && cnt < limit)
+      {
+        i32x4_u *dst = (i32x4_u *)(row + (cnt - (ROW_STAGE - 1)));
+#pragma unroll
+        for (int g = 0; g < ROW_STAGE / 4; ++g)
+          dst[g] = (i32x4_u){st[(4 * g) * 256], st[(4 * g + 1) * 256], st[(4 * g + 2) * 256],
+                             st[(4 * g + 3) * 256]};
+      }
+    }
+    ++cnt;
+    return true;
+  }
+  // Writes the last partial chunk (nothing for a row that overflowed: its
+  // count sends it to be walked again).
+  AXB_D void finish()
+  {
+    if (MODE == MODE_COUNT || cnt > limit)
+      return;
+    const uint32_t rem = cnt & (ROW_STAGE - 1);
+    int32_t *at = row + (cnt - rem);
+    for (uint32_t j = 0; j < rem; ++j)
+      at[j] = st[j * 256];
+  }
+};
+
+template <int MODE, template <int> class W>
+AXB_D void row_epilogue(const TreeView &tv, W<MODE> &rw, int64_t *offsets, int64_t q,
                         uint32_t v_int, uint32_t v_leaf)
 {
   rw.finish();

@@ -61,11 +61,22 @@ __global__ __launch_bounds__(256) void k_spatial(
     const float *p = preds + (PK == PRED_SPHERES ? 4 : 3) * q;
     pr = sphere_pred(p[0], p[1], p[2], PK == PRED_SPHERES ? p[3] : radius);
   }
-  RowWriter<MODE> rw;
-  rw.init(out, offsets, q, buffer);
   uint32_t v_int = 0, v_leaf = 0;
-  rope_walk<F, G, PK, SLOTS>(tv, pr, 0u, tv.total, ~0u, rw, v_int, v_leaf);
-  row_epilogue(tv, rw, offsets, q, COUNT ? v_int : 0u, COUNT ? v_leaf : 0u);
+  if constexpr (G == GEOM_POINTS && PK != PRED_BOXES && MODE != MODE_COUNT)
+  {
+    __shared__ int32_t s_stage[ROW_STAGE * 256];
+    StagedRow<MODE> rw;
+    rw.init(out, offsets, q, buffer, s_stage + threadIdx.x);
+    rope_walk<F, G, PK, SLOTS>(tv, pr, 0u, tv.total, ~0u, rw, v_int, v_leaf);
+    row_epilogue(tv, rw, offsets, q, COUNT ? v_int : 0u, COUNT ? v_leaf : 0u);
+  }
+  else
+  {
+    RowWriter<MODE> rw;
+    rw.init(out, offsets, q, buffer);
+    rope_walk<F, G, PK, SLOTS>(tv, pr, 0u, tv.total, ~0u, rw, v_int, v_leaf);
+    row_epilogue(tv, rw, offsets, q, COUNT ? v_int : 0u, COUNT ? v_leaf : 0u);
+  }
 }
 
 // Rows of leaf slots hit by each ray -> rows of values in (distance, index)
