@@ -1440,27 +1440,55 @@ AXB_HD void select_keys(const Src &src, int c, int m, int32_t *__restrict__ oi,
     }
     bitonic_merge<K>(A);
   }
-  // outputs written as the checks go; a tie rewrites them below
+  // outputs written as the checks go, four at a time (one 16-byte store
+  // each for a full row at 16-byte aligned addresses); a tie rewrites them
+  // below
   bool tie = false;
   uint64_t last = A[0], next = left;
   float d_prev = 0.f, d_last = 0.f;
+  const bool vec = K % 4 == 0 && m == K &&
+                   (((uintptr_t)oi | (uintptr_t)(od ? od : reinterpret_cast<float *>(oi))) &
+                    15u) == 0;
 #pragma unroll
-  for (int j = 0; j < K; ++j)
+  for (int g = 0; g < K; g += 4)
   {
-    if (j < m)
+    int32_t ii[4] = {0, 0, 0, 0};
+    float dd[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
     {
-      const float dj = sqrt_rn(bits_f((uint32_t)(A[j] >> 32)));
-      if (j > 0)
-        tie = tie || (dj == d_prev && (uint32_t)(A[j] >> 32) != (uint32_t)(A[j - 1] >> 32));
-      oi[j] = (int32_t)(uint32_t)A[j];
-      if (od)
-        od[j] = dj;
-      d_prev = dj;
-      if (j == m - 1)
-        last = A[j], d_last = dj;
+      const int j = g + u;
+      if (j < K && j < m)
+      {
+        const float dj = sqrt_rn(bits_f((uint32_t)(A[j] >> 32)));
+        if (j > 0)
+          tie = tie || (dj == d_prev && (uint32_t)(A[j] >> 32) != (uint32_t)(A[j - 1] >> 32));
+        ii[u] = (int32_t)(uint32_t)A[j];
+        dd[u] = dj;
+        d_prev = dj;
+        if (j == m - 1)
+          last = A[j], d_last = dj;
+      }
+      if (j < K && j == m)
+        next = A[j];
     }
-    if (j == m)
-      next = A[j];
+    if (vec)
+    {
+      *reinterpret_cast<int4 *>(oi + g) = make_int4(ii[0], ii[1], ii[2], ii[3]);
+      if (od)
+        *reinterpret_cast<float4 *>(od + g) = make_float4(dd[0], dd[1], dd[2], dd[3]);
+    }
+    else
+    {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (g + u < K && g + u < m)
+        {
+          oi[g + u] = ii[u];
+          if (od)
+            od[g + u] = dd[u];
+        }
+    }
   }
   // a left-out candidate with the last distance but another d2 (an exact
   // repeat of the last d2 is ordered by index already; one behind it may
