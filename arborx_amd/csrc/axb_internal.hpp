@@ -244,11 +244,11 @@ struct RowWriter
   }
 };
 
-// Row writer of the point walks (sphere predicates over point leaves): hits
-// gather in LDS, ROW_STAGE per lane (entry j of lane l at st[j * 256], st
-// = the block's stage + l), and leave for the row as whole 64-byte chunks,
-// written back to back: a row's 16-byte groups written at different steps
-// left its lines to be written back partially, several times.
+// Row writer of the spatial walks: hits gather in LDS, ROW_STAGE per lane
+// (entry j of lane l at st[j * 256], st = the block's stage + l), and leave
+// for the row as whole 64-byte chunks, written back to back (a row's 16-byte
+// groups written at different steps left its lines to be written back
+// partially, several times); the walk's loop carries only the count.
 constexpr int ROW_STAGE = 16;
 template <int MODE>
 struct StagedRow

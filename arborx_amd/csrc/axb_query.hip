@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void k_spatial(
     pr = sphere_pred(p[0], p[1], p[2], PK == PRED_SPHERES ? p[3] : radius);
   }
   uint32_t v_int = 0, v_leaf = 0;
-  if constexpr (G == GEOM_POINTS && PK != PRED_BOXES && MODE != MODE_COUNT)
+  if constexpr (MODE != MODE_COUNT)
   {
     __shared__ int32_t s_stage[ROW_STAGE * 256];
     StagedRow<MODE> rw;

@@ -690,7 +690,8 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
   constexpr int LG = leaf_group<G>();
   WalkFrame fr;
   walk_frame_init(tv, fr);
-  while (p < end)
+  uint32_t step = 1; // 0 after a zero rope (reported below), which ends the loop
+  while (step != 0 && p < end)
   {
     const uint4 s0 = tv.slots[p];
     const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
@@ -733,15 +734,13 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
       const bool pass = PK == PRED_RAYS    ? ray_hits(pr.ray, bx)
                         : PK == PRED_BOXES ? intersects_bb(pr.qb, bx)
                                            : dist2_pnode(pr.cx, pr.cy, pr.cz, bx) <= pr.thr;
-      if (!pass && w == 0)
-      {
-        raise_error(tv.err, KERR_LAYOUT);
-        break;
-      }
       walk_frame_next<F>(tv, pass, bx, fr);
-      p += pass ? SI : (uint32_t)w;
+      step = pass ? SI : (uint32_t)w;
+      p += step;
     }
   }
+  if (step == 0)
+    raise_error(tv.err, KERR_LAYOUT);
 }
 
 // ------------------------------------------------------- ordered walks
