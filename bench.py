@@ -59,7 +59,8 @@ def parse():
     ap.add_argument("--fcc-cells", default="80,80,78",
                     help="FCC cubic cells of the pair-energy case ('' = skip)")
     ap.add_argument("--e2e-steps", type=int, default=3,
-                    help="steps of the moving-points loop (0 = skip)")
+                    help="timed steps of the moving-points loop, after one untimed step "
+                         "(0 = skip)")
     ap.add_argument("--e2e-n", type=int, default=100_000_000)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this size first, as a known-size "
@@ -288,7 +289,10 @@ def bench_energy(ax, args, torch, synthetic, stream):
 def bench_e2e(ax, args, torch, synthetic, stream, fmt):
     """Moving-points loop: each step displaces every point by N(0, 1e-4) per
     axis, rebuilds the tree, runs kNN k=32 and the pair energy forward +
-    backward with the CRS recomputed.  The energy sees the coordinates scaled
+    backward with the CRS recomputed: built from the moved points inside the
+    forward pass (its own Q16 tree and full neighbour list) and freed when
+    the forward pass returns; the fused kernel gives E and dE/dx in that one
+    pass, and backward applies the kept gradient.  The energy sees the coordinates scaled
     by 2.5 / r32 (r32 = radius of about 32 neighbours), i.e. LJ with
     sigma = r32 / 2.5 and cutoff 2.5 sigma, so that pair terms stay finite."""
     n = args.e2e_n
@@ -306,7 +310,9 @@ def bench_e2e(ax, args, torch, synthetic, stream, fmt):
     tb = tk = te = 0.0
     finite = True
     t0 = time.perf_counter()
-    for s in range(args.e2e_steps):
+    # step 0 is untimed: it grows the library's pools and torch's cache from
+    # empty (the device-memory peak below still includes it)
+    for s in range(args.e2e_steps + 1):
         gen.manual_seed(8 + s)
         x.add_(torch.randn(x.shape, generator=gen, device="cuda") * 1e-4)
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
@@ -318,14 +324,15 @@ def bench_e2e(ax, args, torch, synthetic, stream, fmt):
         ev[2].record(stream)
         del bvh
         xr = x.detach().requires_grad_(True)
-        e = ax.PairEnergy.apply(xr * scale, rc, None, None, True)
+        e = ax.PairEnergy.apply(xr * scale, rc, None, None, False)
         e.backward()
         ev[3].record(stream)
         torch.cuda.synchronize()
         finite = finite and bool(torch.isfinite(e).item())
-        tb += ev[0].elapsed_time(ev[1])
-        tk += ev[1].elapsed_time(ev[2])
-        te += ev[2].elapsed_time(ev[3])
+        if s > 0:
+            tb += ev[0].elapsed_time(ev[1])
+            tk += ev[1].elapsed_time(ev[2])
+            te += ev[2].elapsed_time(ev[3])
         del xr, e
         if s % 10 == 0:
             log(f"[bench] e2e/{fmt} step {s}: {time.perf_counter() - t0:.1f}s")
