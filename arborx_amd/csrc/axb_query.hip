@@ -668,11 +668,25 @@ AXB_D void knn_traverse(const TreeView &tv, float qx, float qy, float qz, int m,
 //           adaptive walk (knn_traverse), once after all batches.
 // The walk of `collect` carries no list, so it runs at the register budget
 // of the spatial traversal.
-// Local subtree of a query: the deepest node on its greedy descent path that
-// still holds at least W leaves (slot, slots, decoded box).
+
+// Index of slot s (< 256) of a wavefront's seed window in its block's
+// candidate stage seen as 1024 uint4: the bytes of the stage entries of the
+// wavefront's own 64 lanes (entry j of lane l at uint2 j * 256 + l), 32
+// slots in each of the 8 entry rows, so that no other wavefront's entries
+// are touched.
+AXB_D uint32_t seed_window_index(uint32_t s, uint32_t wave)
+{
+  return ((s >> 5) << 7) + (wave << 5) + (s & 31u);
+}
+
+// Seed radius of a query: the k-th smallest distance over the leaves of a
+// window of about W leaves around the leaf its greedy descent reaches (an
+// upper bound of the k-th nearest distance); +inf when the window holds
+// fewer than k leaves.  stage: the block's candidate stage (KNN_STAGE x 256
+// uint2, free during the seed) or null.
 template <int F, int G, int K>
 AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, int W,
-                     uint32_t &v_int, uint32_t &v_leaf)
+                     uint32_t &v_int, uint32_t &v_leaf, uint2 *stage)
 {
   Subtree X;
   const uint32_t p = greedy_descent<F, G>(tv, qx, qy, qz, UINT32_MAX, X, v_int, v_leaf);
@@ -696,6 +710,35 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
     for (int j = 0; j < K; ++j)
       A[j] = j < K - k ? 0u : ~0u;
     uint32_t leaves = 0;
+    // The windows of a wavefront's queries (consecutive in Morton order)
+    // overlap: when the wavefront is full and the union of its windows has
+    // at most 256 slots, the wavefront loads the union once into LDS (every
+    // load in flight together) and each lane scans its window there.
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    uint32_t ua = a, ue = e;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+      const uint32_t oa = (uint32_t)__shfl_xor((int)ua, o, 64);
+      const uint32_t oe = (uint32_t)__shfl_xor((int)ue, o, 64);
+      ua = oa < ua ? oa : ua;
+      ue = oe > ue ? oe : ue;
+    }
+    uint4 *win = reinterpret_cast<uint4 *>(stage);
+    const bool coop = stage != nullptr && __ballot(1) == ~0ull && ue - ua <= 256u;
+    if (coop)
+    {
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j)
+      {
+        const uint32_t sl = j * 64u + lane;
+        if (ua + sl < ue)
+          win[seed_window_index(sl, wv)] = tv.slots[ua + sl];
+      }
+      // the wavefront's LDS writes complete before any lane reads
+      __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0)
+      __builtin_amdgcn_wave_barrier();
+    }
     for (uint32_t b = a; b < e; b += NB)
     {
       uint32_t B[NB];
@@ -703,7 +746,8 @@ AXB_D float knn_seed(const TreeView &tv, float qx, float qy, float qz, int k, in
       for (int j = 0; j < NB; ++j)
       {
         const uint32_t slot = b + (uint32_t)j;
-        const uint4 r = tv.slots[slot < e ? slot : e - 1];
+        const uint32_t sc = slot < e ? slot : e - 1;
+        const uint4 r = coop ? win[seed_window_index(sc - ua, wv)] : tv.slots[sc];
         const bool leaf = slot < e && (int32_t)r.w < 0;
         B[j] = leaf ? f_bits(dist2_pp(qx, qy, qz, bits_f(r.x), bits_f(r.y), bits_f(r.z)))
                     : ~0u;
@@ -959,7 +1003,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 &&
     int max_rounds, uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_n,
     int32_t *__restrict__ oi, float *__restrict__ od)
 {
-  __shared__ uint2 s_stage[G == GEOM_POINTS ? KNN_STAGE * 256 : 1];
+  __shared__ __align__(16) uint2 s_stage[G == GEOM_POINTS ? KNN_STAGE * 256 : 1];
   const int64_t g = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= tn)
     return;
@@ -969,7 +1013,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((K <= 16 &&
   const float qx = pts[3 * q], qy = pts[3 * q + 1], qz = pts[3 * q + 2];
   const int m = k < tv.n ? k : tv.n;
   uint32_t v_int = 0, v_leaf = 0;
-  float R = knn_seed<F, G, K>(tv, qx, qy, qz, k, W, v_int, v_leaf);
+  float R = knn_seed<F, G, K>(tv, qx, qy, qz, k, W, v_int, v_leaf,
+                             G == GEOM_POINTS ? s_stage : nullptr);
   const uint32_t row_at = (uint32_t)(g * cap); // batch * cap + cap <= 2^32 - 1
   int c = 0;
   bool done = false;
