@@ -171,10 +171,12 @@ AXB_HD float dist2_pnode(float px, float py, float pz, const Box6 &b)
                   clamp_box(pz, b.lo[2], b.hi[2]));
 }
 // ArborX_Intersects.hpp:51-65.
+// (Bitwise ors: every comparison is evaluated, so a walk's loads of the box
+// are issued together instead of one after the other behind branches.)
 AXB_HD bool intersects_bb(const Box6 &a, const Box6 &b)
 {
-  return !(a.lo[0] > b.hi[0] || a.hi[0] < b.lo[0] || a.lo[1] > b.hi[1] ||
-           a.hi[1] < b.lo[1] || a.lo[2] > b.hi[2] || a.hi[2] < b.lo[2]);
+  return !((a.lo[0] > b.hi[0]) | (a.hi[0] < b.lo[0]) | (a.lo[1] > b.hi[1]) |
+           (a.hi[1] < b.lo[1]) | (a.lo[2] > b.hi[2]) | (a.hi[2] < b.lo[2]));
 }
 
 AXB_HD float bits_f(uint32_t u)
@@ -605,8 +607,8 @@ AXB_HD bool leaf_hit(const SpatialPred &pr, uint4 s0, uint4 s1)
   if (G == GEOM_POINTS)
   {
     if (PK == PRED_BOXES)
-      return !(x > pr.qb.hi[0] || x < pr.qb.lo[0] || y > pr.qb.hi[1] || y < pr.qb.lo[1] ||
-               z > pr.qb.hi[2] || z < pr.qb.lo[2]);
+      return !((x > pr.qb.hi[0]) | (x < pr.qb.lo[0]) | (y > pr.qb.hi[1]) |
+               (y < pr.qb.lo[1]) | (z > pr.qb.hi[2]) | (z < pr.qb.lo[2]));
     return dist2_pp(pr.cx, pr.cy, pr.cz, x, y, z) <= pr.thr;
   }
   const Box6 lb = {{x, y, z}, {bits_f(s1.x), bits_f(s1.y), bits_f(s1.z)}};
@@ -675,6 +677,60 @@ AXB_HD void rope_walk_points(const TV &tv, const SpatialPred &pr, uint32_t p, ui
     raise_error(tv.err, KERR_LAYOUT);
 }
 
+// Box leaves with box predicates: one record per step, leaf and internal
+// node through the same straight-line code, as rope_walk_points does for
+// points.  Both are tested by intersects(box, query box); the box is the
+// leaf's exact one (slots p and p + 1) or the internal record's decoded one
+// (FP32: the same two slots), picked by selects, so the lanes of a
+// wavefront, which disagree on the record kind in most steps, run one path.
+// The read of slot p + 1 after a Q16 internal record stays inside the
+// array's zeroed tail.
+template <int F, bool SLOTS, class TV, class Sink>
+AXB_HD void rope_walk_boxes(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
+                            uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
+{
+  constexpr uint32_t SL = leaf_slots<GEOM_BOXES>(), SI = internal_slots<F>();
+  WalkFrame fr;
+  walk_frame_init(tv, fr);
+  uint32_t step = 1; // 0 after a zero rope (reported below), which ends the loop
+  while (step != 0 && p < end)
+  {
+    const uint4 s0 = tv.slots[p];
+    const uint4 s1 = tv.slots[p + 1];
+    const int32_t w = (int32_t)s0.w;
+    const bool leaf = w < 0;
+    Box6 bx;
+    walk_node_box<F>(s0, s1, fr, bx);
+    if (F == NODES_Q16)
+    {
+      bx.lo[0] = leaf ? bits_f(s0.x) : bx.lo[0];
+      bx.lo[1] = leaf ? bits_f(s0.y) : bx.lo[1];
+      bx.lo[2] = leaf ? bits_f(s0.z) : bx.lo[2];
+      bx.hi[0] = leaf ? bits_f(s1.x) : bx.hi[0];
+      bx.hi[1] = leaf ? bits_f(s1.y) : bx.hi[1];
+      bx.hi[2] = leaf ? bits_f(s1.z) : bx.hi[2];
+    }
+    const bool pass = intersects_bb(pr.qb, bx);
+    v_leaf += leaf;
+    v_int += !leaf;
+    if (leaf && pass && p != skip && !sink.hit(SLOTS ? (int32_t)p : ~w))
+      return;
+    walk_frame_next<F>(tv, !leaf && pass, bx, fr);
+    step = leaf ? SL : (pass ? SI : (uint32_t)w);
+    p += step;
+  }
+  if (step == 0)
+    raise_error(tv.err, KERR_LAYOUT);
+}
+
+#ifndef AXB_BOX_WALK_FP32
+#define AXB_BOX_WALK_FP32 1
+#endif
+#ifndef AXB_BOX_WALK_Q16
+#define AXB_BOX_WALK_Q16 1
+#endif
+constexpr bool BOX_WALK_ONE_PATH[2] = {AXB_BOX_WALK_FP32, AXB_BOX_WALK_Q16};
+
 // SLOTS: the sink receives the slot of each hit leaf record instead of its
 // value (read back by sort_ray_row).
 template <int F, int G, int PK, bool SLOTS = false, class TV, class Sink>
@@ -684,6 +740,11 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
   if constexpr (G == GEOM_POINTS && PK != PRED_BOXES)
   {
     rope_walk_points<F>(tv, pr, p, end, skip, sink, v_int, v_leaf);
+    return;
+  }
+  if constexpr (G == GEOM_BOXES && PK == PRED_BOXES && BOX_WALK_ONE_PATH[F])
+  {
+    rope_walk_boxes<F, SLOTS>(tv, pr, p, end, skip, sink, v_int, v_leaf);
     return;
   }
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
@@ -716,8 +777,8 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
         if (run)
         {
           ++v_leaf;
-          if (p != skip && leaf_hit<G, PK>(pr, a[j], b[j]) &&
-              !sink.hit(SLOTS ? (int32_t)p : ~(int32_t)a[j].w))
+          const bool hit = leaf_hit<G, PK>(pr, a[j], b[j]);
+          if (hit && p != skip && !sink.hit(SLOTS ? (int32_t)p : ~(int32_t)a[j].w))
             stop = true;
           p += SL;
         }

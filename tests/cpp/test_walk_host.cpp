@@ -184,6 +184,59 @@ static void run_case(int geometry, int64_t n, unsigned seed, float r, int node_f
   std::snprintf(what, sizeof what, "rows: geometry %d n %lld format %d", geometry, (long long)n,
                 node_format);
   CHECK(ok && err == 0, what);
+
+  // box predicates: boxes around the same centres, of three sizes; every
+  // fifth has its lower corner on a value's coordinates (faces that touch)
+  std::vector<float> qb(nq * 6);
+  for (int64_t q = 0; q < nq; ++q)
+  {
+    const float h = r * (0.5f + 0.5f * (float)(q % 3));
+    for (int d = 0; d < 3; ++d)
+    {
+      qb[6 * q + d] = sph[4 * q + d] - h;
+      qb[6 * q + 3 + d] = sph[4 * q + d] + h;
+    }
+    if (q % 5 == 2)
+      for (int d = 0; d < 3; ++d)
+        qb[6 * q + d] = v[vs * ((q * 7) % n) + 3 * (geometry == 1) + d];
+  }
+  const int64_t bnnz = orc_query_spatial(t, 1, qb.data(), nq, off.data(), nullptr, nullptr,
+                                         nullptr);
+  std::vector<int32_t> bref(bnnz > 0 ? bnnz : 1);
+  orc_query_spatial(t, 1, qb.data(), nq, off.data(), bref.data(), nullptr, nullptr);
+  bool ok_box = true;
+  err = 0;
+  for (int64_t q = 0; q < nq && ok_box; ++q)
+  {
+    const float *c = &qb[6 * q];
+    const axb::SpatialPred pr = axb::box_pred(axb::Box6{{c[0], c[1], c[2]}, {c[3], c[4], c[5]}});
+    std::vector<int32_t> row;
+    RowSink sink{&row};
+    uint32_t vi = 0, vl = 0;
+    if (geometry == 0)
+    {
+      if (node_format == 0)
+        axb::rope_walk<axb::NODES_FP32, axb::GEOM_POINTS, axb::PRED_BOXES>(
+            tv, pr, 0u, (uint32_t)total, ~0u, sink, vi, vl);
+      else
+        axb::rope_walk<axb::NODES_Q16, axb::GEOM_POINTS, axb::PRED_BOXES>(
+            tv, pr, 0u, (uint32_t)total, ~0u, sink, vi, vl);
+    }
+    else
+    {
+      if (node_format == 0)
+        axb::rope_walk<axb::NODES_FP32, axb::GEOM_BOXES, axb::PRED_BOXES>(
+            tv, pr, 0u, (uint32_t)total, ~0u, sink, vi, vl);
+      else
+        axb::rope_walk<axb::NODES_Q16, axb::GEOM_BOXES, axb::PRED_BOXES>(
+            tv, pr, 0u, (uint32_t)total, ~0u, sink, vi, vl);
+    }
+    ok_box = row.size() == (size_t)(off[q + 1] - off[q]) &&
+             std::equal(row.begin(), row.end(), bref.begin() + off[q]);
+  }
+  std::snprintf(what, sizeof what, "box rows: geometry %d n %lld format %d", geometry,
+                (long long)n, node_format);
+  CHECK(ok_box && err == 0 && bnnz > 0, what);
   if (geometry == 0)
   {
     std::snprintf(what, sizeof what, "skip/stop forms: n %lld format %d", (long long)n,
