@@ -677,15 +677,16 @@ AXB_HD void rope_walk_points(const TV &tv, const SpatialPred &pr, uint32_t p, ui
     raise_error(tv.err, KERR_LAYOUT);
 }
 
-// Box leaves with box predicates: one record per step, leaf and internal
-// node through the same straight-line code, as rope_walk_points does for
-// points.  Both are tested by intersects(box, query box); the box is the
-// leaf's exact one (slots p and p + 1) or the internal record's decoded one
-// (FP32: the same two slots), picked by selects, so the lanes of a
-// wavefront, which disagree on the record kind in most steps, run one path.
-// The read of slot p + 1 after a Q16 internal record stays inside the
-// array's zeroed tail.
-template <int F, bool SLOTS, class TV, class Sink>
+// Box leaves: one record per step, leaf and internal node through the same
+// straight-line code, as rope_walk_points does for points.  The box tested
+// is the leaf's exact one (slots p and p + 1) or the internal record's
+// decoded one (FP32: the same two slots), picked by selects, so the lanes
+// of a wavefront, which disagree on the record kind in most steps, run one
+// path.  Box and ray predicates test both kinds of box alike; a sphere
+// tests a leaf box with the reference's comparisons (leaf_hit) and a node
+// box with clamps, both computed.  The read of slot p + 1 after a Q16
+// internal record stays inside the array's zeroed tail.
+template <int F, int PK, bool SLOTS, class TV, class Sink>
 AXB_HD void rope_walk_boxes(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
                             uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
 {
@@ -710,7 +711,14 @@ AXB_HD void rope_walk_boxes(const TV &tv, const SpatialPred &pr, uint32_t p, uin
       bx.hi[1] = leaf ? bits_f(s1.y) : bx.hi[1];
       bx.hi[2] = leaf ? bits_f(s1.z) : bx.hi[2];
     }
-    const bool pass = intersects_bb(pr.qb, bx);
+    bool pass;
+    if (PK == PRED_RAYS)
+      pass = ray_hits(pr.ray, bx);
+    else if (PK == PRED_BOXES)
+      pass = intersects_bb(pr.qb, bx);
+    else
+      pass = leaf ? dist2_pbox(pr.cx, pr.cy, pr.cz, bx) <= pr.thr
+                  : dist2_pnode(pr.cx, pr.cy, pr.cz, bx) <= pr.thr;
     v_leaf += leaf;
     v_int += !leaf;
     if (leaf && pass && p != skip && !sink.hit(SLOTS ? (int32_t)p : ~w))
@@ -723,13 +731,10 @@ AXB_HD void rope_walk_boxes(const TV &tv, const SpatialPred &pr, uint32_t p, uin
     raise_error(tv.err, KERR_LAYOUT);
 }
 
-#ifndef AXB_BOX_WALK_FP32
-#define AXB_BOX_WALK_FP32 1
+#ifndef AXB_BOX_WALK_RAYS
+#define AXB_BOX_WALK_RAYS 1
 #endif
-#ifndef AXB_BOX_WALK_Q16
-#define AXB_BOX_WALK_Q16 1
-#endif
-constexpr bool BOX_WALK_ONE_PATH[2] = {AXB_BOX_WALK_FP32, AXB_BOX_WALK_Q16};
+constexpr bool BOX_WALK_ONE_PATH[2] = {true, AXB_BOX_WALK_RAYS};
 
 // SLOTS: the sink receives the slot of each hit leaf record instead of its
 // value (read back by sort_ray_row).
@@ -742,9 +747,9 @@ AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t 
     rope_walk_points<F>(tv, pr, p, end, skip, sink, v_int, v_leaf);
     return;
   }
-  if constexpr (G == GEOM_BOXES && PK == PRED_BOXES && BOX_WALK_ONE_PATH[F])
+  if constexpr (G == GEOM_BOXES && BOX_WALK_ONE_PATH[PK == PRED_RAYS])
   {
-    rope_walk_boxes<F, SLOTS>(tv, pr, p, end, skip, sink, v_int, v_leaf);
+    rope_walk_boxes<F, PK, SLOTS>(tv, pr, p, end, skip, sink, v_int, v_leaf);
     return;
   }
   constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
