@@ -618,9 +618,10 @@ AXB_HD bool leaf_hit(const SpatialPred &pr, uint4 s0, uint4 s1)
                           : dist2_pbox(pr.cx, pr.cy, pr.cz, lb) <= pr.thr;
 }
 
-// Leaf records handled per step of a walk: a run of consecutive leaves is
-// read together (64 bytes).  The node array keeps LEAF_GROUP * SL - 1 zeroed
-// slots after its last record for this read-ahead.
+// Leaf records handled per step of the nearest candidates' walk over box
+// leaves: a run of consecutive leaves is read together (64 bytes).  The
+// node array keeps LEAF_GROUP * SL - 1 zeroed slots after its last record
+// for this read-ahead (and for the spatial walks' read of slot p + 1).
 template <int G>
 constexpr int leaf_group()
 {
@@ -636,14 +637,16 @@ constexpr int leaf_group()
 // Callbacks.hpp:96-118).  The leaf record at `skip` is not reported.  A Q16
 // walk starts at the root or at a record whose frame is the scene box.
 // TV: a tree view with `const uint4 *slots`, `Box6 scene`, `float ss[3]` and
-// `uint32_t *err`.  PK: PRED_BOXES tests the box, otherwise the sphere.
-// Point leaves with sphere predicates: one record per step, leaf and
-// internal node through the same straight-line code.  The point test
-// d2(c, p) and the node test d2(c, clamp(c, box)) are one computation once
-// the closest point is chosen (the point itself for a leaf), so the lanes
-// of a wavefront, which disagree on the record kind in most steps, share
-// the decode and distance instructions instead of running both paths.
-template <int F, class TV, class Sink>
+// `uint32_t *err`.  PK: PRED_BOXES tests the box, PRED_RAYS the ray (box
+// leaves only), otherwise the sphere.
+// Point leaves: one record per step, leaf and internal node through the
+// same straight-line code.  With a sphere, the point test d2(c, p) and the
+// node test d2(c, clamp(c, box)) are one computation once the closest point
+// is chosen (the point itself for a leaf); with a box, the point test is
+// the box test against [p, p] (the same six comparisons).  So the lanes of
+// a wavefront, which disagree on the record kind in most steps, share the
+// decode and test instructions instead of running both paths.
+template <int F, int PK, class TV, class Sink>
 AXB_HD void rope_walk_points(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
                              uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
 {
@@ -661,10 +664,21 @@ AXB_HD void rope_walk_points(const TV &tv, const SpatialPred &pr, uint32_t p, ui
     const bool leaf = w < 0;
     Box6 bx; // meaningless for a leaf record, and unused then
     walk_node_box<F>(s0, s1, fr, bx);
-    const float cx = leaf ? bits_f(s0.x) : clamp_box(pr.cx, bx.lo[0], bx.hi[0]);
-    const float cy = leaf ? bits_f(s0.y) : clamp_box(pr.cy, bx.lo[1], bx.hi[1]);
-    const float cz = leaf ? bits_f(s0.z) : clamp_box(pr.cz, bx.lo[2], bx.hi[2]);
-    const bool pass = dist2_pp(pr.cx, pr.cy, pr.cz, cx, cy, cz) <= pr.thr;
+    bool pass;
+    if (PK == PRED_BOXES)
+    {
+      const float x = bits_f(s0.x), y = bits_f(s0.y), z = bits_f(s0.z);
+      const Box6 tb = {{leaf ? x : bx.lo[0], leaf ? y : bx.lo[1], leaf ? z : bx.lo[2]},
+                       {leaf ? x : bx.hi[0], leaf ? y : bx.hi[1], leaf ? z : bx.hi[2]}};
+      pass = intersects_bb(pr.qb, tb);
+    }
+    else
+    {
+      const float cx = leaf ? bits_f(s0.x) : clamp_box(pr.cx, bx.lo[0], bx.hi[0]);
+      const float cy = leaf ? bits_f(s0.y) : clamp_box(pr.cy, bx.lo[1], bx.hi[1]);
+      const float cz = leaf ? bits_f(s0.z) : clamp_box(pr.cz, bx.lo[2], bx.hi[2]);
+      pass = dist2_pp(pr.cx, pr.cy, pr.cz, cx, cy, cz) <= pr.thr;
+    }
     v_leaf += leaf;
     v_int += !leaf;
     if (leaf && pass && p != skip && !sink.hit(~w))
@@ -731,82 +745,18 @@ AXB_HD void rope_walk_boxes(const TV &tv, const SpatialPred &pr, uint32_t p, uin
     raise_error(tv.err, KERR_LAYOUT);
 }
 
-#ifndef AXB_BOX_WALK_RAYS
-#define AXB_BOX_WALK_RAYS 1
-#endif
-constexpr bool BOX_WALK_ONE_PATH[2] = {true, AXB_BOX_WALK_RAYS};
-
 // SLOTS: the sink receives the slot of each hit leaf record instead of its
 // value (read back by sort_ray_row).
 template <int F, int G, int PK, bool SLOTS = false, class TV, class Sink>
 AXB_HD void rope_walk(const TV &tv, const SpatialPred &pr, uint32_t p, uint32_t end,
                      uint32_t skip, Sink &sink, uint32_t &v_int, uint32_t &v_leaf)
 {
-  if constexpr (G == GEOM_POINTS && PK != PRED_BOXES)
-  {
-    rope_walk_points<F>(tv, pr, p, end, skip, sink, v_int, v_leaf);
-    return;
-  }
-  if constexpr (G == GEOM_BOXES && BOX_WALK_ONE_PATH[PK == PRED_RAYS])
-  {
+  static_assert(!(G == GEOM_POINTS && (PK == PRED_RAYS || SLOTS)),
+                "rays and slot rows are walks over box leaves");
+  if constexpr (G == GEOM_POINTS)
+    rope_walk_points<F, PK>(tv, pr, p, end, skip, sink, v_int, v_leaf);
+  else
     rope_walk_boxes<F, PK, SLOTS>(tv, pr, p, end, skip, sink, v_int, v_leaf);
-    return;
-  }
-  constexpr uint32_t SL = leaf_slots<G>(), SI = internal_slots<F>();
-  constexpr int LG = leaf_group<G>();
-  WalkFrame fr;
-  walk_frame_init(tv, fr);
-  uint32_t step = 1; // 0 after a zero rope (reported below), which ends the loop
-  while (step != 0 && p < end)
-  {
-    const uint4 s0 = tv.slots[p];
-    const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
-    const int32_t w = (int32_t)s0.w;
-    if (w < 0)
-    {
-      // this leaf and the leaf records right after it, in order
-      uint4 a[LG], b[LG];
-      a[0] = s0, b[0] = s1;
-#pragma unroll
-      for (int j = 1; j < LG; ++j)
-      {
-        a[j] = tv.slots[p + j * SL];
-        b[j] = G == GEOM_BOXES ? tv.slots[p + j * SL + 1] : a[j];
-      }
-      // p advances with each leaf handled, so it is record j's slot here
-      bool run = true, stop = false;
-#pragma unroll
-      for (int j = 0; j < LG; ++j)
-      {
-        run = run && !stop && p < end && (int32_t)a[j].w < 0;
-        if (run)
-        {
-          ++v_leaf;
-          const bool hit = leaf_hit<G, PK>(pr, a[j], b[j]);
-          if (hit && p != skip && !sink.hit(SLOTS ? (int32_t)p : ~(int32_t)a[j].w))
-            stop = true;
-          p += SL;
-        }
-      }
-      if (stop)
-        break;
-      walk_frame_next<F>(tv, false, Box6{}, fr);
-    }
-    else
-    {
-      ++v_int;
-      Box6 bx;
-      walk_node_box<F>(s0, s1, fr, bx);
-      const bool pass = PK == PRED_RAYS    ? ray_hits(pr.ray, bx)
-                        : PK == PRED_BOXES ? intersects_bb(pr.qb, bx)
-                                           : dist2_pnode(pr.cx, pr.cy, pr.cz, bx) <= pr.thr;
-      walk_frame_next<F>(tv, pass, bx, fr);
-      step = pass ? SI : (uint32_t)w;
-      p += step;
-    }
-  }
-  if (step == 0)
-    raise_error(tv.err, KERR_LAYOUT);
 }
 
 // ------------------------------------------------------- ordered walks
