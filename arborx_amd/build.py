@@ -7,7 +7,8 @@ Every translation unit is compiled by hipcc for gfx950 only with
   -Xclang -target-feature -Xclang -packed-fp32-ops
                                             no packed-FP32 instructions
 and linked into one shared library with a plain C ABI
-(include/arborx_mi355x.h).
+(include/arborx_mi355x.h), against ROCm's roctx library for the named
+profiling ranges.
 """
 import concurrent.futures
 import os
@@ -60,7 +61,9 @@ def build(verbose=True):
         if err.strip() and verbose:
             print(err, file=sys.stderr)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(HIPCC)), "lib")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp,
+           "-L" + rocm_lib, "-lrocprofiler-sdk-roctx", "-Wl,-rpath," + rocm_lib]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

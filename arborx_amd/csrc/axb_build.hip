@@ -852,6 +852,7 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
     return hipErrorOutOfMemory;
 
   // ---- scene box
+  Region phase("ArborX::BVH::BVH::calculate_scene_bounding_box");
   if (geometry == GEOM_POINTS)
     AXB_LAUNCH(k_scene_partial<GEOM_POINTS>, dim3(partial_blocks), dim3(256), 0,
                        stream, values, n, partials);
@@ -862,6 +863,7 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
              out.scene);
 
   // ---- Morton codes + digit histograms
+  phase.next("ArborX::BVH::BVH::compute_linear_ordering");
   e0 = fill_u32(hist, 0u, 8 * 256, stream);
   if (e0 != hipSuccess)
     return e0;
@@ -876,11 +878,13 @@ hipError_t build_tree(const float *values, int64_t n, int geometry,
   }
 
   // ---- sort (8 passes of 8 bits; result back in keys0 / vals0)
+  phase.next("ArborX::BVH::BVH::sort_linearized_order");
   hipError_t e = radix_sort_pairs_u64(keys0, keys1, vals0, vals1, n, 8, hist, ws, err, stream);
   if (e != hipSuccess)
     return e;
 
   // ---- hierarchy
+  phase.next("ArborX::BVH::BVH::generate_hierarchy");
   e = fill_u32((uint32_t *)c, 0u, n, stream);
   if (e != hipSuccess)
     return e;

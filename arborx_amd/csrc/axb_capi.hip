@@ -10,6 +10,8 @@
 #include "arborx_mi355x.h"
 #include "axb_internal.hpp"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <algorithm>
 #include <array>
 #include <map>
@@ -64,6 +66,14 @@ ProfScope::~ProfScope()
     std::lock_guard<std::mutex> lock(g_prof_mutex);
     g_prof_recs.push_back({name_, a_, b_, -1.0});
   }
+}
+
+Region::Region(const char *label) { roctxRangePushA(label); }
+Region::~Region() { roctxRangePop(); }
+void Region::next(const char *label)
+{
+  roctxRangePop();
+  roctxRangePushA(label);
 }
 
 HostScope::HostScope(const char *name) : name_(name), on_(g_prof_timing.load())
@@ -448,6 +458,7 @@ uint64_t axb_morton64_host(const float p[3], const float scene[6])
 axb_status axb_bvh_create(const float *values, int64_t n, int32_t geometry, int32_t node_format,
                           int32_t flags, axb_stream_t stream_, axb_bvh **out)
 {
+  Region region("ArborX::BVH::BVH");
   if (!out)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_create: out is NULL");
   *out = nullptr;
@@ -674,6 +685,7 @@ static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy po
                             hipStream_t s, const char *what, uint32_t *tolerated = nullptr,
                             int64_t auto_rows = 64)
 {
+  Region region("ArborX::CrsGraphWrapper::two_pass");
   const bool hard = policy.buffer_size < 0;
   int64_t B = policy.buffer_size < 0 ? -(int64_t)policy.buffer_size : policy.buffer_size;
   if (B == 0)
@@ -838,6 +850,9 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
                                 int32_t **indices, int64_t *nnz_host, axb_stream_t stream_,
                                 int32_t limit = INT32_MAX, uint32_t *tolerated = nullptr)
 {
+  Region region("ArborX::BVH::query_crs");
+  Region tag_region(pred_kind == AXB_PREDICATE_ORDERED_RAYS ? "ArborX::BVH::query::ordered_spatial"
+                                                           : "ArborX::BVH::query::spatial");
   if (!t || !offsets || !indices || !nnz_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: NULL argument");
   const bool ordered = pred_kind == AXB_PREDICATE_ORDERED_RAYS;
@@ -870,6 +885,8 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
   void *order_blk = nullptr;
   if (policy.sort_predicates)
   {
+    Region perm_region(ordered ? "ArborX::BVH::query::ordered_spatial::compute_permutation"
+                               : "ArborX::BVH::query::spatial::compute_permutation");
     axb_status st = predicate_order(t, err.p, ordered ? PRED_RAYS : pred_kind, preds, nq, s,
                                     &order, &order_blk);
     if (st != AXB_OK)
@@ -1152,6 +1169,8 @@ axb_status axb_bvh_neighbor_list(const axb_bvh *t, float radius, int32_t full,
                                  axb_traversal_policy policy, int64_t *offsets,
                                  int32_t **indices, int64_t *nnz_host, axb_stream_t stream_)
 {
+  Region region(full ? "ArborX::Experimental::FullNeighborList"
+                     : "ArborX::Experimental::HalfNeighborList");
   if (!t || !offsets || !indices || !nnz_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_neighbor_list: NULL argument");
   if (t->geometry != AXB_GEOMETRY_POINTS)
@@ -1214,6 +1233,7 @@ axb_status axb_expand_half_to_full(const int64_t *offsets, const int32_t *indice
                                    int64_t *out_offsets, int32_t **out_indices,
                                    int64_t *nnz_host, axb_stream_t stream_)
 {
+  Region region("ArborX::Experimental::HalfToFull");
   if (!offsets || !out_offsets || !out_indices || !nnz_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_expand_half_to_full: NULL argument");
   if (n < 0 || n >= (int64_t(1) << 31))
@@ -1274,6 +1294,8 @@ axb_status axb_bvh_query_nearest(const axb_bvh *t, const float *points, int64_t 
                                  int32_t **indices, float **distances, int64_t *nnz_host,
                                  axb_stream_t stream_)
 {
+  Region region("ArborX::BVH::query_crs");
+  Region tag_region("ArborX::BVH::query::nearest");
   if (!t || !offsets || !indices || !nnz_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bvh_query_nearest: NULL argument");
   if (nq < 0 || nq >= (int64_t(1) << 31))
@@ -1318,6 +1340,7 @@ axb_status axb_bvh_query_nearest(const axb_bvh *t, const float *points, int64_t 
     }
     if (e == hipSuccess && policy.sort_predicates)
     {
+      Region perm_region("ArborX::BVH::query::nearest::compute_permutation");
       axb_status st = predicate_order(t, err.p, PRED_RADIUS, points, nq, s, &order, &order_blk);
       if (st != AXB_OK)
       {
@@ -1534,6 +1557,7 @@ struct axb_bruteforce
 axb_status axb_bruteforce_create(const float *values, int64_t n, int32_t geometry,
                                  axb_stream_t stream_, axb_bruteforce **out)
 {
+  Region region("ArborX::BruteForce::BruteForce");
   if (!out)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_create: out is NULL");
   *out = nullptr;
@@ -1609,6 +1633,8 @@ static axb_status bf_query_spatial(const axb_bruteforce *bf, int pred_kind, cons
                                    int64_t *offsets, int32_t **indices, int64_t *nnz_host,
                                    axb_stream_t stream_)
 {
+  Region region("ArborX::BruteForce::query_crs");
+  Region tag_region("ArborX::BruteForce::query::spatial");
   if (!bf || !offsets || !indices || !nnz_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "bruteforce query: NULL argument");
   if (nq < 0 || nq >= (int64_t(1) << 31))
@@ -1678,6 +1704,8 @@ axb_status axb_bruteforce_query_nearest_rays(const axb_bruteforce *bf, const flo
                                              int64_t *offsets, int32_t **indices,
                                              int64_t *nnz_host, axb_stream_t stream_)
 {
+  Region region("ArborX::BruteForce::query_crs");
+  Region tag_region("ArborX::BruteForce::query::nearest");
   if (!bf || !offsets || !indices || !nnz_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_query_nearest_rays: NULL argument");
   if (nq < 0 || nq >= (int64_t(1) << 31))
@@ -1719,6 +1747,8 @@ axb_status axb_bruteforce_query_nearest(const axb_bruteforce *bf, const float *p
                                        int64_t *offsets, int32_t **indices, float **distances,
                                        int64_t *nnz_host, axb_stream_t stream_)
 {
+  Region region("ArborX::BruteForce::query_crs");
+  Region tag_region("ArborX::BruteForce::query::nearest");
   (void)policy; // no traversal order to choose: every value is visited
   if (!bf || !offsets || !indices || !nnz_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_bruteforce_query_nearest: NULL argument");

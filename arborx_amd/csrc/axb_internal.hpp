@@ -44,6 +44,20 @@ struct ProfScope
   bool on_;
   hipEvent_t a_, b_;
 };
+// Named range on the host's timeline, around a public entry point or one of
+// its phases, under the reference's profiling region labels (Kokkos regions,
+// e.g. ArborX_LinearBVH.hpp:201-256, ArborX_CrsGraphWrapperImpl.hpp:391-410):
+// roctx ranges, which `rocprofv3 --marker-trace` records and which cost one
+// call each when no tool listens.  next() ends the range and opens the
+// following phase's.
+struct Region
+{
+  explicit Region(const char *label);
+  ~Region();
+  void next(const char *label);
+  Region(const Region &) = delete;
+  Region &operator=(const Region &) = delete;
+};
 // Host-side interval recorded with the kernel timings (name "host:...").
 struct HostScope
 {

@@ -31,6 +31,24 @@ def test_library_exports_every_declared_symbol():
     assert set(arborx_amd.EXPORTED_SYMBOLS) == declared
 
 
+def test_library_marks_ranges_with_roctx():
+    """The entry points open roctx ranges (the reference's profiling
+    region labels): the library imports roctx's push and pop and holds the
+    labels of the build phases and the query forms."""
+    import subprocess
+    und = subprocess.run(["nm", "-D", "--undefined-only", arborx_amd.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    assert "roctxRangePushA" in und and "roctxRangePop" in und
+    blob = open(arborx_amd.LIB_PATH, "rb").read()
+    for label in (b"ArborX::BVH::BVH", b"ArborX::BVH::BVH::sort_linearized_order",
+                  b"ArborX::BVH::BVH::generate_hierarchy", b"ArborX::BVH::query_crs",
+                  b"ArborX::BVH::query::spatial", b"ArborX::BVH::query::nearest",
+                  b"ArborX::CrsGraphWrapper::two_pass",
+                  b"ArborX::Experimental::FullNeighborList",
+                  b"ArborX::BruteForce::BruteForce"):
+        assert label + b"\0" in blob, label
+
+
 def test_version_string():
     assert b"gfx950" in arborx_amd.lib().axb_version()
 
