@@ -697,6 +697,7 @@ static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy po
   WsLease rows;
   int32_t *buf = nullptr;
   uint32_t *ovf = nullptr, *ovf_n = nullptr;
+  uint8_t *over_flags = nullptr; // a bit per row: its count exceeds B
   axb_status st = AXB_OK;
   hipError_t e = scan_ws ? hipSuccess : hipErrorOutOfMemory;
   int64_t nnz = 0;
@@ -707,7 +708,8 @@ static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy po
     buf = (int32_t *)rows.p;
     ovf = (uint32_t *)dalloc((size_t)nq * sizeof(uint32_t), s);
     ovf_n = (uint32_t *)dalloc(sizeof(uint32_t), s);
-    e = (buf && ovf && ovf_n) ? hipSuccess : hipErrorOutOfMemory;
+    over_flags = (uint8_t *)dalloc((size_t)div_up(nq + 1, 8), s);
+    e = (buf && ovf && ovf_n && over_flags) ? hipSuccess : hipErrorOutOfMemory;
     if (e == hipSuccess)
       e = first(MODE_BUFFER, buf, (int32_t)B);
   }
@@ -717,13 +719,12 @@ static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy po
   if (e == hipSuccess)
     e = fill_u64((uint64_t *)(offsets + nq), 0ull, 1, s);
   if (e == hipSuccess)
-    e = exclusive_scan_i64(offsets, offsets, nq + 1, scan_ws, d_err, s);
+    e = exclusive_scan_i64(offsets, offsets, nq + 1, scan_ws, d_err, s, over_flags, B);
   if (e == hipSuccess && B > 0)
   {
     e = fill_u32(ovf_n, 0u, 1, s);
     if (e == hipSuccess)
-      e = launch_overflow_list(offsets, ovf_order, nq, (int32_t)B, ovf_emit_index, ovf, ovf_n,
-                               s);
+      e = launch_overflow_list(over_flags, ovf_order, nq, ovf_emit_index, ovf, ovf_n, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(&n_ovf, ovf_n, sizeof(n_ovf), hipMemcpyDeviceToHost, s);
   }
@@ -760,6 +761,7 @@ static axb_status crs_query(uint32_t *d_err, int64_t nq, axb_traversal_policy po
   dfree(scan_ws, s);
   dfree(ovf, s);
   dfree(ovf_n, s);
+  dfree(over_flags, s);
   if (e != hipSuccess)
   {
     out_free(*indices, s);

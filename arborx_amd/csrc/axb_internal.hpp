@@ -144,11 +144,13 @@ hipError_t radix_sort_pairs_u32(uint32_t *keys0, uint32_t *keys1,
                                 int passes, uint32_t *hist, void *workspace,
                                 uint32_t *err, hipStream_t stream);
 
-// Device-wide scans with decoupled look-back (in place allowed).
+// Device-wide scans, reduce then scan (in place allowed).  over_flags (when
+// given, (n + 7) / 8 bytes): bit q % 8 of byte q / 8 is set when input q
+// exceeds over_limit.
 size_t scan_workspace_bytes(int64_t n);
-hipError_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n,
-                              void *workspace, uint32_t *err,
-                              hipStream_t stream);
+hipError_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, void *workspace,
+                              uint32_t *err, hipStream_t stream, uint8_t *over_flags = nullptr,
+                              int64_t over_limit = 0);
 hipError_t inclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n,
                               void *workspace, uint32_t *err,
                               hipStream_t stream);
@@ -401,9 +403,8 @@ hipError_t launch_pred_morton32(int pred_kind, const float *preds, int64_t nq,
                                 hipStream_t s);
 hipError_t launch_compact(const int32_t *buf, int64_t nq, int32_t buffer,
                           const int64_t *offsets, int32_t *out, hipStream_t s);
-hipError_t launch_overflow_list(const int64_t *offsets, const uint32_t *order, int64_t nq,
-                                int32_t buffer, bool emit_index, uint32_t *ovf,
-                                uint32_t *ovf_n, hipStream_t s);
+hipError_t launch_overflow_list(const uint8_t *over_flags, const uint32_t *order, int64_t nq,
+                                bool emit_index, uint32_t *ovf, uint32_t *ovf_n, hipStream_t s);
 // Neighbour list of a point tree: lane t takes leaf position list[t] (t when
 // list is nullptr); rows are the leaves' values.  rowof (leaf position ->
 // row) is written when non-null.

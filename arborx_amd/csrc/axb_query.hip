@@ -1186,33 +1186,61 @@ __global__ __launch_bounds__(256) void k_pred_morton32(
 // Buffered first pass -> CRS: row q's results sit at buf + q * buffer when
 // they fit (CrsGraphWrapperImpl.hpp:296-318, copy_valid_values); rows that
 // did not fit are written by a second traversal of those queries.
-// Scratch rows that fit -> their CRS rows: 16 lanes per row.
-__global__ void k_compact(const int32_t *__restrict__ buf, int64_t nq, int32_t buffer,
-                          const int64_t *__restrict__ offsets, int32_t *__restrict__ out)
+// Scratch rows that fit -> their CRS rows.  One wavefront per run of
+// COMPACT_ROWS consecutive rows, whose CRS rows are one contiguous range:
+// it writes that range in order, lane l of each step at position
+// start + l + 64 i (whole lines, no partial writes at the rows' unaligned
+// starts), each lane following the row its position falls in through the
+// run's offsets in LDS.  Positions of rows that did not fit are passed over
+// (the second traversal writes them).
+constexpr int COMPACT_ROWS = 64;
+__global__ __launch_bounds__(256) void k_compact(const int32_t *__restrict__ buf, int64_t nq,
+                                                 int32_t buffer,
+                                                 const int64_t *__restrict__ offsets,
+                                                 int32_t *__restrict__ out)
 {
-  const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
-  if (q >= nq)
+  __shared__ int64_t s_off[4][COMPACT_ROWS + 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t q0 = ((int64_t)blockIdx.x * 4 + wave) * COMPACT_ROWS;
+  const int rows = q0 >= nq ? 0 : (int)(nq - q0 < COMPACT_ROWS ? nq - q0 : COMPACT_ROWS);
+  int64_t *so = s_off[wave];
+  // so[r] = offsets[q0 + r] for r <= rows, then offsets[q0 + rows] (empty
+  // rows); offsets holds nq + 1 entries
+  if (rows > 0)
+  {
+    so[lane] = offsets[q0 + (lane < rows ? lane : rows)];
+    if (lane == 0)
+      so[COMPACT_ROWS] = offsets[q0 + rows];
+  }
+  __syncthreads();
+  if (rows == 0)
     return;
-  const int l = threadIdx.x & 15;
-  const int64_t o = offsets[q];
-  const int64_t cnt = offsets[q + 1] - o;
-  if (cnt > buffer)
-    return;
-  const int32_t *src = buf + q * (int64_t)buffer;
-  // one int per lane: the 16 lanes of a row read and write 64 contiguous
-  // bytes per step (16-byte accesses at the rows' unaligned starts measured
-  // 2% slower over the whole radius query)
-  for (int64_t j = l; j < cnt; j += 16)
-    out[o + j] = src[j];
+  const int64_t end = so[rows];
+  int r = 0;
+  for (int64_t j = so[0] + lane; j < end;)
+  {
+    // j < end = so[rows] and so is non-decreasing: r stops below rows
+    while (so[r + 1] <= j)
+      ++r;
+    const int64_t start = so[r], stop = so[r + 1];
+    if (stop - start <= buffer)
+    {
+      out[j] = buf[(q0 + r) * (int64_t)buffer + (j - start)];
+      j += 64;
+    }
+    else // past the row, keeping j's place in the line pattern
+      j += (stop - j + 63) / 64 * 64;
+  }
 }
 
 // Rows whose result count exceeds the buffer, listed in processing
 // (Morton) order: one atomic per wavefront, lanes in order behind it.  The
 // list holds the rows, or the processing indices when emit_index is set.
-__global__ void k_overflow_list(const int64_t *__restrict__ offsets,
-                                const uint32_t *__restrict__ order, int64_t nq, int32_t buffer,
-                                int emit_index, uint32_t *__restrict__ ovf,
-                                uint32_t *__restrict__ ovf_n)
+// over_flags: one bit per row (set by the offsets' scan), 1/64 of the
+// offsets' bytes, so that the lookups in processing order stay in L2.
+__global__ void k_overflow_list(const uint8_t *__restrict__ over_flags,
+                                const uint32_t *__restrict__ order, int64_t nq, int emit_index,
+                                uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_n)
 {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -1221,7 +1249,7 @@ __global__ void k_overflow_list(const int64_t *__restrict__ offsets,
   if (t < nq)
   {
     q = order ? order[t] : (uint32_t)t;
-    f = offsets[q + 1] - offsets[q] > buffer;
+    f = (over_flags[q >> 3] >> (q & 7)) & 1;
   }
   const uint64_t m = __ballot(f);
   if (m == 0)
@@ -1665,18 +1693,17 @@ hipError_t launch_compact(const int32_t *buf, int64_t nq, int32_t buffer,
 {
   if (nq <= 0)
     return hipSuccess;
-  AXB_LAUNCH(k_compact, dim3((unsigned)div_up(nq, 16)), dim3(256), 0, s, buf, nq, buffer,
-             offsets, out);
+  AXB_LAUNCH(k_compact, dim3((unsigned)div_up(nq, 4 * COMPACT_ROWS)), dim3(256), 0, s, buf, nq,
+             buffer, offsets, out);
   return hipGetLastError();
 }
 
-hipError_t launch_overflow_list(const int64_t *offsets, const uint32_t *order, int64_t nq,
-                                int32_t buffer, bool emit_index, uint32_t *ovf,
-                                uint32_t *ovf_n, hipStream_t s)
+hipError_t launch_overflow_list(const uint8_t *over_flags, const uint32_t *order, int64_t nq,
+                                bool emit_index, uint32_t *ovf, uint32_t *ovf_n, hipStream_t s)
 {
   if (nq <= 0)
     return hipSuccess;
-  AXB_LAUNCH(k_overflow_list, dim3(grid_of(nq)), dim3(256), 0, s, offsets, order, nq, buffer,
+  AXB_LAUNCH(k_overflow_list, dim3(grid_of(nq)), dim3(256), 0, s, over_flags, order, nq,
              emit_index ? 1 : 0, ovf, ovf_n);
   return hipGetLastError();
 }

@@ -296,90 +296,138 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
 }
 
 // ------------------------------------------------------------ scans
-constexpr uint64_t SFLAG_AGG = 1ull << 62;
-constexpr uint64_t SFLAG_PREFIX = 2ull << 62;
-constexpr uint64_t SVALUE_MASK = (1ull << 62) - 1ull;
+// Reduce, then scan: (1) the sum of each tile of SCAN_TILE elements,
+// (2) one block turns the tile sums into exclusive tile offsets, (3) each
+// tile scans itself from its offset.  The input is read twice and no block
+// waits on another.  A thread owns SCAN_ITEMS consecutive elements.
 
-template <class Tin, class Tout, bool INCLUSIVE>
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(
-    const Tin *in, Tout *out, int64_t n, uint64_t *__restrict__ status,
-    uint32_t *__restrict__ tile_counter, uint32_t *__restrict__ err)
+// Elements [base, base + SCAN_ITEMS) of a thread, 0 past n; running sums in
+// v.  A whole, 16-byte aligned run is read with 16-byte loads.
+template <class Tin>
+AXB_D int64_t scan_load_items(const Tin *in, int64_t n, int64_t base, int64_t v[SCAN_ITEMS])
 {
-  constexpr int WAVES = SCAN_BLOCK / 64;
-  __shared__ uint32_t s_tile;
-  __shared__ int64_t s_wsum[WAVES];
-  __shared__ int64_t s_prefix;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (tid == 0)
-    s_tile = atomicAdd(tile_counter, 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const int64_t base = (int64_t)tile * SCAN_TILE + (int64_t)tid * SCAN_ITEMS;
-  int64_t v[SCAN_ITEMS];
+  static_assert(SCAN_ITEMS * sizeof(Tin) % 16 == 0, "runs of whole 16-byte words");
+  Tin x[SCAN_ITEMS];
+  if (base + SCAN_ITEMS <= n && ((uintptr_t)(in + base) & 15u) == 0)
+  {
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i += 16 / (int)sizeof(Tin))
+      *(uint4 *)&x[i] = *(const uint4 *)(in + base + i);
+  }
+  else
+  {
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+      x[i] = (base + i < n) ? in[base + i] : Tin(0);
+  }
   int64_t run = 0;
 #pragma unroll
   for (int i = 0; i < SCAN_ITEMS; ++i)
   {
-    int64_t x = (base + i < n) ? (int64_t)in[base + i] : 0;
-    run += x;
-    v[i] = run; // inclusive within the thread
+    run += (int64_t)x[i];
+    v[i] = run;
   }
-  int64_t inc = wave_inclusive_sum(run);
+  return run;
+}
+
+// Exclusive prefix of x over the block's threads, and the block's total
+// (s_wsum: one slot per wavefront).
+template <int BLOCK>
+AXB_D int64_t block_exclusive_sum(int64_t x, int64_t *s_wsum, int64_t &total)
+{
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t inc = wave_inclusive_sum(x);
   if (lane == 63)
     s_wsum[wave] = inc;
   __syncthreads();
-  int64_t add = 0, block_total = 0;
-  for (int w = 0; w < WAVES; ++w)
+  int64_t add = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < BLOCK / 64; ++w)
   {
-    if (w < wave)
-      add += s_wsum[w];
-    block_total += s_wsum[w];
+    add += w < wave ? s_wsum[w] : 0;
+    total += s_wsum[w];
   }
-  const int64_t thread_excl = inc - run + add;
-  if (tid == 0)
+  return inc - x + add;
+}
+
+template <class Tin>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile_sums(const Tin *in, int64_t n,
+                                                               int64_t *__restrict__ sums)
+{
+  __shared__ int64_t s_wsum[SCAN_BLOCK / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int64_t v[SCAN_ITEMS];
+  const int64_t run = scan_load_items(in, n, base, v);
+  int64_t total;
+  block_exclusive_sum<SCAN_BLOCK>(run, s_wsum, total);
+  if (threadIdx.x == 0)
+    sums[blockIdx.x] = total;
+}
+
+// One block: sums[0, tiles) -> their exclusive prefix, in place.
+constexpr int SCAN_OFFSETS_BLOCK = 1024;
+__global__ __launch_bounds__(SCAN_OFFSETS_BLOCK) void k_scan_tile_offsets(int64_t *sums,
+                                                                          int64_t tiles)
+{
+  __shared__ int64_t s_wsum[SCAN_OFFSETS_BLOCK / 64];
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < tiles; c0 += (int64_t)SCAN_OFFSETS_BLOCK * SCAN_ITEMS)
   {
-    uint64_t *my = status + tile;
-    int64_t excl = 0;
-    if (tile == 0)
-      st_agent64(my, SFLAG_PREFIX | (uint64_t)block_total);
-    else
-    {
-      st_agent64(my, SFLAG_AGG | (uint64_t)block_total);
-      int64_t j = (int64_t)tile - 1;
-      uint32_t spins = 0;
-      while (j >= 0)
-      {
-        uint64_t s = ld_agent64(status + j);
-        uint64_t flag = s & ~SVALUE_MASK;
-        if (flag == 0)
-        {
-          if (++spins >= SPIN_LIMIT)
-          {
-            raise_error(err, KERR_SPIN);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        excl += (int64_t)(s & SVALUE_MASK);
-        if (flag == SFLAG_PREFIX)
-          break;
-        --j;
-      }
-      st_agent64(my, SFLAG_PREFIX | (uint64_t)(excl + block_total));
-    }
-    s_prefix = excl;
+    const int64_t base = c0 + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int64_t v[SCAN_ITEMS];
+    const int64_t run = scan_load_items(sums, tiles, base, v);
+    int64_t total;
+    const int64_t pre = carry + block_exclusive_sum<SCAN_OFFSETS_BLOCK>(run, s_wsum, total);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+      if (base + i < tiles)
+        sums[base + i] = pre + (i == 0 ? 0 : v[i - 1]);
+    carry += total;
+    __syncthreads(); // s_wsum is rewritten by the next chunk
   }
-  __syncthreads();
-  const int64_t pre = s_prefix + thread_excl;
+}
+
+// flags (when given): bit i of byte e is set when input e * 8 + i exceeds
+// limit (the CRS's rows that overflow their scratch row, read from the
+// counts as they are scanned).
+template <class Tin, class Tout, bool INCLUSIVE>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const Tin *in, Tout *out, int64_t n,
+                                                           const int64_t *__restrict__ offs,
+                                                           uint8_t *__restrict__ flags,
+                                                           int64_t limit)
+{
+  static_assert(SCAN_ITEMS == 8, "one flag byte per thread");
+  __shared__ int64_t s_wsum[SCAN_BLOCK / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int64_t v[SCAN_ITEMS];
+  const int64_t run = scan_load_items(in, n, base, v);
+  if (flags && base < n)
+  {
+    uint32_t bits = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+      bits |= (uint32_t)(base + i < n && v[i] - (i == 0 ? 0 : v[i - 1]) > limit) << i;
+    flags[base >> 3] = (uint8_t)bits;
+  }
+  int64_t total;
+  const int64_t pre = offs[blockIdx.x] + block_exclusive_sum<SCAN_BLOCK>(run, s_wsum, total);
+  Tout y[SCAN_ITEMS];
 #pragma unroll
   for (int i = 0; i < SCAN_ITEMS; ++i)
+    y[i] = (Tout)(pre + (INCLUSIVE ? v[i] : (i == 0 ? 0 : v[i - 1])));
+  if (base + SCAN_ITEMS <= n && ((uintptr_t)(out + base) & 15u) == 0)
   {
-    if (base + i < n)
-    {
-      int64_t x = INCLUSIVE ? v[i] : (i == 0 ? 0 : v[i - 1]);
-      out[base + i] = (Tout)(pre + x);
-    }
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i += 16 / (int)sizeof(Tout))
+      *(uint4 *)(out + base + i) = *(const uint4 *)&y[i];
+  }
+  else
+  {
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+      if (base + i < n)
+        out[base + i] = y[i];
   }
 }
 
@@ -462,33 +510,33 @@ hipError_t radix_sort_pairs_u32(uint32_t *keys0, uint32_t *keys1,
 
 size_t scan_workspace_bytes(int64_t n)
 {
-  return sizeof(uint64_t) * (size_t)(div_up(n, SCAN_TILE) + 1);
+  return sizeof(int64_t) * (size_t)(div_up(n, SCAN_TILE) + 1);
 }
 
+// In place (out == in) is allowed: each element is read and written by the
+// same thread.  n < 2^31 * SCAN_TILE (the grid's x limit).
 template <class Tin, class Tout, bool INC>
-static hipError_t scan_impl(const Tin *in, Tout *out, int64_t n,
-                            void *workspace, uint32_t *err, hipStream_t stream)
+static hipError_t scan_impl(const Tin *in, Tout *out, int64_t n, void *workspace, uint32_t *,
+                            hipStream_t stream, uint8_t *flags = nullptr, int64_t limit = 0)
 {
   if (n <= 0)
     return hipSuccess;
   const int64_t tiles = div_up(n, SCAN_TILE);
-  uint64_t *status = (uint64_t *)workspace;
-  // counter lives in the last word
-  AXB_LAUNCH(k_fill_u64, dim3(fill_grid(tiles + 1)), dim3(256), 0,
-                     stream, status, 0ull, tiles + 1);
-  uint32_t *counter = (uint32_t *)(status + tiles);
-  AXB_LAUNCH((k_scan<Tin, Tout, INC>), dim3((unsigned)tiles),
-                     dim3(SCAN_BLOCK), 0, stream, in, out, n, status, counter,
-                     err);
+  int64_t *sums = (int64_t *)workspace;
+  AXB_LAUNCH(k_scan_tile_sums<Tin>, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, stream, in, n,
+             sums);
+  AXB_LAUNCH(k_scan_tile_offsets, dim3(1), dim3(SCAN_OFFSETS_BLOCK), 0, stream, sums, tiles);
+  AXB_LAUNCH((k_scan_tiles<Tin, Tout, INC>), dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, stream,
+             in, out, n, sums, flags, limit);
   return hipGetLastError();
 }
 
-hipError_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n,
-                              void *workspace, uint32_t *err,
-                              hipStream_t stream)
+hipError_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, void *workspace,
+                              uint32_t *err, hipStream_t stream, uint8_t *over_flags,
+                              int64_t over_limit)
 {
-  return scan_impl<int64_t, int64_t, false>(in, out, n, workspace, err,
-                                            stream);
+  return scan_impl<int64_t, int64_t, false>(in, out, n, workspace, err, stream, over_flags,
+                                            over_limit);
 }
 hipError_t inclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n,
                               void *workspace, uint32_t *err,
