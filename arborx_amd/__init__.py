@@ -63,7 +63,7 @@ EXPORTED_SYMBOLS = (
     "axb_bruteforce_query_intersects_radius", "axb_bruteforce_query_nearest",
     "axb_bruteforce_query_nearest_rays",
     "axb_bruteforce_view",
-    "axb_bvh_view", "axb_bvh_predicate_order", "axb_exclusive_scan_i64",
+    "axb_bvh_view", "axb_bvh_predicate_order", "axb_exclusive_scan_i64", "axb_gather_pairs",
     "axb_bvh_num_slots", "axb_bvh_slots", "axb_bvh_export_topology",
     "axb_sqrt_threshold", "axb_morton64_host", "axb_q16_encode_host", "axb_profile_enable",
     "axb_profile_clear",

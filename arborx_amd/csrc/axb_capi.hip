@@ -1812,3 +1812,77 @@ axb_status axb_bruteforce_query_nearest(const axb_bruteforce *bf, const float *p
 }
 
 } // extern "C"
+
+namespace
+{
+// Record j of out = {values[indices[j]] (VF floats), indices[j]}; an index
+// outside [0, n_values) sets *bad and leaves its record unwritten.
+template <int VF>
+__global__ __launch_bounds__(256) void k_gather_pairs(const float *__restrict__ values,
+                                                      int64_t n_values,
+                                                      const int32_t *__restrict__ indices,
+                                                      int64_t nnz, uint32_t *__restrict__ out,
+                                                      uint32_t *__restrict__ bad)
+{
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnz)
+    return;
+  const int32_t i = indices[j];
+  if (i < 0 || i >= n_values)
+  {
+    *bad = 1u;
+    return;
+  }
+  uint32_t *o = out + j * (VF + 1);
+#pragma unroll
+  for (int d = 0; d < VF; ++d)
+    o[d] = __float_as_uint(values[(int64_t)i * VF + d]);
+  o[VF] = (uint32_t)i;
+}
+} // namespace
+
+extern "C" {
+
+axb_status axb_gather_pairs(const float *values, int32_t value_floats, int64_t n_values,
+                            const int32_t *indices, int64_t nnz, void *out,
+                            axb_stream_t stream_)
+{
+  if (value_floats != 3 && value_floats != 6)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT,
+             "axb_gather_pairs: values are points (3 floats) or boxes (6 floats)");
+  if (nnz < 0 || n_values < 0 || (nnz > 0 && (!values || !indices || !out)))
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_gather_pairs: bad argument");
+  if (nnz == 0)
+    return AXB_OK;
+  hipStream_t s = (hipStream_t)stream_;
+  uint32_t *bad = (uint32_t *)dalloc(sizeof(uint32_t), s);
+  if (!bad)
+    AXB_FAIL(AXB_ERR_OUT_OF_MEMORY, "axb_gather_pairs: out of device memory");
+  uint32_t bad_host = 0;
+  hipError_t e = fill_u32(bad, 0u, 1, s);
+  const unsigned grid = (unsigned)div_up(nnz, 256);
+  if (e == hipSuccess && value_floats == 3)
+  {
+    AXB_LAUNCH(k_gather_pairs<3>, dim3(grid), dim3(256), 0, s, values, n_values, indices, nnz,
+               (uint32_t *)out, bad);
+    e = hipGetLastError();
+  }
+  else if (e == hipSuccess)
+  {
+    AXB_LAUNCH(k_gather_pairs<6>, dim3(grid), dim3(256), 0, s, values, n_values, indices, nnz,
+               (uint32_t *)out, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&bad_host, bad, sizeof(bad_host), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = sync_stream(s);
+  dfree(bad, s);
+  if (e != hipSuccess)
+    return hip_fail(e, "axb_gather_pairs");
+  if (bad_host)
+    AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "axb_gather_pairs: an index is outside [0, n_values)");
+  return AXB_OK;
+}
+
+} // extern "C"

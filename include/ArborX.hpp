@@ -228,6 +228,36 @@ private:
   hipStream_t _stream = nullptr;
 };
 
+// ArborX_PairValueIndex.hpp:22-46: a value with its position in the values
+// the index was built from (the values of a tree built over
+// attach_indices(values)).  32-bit indices.
+template <class Value, class Index = unsigned>
+struct PairValueIndex
+{
+  static_assert(std::is_integral_v<Index> && sizeof(Index) == 4, "32-bit indices");
+  using value_type = Value;
+  using index_type = Index;
+
+  Value value;
+  Index index;
+};
+
+namespace Experimental
+{
+// ArborX_AttachIndices.hpp:23-65: values whose positions travel with them;
+// a tree built over them returns PairValueIndex<Value, Index> values.
+template <class Values, class Index>
+struct AttachIndices
+{
+  Values const &_values;
+};
+template <class Index = unsigned, class Values>
+AttachIndices<Values, Index> attach_indices(Values const &values)
+{
+  return AttachIndices<Values, Index>{values};
+}
+} // namespace Experimental
+
 namespace Experimental
 {
 // detail/ArborX_TraversalPolicy.hpp:19-47
@@ -434,12 +464,31 @@ public:
     build(space, (float const *)values.data(), (int64_t)values.size(), AXB_GEOMETRY_BOXES,
           fmt);
   }
+  // Built over attach_indices(values) (ArborX_LinearBVH.hpp:50-57 with
+  // PairValueIndex values): the tree keeps a copy of the values, and the
+  // query overload taking DeviceVector<PairValueIndex<...>> returns them
+  // with their indices.
+  template <class G, class Index>
+  BoundingVolumeHierarchy(HipStream const &space,
+                          Experimental::AttachIndices<DeviceVector<G>, Index> const &values,
+                          NodeFormat fmt = NodeFormat::FP32)
+      : BoundingVolumeHierarchy(space, values._values, fmt)
+  {
+    static_assert(std::is_same_v<G, Point> || std::is_same_v<G, Box>, "points or boxes");
+    DeviceVector<G> const &v = values._values;
+    _values.resize(v.size() * (sizeof(G) / sizeof(float)));
+    if (v.size())
+      Details::hip_check(hipMemcpyAsync(_values.data(), v.data(), v.size() * sizeof(G),
+                                        hipMemcpyDeviceToDevice, space.stream),
+                         "hipMemcpyAsync");
+    _value_floats = (int)(sizeof(G) / sizeof(float));
+  }
   BoundingVolumeHierarchy(BoundingVolumeHierarchy const &) = delete;
   BoundingVolumeHierarchy &operator=(BoundingVolumeHierarchy const &) = delete;
-  BoundingVolumeHierarchy(BoundingVolumeHierarchy &&o) noexcept { std::swap(_h, o._h); }
+  BoundingVolumeHierarchy(BoundingVolumeHierarchy &&o) noexcept { swap(o); }
   BoundingVolumeHierarchy &operator=(BoundingVolumeHierarchy &&o) noexcept
   {
-    std::swap(_h, o._h);
+    swap(o);
     return *this;
   }
   ~BoundingVolumeHierarchy()
@@ -565,6 +614,28 @@ public:
     nearest_impl(space, preds, indices, offsets, &distances, policy);
   }
 
+  // query(space, predicates, values, offsets[, policy]) for a tree built
+  // over attach_indices: each result as PairValueIndex{value, index}, rows
+  // as the index form's.
+  template <class Predicates, class V, class I>
+  void query(HipStream const &space, Predicates const &preds,
+             DeviceVector<PairValueIndex<V, I>> &values, DeviceVector<int64_t> &offsets,
+             Experimental::TraversalPolicy const &policy = {}) const
+  {
+    static_assert(sizeof(PairValueIndex<V, I>) == sizeof(V) + 4 && sizeof(V) % 4 == 0,
+                  "packed (value, index) records");
+    if (!empty() && _value_floats != (int)(sizeof(V) / sizeof(float)))
+      throw std::invalid_argument(
+          "BVH::query: the tree was not built over attach_indices of values of this type");
+    DeviceVector<int> idx;
+    query(space, preds, idx, offsets, policy);
+    values.resize(idx.size());
+    if (idx.size())
+      Details::check(axb_gather_pairs(_values.data(), _value_floats, size(), idx.data(),
+                                      (int64_t)idx.size(), values.data(),
+                                      (axb_stream_t)space.stream));
+  }
+
   axb_bvh const *c_handle() const { return _h; }
 
 #if defined(__HIPCC__)
@@ -667,7 +738,16 @@ private:
       distances->adopt(dist, (size_t)nnz, space.stream);
   }
 
+  void swap(BoundingVolumeHierarchy &o) noexcept
+  {
+    std::swap(_h, o._h);
+    std::swap(_values, o._values);
+    std::swap(_value_floats, o._value_floats);
+  }
+
   axb_bvh *_h = nullptr;
+  DeviceVector<float> _values; // attach_indices: a copy of the values
+  int _value_floats = 0;       // 3 (points) or 6 (boxes); 0 without them
 };
 
 using BVH = BoundingVolumeHierarchy;

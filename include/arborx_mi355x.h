@@ -374,6 +374,18 @@ axb_status axb_bvh_predicate_order(const axb_bvh *bvh, int32_t predicate_kind,
                                    const float *predicates, int64_t nq,
                                    uint32_t *order, axb_stream_t stream);
 
+/* Results as (value, index) records: the PairValueIndex values of a tree
+ * built over attach_indices(values) (ArborX_PairValueIndex.hpp:22-46,
+ * ArborX_AttachIndices.hpp:23-65), for the C++ facade's query overload
+ * that returns them.  Record j of out (device, nnz records of
+ * 4 * (value_floats + 1) bytes) = {values[indices[j]] (value_floats floats:
+ * 3 for points, 6 for boxes), indices[j] (32-bit)}.  values: device,
+ * n_values x value_floats floats; an index outside [0, n_values) fails with
+ * AXB_ERR_INVALID_ARGUMENT.  Synchronises the stream. */
+axb_status axb_gather_pairs(const float *values, int32_t value_floats, int64_t n_values,
+                            const int32_t *indices, int64_t nnz, void *out,
+                            axb_stream_t stream);
+
 /* Exclusive prefix sum of n int64 values (device; in place allowed),
  * the scan of the two-pass CRS (KokkosExt::exclusive_scan). */
 axb_status axb_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n,

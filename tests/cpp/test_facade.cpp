@@ -305,6 +305,88 @@ static void brute_force(HipStream const &space, Make make)
   CHECK(ok_k, "kNN equals brute force (distance, index)");
 }
 
+// Trees over attach_indices(values) (tstAttachIndices.cpp:19-29: unsigned
+// indices by default, index = position): queries returning
+// PairValueIndex{value, index} give the index form's rows with each value
+// equal to values[index]; boxes keep their six floats; an empty tree gives
+// empty rows; values of another type are refused.
+static void attach_indices_values(HipStream const &space, NodeFormat fmt)
+{
+  static_assert(std::is_same_v<decltype(PairValueIndex<Point>{}.index), unsigned>);
+  std::mt19937 gen(7);
+  std::uniform_real_distribution<float> U(0.f, 1.f);
+  const int n = 3000, nq = 500, k = 5;
+  std::vector<Point> pts(n), qp(nq);
+  for (auto &p : pts)
+    p = {{U(gen), U(gen), U(gen)}};
+  for (auto &p : qp)
+    p = {{U(gen), U(gen), U(gen)}};
+  DeviceVector<Point> v(pts), q(qp);
+  BVH plain(space, v, fmt);
+  BVH tree(space, Experimental::attach_indices(v), fmt);
+  DeviceVector<int> I;
+  DeviceVector<int64_t> O, O2;
+  DeviceVector<PairValueIndex<Point>> P;
+  bool ok = true;
+  for (int form = 0; form < 2; ++form)
+  {
+    if (form == 0)
+    {
+      plain.query(space, Experimental::make_intersects(q, 0.08f), I, O);
+      tree.query(space, Experimental::make_intersects(q, 0.08f), P, O2);
+    }
+    else
+    {
+      plain.query(space, Experimental::make_nearest(q, k), I, O);
+      tree.query(space, Experimental::make_nearest(q, k), P, O2);
+    }
+    auto i = I.to_host();
+    auto pv = P.to_host();
+    ok = ok && O.to_host() == O2.to_host() && pv.size() == i.size() && !i.empty();
+    for (size_t j = 0; ok && j < i.size(); ++j)
+      ok = pv[j].index == (unsigned)i[j] && pv[j].value[0] == pts[i[j]][0] &&
+           pv[j].value[1] == pts[i[j]][1] && pv[j].value[2] == pts[i[j]][2];
+  }
+  CHECK(ok, "attach_indices points: PairValueIndex rows = index rows with their values");
+
+  std::vector<Box> bx(n);
+  for (int j = 0; j < n; ++j)
+    bx[j] = Box({{pts[j][0], pts[j][1], pts[j][2]}},
+                {{pts[j][0] + 0.01f, pts[j][1] + 0.02f, pts[j][2] + 0.03f}});
+  DeviceVector<Box> vb(bx);
+  BVH btree(space, Experimental::attach_indices<int>(vb), fmt);
+  DeviceVector<PairValueIndex<Box, int>> PB;
+  btree.query(space, Experimental::make_intersects(q, 0.05f), PB, O2);
+  BVH bplain(space, vb, fmt);
+  bplain.query(space, Experimental::make_intersects(q, 0.05f), I, O);
+  auto i = I.to_host();
+  auto pb = PB.to_host();
+  bool okb = O.to_host() == O2.to_host() && pb.size() == i.size() && !i.empty();
+  for (size_t j = 0; okb && j < i.size(); ++j)
+    okb = pb[j].index == i[j] && pb[j].value.minCorner()[0] == bx[i[j]].minCorner()[0] &&
+          pb[j].value.maxCorner()[2] == bx[i[j]].maxCorner()[2];
+  CHECK(okb, "attach_indices boxes: PairValueIndex<Box, int> rows");
+
+  DeviceVector<Point> none;
+  BVH empty_tree(space, Experimental::attach_indices(none), fmt);
+  empty_tree.query(space, Experimental::make_intersects(q, 0.08f), P, O2);
+  auto o = O2.to_host();
+  CHECK(P.size() == 0 && o.size() == (size_t)nq + 1 && o.back() == 0,
+        "attach_indices over no values: empty rows");
+
+  bool threw = false;
+  try
+  {
+    DeviceVector<PairValueIndex<Point>> wrong;
+    btree.query(space, Experimental::make_intersects(q, 0.05f), wrong, O2);
+  }
+  catch (std::invalid_argument const &)
+  {
+    threw = true;
+  }
+  CHECK(threw, "PairValueIndex of another value type is refused");
+}
+
 // tstNeighborList.cpp:116-176: four collinear points, full list and half
 // list expanded to full.
 static void neighbor_lists(HipStream const &space)
@@ -410,6 +492,7 @@ int main()
     manufactured(space, MakeBVH{fmt});
     brute_force(space, MakeBVH{fmt});
     rays(space, MakeBVH{fmt});
+    attach_indices_values(space, fmt);
   }
   rays(space, MakeBruteForce{});
   policy(space, MakeBVH{NodeFormat::FP32});
