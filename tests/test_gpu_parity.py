@@ -289,7 +289,9 @@ def test_buffer_policy():
     o_ref, i_ref = t.query_spatial(0, spheres_from_points(v, r))
     bvh = ax.BoundingVolumeHierarchy(dev(v))
     pts = dev(v)
-    for b in (64, 8, 1):  # fits, overflows (second pass), overflows
+    # fits; then rows overflow (second pass) at sizes that split the
+    # compaction's 64-wide steps unevenly
+    for b in (64, 37, 8, 3, 1):
         i, o = bvh.query(ax.make_intersects(pts, r), ax.TraversalPolicy().setBufferSize(b))
         np.testing.assert_array_equal(host(o), o_ref)
         np.testing.assert_array_equal(host(i), i_ref)
