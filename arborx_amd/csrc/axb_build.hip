@@ -704,8 +704,8 @@ __global__ void k_single_leaf(const float *values, uint4 *slots, float *scene)
     scene[d] = lo[d], scene[3 + d] = hi[d];
 }
 
-// Zeroes the SLOT_PAD slots after the last stored record (the read-ahead of
-// the walks' leaf groups sees records of neither kind there).
+// Zeroes the SLOT_PAD slots after the last stored record (read as slot
+// p + 1 by the box walks after a record at the end of the array).
 template <int SL, int SI>
 __global__ void k_pad_tail(uint4 *slots, const int32_t *C, int n)
 {

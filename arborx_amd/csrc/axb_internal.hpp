@@ -334,7 +334,7 @@ AXB_D void row_epilogue(const TreeView &tv, W<MODE> &rw, int64_t *offsets, int64
 }
 
 
-constexpr int SLOT_PAD = 4; // zeroed slots after the array (read-ahead of leaf groups)
+constexpr int SLOT_PAD = 4; // zeroed slots after the array (the box walks read slot p + 1)
 int64_t total_slots_for(int64_t n, int geometry, int node_format);
 // Scene box {lo xyz, hi xyz} of n points or boxes (TreeConstruction.hpp:26-39
 // semantics; the empty box {+FLT_MAX, -FLT_MAX} for n = 0) into device scene.

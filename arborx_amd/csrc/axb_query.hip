@@ -912,75 +912,49 @@ AXB_D int knn_collect_walk(const TreeView &tv, float qx, float qy, float qz, flo
       stage.flush(cand, row_at, c & ~(KNN_STAGE - 1));
     return c;
   }
+  // Box leaves: one record per step, leaf and internal node through the
+  // same code, as in rope_walk_boxes (the box is the leaf's exact one or the
+  // node's decoded one, picked by selects).  The walk stops at cap + 1
+  // candidates (the next round shrinks the radius).
   int c = 0;
   uint32_t p = 0;
   const uint32_t end = tv.total;
-  Box6 par;
-  float ps[3];
-  bool desc = false;
-  while (p < end)
+  uint32_t step = 1; // 0 after a zero rope (reported below), which ends the loop
+  WalkFrame fr;
+  walk_frame_init(tv, fr);
+  while (step != 0 && p < end && c <= cap)
   {
     const uint4 s0 = tv.slots[p];
-    const uint4 s1 = (F == NODES_FP32 || G == GEOM_BOXES) ? tv.slots[p + 1] : s0;
+    const uint4 s1 = tv.slots[p + 1];
     const int32_t w = (int32_t)s0.w;
-    if (w < 0)
+    const bool leaf = w < 0;
+    Box6 bx;
+    walk_node_box<F>(s0, s1, fr, bx);
+    if (F == NODES_Q16)
     {
-      // this leaf and the leaf records right after it
-      constexpr int LG = leaf_group<G>();
-      uint4 a[LG], b[LG];
-      a[0] = s0, b[0] = s1;
-#pragma unroll
-      for (int j = 1; j < LG; ++j)
-      {
-        a[j] = tv.slots[p + j * SL];
-        b[j] = G == GEOM_BOXES ? tv.slots[p + j * SL + 1] : a[j];
-      }
-      // p advances with each leaf handled, so it is record j's slot here
-      bool run = true;
-#pragma unroll
-      for (int j = 0; j < LG; ++j)
-      {
-        run = run && c <= cap && p < end && (int32_t)a[j].w < 0;
-        if (run)
-        {
-          ++v_leaf;
-          const float d2 = leaf_d2<G>(qx, qy, qz, a[j], b[j]);
-          if (d2 <= thr)
-          {
-            if (c < cap)
-              cand[row_at + c] = make_uint2((uint32_t)~(int32_t)a[j].w, f_bits(d2));
-            ++c;
-          }
-          p += SL;
-        }
-      }
-      if (c > cap)
-        break;
-      desc = false;
+      bx.lo[0] = leaf ? bits_f(s0.x) : bx.lo[0];
+      bx.lo[1] = leaf ? bits_f(s0.y) : bx.lo[1];
+      bx.lo[2] = leaf ? bits_f(s0.z) : bx.lo[2];
+      bx.hi[0] = leaf ? bits_f(s1.x) : bx.hi[0];
+      bx.hi[1] = leaf ? bits_f(s1.y) : bx.hi[1];
+      bx.hi[2] = leaf ? bits_f(s1.z) : bx.hi[2];
     }
-    else
+    const float d2 = dist2_pnode(qx, qy, qz, bx); // leaf_d2 for a leaf
+    const bool pass = d2 <= thr;
+    v_leaf += leaf;
+    v_int += !leaf;
+    if (leaf && pass)
     {
-      ++v_int;
-      Box6 b;
-      node_box<F>(tv, s0, s1, desc, par, ps, b);
-      if (dist2_pnode(qx, qy, qz, b) <= thr)
-      {
-        set_frame<F>(b, par, ps);
-        desc = true;
-        p += SI;
-      }
-      else
-      {
-        if (w == 0)
-        {
-          raise_error(tv.err, KERR_LAYOUT);
-          return c;
-        }
-        desc = false;
-        p += (uint32_t)w;
-      }
+      if (c < cap)
+        cand[row_at + c] = make_uint2((uint32_t)~w, f_bits(d2));
+      ++c;
     }
+    walk_frame_next<F>(tv, !leaf && pass, bx, fr);
+    step = leaf ? SL : (pass ? SI : (uint32_t)w);
+    p += step;
   }
+  if (step == 0)
+    raise_error(tv.err, KERR_LAYOUT);
   return c;
 }
 

@@ -618,16 +618,6 @@ AXB_HD bool leaf_hit(const SpatialPred &pr, uint4 s0, uint4 s1)
                           : dist2_pbox(pr.cx, pr.cy, pr.cz, lb) <= pr.thr;
 }
 
-// Leaf records handled per step of the nearest candidates' walk over box
-// leaves: a run of consecutive leaves is read together (64 bytes).  The
-// node array keeps LEAF_GROUP * SL - 1 zeroed slots after its last record
-// for this read-ahead (and for the spatial walks' read of slot p + 1).
-template <int G>
-constexpr int leaf_group()
-{
-  return G == GEOM_POINTS ? 4 : 2;
-}
-
 // (Host-callable as well, so the CPU tests run this very walk over the
 // oracle's node arrays.)
 // Stackless pre-order walk over slots [p, end) (TreeTraversal.hpp:97-119:
