@@ -643,8 +643,8 @@ static axb_status predicate_order(const axb_bvh *t, uint32_t *err, int pred_kind
 } // extern "C"
 
 // Bytes the scratch rows of a buffered CRS query may take: an eighth of the
-// current device's memory (36 GB on MI355X: 64 results per row for up to
-// 140M queries), at least 4 GiB.
+// current device's memory (36 GB on MI355X: 96 results per row for up to
+// 93M queries, shorter rows beyond), at least 4 GiB.
 static int64_t scratch_rows_budget()
 {
   static std::atomic<int64_t> budget[64] = {};
@@ -674,7 +674,8 @@ static int64_t scratch_rows_budget()
 // processing order `ovf_order` (processing index -> row, nullptr: identity)
 // and holds rows, or processing indices when `ovf_emit_index`.
 // TraversalPolicy buffer_size: 0 -> a scratch row of up to `auto_rows`
-// results (64; 96 for rays) when the scratch fits scratch_rows_budget(),
+// results (96 for the tree's queries and neighbour lists, 64 for brute
+// force) when the scratch fits scratch_rows_budget(),
 // else plain count + fill; > 0 -> that many;
 // < 0 -> that many, and a row that overflows is an error
 // (AXB_ERR_BUFFER_OVERFLOW, the reference's SearchException).
@@ -918,13 +919,14 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
     return launch_spatial(tv, t->node_format, t->geometry, pred_kind, preds, n_list, radius,
                           list ? list : order, offsets, out, 0, MODE_FILL, s);
   };
-  // Ray rows have a long tail (1M rays over 5M boxes: mean 10, 0.1% above
-  // 64): the few rows that overflow a scratch row are walked again by a
-  // launch of few, long walks, 5 ms of latency; rows of 96 avoid it
-  // (intersects 21.2 -> 16.1 ms).  Radius rows: 64 (7.64 ms, 96: 7.75 ms).
-  const bool rays = pred_kind == PRED_RAYS || ordered;
+  // Rows of 96: rows that overflow their scratch row are walked again, and
+  // row counts have tails -- rays (1M rays over 5M boxes: mean 10, 0.1%
+  // above 64; intersects 21.2 -> 16.1 ms against rows of 64), radius rows
+  // of clustered points (10M-point Gaussian mixture, ~32 neighbours: 8.6 ->
+  // 7.1 ms) -- while rows of ~32 uniform neighbours take the same time with
+  // 64 or 96 (6.9 ms at 10M points).
   axb_status st = crs_query(err.p, nq, policy, order, false, first, fill, offsets, indices,
-                            nnz_host, s, "query", tolerated, rays ? 96 : 64);
+                            nnz_host, s, "query", tolerated, 96);
   dfree(order_blk, s);
   if (st == AXB_OK && sorted_rows && *nnz_host > 0)
   {
@@ -1208,8 +1210,11 @@ axb_status axb_bvh_neighbor_list(const axb_bvh *t, float radius, int32_t full,
     return launch_nlist(tv, t->node_format, f, radius, n_list, list, offsets, out, 0, nullptr,
                         MODE_FILL, s);
   };
+  // rows of 96, as for queries: the 2M-atom FCC lattice's ~83 neighbours
+  // per row take 5.1 ms with rows of 64 (most rows walked twice), 2.4 ms
+  // with 96; ~32 uniform neighbours 6.2 vs 6.3 ms at 10M points
   axb_status st = crs_query(err.p, n, policy, rowof, true, first, fill, offsets, indices,
-                            nnz_host, s, "axb_bvh_neighbor_list");
+                            nnz_host, s, "axb_bvh_neighbor_list", nullptr, 96);
   dfree(rowof, s);
   return st;
 }
