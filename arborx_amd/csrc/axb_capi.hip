@@ -854,8 +854,9 @@ static axb_status query_spatial(const axb_bvh *t, int pred_kind, const float *pr
                                 int32_t limit = INT32_MAX, uint32_t *tolerated = nullptr)
 {
   Region region("ArborX::BVH::query_crs");
-  Region tag_region(pred_kind == AXB_PREDICATE_ORDERED_RAYS ? "ArborX::BVH::query::ordered_spatial"
-                                                           : "ArborX::BVH::query::spatial");
+  Region tag_region(pred_kind == AXB_PREDICATE_ORDERED_RAYS || ordered_geom_kind(pred_kind) >= 0
+                        ? "ArborX::BVH::query::ordered_spatial"
+                        : "ArborX::BVH::query::spatial");
   if (!t || !offsets || !indices || !nnz_host)
     AXB_FAIL(AXB_ERR_INVALID_ARGUMENT, "query: NULL argument");
   const bool ordered = pred_kind == AXB_PREDICATE_ORDERED_RAYS;
